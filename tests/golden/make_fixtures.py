@@ -1,0 +1,131 @@
+"""Writes the golden fixtures of the solve path.
+
+Each fixture is produced by ``oracle/_ref/refdrv`` (a driver of this
+repository compiled against the reference sources in place, see
+``oracle/Makefile``) and packed here as a compressed ``.npz``:
+
+* ``A_*``      : the condensed operator as scalar CSR exactly as the reference
+                 stores it (``Diffusion_HHO.h:633-634``, RowMajor, int32),
+* ``b``, ``x`` : right-hand side and the reference solver's solution,
+* ``hist``     : normalised residual after each iteration (IterationResult.h:121-139),
+* ``levelL_P``/``levelL_A`` : multigrid prolongations and coarse operators
+                 exported by ``Level::Setup`` (Level.h:116-125),
+* ``agg1``     : first-pass pairwise aggregate of every block row
+                 (AlgebraicMesh.h:54-125, PairwiseAggregation.h:9-54).
+
+Run: ``python tests/golden/make_fixtures.py`` (needs the reference tree).
+"""
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+import scipy.io as sio
+import scipy.sparse as sp
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+REFDRV = os.path.join(REPO, "oracle", "_ref", "refdrv")
+
+
+def read_csr(path):
+    with open(path, "rb") as f:
+        rows, cols, nnz = np.frombuffer(f.read(24), dtype=np.int64)
+        rp = np.frombuffer(f.read(4 * (rows + 1)), dtype=np.int32)
+        ci = np.frombuffer(f.read(4 * nnz), dtype=np.int32)
+        v = np.frombuffer(f.read(8 * nnz), dtype=np.float64)
+    return sp.csr_matrix((v, ci, rp), shape=(int(rows), int(cols)))
+
+
+def read_vec(path, dtype=np.float64):
+    with open(path, "rb") as f:
+        n = int(np.frombuffer(f.read(8), dtype=np.int64)[0])
+        return np.frombuffer(f.read(8 * n), dtype=dtype).copy()
+
+
+def read_info(path):
+    out = {}
+    for line in open(path):
+        k, v = line.split()
+        out[k] = float(v)
+    return out
+
+
+def pack_csr(prefix, M, d):
+    M = M.tocsr()
+    M.sort_indices()
+    d[prefix + "_rowptr"] = M.indptr.astype(np.int32)
+    d[prefix + "_col"] = M.indices.astype(np.int32)
+    d[prefix + "_val"] = M.data.astype(np.float64)
+    d[prefix + "_shape"] = np.array(M.shape, dtype=np.int64)
+
+
+def run(name, args, export_mg=False, keep_A=True, aggreg=None):
+    with tempfile.TemporaryDirectory() as tmp:
+        cmd = [REFDRV] + args + ["-out", tmp]
+        if export_mg:
+            cmd.append("-export-mg")
+        subprocess.run(cmd, check=True, stdout=subprocess.DEVNULL)
+        info = read_info(os.path.join(tmp, "info.txt"))
+        d = {"cmd": np.array(" ".join(["refdrv"] + args))}
+        if keep_A:
+            pack_csr("A", read_csr(os.path.join(tmp, "A.csr")), d)
+            d["b"] = read_vec(os.path.join(tmp, "b.vec"))
+        if os.path.exists(os.path.join(tmp, "x.vec")):
+            d["x"] = read_vec(os.path.join(tmp, "x.vec"))
+            d["hist"] = read_vec(os.path.join(tmp, "hist.vec"))
+        for k, v in info.items():
+            d["info_" + k] = np.array(v)
+        if export_mg:
+            lvl = 0
+            while os.path.exists(os.path.join(tmp, "level%d_A.dat" % lvl)):
+                if lvl > 0:
+                    pack_csr("level%d_A" % lvl, sio.mmread(os.path.join(tmp, "level%d_A.dat" % lvl)), d)
+                pf = os.path.join(tmp, "level%d_P.dat" % lvl)
+                if os.path.exists(pf):
+                    pack_csr("level%d_P" % lvl, sio.mmread(pf), d)
+                lvl += 1
+            d["nlevels"] = np.array(lvl)
+        if aggreg is not None:
+            cmd = [REFDRV] + aggreg + ["-out", tmp, "-aggreg"]
+            subprocess.run(cmd, check=True, stdout=subprocess.DEVNULL)
+            d["agg1"] = read_vec(os.path.join(tmp, "agg1.ivec"), dtype=np.int64).astype(np.int32)
+            d["agg_cmd"] = np.array(" ".join(["refdrv"] + aggreg + ["-aggreg"]))
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), **d)
+        print(name, {k: (v.shape if hasattr(v, "shape") else v) for k, v in d.items() if not k.endswith(("_col", "_val", "_rowptr"))})
+
+
+def main():
+    if not os.path.exists(REFDRV):
+        sys.exit("build oracle/_ref/refdrv first: make -C oracle ref")
+    c1 = ["-dim", "2", "-mesh", "cart", "-k", "1"]
+    # Config-1 problem: 64x64 Cartesian, k=1, CG + GMG V(1,1), block Jacobi
+    run("c1_n64_cg_mg_bj", c1 + ["-n", "64", "-s", "cg", "-prec", "mg", "-cycle", "V,1,1", "-smoothers", "bj,bj"], export_mg=True)
+    run("c1_n64_cg_mg_bj23", c1 + ["-n", "64", "-s", "cg", "-prec", "mg", "-cycle", "V,1,1", "-smoothers", "bj23,bj23"], keep_A=False)
+    # Small 2-D systems for detailed smoother / Krylov checks
+    c1s = ["-dim", "2", "-mesh", "cart", "-k", "1", "-source", "exp"]
+    run("c1_n32_cg_mg_bj23", c1s + ["-n", "32", "-s", "cg", "-prec", "mg", "-cycle", "V,1,1", "-smoothers", "bj23,bj23"], export_mg=True)
+    # undamped block Jacobi stagnates here: the first iterations still pin the cycle arithmetic
+    run("c1_n32_cg_mg_bj", c1s + ["-n", "32", "-s", "cg", "-prec", "mg", "-cycle", "V,1,1", "-smoothers", "bj,bj", "-max-iter", "15"], keep_A=False)
+    run("c1_n32_cg_mg_bgs", c1s + ["-n", "32", "-s", "cg", "-prec", "mg", "-cycle", "V,1,1", "-smoothers", "bgs,rbgs"], keep_A=False)
+    run("c1_n32_cg_bj", c1s + ["-n", "32", "-s", "cg", "-prec", "bj"], keep_A=False)
+    run("c1_n32_cg_none", c1s + ["-n", "32", "-s", "cg", "-prec", ""], keep_A=False)
+    run("c1_n32_fcg_mg_bj23", c1s + ["-n", "32", "-s", "fcg", "-prec", "mg", "-cycle", "V,1,1", "-smoothers", "bj23,bj23"], keep_A=False)
+    run("c1_n32_cg_mg_w_bj23", c1s + ["-n", "32", "-s", "cg", "-prec", "mg", "-cycle", "W,1,1", "-smoothers", "bj23,bj23"], keep_A=False)
+    run("c1_n32_fcg_mg_k_bgs", c1s + ["-n", "32", "-s", "fcg", "-prec", "mg", "-cycle", "K,1,1", "-smoothers", "bgs,rbgs"], keep_A=False)
+    # Aggregation AMG on a scalar (k=0) tetrahedral system: FCG + K-cycle
+    tet0 = ["-dim", "3", "-mesh", "stetra", "-k", "0", "-n", "8", "-source", "sine"]
+    run("tet8_k0_fcg_amg", tet0 + ["-s", "fcg", "-prec", "aggregamg", "-cycle", "K,1,1", "-smoothers", "bgs,rbgs"],
+        export_mg=True, aggreg=tet0 + ["-nosolve"])
+    run("tet8_k0_cg_amg_v", tet0 + ["-s", "cg", "-prec", "aggregamg", "-cycle", "V,1,1", "-smoothers", "bgs,rbgs", "-max-iter", "25"], keep_A=False)
+    # Block (n_f = 6) tetrahedral system: SpMV / block smoothers / block aggregation
+    tet2 = ["-dim", "3", "-mesh", "stetra", "-k", "2", "-n", "4", "-source", "sine"]
+    run("tet4_k2", tet2 + ["-s", "cg", "-prec", "bj"], aggreg=tet2 + ["-nosolve"])
+    # 3-D Cartesian k=2 with GMG V(1,1) bj23
+    run("cart8_k2_cg_mg_bj23", ["-dim", "3", "-mesh", "cart", "-k", "2", "-n", "8", "-s", "cg", "-prec", "mg",
+                                "-cycle", "V,1,1", "-smoothers", "bj23,bj23"], export_mg=True)
+
+
+if __name__ == "__main__":
+    main()
