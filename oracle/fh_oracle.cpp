@@ -1,0 +1,704 @@
+// CPU oracle of the condensed-HHO solve path -- TEST INFRASTRUCTURE ONLY.
+//
+// A plain restatement of the reference algorithms, written for this
+// repository from their description (no reference source is copied).  Each
+// function names the reference file:line it follows.  Tests, the smoke check
+// and the CPU-baseline leg of bench.py are its only users.
+#include "fh_oracle.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <unordered_map>
+#include <vector>
+
+namespace {
+
+struct Csr {
+    int n = 0, m = 0;
+    std::vector<int> rp, ci;
+    std::vector<double> v;
+    const int* prp = nullptr;
+    const int* pci = nullptr;
+    const double* pv = nullptr;
+    void view(int n_, int m_, const int* r, const int* c, const double* val) {
+        n = n_; m = m_; prp = r; pci = c; pv = val;
+    }
+    void own() { prp = rp.data(); pci = ci.data(); pv = v.data(); }
+};
+
+double dot(int n, const double* a, const double* b) {
+    double s = 0;
+    for (int i = 0; i < n; i++) s += a[i] * b[i];
+    return s;
+}
+double nrm2(int n, const double* a) { return std::sqrt(dot(n, a, a)); }
+
+void spmv(const Csr& A, const double* x, double* y) {
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < A.n; i++) {
+        double s = 0;
+        for (int p = A.prp[i]; p < A.prp[i + 1]; p++) s += A.pv[p] * x[A.pci[p]];
+        y[i] = s;
+    }
+}
+
+// Gauss-Jordan inverse with partial pivoting of an nf x nf row-major block.
+bool invert_block(int nf, double* a, double* inv) {
+    for (int i = 0; i < nf; i++)
+        for (int j = 0; j < nf; j++) inv[i * nf + j] = i == j ? 1.0 : 0.0;
+    for (int c = 0; c < nf; c++) {
+        int p = c;
+        for (int r = c + 1; r < nf; r++)
+            if (std::fabs(a[r * nf + c]) > std::fabs(a[p * nf + c])) p = r;
+        if (a[p * nf + c] == 0.0) return false;
+        if (p != c)
+            for (int j = 0; j < nf; j++) {
+                std::swap(a[p * nf + j], a[c * nf + j]);
+                std::swap(inv[p * nf + j], inv[c * nf + j]);
+            }
+        double d = 1.0 / a[c * nf + c];
+        for (int j = 0; j < nf; j++) { a[c * nf + j] *= d; inv[c * nf + j] *= d; }
+        for (int r = 0; r < nf; r++) {
+            if (r == c) continue;
+            double f = a[r * nf + c];
+            if (f == 0.0) continue;
+            for (int j = 0; j < nf; j++) { a[r * nf + j] -= f * a[c * nf + j]; inv[r * nf + j] -= f * inv[c * nf + j]; }
+        }
+    }
+    return true;
+}
+
+int block_inverse(const Csr& A, int nf, std::vector<double>& invD) {
+    int nb = A.n / nf;
+    invD.assign((size_t)nb * nf * nf, 0.0);
+    int bad = 0;
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < nb; i++) {
+        std::vector<double> D(nf * nf, 0.0);
+        for (int k = 0; k < nf; k++) {
+            int row = i * nf + k;
+            for (int p = A.prp[row]; p < A.prp[row + 1]; p++) {
+                int j = A.pci[p];
+                if (j / nf == i) D[k * nf + (j - i * nf)] = A.pv[p];
+            }
+        }
+        if (!invert_block(nf, D.data(), &invD[(size_t)i * nf * nf])) {
+#pragma omp critical
+            if (!bad) bad = -(1 + i);
+        }
+    }
+    return bad;
+}
+
+// BlockJacobi::ProcessBlockRow (BlockJacobi.h:101-127)
+void bj_sweep(const Csr& A, int nf, const double* invD, double omega, const double* b, double* x) {
+    int nb = A.n / nf;
+    std::vector<double> xold(x, x + A.n);
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < nb; i++) {
+        double tmp[64];
+        for (int k = 0; k < nf; k++) {
+            int row = i * nf + k;
+            double t = omega * b[row];
+            for (int p = A.prp[row]; p < A.prp[row + 1]; p++) {
+                int j = A.pci[p];
+                if (j / nf == i) t += (1 - omega) * A.pv[p] * xold[j];
+                else t += -omega * A.pv[p] * xold[j];
+            }
+            tmp[k] = t;
+        }
+        const double* Di = invD + (size_t)i * nf * nf;
+        for (int k = 0; k < nf; k++) {
+            double s = 0;
+            for (int l = 0; l < nf; l++) s += Di[k * nf + l] * tmp[l];
+            x[i * nf + k] = s;
+        }
+    }
+}
+
+// BlockGaussSeidel::ProcessBlockRow (BlockGaussSeidel.h:193-223)
+inline void bgs_row(const Csr& A, int nf, const double* invD, int i, const double* b, double* x) {
+    double tmp[64];
+    for (int k = 0; k < nf; k++) {
+        int row = i * nf + k;
+        double t = b[row];
+        for (int p = A.prp[row]; p < A.prp[row + 1]; p++) {
+            int j = A.pci[p];
+            if (j / nf != i) t -= A.pv[p] * x[j];
+        }
+        tmp[k] = t;
+    }
+    const double* Di = invD + (size_t)i * nf * nf;
+    for (int k = 0; k < nf; k++) {
+        double s = 0;
+        for (int l = 0; l < nf; l++) s += Di[k * nf + l] * tmp[l];
+        x[i * nf + k] = s;
+    }
+}
+
+// BlockGaussSeidel::ExecuteOneIteration, non-hybrid (BlockGaussSeidel.h:84-123)
+void bgs_sweep(const Csr& A, int nf, const double* invD, int dir, const double* b, double* x) {
+    int nb = A.n / nf;
+    if (dir == 0 || dir == 2)
+        for (int i = 0; i < nb; i++) bgs_row(A, nf, invD, i, b, x);
+    if (dir == 1 || dir == 2)
+        for (int i = nb - 1; i >= 0; i--) bgs_row(A, nf, invD, i, b, x);
+}
+
+// Dense Cholesky (coarse solver; the reference uses SimplicialLDLT,
+// EigenSparseCholesky.h:19-41, which solves the same SPD system).
+struct Dense {
+    int n = 0;
+    std::vector<double> L;
+    bool factor(const Csr& A) {
+        n = A.n;
+        L.assign((size_t)n * n, 0.0);
+        for (int i = 0; i < n; i++)
+            for (int p = A.prp[i]; p < A.prp[i + 1]; p++) L[(size_t)i * n + A.pci[p]] = A.pv[p];
+        for (int j = 0; j < n; j++) {
+            double d = L[(size_t)j * n + j];
+            for (int k = 0; k < j; k++) d -= L[(size_t)j * n + k] * L[(size_t)j * n + k];
+            if (d <= 0) return false;
+            d = std::sqrt(d);
+            L[(size_t)j * n + j] = d;
+            for (int i = j + 1; i < n; i++) {
+                double s = L[(size_t)i * n + j];
+                for (int k = 0; k < j; k++) s -= L[(size_t)i * n + k] * L[(size_t)j * n + k];
+                L[(size_t)i * n + j] = s / d;
+            }
+        }
+        return true;
+    }
+    void solve(const double* b, double* x) const {
+        std::vector<double> y(n);
+        for (int i = 0; i < n; i++) {
+            double s = b[i];
+            for (int k = 0; k < i; k++) s -= L[(size_t)i * n + k] * y[k];
+            y[i] = s / L[(size_t)i * n + i];
+        }
+        for (int i = n - 1; i >= 0; i--) {
+            double s = y[i];
+            for (int k = i + 1; k < n; k++) s -= L[(size_t)k * n + i] * x[k];
+            x[i] = s / L[(size_t)i * n + i];
+        }
+    }
+};
+
+// ----------------------------------------------------- pairwise aggregation
+struct AElem {
+    std::vector<std::pair<int, double>> nbr;  // sorted by ascending coupling
+    std::vector<int> strong;
+    int count = 0;   // NElementsIAmStrongNeighbourOf
+    int agg = -1;
+};
+
+// AlgebraicMesh::Build (AlgebraicMesh.h:54-113), CouplingValue (:133-148),
+// IsStronglyCoupled (:150-153)
+void build_algebraic_mesh(const Csr& A, int nf, double theta, std::vector<AElem>& E) {
+    int nb = A.n / nf;
+    E.assign(nb, AElem());
+    for (int i = 0; i < nb; i++) {
+        AElem& e = E[i];
+        for (int k = 0; k < nf; k++) {
+            int row = i * nf + k;
+            for (int p = A.prp[row]; p < A.prp[row + 1]; p++) {
+                int nbn = A.pci[p] / nf;
+                if (nbn == i) continue;
+                bool seen = false;
+                for (auto& q : e.nbr) if (q.first == nbn) { seen = true; break; }
+                if (seen) continue;
+                // trace of block (i, nbn), summed over the diagonal in order
+                double tr = 0;
+                bool first = true;
+                for (int a = 0; a < nf; a++) {
+                    int r = i * nf + a, c = nbn * nf + a;
+                    double val = 0;
+                    for (int q = A.prp[r]; q < A.prp[r + 1]; q++) if (A.pci[q] == c) { val = A.pv[q]; break; }
+                    if (first) { tr = val; first = false; } else tr += val;
+                }
+                e.nbr.push_back({nbn, tr});
+            }
+        }
+        std::sort(e.nbr.begin(), e.nbr.end(),
+                  [](const std::pair<int, double>& a, const std::pair<int, double>& b) { return a.second < b.second; });
+    }
+    for (int i = 0; i < nb; i++) {
+        AElem& e = E[i];
+        for (auto& q : e.nbr) {
+            if (q.second < 0 && q.second < theta * e.nbr[0].second) {
+                e.strong.push_back(q.first);
+                E[q.first].count++;
+            } else
+                break;
+        }
+    }
+}
+
+// PairwiseAggregation::Perform / OrderElementsByPriority / NumberNeighbours /
+// StrongestAvailableNeighbour / AddToAggregate (PairwiseAggregation.h:9-205)
+int pairwise_aggregate(std::vector<AElem>& E, bool& coarsest) {
+    int n = (int)E.size();
+    std::vector<int> order(n, -1);
+    std::vector<char> numbered(n, 0);
+    int on = 0;
+    int start = 0, mn = E[0].count;
+    for (int i = 0; i < n; i++) if (E[i].count < mn) { start = i; mn = E[i].count; }
+    order[on++] = start;
+    numbered[start] = 1;
+    for (int i = 0; i < n; i++) {
+        if (i >= on) {  // disconnected graph: restart from the next unnumbered element
+            int s = 0;
+            while (numbered[s]) s++;
+            order[on++] = s;
+            numbered[s] = 1;
+        }
+        const AElem& e = E[order[i]];
+        std::vector<int> un;
+        for (auto& q : e.nbr) if (!numbered[q.first]) un.push_back(q.first);
+        if (un.empty()) continue;
+        std::sort(un.begin(), un.end(), [&E](int a, int b) { return E[a].count < E[b].count; });
+        for (int u : un) { order[on++] = u; numbered[u] = 1; }
+    }
+    int nagg = 0;
+    int remaining = n;
+    auto add = [&E](int el, int a) {
+        E[el].agg = a;
+        for (int s : E[el].strong) if (E[s].agg < 0) E[s].count--;
+    };
+    for (int idx = 0; idx < n; idx++) {
+        int el = order[idx];
+        if (E[el].agg >= 0) continue;
+        int a = nagg++;
+        add(el, a);
+        remaining--;
+        double mnc = 100000000;
+        int chosen = -1;
+        for (int s : E[el].strong)
+            if (E[s].agg < 0 && E[s].count < mnc) { chosen = s; mnc = E[s].count; }
+        if (chosen >= 0) { add(chosen, a); remaining--; }
+        if (remaining == 0) break;
+    }
+    coarsest = (double)n / (double)nagg <= 1.01;
+    return nagg;
+}
+
+// AggregLevel::GalerkinOperator (AggregAMG.h:156-180) extended to blocks:
+// Ac(I,J) = sum_{k in I, l in J} A_kl, entries |a| <= 1e-15 dropped before
+// summation (NonZeroCoefficients.h:36-40), summed in row-major order.
+void galerkin(const Csr& A, int nf, const std::vector<int>& agg, int nagg, Csr& Ac) {
+    int nb = A.n / nf;
+    std::vector<std::vector<int>> members(nagg);
+    for (int i = 0; i < nb; i++) members[agg[i]].push_back(i);
+    Ac.n = Ac.m = nagg * nf;
+    Ac.rp.assign(Ac.n + 1, 0);
+    Ac.ci.clear();
+    Ac.v.clear();
+    std::vector<double> acc((size_t)Ac.n, 0.0);
+    std::vector<char> used((size_t)Ac.n, 0);
+    std::vector<int> cols;
+    for (int I = 0; I < nagg; I++) {
+        for (int a = 0; a < nf; a++) {
+            cols.clear();
+            for (int k : members[I]) {
+                int row = k * nf + a;
+                for (int p = A.prp[row]; p < A.prp[row + 1]; p++) {
+                    double val = A.pv[p];
+                    if (!(std::fabs(val) > 1e-15)) continue;
+                    int l = A.pci[p];
+                    int c = agg[l / nf] * nf + (l % nf);
+                    if (!used[c]) { used[c] = 1; acc[c] = val; cols.push_back(c); }
+                    else acc[c] += val;
+                }
+            }
+            std::sort(cols.begin(), cols.end());
+            for (int c : cols) { Ac.ci.push_back(c); Ac.v.push_back(acc[c]); used[c] = 0; }
+            Ac.rp[I * nf + a + 1] = (int)Ac.ci.size();
+        }
+    }
+    Ac.own();
+}
+
+// ----------------------------------------------------------------- multigrid
+enum { SM_BJ = 1, SM_BGS_F = 2, SM_BGS_B = 3, SM_BGS_S = 4 };
+
+struct Level {
+    Csr A;
+    Csr P, R;            // GMG transfer (level -> coarser)
+    std::vector<int> agg; int nagg = 0;  // AMG transfer
+    bool amg = false;
+    std::vector<double> invD;
+};
+
+struct MG {
+    std::vector<Level> lv;
+    Dense coarse;
+    const or_solver_cfg* cfg = nullptr;
+    int nf = 1;
+
+    void smooth(Level& L, int kind, double omega, int nu, const double* b, double* x) {
+        for (int s = 0; s < nu; s++) {
+            if (kind == SM_BJ) bj_sweep(L.A, nf, L.invD.data(), omega, b, x);
+            else bgs_sweep(L.A, nf, L.invD.data(), kind - SM_BGS_F, b, x);
+        }
+    }
+    void restrict_(int l, const std::vector<double>& r, std::vector<double>& rc) {
+        Level& L = lv[l];
+        if (!L.amg) {
+            rc.assign(L.R.n, 0.0);
+            spmv(L.R, r.data(), rc.data());
+        } else {
+            // AggregLevel::Restrict (AggregAMG.h:237-271), blockwise
+            rc.assign((size_t)L.nagg * nf, 0.0);
+            int nb = L.A.n / nf;
+            std::vector<char> first((size_t)L.nagg, 1);
+            for (int i = 0; i < nb; i++)
+                for (int a = 0; a < nf; a++) rc[(size_t)L.agg[i] * nf + a] += r[(size_t)i * nf + a];
+        }
+    }
+    void prolong_add(int l, const std::vector<double>& ec, double* x) {
+        Level& L = lv[l];
+        if (!L.amg) {
+            std::vector<double> t(L.P.n);
+            spmv(L.P, ec.data(), t.data());
+            for (int i = 0; i < L.P.n; i++) x[i] += t[i];
+        } else {
+            int nb = L.A.n / nf;
+            for (int i = 0; i < nb; i++)
+                for (int a = 0; a < nf; a++) x[(size_t)i * nf + a] += ec[(size_t)L.agg[i] * nf + a];
+        }
+    }
+    // Multigrid::MultigridCycle (Multigrid.h:438-608)
+    void cycle(int l, const double* b, double* x, double* Ax) {
+        Level& L = lv[l];
+        int n = L.A.n;
+        if (l == (int)lv.size() - 1) {
+            coarse.solve(b, x);
+            if (Ax) spmv(L.A, x, Ax);
+            return;
+        }
+        smooth(L, cfg->pre_kind, cfg->pre_omega, cfg->nu1, b, x);
+        std::vector<double> r(n);
+        spmv(L.A, x, r.data());
+        for (int i = 0; i < n; i++) r[i] = b[i] - r[i];
+        std::vector<double> rc, ec;
+        restrict_(l, r, rc);
+        int nc = (int)rc.size();
+        ec.assign(nc, 0.0);
+        if (cfg->cycle == 'K' && l + 1 < (int)lv.size() - 1)
+            fcg_kcycle(l + 1, rc, ec);
+        else {
+            int loops = cfg->cycle == 'W' ? std::max(2, cfg->wloops) : 1;
+            for (int w = 0; w < loops; w++) {
+                cycle(l + 1, rc.data(), ec.data(), nullptr);
+                if (l + 1 == (int)lv.size() - 1) break;
+            }
+        }
+        prolong_add(l, ec, x);
+        smooth(L, cfg->post_kind, cfg->post_omega, cfg->nu2, b, x);
+        if (Ax) spmv(L.A, x, Ax);
+    }
+    // Multigrid::FCGForKCycle (Multigrid.h:611-656); r is overwritten
+    void fcg_kcycle(int l, std::vector<double>& r, std::vector<double>& out) {
+        const double t = 0.25;
+        int n = (int)r.size();
+        std::vector<double> c(n, 0.0), v(n);
+        cycle(l, r.data(), c.data(), v.data());
+        double rho1 = dot(n, c.data(), v.data());
+        double alpha1 = dot(n, c.data(), r.data());
+        double rold = nrm2(n, r.data());
+        for (int i = 0; i < n; i++) r[i] -= (alpha1 / rho1) * v[i];
+        double rnew = nrm2(n, r.data());
+        if (rnew <= t * rold) {
+            for (int i = 0; i < n; i++) out[i] = (alpha1 / rho1) * c[i];
+            return;
+        }
+        std::vector<double> d(n, 0.0), w(n);
+        cycle(l, r.data(), d.data(), w.data());
+        double gamma = dot(n, d.data(), v.data());
+        double beta = dot(n, d.data(), w.data());
+        double alpha2 = dot(n, d.data(), r.data());
+        double rho2 = beta - gamma * gamma / rho1;
+        double ca = alpha1 / rho1 - gamma * alpha2 / (rho1 * rho2), cb = alpha2 / rho2;
+        for (int i = 0; i < n; i++) out[i] = ca * c[i] + cb * d[i];
+    }
+};
+
+double g_setup_s = 0, g_solve_s = 0;
+
+int build_amg(MG& mg, const Csr& A0, int nf, double theta, int max_coarse, int max_levels) {
+    mg.lv.clear();
+    mg.lv.emplace_back();
+    mg.lv[0].A = A0;
+    while ((int)mg.lv.size() < max_levels && mg.lv.back().A.n > max_coarse) {
+        Level& L = mg.lv.back();
+        // DoublePairwiseAggregation (AggregAMG.h:40-100)
+        std::vector<AElem> E1;
+        build_algebraic_mesh(L.A, nf, theta, E1);
+        bool coarsest = false;
+        int n1 = pairwise_aggregate(E1, coarsest);
+        if (coarsest) break;
+        std::vector<int> a1(E1.size());
+        for (size_t i = 0; i < E1.size(); i++) a1[i] = E1[i].agg;
+        Csr A1;
+        galerkin(L.A, nf, a1, n1, A1);
+        std::vector<AElem> E2;
+        build_algebraic_mesh(A1, nf, theta, E2);
+        int n2 = pairwise_aggregate(E2, coarsest);
+        if (coarsest) break;
+        std::vector<int> a2(E2.size());
+        for (size_t i = 0; i < E2.size(); i++) a2[i] = E2[i].agg;
+        L.amg = true;
+        L.agg.resize(a1.size());
+        for (size_t i = 0; i < a1.size(); i++) L.agg[i] = a2[a1[i]];
+        L.nagg = n2;
+        Level C;
+        galerkin(A1, nf, a2, n2, C.A);
+        mg.lv.push_back(std::move(C));
+        // re-own pointers after the move of the vector storage
+        for (auto& q : mg.lv) if (!q.A.rp.empty()) q.A.own();
+    }
+    return (int)mg.lv.size();
+}
+
+}  // namespace
+
+extern "C" {
+
+void or_spmv(int n, const int* rp, const int* ci, const double* v, const double* x, double* y) {
+    Csr A; A.view(n, n, rp, ci, v);
+    spmv(A, x, y);
+}
+
+int or_block_inverse(int n, int nf, const int* rp, const int* ci, const double* v, double* invD) {
+    Csr A; A.view(n, n, rp, ci, v);
+    std::vector<double> inv;
+    int rc = block_inverse(A, nf, inv);
+    std::memcpy(invD, inv.data(), inv.size() * sizeof(double));
+    return rc;
+}
+
+void or_bj_sweep(int n, int nf, const int* rp, const int* ci, const double* v, const double* invD,
+                 double omega, const double* b, double* x) {
+    Csr A; A.view(n, n, rp, ci, v);
+    bj_sweep(A, nf, invD, omega, b, x);
+}
+
+void or_bgs_sweep(int n, int nf, const int* rp, const int* ci, const double* v, const double* invD,
+                  int dir, const double* b, double* x) {
+    Csr A; A.view(n, n, rp, ci, v);
+    bgs_sweep(A, nf, invD, dir, b, x);
+}
+
+void or_mcbgs_sweep(int n, int nf, const int* rp, const int* ci, const double* v, const double* invD,
+                    int ncol, const int* colour_ptr, const int* colour_rows, int dir,
+                    const double* b, double* x) {
+    Csr A; A.view(n, n, rp, ci, v);
+    for (int cc = 0; cc < ncol; cc++) {
+        int c = dir == 0 ? cc : ncol - 1 - cc;
+        std::vector<double> xs(x, x + n);  // values of the previous colours
+        for (int q = colour_ptr[c]; q < colour_ptr[c + 1]; q++) {
+            int i = colour_rows[q];
+            double tmp[64];
+            for (int k = 0; k < nf; k++) {
+                int row = i * nf + k;
+                double t = b[row];
+                for (int p = rp[row]; p < rp[row + 1]; p++)
+                    if (ci[p] / nf != i) t -= v[p] * xs[ci[p]];
+                tmp[k] = t;
+            }
+            const double* Di = invD + (size_t)i * nf * nf;
+            for (int k = 0; k < nf; k++) {
+                double s = 0;
+                for (int l = 0; l < nf; l++) s += Di[k * nf + l] * tmp[l];
+                x[i * nf + k] = s;
+            }
+        }
+    }
+}
+
+int or_greedy_colouring(int nb, const int* brp, const int* bci, int* colour) {
+    int ncol = 0;
+    std::vector<int> mark;
+    for (int i = 0; i < nb; i++) {
+        mark.assign(ncol + 1, 0);
+        for (int p = brp[i]; p < brp[i + 1]; p++) {
+            int j = bci[p];
+            if (j < i) mark[colour[j]] = 1;
+        }
+        int c = 0;
+        while (mark[c]) c++;
+        colour[i] = c;
+        if (c + 1 > ncol) ncol = c + 1;
+    }
+    return ncol;
+}
+
+int or_pairwise_aggregate(int n, int nf, const int* rp, const int* ci, const double* v, double theta,
+                          int* agg, int* coarsest) {
+    Csr A; A.view(n, n, rp, ci, v);
+    std::vector<AElem> E;
+    build_algebraic_mesh(A, nf, theta, E);
+    bool c = false;
+    int na = pairwise_aggregate(E, c);
+    for (size_t i = 0; i < E.size(); i++) agg[i] = E[i].agg;
+    *coarsest = c;
+    return na;
+}
+
+int or_amg_hierarchy(int n, int nf, const int* rp, const int* ci, const double* v, double theta,
+                     int max_coarse, int* agg0, int* level_n, int max_levels) {
+    Csr A; A.view(n, n, rp, ci, v);
+    MG mg;
+    int nl = build_amg(mg, A, nf, theta, max_coarse, max_levels);
+    for (int l = 0; l < nl; l++) level_n[l] = mg.lv[l].A.n;
+    if (nl > 1) std::memcpy(agg0, mg.lv[0].agg.data(), mg.lv[0].agg.size() * sizeof(int));
+    return nl;
+}
+
+void or_last_times(double* setup_s, double* solve_s) {
+    *setup_s = g_setup_s;
+    *solve_s = g_solve_s;
+}
+
+int or_solve(const or_solver_cfg* cfg, const or_level_desc* levels, const double* b, double* x,
+             double* hist, or_solver_out* out) {
+    auto t0 = std::chrono::steady_clock::now();
+    const int nf = cfg->nf;
+    Csr A; A.view(levels[0].n, levels[0].n, levels[0].rp, levels[0].ci, levels[0].v);
+    const int n = A.n;
+    MG mg;
+    mg.cfg = cfg;
+    mg.nf = nf;
+    std::vector<double> invD;
+    if (cfg->prec == 1 || cfg->prec == 2) {
+        if (block_inverse(A, nf, invD)) return -2;
+    } else if (cfg->prec == 3) {
+        mg.lv.resize(cfg->nlevels);
+        for (int l = 0; l < cfg->nlevels; l++) {
+            const or_level_desc& d = levels[l];
+            mg.lv[l].A.view(d.n, d.n, d.rp, d.ci, d.v);
+            if (l + 1 < cfg->nlevels) {
+                Level& L = mg.lv[l];
+                L.P.view(d.n, d.nc, d.prp, d.pci, d.pv);
+                // R = P^T (LevelForHHO.h:535-543)
+                L.R.n = d.nc; L.R.m = d.n;
+                L.R.rp.assign(d.nc + 1, 0);
+                for (int p = 0; p < d.prp[d.n]; p++) L.R.rp[d.pci[p] + 1]++;
+                for (int i = 0; i < d.nc; i++) L.R.rp[i + 1] += L.R.rp[i];
+                L.R.ci.resize(d.prp[d.n]); L.R.v.resize(d.prp[d.n]);
+                std::vector<int> pos(L.R.rp.begin(), L.R.rp.end() - 1);
+                for (int i = 0; i < d.n; i++)
+                    for (int p = d.prp[i]; p < d.prp[i + 1]; p++) {
+                        int q = pos[d.pci[p]]++;
+                        L.R.ci[q] = i; L.R.v[q] = d.pv[p];
+                    }
+                L.R.own();
+            }
+        }
+    } else if (cfg->prec == 4) {
+        build_amg(mg, A, nf, cfg->theta, cfg->max_coarse, 32);
+    }
+    if (cfg->prec >= 3) {
+        for (size_t l = 0; l + 1 < mg.lv.size(); l++)
+            if (block_inverse(mg.lv[l].A, nf, mg.lv[l].invD)) return -2;
+        if (!mg.coarse.factor(mg.lv.back().A)) return -3;
+        out->nlevels = (int)mg.lv.size();
+        for (size_t l = 0; l < mg.lv.size() && l < 32; l++) out->level_n[l] = mg.lv[l].A.n;
+    }
+    // MG::CanOptimizeResidualComputation (Multigrid.h:74-77): the point
+    // Gauss-Seidel post-smoother returns A x (GaussSeidel.h:53-56).
+    bool fused = cfg->prec >= 3 && mg.lv.size() > 1 && nf == 1 && cfg->post_kind >= SM_BGS_F && cfg->nu2 > 0;
+    auto t1 = std::chrono::steady_clock::now();
+
+    auto apply = [&](const double* r, double* z, double* Az) {
+        std::fill(z, z + n, 0.0);
+        if (cfg->prec == 0) { std::copy(r, r + n, z); return; }
+        if (cfg->prec == 1) { bj_sweep(A, nf, invD.data(), cfg->omega, r, z); return; }
+        if (cfg->prec == 2) { bgs_sweep(A, nf, invD.data(), 0, r, z); return; }
+        mg.cycle(0, r, z, Az);
+    };
+
+    double bn = nrm2(n, b);
+    auto nres = [&](const double* r) { double rn = nrm2(n, r); return bn > 0 ? rn / bn : rn; };
+    std::fill(x, x + n, 0.0);
+    int it = 0;
+    int nh = 0;
+    double res = 1.0;
+    hist[nh++] = res;
+    int status = 1;
+    std::vector<double> r(b, b + n), z(n), d(n), Ad(n), Az(n);
+    auto stop = [&]() {
+        if (it == 0) return false;
+        if (!std::isfinite(res)) { status = -4; return true; }
+        if (res < cfg->tol) { status = 0; return true; }
+        if (it >= cfg->max_iter) { status = 1; return true; }
+        return false;
+    };
+    if (cfg->krylov == 1) {
+        // ConjugateGradient::Solve (ConjugateGradient.h:27-104)
+        apply(r.data(), z.data(), nullptr);
+        d = z;
+        double rz = dot(n, r.data(), z.data()), beta = 0;
+        while (!stop()) {
+            if (it > 0) for (int i = 0; i < n; i++) d[i] = z[i] + beta * d[i];
+            spmv(A, d.data(), Ad.data());
+            double alpha = rz / dot(n, d.data(), Ad.data());
+            for (int i = 0; i < n; i++) x[i] += alpha * d[i];
+            double old = rz;
+            for (int i = 0; i < n; i++) r[i] -= alpha * Ad[i];
+            apply(r.data(), z.data(), nullptr);
+            rz = dot(n, r.data(), z.data());
+            beta = rz / old;
+            it++;
+            res = nres(r.data());
+            hist[nh++] = res;
+        }
+    } else if (cfg->krylov == 2) {
+        // FlexibleConjugateGradient::Solve, truncation 1 (FlexibleConjugateGradient.h:47-171)
+        std::vector<double> dk, Adk;
+        double dkAdk = 0;
+        bool have = false;
+        while (!stop()) {
+            apply(r.data(), z.data(), fused ? Az.data() : nullptr);
+            d = z;
+            if (fused) Ad = Az;
+            if (have) {
+                double zAdk = dot(n, z.data(), Adk.data());
+                for (int i = 0; i < n; i++) d[i] -= (zAdk / dkAdk) * dk[i];
+                if (fused) for (int i = 0; i < n; i++) Ad[i] -= (zAdk / dkAdk) * Adk[i];
+            }
+            if (!fused) spmv(A, d.data(), Ad.data());
+            double dAd = dot(n, d.data(), Ad.data());
+            double alpha = dot(n, r.data(), d.data()) / dAd;
+            for (int i = 0; i < n; i++) x[i] += alpha * d[i];
+            for (int i = 0; i < n; i++) r[i] -= alpha * Ad[i];
+            dk = d; Adk = Ad; dkAdk = dAd; have = true;
+            it++;
+            res = nres(r.data());
+            hist[nh++] = res;
+        }
+    } else {
+        // Stand-alone iterations of the preconditioner (IterativeSolver.h:148-179)
+        std::vector<double> xx(n);
+        while (!stop()) {
+            if (cfg->prec >= 3) mg.cycle(0, b, x, nullptr);
+            else if (cfg->prec == 1) bj_sweep(A, nf, invD.data(), cfg->omega, b, x);
+            else bgs_sweep(A, nf, invD.data(), 0, b, x);
+            spmv(A, x, xx.data());
+            for (int i = 0; i < n; i++) r[i] = b[i] - xx[i];
+            it++;
+            res = nres(r.data());
+            hist[nh++] = res;
+        }
+    }
+    auto t2 = std::chrono::steady_clock::now();
+    g_setup_s = std::chrono::duration<double>(t1 - t0).count();
+    g_solve_s = std::chrono::duration<double>(t2 - t1).count();
+    out->iterations = it;
+    out->final_res = res;
+    return status;
+}
+
+}  // extern "C"

@@ -1,0 +1,200 @@
+"""Pins the CPU oracle before it is trusted as the checker of the HIP path.
+
+Every oracle function is checked against the reference's golden fixtures
+(tests/golden/*.npz, written by the reference's own solver) and/or against a
+closed form or brute force that a plausible mistake would fail.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+from conftest import csr, gmg_levels, load
+from oracle import oracle as O
+
+GMG_CASES = [
+    # fixture, system fixture, solver, prec, cycle, smoothers, nf, max_iter
+    ("c1_n64_cg_mg_bj", "c1_n64_cg_mg_bj", "cg", "mg", "V", ("bj", "bj"), 2, 200),
+    ("c1_n64_cg_mg_bj23", "c1_n64_cg_mg_bj", "cg", "mg", "V", ("bj23", "bj23"), 2, 200),
+    ("c1_n32_cg_mg_bj23", "c1_n32_cg_mg_bj23", "cg", "mg", "V", ("bj23", "bj23"), 2, 200),
+    ("c1_n32_cg_mg_bj", "c1_n32_cg_mg_bj23", "cg", "mg", "V", ("bj", "bj"), 2, 15),
+    ("c1_n32_cg_mg_bgs", "c1_n32_cg_mg_bj23", "cg", "mg", "V", ("bgs", "rbgs"), 2, 200),
+    ("c1_n32_cg_bj", "c1_n32_cg_mg_bj23", "cg", "bj", "V", ("bj", "bj"), 2, 200),
+    ("c1_n32_cg_none", "c1_n32_cg_mg_bj23", "cg", "none", "V", ("bj", "bj"), 2, 200),
+    ("c1_n32_fcg_mg_bj23", "c1_n32_cg_mg_bj23", "fcg", "mg", "V", ("bj23", "bj23"), 2, 200),
+    ("c1_n32_cg_mg_w_bj23", "c1_n32_cg_mg_bj23", "cg", "mg", "W", ("bj23", "bj23"), 2, 200),
+    ("c1_n32_fcg_mg_k_bgs", "c1_n32_cg_mg_bj23", "fcg", "mg", "K", ("bgs", "rbgs"), 2, 200),
+    ("cart8_k2_cg_mg_bj23", "cart8_k2_cg_mg_bj23", "cg", "mg", "V", ("bj23", "bj23"), 6, 200),
+]
+
+
+@pytest.mark.parametrize("case", GMG_CASES, ids=[c[0] for c in GMG_CASES])
+def test_oracle_gmg_matches_reference(case):
+    name, sysname, solver, prec, cycle, sm, nf, maxit = case
+    f = load(name)
+    d = load(sysname)
+    A = csr(d, "A")
+    x, hist, out = O.solve(A, d["b"], solver, prec, nf=nf, cycle=cycle, smoothers=sm,
+                           levels=gmg_levels(d), max_iter=maxit)
+    assert out.iterations == int(f["info_iterations"])
+    assert np.linalg.norm(x - f["x"]) <= 1e-10 * np.linalg.norm(f["x"])
+    k = min(6, len(hist))
+    np.testing.assert_allclose(hist[:k], f["hist"][:k], rtol=1e-9)
+
+
+AMG_CASES = [
+    ("tet8_k0_fcg_amg", "fcg", "K", 200),
+    ("tet8_k0_cg_amg_v", "cg", "V", 25),
+]
+
+
+@pytest.mark.parametrize("case", AMG_CASES, ids=[c[0] for c in AMG_CASES])
+def test_oracle_amg_matches_reference(case):
+    name, solver, cycle, maxit = case
+    f = load(name)
+    d = load("tet8_k0_fcg_amg")
+    A = csr(d, "A")
+    x, hist, out = O.solve(A, d["b"], solver, "aggregamg", nf=1, cycle=cycle, smoothers=("bgs", "rbgs"),
+                           max_iter=maxit)
+    assert out.iterations == int(f["info_iterations"])
+    assert np.linalg.norm(x - f["x"]) <= 1e-10 * np.linalg.norm(f["x"])
+    np.testing.assert_allclose(hist[:8], f["hist"][:8], rtol=1e-9)
+
+
+def test_oracle_aggregation_bit_exact_scalar():
+    d = load("tet8_k0_fcg_amg")
+    na, agg, coarsest = O.pairwise_aggregate(csr(d, "A"), 1)
+    assert np.array_equal(agg, d["agg1"])
+    assert na == d["agg1"].max() + 1 and not coarsest
+
+
+def test_oracle_aggregation_bit_exact_blocks():
+    d = load("tet4_k2")
+    na, agg, _ = O.pairwise_aggregate(csr(d, "A"), 6)
+    assert np.array_equal(agg, d["agg1"])
+
+
+def test_oracle_amg_levels_match_reference_galerkin():
+    d = load("tet8_k0_fcg_amg")
+    sizes, _ = O.amg_hierarchy(csr(d, "A"), 1)
+    ref = [csr(d, "A").shape[0]] + [csr(d, "level%d_A" % l).shape[0] for l in range(1, int(d["nlevels"]))]
+    assert sizes == ref
+
+
+def test_aggregation_invariants():
+    d = load("tet4_k2")
+    A = csr(d, "A")
+    na, agg, _ = O.pairwise_aggregate(A, 6)
+    counts = np.bincount(agg, minlength=na)
+    assert counts.min() >= 1 and counts.max() <= 2  # pairs or singletons
+    assert agg.min() == 0 and agg.max() == na - 1
+
+
+def test_spmv_matches_scipy():
+    d = load("tet4_k2")
+    A = csr(d, "A")
+    x = np.random.default_rng(0).standard_normal(A.shape[0])
+    np.testing.assert_allclose(O.spmv(A, x), A @ x, rtol=1e-13, atol=1e-13 * np.abs(A @ x).max())
+
+
+def _blocks(A, nf):
+    nb = A.shape[0] // nf
+    Ad = A.toarray()
+    return [Ad[i * nf:(i + 1) * nf, i * nf:(i + 1) * nf] for i in range(nb)], Ad
+
+
+def test_block_inverse_is_inverse():
+    d = load("tet4_k2")
+    A = csr(d, "A")
+    nf = 6
+    inv = O.block_inverse(A, nf).reshape(-1, nf, nf)
+    D, _ = _blocks(A, nf)
+    for i in range(0, len(D), 37):
+        np.testing.assert_allclose(D[i] @ inv[i], np.eye(nf), atol=1e-10)
+
+
+def test_bj_sweep_residual_form():
+    """Reference form w b + (1-w) D x - w (L+U) x, then D^-1, equals the
+    residual form x + w D^-1 (b - A x)."""
+    d = load("c1_n32_cg_mg_bj23")
+    A = csr(d, "A")
+    nf, w = 2, 2.0 / 3.0
+    rng = np.random.default_rng(1)
+    b, x = rng.standard_normal(A.shape[0]), rng.standard_normal(A.shape[0])
+    inv = O.block_inverse(A, nf)
+    got = O.bj_sweep(A, nf, inv, w, b, x)
+    Dinv = sp.block_diag(list(inv.reshape(-1, nf, nf)))
+    want = x + w * (Dinv @ (b - A @ x))
+    np.testing.assert_allclose(got, want, rtol=1e-10, atol=1e-12)
+
+
+def test_bgs_forward_solves_block_triangular_system():
+    d = load("c1_n32_cg_mg_bj23")
+    A = csr(d, "A")
+    nf = 2
+    rng = np.random.default_rng(2)
+    b, x = rng.standard_normal(A.shape[0]), rng.standard_normal(A.shape[0])
+    inv = O.block_inverse(A, nf)
+    Ad = A.toarray()
+    nb = A.shape[0] // nf
+    mask = np.kron(np.tril(np.ones((nb, nb))), np.ones((nf, nf)))
+    LD, U = Ad * mask, Ad * (1 - mask)
+    got = O.bgs_sweep(A, nf, inv, 0, b, x)
+    np.testing.assert_allclose(LD @ got, b - U @ x, rtol=1e-9, atol=1e-9)
+    got = O.bgs_sweep(A, nf, inv, 1, b, x)
+    maskU = np.kron(np.triu(np.ones((nb, nb))), np.ones((nf, nf)))
+    np.testing.assert_allclose((Ad * maskU) @ got, b - (Ad * (1 - maskU)) @ x, rtol=1e-9, atol=1e-9)
+
+
+def _block_graph(A, nf):
+    B = (abs(A) > 0).astype(np.int8)
+    nb = A.shape[0] // nf
+    R = sp.kron(sp.eye(nb), np.ones((1, nf))).tocsr()
+    G = (R @ B @ R.T).tocsr()
+    G.sort_indices()
+    return G.indptr.astype(np.int32), G.indices.astype(np.int32)
+
+
+def test_greedy_colouring_is_valid():
+    d = load("tet4_k2")
+    brp, bci = _block_graph(csr(d, "A"), 6)
+    nc, col = O.greedy_colouring(brp, bci)
+    for i in range(len(brp) - 1):
+        for j in bci[brp[i]:brp[i + 1]]:
+            assert j == i or col[j] != col[i]
+    assert nc == col.max() + 1
+
+
+def test_multicolour_gs_reduces_to_lexicographic():
+    """One colour per block row, visited in row order = lexicographic GS."""
+    d = load("c1_n32_cg_mg_bj23")
+    A = csr(d, "A")
+    nf = 2
+    nb = A.shape[0] // nf
+    rng = np.random.default_rng(3)
+    b, x = rng.standard_normal(A.shape[0]), rng.standard_normal(A.shape[0])
+    inv = O.block_inverse(A, nf)
+    cp = np.arange(nb + 1, dtype=np.int32)
+    rows = np.arange(nb, dtype=np.int32)
+    np.testing.assert_allclose(O.mcbgs_sweep(A, nf, inv, cp, rows, 0, b, x), O.bgs_sweep(A, nf, inv, 0, b, x),
+                               rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(O.mcbgs_sweep(A, nf, inv, cp, rows, 1, b, x), O.bgs_sweep(A, nf, inv, 1, b, x),
+                               rtol=1e-12, atol=1e-12)
+
+
+def test_fcg_equals_cg_with_fixed_spd_preconditioner():
+    """With a fixed SPD preconditioner FCG(1) reproduces CG (Notay)."""
+    d = load("c1_n32_cg_mg_bj23")
+    A = csr(d, "A")
+    x1, h1, o1 = O.solve(A, d["b"], "cg", "mg", nf=2, smoothers=("bj23", "bj23"), levels=gmg_levels(d))
+    x2, h2, o2 = O.solve(A, d["b"], "fcg", "mg", nf=2, smoothers=("bj23", "bj23"), levels=gmg_levels(d))
+    assert o1.iterations == o2.iterations
+    np.testing.assert_allclose(h1, h2, rtol=1e-6)
+
+
+def test_cg_solution_matches_direct_solve():
+    d = load("c1_n32_cg_mg_bj23")
+    A = csr(d, "A")
+    x, _, _ = O.solve(A, d["b"], "cg", "none", nf=2, tol=1e-12, max_iter=1000)
+    xd = spla.spsolve(A.tocsc(), d["b"])
+    assert np.linalg.norm(x - xd) <= 1e-9 * np.linalg.norm(xd)
