@@ -1,0 +1,374 @@
+// extern "C" boundary of the MI355X solve path (declared in include/fhgpu.h).
+#include <cstring>
+#include <string>
+
+#include "../../include/fhgpu.h"
+#include "fh_kernels.h"
+#include "fh_matrix.h"
+#include "fh_solver.h"
+
+struct fh_ctx {
+    fh::Ctx c;
+    explicit fh_ctx(int dev) : c(dev) {}
+};
+struct fh_mat {
+    std::unique_ptr<fh::Bcsr> A;
+};
+struct fh_bjac {
+    fh::LevelSmoother S;
+};
+struct fh_coloring {
+    int ncolours = 0;
+    std::vector<int> colour;
+    fh::LevelSmoother S;  // colour lists on the device
+};
+struct fh_solver {
+    fh_ctx* ctx;
+    std::unique_ptr<fh::Solver> s;
+};
+
+namespace {
+thread_local std::string g_err;
+
+template <class F>
+int guard(F&& f) {
+    try {
+        f();
+        return 0;
+    } catch (const fh::Error& e) {
+        g_err = e.what();
+        return e.code < 0 ? e.code : -1;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+
+fh::Prec P(int p) {
+    if (p < 0 || p > 2) throw fh::Error(-2, "invalid precision code " + std::to_string(p));
+    return (fh::Prec)p;
+}
+#define NEED(ptr) FH_CHECK((ptr) != nullptr, -2, #ptr " is null")
+#define NEED_DEV(ctx) FH_CHECK(!(ctx)->c.host_only, -9, "this entry point needs a device context")
+}  // namespace
+
+extern "C" {
+
+const char* fh_last_error(void) { return g_err.c_str(); }
+
+int fh_ctx_create(int device, fh_ctx** out) {
+    return guard([&] {
+        NEED(out);
+        *out = new fh_ctx(device);
+    });
+}
+int fh_ctx_destroy(fh_ctx* ctx) {
+    return guard([&] { delete ctx; });
+}
+int fh_ctx_sync(fh_ctx* ctx) {
+    return guard([&] { NEED(ctx); ctx->c.sync(); });
+}
+
+int fh_mat_from_csr(fh_ctx* ctx, int nrows, int ncols, int bs, const int* rowptr, const int* colidx,
+                    const double* vals, int keep_host, fh_mat** out) {
+    return guard([&] {
+        NEED(ctx); NEED(rowptr); NEED(out);
+        FH_CHECK(rowptr[nrows] == 0 || (colidx && vals), -2, "null column or value array");
+        auto m = new fh_mat;
+        m->A = fh::bcsr_from_csr(ctx->c, nrows, ncols, bs, rowptr, colidx, vals, keep_host != 0);
+        *out = m;
+    });
+}
+int fh_mat_from_blocks(fh_ctx* ctx, int nbr, int nbc, int bs, const int* rowptr, const int* colidx,
+                       const double* vals, int keep_host, fh_mat** out) {
+    return guard([&] {
+        NEED(ctx); NEED(rowptr); NEED(out);
+        FH_CHECK(rowptr[nbr] == 0 || (colidx && vals), -2, "null column or value array");
+        for (int p = 0; p < rowptr[nbr]; p++)
+            FH_CHECK(colidx[p] >= 0 && colidx[p] < nbc, -2, "block column index out of range");
+        auto m = new fh_mat;
+        m->A = fh::bcsr_from_blocks(ctx->c, nbr, nbc, bs, bs, rowptr, colidx, vals, keep_host != 0);
+        *out = m;
+    });
+}
+int fh_mat_convert(fh_ctx* ctx, const fh_mat* A, int prec, fh_mat** out) {
+    return guard([&] {
+        NEED(ctx); NEED_DEV(ctx); NEED(A); NEED(out);
+        auto m = new fh_mat;
+        m->A = fh::bcsr_convert(ctx->c, *A->A, P(prec));
+        *out = m;
+    });
+}
+int fh_mat_destroy(fh_mat* A) {
+    return guard([&] { delete A; });
+}
+int fh_mat_info(const fh_mat* A, int* nbr, int* nbc, int* bs, int64_t* nnzb, int* prec) {
+    return guard([&] {
+        NEED(A);
+        if (nbr) *nbr = A->A->nbr;
+        if (nbc) *nbc = A->A->nbc;
+        if (bs) *bs = A->A->bs_r;
+        if (nnzb) *nnzb = A->A->nnzb;
+        if (prec) *prec = (int)A->A->prec;
+    });
+}
+int fh_mat_device_arrays(const fh_mat* A, const int** rowptr, const int** colidx, const void** vals) {
+    return guard([&] {
+        NEED(A);
+        if (rowptr) *rowptr = A->A->rowptr.as<int>();
+        if (colidx) *colidx = A->A->colidx.as<int>();
+        if (vals) *vals = A->A->vals.p;
+    });
+}
+
+int fh_spmv(fh_ctx* ctx, const fh_mat* A, double alpha, const void* x, double beta, void* y, int vprec) {
+    return guard([&] {
+        NEED(ctx); NEED_DEV(ctx); NEED(A); NEED(x); NEED(y);
+        fh::launch_bcsr(ctx->c, *A->A, fh::OP_SPMV, P(vprec), x, nullptr, y, alpha, beta, 0.0, nullptr, P(vprec),
+                        nullptr, 0);
+    });
+}
+int fh_residual(fh_ctx* ctx, const fh_mat* A, const void* b, const void* x, void* r, int vprec) {
+    return guard([&] {
+        NEED(ctx); NEED_DEV(ctx); NEED(A); NEED(b); NEED(x); NEED(r);
+        FH_CHECK(A->A->nbr == A->A->nbc, -2, "residual needs a square operator");
+        fh::launch_bcsr(ctx->c, *A->A, fh::OP_RESID, P(vprec), x, b, r, 1.0, 0.0, 0.0, nullptr, P(vprec), nullptr, 0);
+    });
+}
+int fh_dot(fh_ctx* ctx, int64_t n, const void* x, const void* y, int vprec, double* out) {
+    return guard([&] {
+        NEED(ctx); NEED_DEV(ctx); NEED(x); NEED(y); NEED(out);
+        const void* xs[1] = {x};
+        const void* ys[1] = {y};
+        int slot[1] = {fh::Ctx::kScalars - 1};
+        fh::launch_dots(ctx->c, n, P(vprec), 1, xs, ys, slot);
+        FH_HIP(hipMemcpyAsync(out, ctx->c.d_scalars + slot[0], sizeof(double), hipMemcpyDeviceToHost, ctx->c.stream));
+        ctx->c.sync();
+    });
+}
+int fh_axpby(fh_ctx* ctx, int64_t n, double a, const void* x, double b, void* y, int vprec) {
+    return guard([&] {
+        NEED(ctx); NEED_DEV(ctx); NEED(x); NEED(y);
+        fh::launch_axpby(ctx->c, n, P(vprec), x, y, a, -1, -1, b, -1, -1);
+    });
+}
+int fh_convert(fh_ctx* ctx, int64_t n, const void* src, int sprec, void* dst, int dprec) {
+    return guard([&] {
+        NEED(ctx); NEED_DEV(ctx); NEED(src); NEED(dst);
+        fh::launch_convert(ctx->c, n, src, P(sprec), dst, P(dprec));
+    });
+}
+
+int fh_bjac_create(fh_ctx* ctx, const fh_mat* A, double omega, int dprec, fh_bjac** out) {
+    return guard([&] {
+        NEED(ctx); NEED_DEV(ctx); NEED(A); NEED(out);
+        FH_CHECK(A->A->prec == fh::Prec::F64, -2, "block inverses are computed from an FP64 matrix");
+        FH_CHECK(omega > 0 && omega < 2, -2, "omega must lie in (0, 2)");
+        auto b = new fh_bjac;
+        b->S.spec.kind = fh::SmootherKind::BlockJacobi;
+        b->S.spec.omega = omega;
+        try {
+            fh::smoother_setup(ctx->c, *A->A, b->S, P(dprec));
+        } catch (...) {
+            delete b;
+            throw;
+        }
+        *out = b;
+    });
+}
+int fh_bjac_destroy(fh_bjac* bj) {
+    return guard([&] { delete bj; });
+}
+int fh_bjac_inverses(fh_ctx* ctx, const fh_bjac* bj, double* host_out) {
+    return guard([&] {
+        NEED(ctx); NEED_DEV(ctx); NEED(bj); NEED(host_out);
+        FH_CHECK(bj->S.dprec == fh::Prec::F64, -2, "download available for FP64 inverses");
+        FH_HIP(hipMemcpyAsync(host_out, bj->S.invD.p, bj->S.invD.bytes, hipMemcpyDeviceToHost, ctx->c.stream));
+        ctx->c.sync();
+    });
+}
+int fh_bjac_sweep(fh_ctx* ctx, const fh_bjac* bj, const fh_mat* A, const void* b, void* x, void* work, int nsweeps,
+                  int x_is_zero, int vprec) {
+    return guard([&] {
+        NEED(ctx); NEED_DEV(ctx); NEED(bj); NEED(A); NEED(b); NEED(x); NEED(work);
+        const fh::Bcsr& M = *A->A;
+        fh::Prec vp = P(vprec);
+        const int nf = M.bs_r;
+        void* cur = x;
+        void* nxt = work;
+        bool xz = x_is_zero != 0;
+        for (int s = 0; s < nsweeps; s++) {
+            if (xz)
+                fh::launch_bjac_zero(ctx->c, M.nbr, nf, vp, b, cur, bj->S.spec.omega, bj->S.invD.p, bj->S.dprec,
+                                     nullptr, 0);
+            else {
+                fh::launch_bcsr(ctx->c, M, fh::OP_SMOOTH, vp, cur, b, nxt, 1.0, 0.0, bj->S.spec.omega, bj->S.invD.p,
+                                bj->S.dprec, nullptr, 0);
+                std::swap(cur, nxt);
+            }
+            xz = false;
+        }
+        if (cur != x)
+            FH_HIP(hipMemcpyAsync(x, cur, (size_t)M.rows() * fh::prec_bytes(vp), hipMemcpyDeviceToDevice,
+                                  ctx->c.stream));
+    });
+}
+int fh_coloring_create(fh_ctx* ctx, const fh_mat* A, fh_coloring** out) {
+    return guard([&] {
+        NEED(ctx); NEED(A); NEED(out);
+        auto col = new fh_coloring;
+        col->ncolours = fh::greedy_colouring(*A->A, col->colour);
+        auto& S = col->S;
+        S.ncolours = col->ncolours;
+        S.colour_ptr.assign(S.ncolours + 1, 0);
+        for (int cc : col->colour) S.colour_ptr[cc + 1]++;
+        for (int k = 0; k < S.ncolours; k++) S.colour_ptr[k + 1] += S.colour_ptr[k];
+        std::vector<int> rows(A->A->nbr), pos(S.colour_ptr.begin(), S.colour_ptr.end() - 1);
+        for (int i = 0; i < A->A->nbr; i++) rows[pos[col->colour[i]]++] = i;
+        if (!ctx->c.host_only) {
+            S.colour_rows.alloc(sizeof(int) * rows.size());
+            FH_HIP(hipMemcpyAsync(S.colour_rows.p, rows.data(), sizeof(int) * rows.size(), hipMemcpyHostToDevice,
+                                  ctx->c.stream));
+            ctx->c.sync();
+        }
+        *out = col;
+    });
+}
+int fh_coloring_destroy(fh_coloring* col) {
+    return guard([&] { delete col; });
+}
+int fh_coloring_get(const fh_coloring* col, int* ncolours, int* colour_of_row) {
+    return guard([&] {
+        NEED(col);
+        if (ncolours) *ncolours = col->ncolours;
+        if (colour_of_row) std::memcpy(colour_of_row, col->colour.data(), sizeof(int) * col->colour.size());
+    });
+}
+int fh_mcgs_sweep(fh_ctx* ctx, const fh_mat* A, const fh_bjac* bj, const fh_coloring* col, int dir, const void* b,
+                  void* x, int vprec) {
+    return guard([&] {
+        NEED(ctx); NEED_DEV(ctx); NEED(A); NEED(bj); NEED(col); NEED(b); NEED(x);
+        FH_CHECK(dir >= 0 && dir <= 2, -2, "dir must be 0, 1 or 2");
+        const auto& S = col->S;
+        auto sweep = [&](bool backward) {
+            for (int k = 0; k < S.ncolours; k++) {
+                int cc = backward ? S.ncolours - 1 - k : k;
+                int cnt = S.colour_ptr[cc + 1] - S.colour_ptr[cc];
+                fh::launch_bcsr(ctx->c, *A->A, fh::OP_SMOOTH, P(vprec), x, b, x, 1.0, 0.0, 1.0, bj->S.invD.p,
+                                bj->S.dprec, S.colour_rows.as<int>() + S.colour_ptr[cc], cnt);
+            }
+        };
+        if (dir == 0 || dir == 2) sweep(false);
+        if (dir == 1 || dir == 2) sweep(true);
+    });
+}
+
+int fh_aggregate_pairwise(fh_ctx* ctx, const fh_mat* A, double theta, int* agg_out, int* nagg, int* coarsest) {
+    return guard([&] {
+        NEED(ctx); NEED(A); NEED(agg_out);
+        std::vector<int> agg;
+        std::vector<std::vector<int>> mem;
+        bool c = false;
+        int na = fh::pairwise_aggregate(*A->A, theta, agg, mem, c);
+        std::memcpy(agg_out, agg.data(), sizeof(int) * agg.size());
+        if (nagg) *nagg = na;
+        if (coarsest) *coarsest = c ? 1 : 0;
+    });
+}
+int fh_galerkin_aggregate(fh_ctx* ctx, const fh_mat* A, const int* agg, int nagg, fh_mat** out) {
+    return guard([&] {
+        NEED(ctx); NEED(A); NEED(agg); NEED(out);
+        std::vector<int> a(agg, agg + A->A->nbr);
+        for (int v : a) FH_CHECK(v >= 0 && v < nagg, -2, "aggregate index out of range");
+        auto m = new fh_mat;
+        m->A = fh::galerkin_aggregate(ctx->c, *A->A, a, nagg);
+        *out = m;
+    });
+}
+
+int fh_solver_create(fh_ctx* ctx, const fh_solver_args* a, fh_solver** out) {
+    return guard([&] {
+        NEED(ctx); NEED(a); NEED(out);
+        fh::SolverArgs s;
+        if (a->solver) s.solver = a->solver;
+        if (a->prec) s.prec = a->prec;
+        s.cycle = a->cycle ? a->cycle : 'V';
+        FH_CHECK(s.cycle == 'V' || s.cycle == 'W' || s.cycle == 'K', -2, "cycle must be V, W or K");
+        s.wloops = a->wloops > 0 ? a->wloops : (s.cycle == 'W' ? 2 : 1);
+        s.nu1 = a->nu1;
+        s.nu2 = a->nu2;
+        if (a->pre_smoother) s.pre_smoother = a->pre_smoother;
+        if (a->post_smoother) s.post_smoother = a->post_smoother;
+        s.relax = a->relax > 0 ? a->relax : 1.0;
+        s.tol = a->tol > 0 ? a->tol : 1e-8;
+        s.max_iter = a->max_iter > 0 ? a->max_iter : 200;
+        s.max_coarse = a->max_coarse > 0 ? a->max_coarse : 1000;
+        s.theta = a->theta > 0 ? a->theta : 0.25;
+        s.mg_prec = P(a->mg_prec);
+        s.mg_store = P(a->mg_store);
+        FH_CHECK(s.mg_prec != fh::Prec::BF16, -2, "multigrid vectors are FP64 or FP32");
+        FH_CHECK(s.mg_prec == fh::Prec::F32 || s.mg_store == fh::Prec::F64, -2,
+                 "FP32/BF16 level matrices need FP32 multigrid vectors");
+        FH_CHECK(s.mg_prec == fh::Prec::F64 || s.mg_store != fh::Prec::F64, -2,
+                 "FP32 multigrid vectors need FP32 or BF16 level matrices");
+        auto sv = new fh_solver;
+        sv->ctx = ctx;
+        sv->s = std::make_unique<fh::Solver>(ctx->c, s);
+        *out = sv;
+    });
+}
+int fh_solver_destroy(fh_solver* s) {
+    return guard([&] { delete s; });
+}
+int fh_solver_set_level(fh_solver* s, int level, const fh_mat* A_level, const fh_mat* P_level) {
+    return guard([&] {
+        NEED(s);
+        FH_CHECK(level >= 0 && level < 64, -2, "level out of range");
+        s->s->SetLevel(level, A_level ? A_level->A.get() : nullptr, P_level ? P_level->A.get() : nullptr);
+    });
+}
+int fh_solver_setup(fh_solver* s, const fh_mat* A) {
+    return guard([&] {
+        NEED(s); NEED(A);
+        FH_CHECK(!s->ctx->c.host_only, -9, "solver set-up needs a device context");
+        FH_CHECK(A->A->prec == fh::Prec::F64, -2, "the system matrix must be FP64");
+        s->s->Setup(*A->A);
+    });
+}
+int fh_solver_solve(fh_solver* s, const double* b, double* x, fh_solve_stats* st) {
+    return guard([&] {
+        NEED(s); NEED(b); NEED(x);
+        s->s->Solve(b, x);
+        if (st) {
+            const auto& t = s->s->Stats();
+            st->iterations = t.iterations;
+            st->converged = t.converged;
+            st->levels = t.levels;
+            st->final_res = t.final_res;
+            st->setup_s = t.setup_s;
+            st->solve_s = t.solve_s;
+            st->op_complexity = t.op_complexity;
+        }
+    });
+}
+int fh_solver_history(const fh_solver* s, double* out, int cap) {
+    int n = 0;
+    int rc = guard([&] {
+        NEED(s);
+        const auto& h = s->s->History();
+        n = (int)std::min<size_t>(h.size(), (size_t)cap);
+        if (out) std::memcpy(out, h.data(), sizeof(double) * n);
+    });
+    return rc ? rc : n;
+}
+int fh_solver_level_size(const fh_solver* s, int level) {
+    int n = 0;
+    int rc = guard([&] {
+        NEED(s);
+        FH_CHECK(level >= 0 && level < s->s->NumLevels(), -2, "level out of range");
+        n = s->s->LevelUnknowns(level);
+    });
+    return rc ? rc : n;
+}
+
+}  // extern "C"

@@ -1,0 +1,118 @@
+// Shared definitions of the MI355X solve-path library (host side).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace fh {
+
+// Storage precision of matrix values and vectors.
+enum class Prec : int { F64 = 0, F32 = 1, BF16 = 2 };
+
+inline size_t prec_bytes(Prec p) { return p == Prec::F64 ? 8 : (p == Prec::F32 ? 4 : 2); }
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define FH_HIP(call)                                                                    \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            throw ::fh::Error(-10, std::string("HIP error ") + hipGetErrorString(e_) +  \
+                                       " at " __FILE__ ":" + std::to_string(__LINE__)); \
+    } while (0)
+
+#define FH_CHECK(cond, code, msg) \
+    do {                          \
+        if (!(cond)) throw ::fh::Error(code, msg); \
+    } while (0)
+
+// Execution context: one device, one stream (default priority), scratch for
+// reductions and a pinned mailbox for scalars read back by the host.
+struct Ctx {
+    int device = 0;
+    bool host_only = false;         // device < 0: host set-up algorithms only, no HIP calls
+    hipStream_t stream = nullptr;
+    double* d_partials = nullptr;   // reduction partials (kMaxPartials per slot)
+    double* d_scalars = nullptr;    // device scalars (kScalars)
+    double* h_scalars = nullptr;    // pinned host mirror
+    static constexpr int kMaxPartials = 2048;
+    static constexpr int kPartialSlots = 4;
+    static constexpr int kScalars = 512;  // K-cycle uses 32 + 8 * level + 7 (levels < 40)
+    explicit Ctx(int dev);
+    ~Ctx();
+    void sync() {
+        if (!host_only) FH_HIP(hipStreamSynchronize(stream));
+    }
+};
+
+// Owning device allocation.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    explicit DevBuf(size_t b) { alloc(b); }
+    void alloc(size_t b) {
+        release();
+        bytes = b;
+        if (b) FH_HIP(hipMalloc(&p, b));
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    ~DevBuf() { release(); }
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+    DevBuf& operator=(DevBuf&& o) noexcept {
+        if (this != &o) { release(); p = o.p; bytes = o.bytes; o.p = nullptr; o.bytes = 0; }
+        return *this;
+    }
+    template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+// Device vector of a given precision.
+struct DVec {
+    int64_t n = 0;
+    Prec prec = Prec::F64;
+    DevBuf buf;
+    DVec() = default;
+    DVec(int64_t n_, Prec p) { resize(n_, p); }
+    void resize(int64_t n_, Prec p) {
+        if (n_ == n && p == prec && buf.p) return;
+        n = n_; prec = p;
+        buf.alloc((size_t)n * prec_bytes(p));
+    }
+    void* data() const { return buf.p; }
+};
+
+// Block-CSR matrix with dense nf_r x nf_c blocks stored row-major, block
+// rows contiguous.  Value offsets are 64-bit (block index * nf_r * nf_c).
+struct Bcsr {
+    int nbr = 0, nbc = 0, bs_r = 1, bs_c = 1;
+    int64_t nnzb = 0;
+    Prec prec = Prec::F64;
+    DevBuf rowptr, colidx, vals;
+    // host mirror of the structure (kept for set-up: aggregation, colouring)
+    std::vector<int> h_rowptr, h_colidx;
+    // host copy of FP64 values (kept only when a host set-up step needs it)
+    std::vector<double> h_vals;
+    // stored-entry mask of the scalar pattern inside the blocks (1 = the
+    // entry exists in the scalar sparse matrix, even if its value is 0);
+    // empty: an entry is stored iff its value is non-zero
+    std::vector<uint8_t> h_mask;
+    int64_t rows() const { return (int64_t)nbr * bs_r; }
+    int64_t cols() const { return (int64_t)nbc * bs_c; }
+    size_t value_bytes() const { return (size_t)nnzb * bs_r * bs_c * prec_bytes(prec); }
+};
+
+}  // namespace fh

@@ -1,0 +1,60 @@
+// Host-side launchers of the gfx950 kernels (fh_kernels.hip).
+#pragma once
+#include "fh_common.h"
+
+namespace fh {
+
+// Operation of the block-row kernel.
+enum BcsrMode : int {
+    OP_SPMV = 0,    // y = alpha * A x + beta * y
+    OP_RESID = 1,   // y = b - A x
+    OP_SMOOTH = 2,  // y = x + omega * D^-1 (b - A x)   (block Jacobi / GS colour)
+};
+
+// Block-row operation over all block rows, or over `rows[0..nrows)` when
+// rows != nullptr (a colour of the multicolour Gauss-Seidel; then y may
+// alias x).  Vectors are in `vprec` (F64 or F32); matrix values in A.prec;
+// invD (nbr blocks nf x nf) in dprec.
+void launch_bcsr(Ctx& c, const Bcsr& A, BcsrMode mode, Prec vprec, const void* x, const void* b, void* y,
+                 double alpha, double beta, double omega, const void* invD, Prec dprec,
+                 const int* rows, int nrows);
+
+// y = omega * D^-1 b over all block rows (smoothing from a zero guess).
+void launch_bjac_zero(Ctx& c, int nbr, int nf, Prec vprec, const void* b, void* y, double omega,
+                      const void* invD, Prec dprec, const int* rows, int nrows);
+
+// Batched inverse of the diagonal blocks of A (FP64 values, Gauss-Jordan
+// with partial pivoting per block); diag[i] = block index of A_ii.  Writes
+// nbr * nf * nf entries in dprec.  *d_flag (device int) is set to i+1 when
+// block i is singular.
+void launch_block_inverse(Ctx& c, const Bcsr& A, const int* diag, void* invD, Prec dprec, int* d_flag);
+
+// Conversions / BLAS-1 over n entries.
+void launch_convert(Ctx& c, int64_t n, const void* src, Prec sp, void* dst, Prec dp);
+void launch_fill(Ctx& c, int64_t n, void* x, Prec p, double v);
+// y = a*x + b*y, with a = ha * (ia >= 0 ? s[ia] : 1) * (ib >= 0 ? 1/s[ib] : 1)
+// (device scalars s; ia, ib < 0 use the host value only); same for b with
+// (hb, ic, id).
+void launch_axpby(Ctx& c, int64_t n, Prec p, const void* x, void* y, double ha, int ia, int ib, double hb,
+                  int ic, int id);
+// x += a d ; r -= a q with a = s[inum] / s[iden]  (CG / FCG update).
+void launch_cg_update(Ctx& c, int64_t n, Prec p, void* x, const void* d, void* r, const void* q, int inum, int iden,
+                      double sign);
+// Deterministic dot products: s[slot_k] = <x_k, y_k>, k < nd (nd <= 3),
+// accumulated in FP64 with a fixed reduction tree.
+void launch_dots(Ctx& c, int64_t n, Prec p, int nd, const void* const* xs, const void* const* ys, const int* slots);
+// Device scalar arithmetic: s[dst] = op(s[a], s[b]); op 0: a/b, 1: sqrt(a),
+// 2: a*b, 3: a-b, 4: copy a.
+void launch_scalar_op(Ctx& c, int op, int dst, int a, int b);
+
+// Aggregation transfers (piecewise-constant prolongation, block identity).
+// restrict: rc[I*nf+a] = sum_{k in members(I)} r[k*nf+a]
+void launch_agg_restrict(Ctx& c, int nagg, int nf, Prec p, const int* mptr, const int* members, const void* r,
+                         void* rc);
+// prolong-add: x[k*nf+a] += ec[agg[k]*nf+a]
+void launch_agg_prolong_add(Ctx& c, int nb, int nf, Prec p, const int* agg, const void* ec, void* x);
+
+// y = M x with M dense symmetric n x n (precision mp), vectors in vp.
+void launch_dense_symv(Ctx& c, int n, const void* M, Prec mp, Prec vp, const void* x, void* y);
+
+}  // namespace fh

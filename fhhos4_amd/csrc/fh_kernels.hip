@@ -1,0 +1,536 @@
+// gfx950 kernels of the condensed-HHO solve path.
+//
+// All kernels are written for CDNA wave64.  The block-row kernel is the
+// workhorse: one wavefront per block row, lane = (block slot, row in block),
+// so a wavefront streams SLOTS consecutive n_f x n_f blocks of the row as one
+// contiguous span of HBM (values are stored block row after block row), and
+// the per-row partial sums are reduced across slots with DPP/shuffles.
+// Workgroup ids are remapped so that each XCD walks a contiguous range of
+// block rows (better reuse of x in that XCD's L2).
+#include "fh_kernels.h"
+
+namespace fh {
+namespace {
+
+constexpr int kWG = 256;          // 4 wavefronts per workgroup
+constexpr int kDotBlocks = 1024;  // fixed reduction grid: deterministic dots
+
+struct bf16_t {
+    uint16_t u;
+};
+
+__device__ __forceinline__ double ldv(const double* p) { return *p; }
+__device__ __forceinline__ float ldv(const float* p) { return *p; }
+__device__ __forceinline__ float ldv(const bf16_t* p) { return __uint_as_float(((uint32_t)p->u) << 16); }
+
+__device__ __forceinline__ void stv(double* p, double v) { *p = v; }
+__device__ __forceinline__ void stv(float* p, double v) { *p = (float)v; }
+__device__ __forceinline__ void stv(bf16_t* p, double v) {
+    // round to nearest even
+    float f = (float)v;
+    uint32_t u = __float_as_uint(f);
+    uint32_t lsb = (u >> 16) & 1u;
+    u += 0x7FFFu + lsb;
+    p->u = (uint16_t)(u >> 16);
+}
+
+template <class TV> struct Acc { using T = double; };
+template <> struct Acc<float> { using T = float; };
+
+// Workgroup id -> logical id such that XCD x (= bid % 8) owns a contiguous
+// range of logical ids.
+__device__ __forceinline__ int xcd_remap(int bid, int nwg) {
+    int xcd = bid & 7, idx = bid >> 3;
+    int per = nwg >> 3, rem = nwg & 7;
+    return xcd * per + (xcd < rem ? xcd : rem) + idx;
+}
+
+template <int NF, class TA, class TV, class TD, int MODE>
+__global__ __launch_bounds__(kWG) void k_bcsr(int nbr, const int* __restrict__ rp, const int* __restrict__ ci,
+                                               const TA* __restrict__ val, const TV* x, const TV* __restrict__ b,
+                                               TV* y, double alpha, double beta, double omega,
+                                               const TD* __restrict__ invD, const int* __restrict__ rows,
+                                               int nrows) {
+    using AT = typename Acc<TV>::T;
+    constexpr int SLOTS = 64 / NF;
+    constexpr int ACT = SLOTS * NF;
+    const int lane = threadIdx.x & 63;
+    const int wg = xcd_remap(blockIdx.x, gridDim.x);
+    const int w = wg * (kWG / 64) + (threadIdx.x >> 6);
+    const int count = rows ? nrows : nbr;
+    if (w >= count) return;  // wave-uniform
+    const int br = rows ? rows[w] : w;
+    const int slot = lane / NF;
+    const int i = lane - slot * NF;
+    AT acc = 0;
+    if (lane < ACT) {
+        const int s = rp[br], e = rp[br + 1];
+        for (int p = s + slot; p < e; p += SLOTS) {
+            const int c = ci[p];
+            const TA* a = val + ((int64_t)p * NF + i) * NF;
+            const TV* xs = x + (int64_t)c * NF;
+#pragma unroll
+            for (int j = 0; j < NF; j++) acc += (AT)ldv(a + j) * (AT)xs[j];
+        }
+    }
+#pragma unroll
+    for (int off = NF; off < ACT; off <<= 1) {
+        AT o = __shfl_down(acc, off, 64);
+        acc += (lane + off < 64) ? o : (AT)0;
+    }
+    const int64_t row = (int64_t)br * NF + i;
+    if (MODE == OP_SPMV) {
+        if (lane < NF) {
+            AT v = (AT)alpha * acc;
+            if (beta != 0.0) v += (AT)beta * (AT)y[row];
+            y[row] = (TV)v;
+        }
+    } else if (MODE == OP_RESID) {
+        if (lane < NF) y[row] = (TV)((AT)b[row] - acc);
+    } else {
+        AT r = (lane < NF) ? (AT)b[row] - acc : (AT)0;
+        AT z = 0;
+        const TD* D = invD + ((int64_t)br * NF + (lane < NF ? i : 0)) * NF;
+#pragma unroll
+        for (int j = 0; j < NF; j++) {
+            AT rj = __shfl(r, j, 64);
+            if (lane < NF) z += (AT)ldv(D + j) * rj;
+        }
+        if (lane < NF) y[row] = (TV)((AT)x[row] + (AT)omega * z);
+    }
+}
+
+template <class TV, class TD>
+__global__ void k_bjac_zero(int count, int nf, const TV* __restrict__ b, TV* __restrict__ y, double omega,
+                            const TD* __restrict__ invD, const int* __restrict__ rows) {
+    using AT = typename Acc<TV>::T;
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)count * nf) return;
+    int w = (int)(t / nf), i = (int)(t - (int64_t)w * nf);
+    int br = rows ? rows[w] : w;
+    const TD* D = invD + ((int64_t)br * nf + i) * nf;
+    const TV* bb = b + (int64_t)br * nf;
+    AT z = 0;
+    for (int j = 0; j < nf; j++) z += (AT)ldv(D + j) * (AT)bb[j];
+    y[(int64_t)br * nf + i] = (TV)((AT)omega * z);
+}
+
+template <class TD>
+__global__ void k_block_inverse(int nbr, int nf, const double* __restrict__ val, const int* __restrict__ diag,
+                                TD* __restrict__ invD, int* flag) {
+    int br = blockIdx.x * blockDim.x + threadIdx.x;
+    if (br >= nbr) return;
+    double a[100], inv[100];
+    const double* D = val + (int64_t)diag[br] * nf * nf;
+    for (int k = 0; k < nf * nf; k++) a[k] = D[k];
+    for (int r = 0; r < nf; r++)
+        for (int c = 0; c < nf; c++) inv[r * nf + c] = r == c ? 1.0 : 0.0;
+    for (int c = 0; c < nf; c++) {
+        int p = c;
+        for (int r = c + 1; r < nf; r++)
+            if (fabs(a[r * nf + c]) > fabs(a[p * nf + c])) p = r;
+        if (a[p * nf + c] == 0.0) {
+            atomicCAS(flag, 0, br + 1);
+            return;
+        }
+        if (p != c)
+            for (int j = 0; j < nf; j++) {
+                double t = a[p * nf + j]; a[p * nf + j] = a[c * nf + j]; a[c * nf + j] = t;
+                t = inv[p * nf + j]; inv[p * nf + j] = inv[c * nf + j]; inv[c * nf + j] = t;
+            }
+        double d = 1.0 / a[c * nf + c];
+        for (int j = 0; j < nf; j++) { a[c * nf + j] *= d; inv[c * nf + j] *= d; }
+        for (int r = 0; r < nf; r++) {
+            if (r == c) continue;
+            double f = a[r * nf + c];
+            if (f == 0.0) continue;
+            for (int j = 0; j < nf; j++) { a[r * nf + j] -= f * a[c * nf + j]; inv[r * nf + j] -= f * inv[c * nf + j]; }
+        }
+    }
+    TD* out = invD + (int64_t)br * nf * nf;
+    for (int k = 0; k < nf * nf; k++) stv(out + k, inv[k]);
+}
+
+template <class TS, class TDd>
+__global__ void k_convert(int64_t n, const TS* __restrict__ s, TDd* __restrict__ d) {
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x)
+        stv(d + t, (double)ldv(s + t));
+}
+
+template <class T>
+__global__ void k_fill(int64_t n, T* x, double v) {
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x)
+        stv(x + t, v);
+}
+
+__device__ __forceinline__ double scal(const double* s, double h, int ia, int ib) {
+    double v = h;
+    if (ia >= 0) v *= s[ia];
+    if (ib >= 0) v /= s[ib];
+    return v;
+}
+
+template <class T>
+__global__ void k_axpby(int64_t n, const T* __restrict__ x, T* __restrict__ y, const double* s, double ha, int ia,
+                        int ib, double hb, int ic, int id) {
+    using AT = typename Acc<T>::T;
+    const AT a = (AT)scal(s, ha, ia, ib);
+    const AT bb = (AT)scal(s, hb, ic, id);
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+        AT v = a * (AT)x[t];
+        if (bb != (AT)0) v += bb * (AT)y[t];
+        y[t] = (T)v;
+    }
+}
+
+template <class T>
+__global__ void k_cg_update(int64_t n, T* __restrict__ x, const T* __restrict__ d, T* __restrict__ r,
+                            const T* __restrict__ q, const double* s, int inum, int iden, double sign) {
+    using AT = typename Acc<T>::T;
+    const double al = sign * s[inum] / s[iden];
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+        x[t] = (T)((AT)x[t] + (AT)al * (AT)d[t]);
+        r[t] = (T)((AT)r[t] - (AT)al * (AT)q[t]);
+    }
+}
+
+template <int ND>
+__device__ __forceinline__ void block_reduce(double (&v)[3], double* sh) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < ND; k++) {
+        double a = v[k];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) a += __shfl_down(a, off, 64);
+        if (lane == 0) sh[k * 16 + wid] = a;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int nw = blockDim.x >> 6;
+#pragma unroll
+        for (int k = 0; k < ND; k++) {
+            double a = 0;
+            for (int w = 0; w < nw; w++) a += sh[k * 16 + w];
+            v[k] = a;
+        }
+    }
+}
+
+template <class T, int ND>
+__global__ __launch_bounds__(kWG) void k_dots(int64_t n, const T* x0, const T* y0, const T* x1, const T* y1,
+                                               const T* x2, const T* y2, double* partials) {
+    __shared__ double sh[3 * 16];
+    double v[3] = {0, 0, 0};
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+        v[0] += (double)x0[t] * (double)y0[t];
+        if (ND > 1) v[1] += (double)x1[t] * (double)y1[t];
+        if (ND > 2) v[2] += (double)x2[t] * (double)y2[t];
+    }
+    block_reduce<ND>(v, sh);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < ND; k++) partials[k * kDotBlocks + blockIdx.x] = v[k];
+}
+
+template <int ND>
+__global__ __launch_bounds__(1024) void k_dots_final(const double* partials, double* s, int s0, int s1, int s2) {
+    __shared__ double sh[3 * 16];
+    double v[3] = {0, 0, 0};
+    const int t = threadIdx.x;
+    for (int k = 0; k < ND; k++) v[k] = partials[k * kDotBlocks + t];
+    block_reduce<ND>(v, sh);
+    if (t == 0) {
+        s[s0] = v[0];
+        if (ND > 1) s[s1] = v[1];
+        if (ND > 2) s[s2] = v[2];
+    }
+}
+
+__global__ void k_scalar_op(double* s, int op, int dst, int a, int b) {
+    double r = 0;
+    switch (op) {
+        case 0: r = s[a] / s[b]; break;
+        case 1: r = sqrt(s[a]); break;
+        case 2: r = s[a] * s[b]; break;
+        case 3: r = s[a] - s[b]; break;
+        default: r = s[a]; break;
+    }
+    s[dst] = r;
+}
+
+template <class T>
+__global__ void k_agg_restrict(int nagg, int nf, const int* __restrict__ mptr, const int* __restrict__ members,
+                               const T* __restrict__ r, T* __restrict__ rc) {
+    using AT = typename Acc<T>::T;
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)nagg * nf) return;
+    int I = (int)(t / nf), a = (int)(t - (int64_t)I * nf);
+    AT s = 0;
+    for (int q = mptr[I]; q < mptr[I + 1]; q++) s += (AT)r[(int64_t)members[q] * nf + a];
+    rc[t] = (T)s;
+}
+
+template <class T>
+__global__ void k_agg_prolong_add(int nb, int nf, const int* __restrict__ agg, const T* __restrict__ ec,
+                                  T* __restrict__ x) {
+    using AT = typename Acc<T>::T;
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)nb * nf) return;
+    int k = (int)(t / nf), a = (int)(t - (int64_t)k * nf);
+    x[t] = (T)((AT)x[t] + (AT)ec[(int64_t)agg[k] * nf + a]);
+}
+
+// One wavefront per row of a dense symmetric matrix.
+template <class TM, class TV>
+__global__ __launch_bounds__(kWG) void k_dense_symv(int n, const TM* __restrict__ M, const TV* __restrict__ x,
+                                                     TV* __restrict__ y) {
+    using AT = typename Acc<TV>::T;
+    const int lane = threadIdx.x & 63;
+    const int row = blockIdx.x * (kWG / 64) + (threadIdx.x >> 6);
+    if (row >= n) return;
+    const TM* m = M + (int64_t)row * n;
+    AT s = 0;
+    for (int j = lane; j < n; j += 64) s += (AT)ldv(m + j) * (AT)x[j];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if (lane == 0) y[row] = (TV)s;
+}
+
+inline int grid_for(int64_t n, int bs = 256, int cap = 65536) {
+    int64_t g = (n + bs - 1) / bs;
+    if (g > cap) g = cap;
+    if (g < 1) g = 1;
+    return (int)g;
+}
+
+// ---------------------------------------------------------------- dispatch
+template <int NF, class TA, class TV, class TD>
+void bcsr_dispatch_mode(Ctx& c, const Bcsr& A, BcsrMode mode, const void* x, const void* b, void* y, double alpha,
+                        double beta, double omega, const void* invD, const int* rows, int nrows) {
+    int count = rows ? nrows : A.nbr;
+    if (count <= 0) return;
+    int nwg = (count + 3) / 4;
+    auto rp = A.rowptr.as<int>();
+    auto ci = A.colidx.as<int>();
+    auto val = A.vals.as<TA>();
+    auto xx = (const TV*)x;
+    auto bb = (const TV*)b;
+    auto yy = (TV*)y;
+    auto D = (const TD*)invD;
+    switch (mode) {
+        case OP_SPMV:
+            hipLaunchKernelGGL((k_bcsr<NF, TA, TV, TD, OP_SPMV>), dim3(nwg), dim3(kWG), 0, c.stream, A.nbr, rp, ci, val,
+                               xx, bb, yy, alpha, beta, omega, D, rows, nrows);
+            break;
+        case OP_RESID:
+            hipLaunchKernelGGL((k_bcsr<NF, TA, TV, TD, OP_RESID>), dim3(nwg), dim3(kWG), 0, c.stream, A.nbr, rp, ci,
+                               val, xx, bb, yy, alpha, beta, omega, D, rows, nrows);
+            break;
+        default:
+            hipLaunchKernelGGL((k_bcsr<NF, TA, TV, TD, OP_SMOOTH>), dim3(nwg), dim3(kWG), 0, c.stream, A.nbr, rp, ci,
+                               val, xx, bb, yy, alpha, beta, omega, D, rows, nrows);
+            break;
+    }
+    FH_HIP(hipGetLastError());
+}
+
+template <int NF>
+void bcsr_dispatch_types(Ctx& c, const Bcsr& A, BcsrMode mode, Prec vp, const void* x, const void* b, void* y,
+                         double alpha, double beta, double omega, const void* invD, Prec dp, const int* rows,
+                         int nrows) {
+    if (vp == Prec::F64) {
+        FH_CHECK(A.prec == Prec::F64 && (mode != OP_SMOOTH || dp == Prec::F64), -3,
+                 "FP64 vectors need FP64 matrix and FP64 block inverses");
+        bcsr_dispatch_mode<NF, double, double, double>(c, A, mode, x, b, y, alpha, beta, omega, invD, rows, nrows);
+    } else if (vp == Prec::F32) {
+        FH_CHECK(mode != OP_SMOOTH || dp == Prec::F32, -3, "FP32 smoothing needs FP32 block inverses");
+        if (A.prec == Prec::F32)
+            bcsr_dispatch_mode<NF, float, float, float>(c, A, mode, x, b, y, alpha, beta, omega, invD, rows, nrows);
+        else if (A.prec == Prec::BF16)
+            bcsr_dispatch_mode<NF, bf16_t, float, float>(c, A, mode, x, b, y, alpha, beta, omega, invD, rows, nrows);
+        else
+            throw Error(-3, "FP32 vectors need an FP32 or BF16 matrix");
+    } else
+        throw Error(-3, "vectors must be FP64 or FP32");
+}
+
+}  // namespace
+
+void launch_bcsr(Ctx& c, const Bcsr& A, BcsrMode mode, Prec vprec, const void* x, const void* b, void* y,
+                 double alpha, double beta, double omega, const void* invD, Prec dprec, const int* rows,
+                 int nrows) {
+    FH_CHECK(A.bs_r == A.bs_c, -3, "block-row kernel needs square blocks");
+    switch (A.bs_r) {
+        case 1: bcsr_dispatch_types<1>(c, A, mode, vprec, x, b, y, alpha, beta, omega, invD, dprec, rows, nrows); break;
+        case 2: bcsr_dispatch_types<2>(c, A, mode, vprec, x, b, y, alpha, beta, omega, invD, dprec, rows, nrows); break;
+        case 3: bcsr_dispatch_types<3>(c, A, mode, vprec, x, b, y, alpha, beta, omega, invD, dprec, rows, nrows); break;
+        case 4: bcsr_dispatch_types<4>(c, A, mode, vprec, x, b, y, alpha, beta, omega, invD, dprec, rows, nrows); break;
+        case 6: bcsr_dispatch_types<6>(c, A, mode, vprec, x, b, y, alpha, beta, omega, invD, dprec, rows, nrows); break;
+        case 10: bcsr_dispatch_types<10>(c, A, mode, vprec, x, b, y, alpha, beta, omega, invD, dprec, rows, nrows); break;
+        default: throw Error(-3, "unsupported block size " + std::to_string(A.bs_r) + " (1,2,3,4,6,10)");
+    }
+}
+
+void launch_bjac_zero(Ctx& c, int nbr, int nf, Prec vp, const void* b, void* y, double omega, const void* invD,
+                      Prec dp, const int* rows, int nrows) {
+    int count = rows ? nrows : nbr;
+    if (count <= 0) return;
+    int64_t nt = (int64_t)count * nf;
+    int g = (int)((nt + 255) / 256);
+    if (vp == Prec::F64 && dp == Prec::F64)
+        hipLaunchKernelGGL((k_bjac_zero<double, double>), dim3(g), dim3(256), 0, c.stream, count, nf, (const double*)b,
+                           (double*)y, omega, (const double*)invD, rows);
+    else if (vp == Prec::F32 && dp == Prec::F32)
+        hipLaunchKernelGGL((k_bjac_zero<float, float>), dim3(g), dim3(256), 0, c.stream, count, nf, (const float*)b,
+                           (float*)y, omega, (const float*)invD, rows);
+    else
+        throw Error(-3, "unsupported precision pair for block Jacobi");
+    FH_HIP(hipGetLastError());
+}
+
+void launch_block_inverse(Ctx& c, const Bcsr& A, const int* diag, void* invD, Prec dp, int* d_flag) {
+    FH_CHECK(A.prec == Prec::F64, -3, "block inverse reads FP64 values");
+    FH_CHECK(A.bs_r <= 10, -3, "block size above 10");
+    int g = (A.nbr + 127) / 128;
+    if (A.nbr == 0) return;
+    if (dp == Prec::F64)
+        hipLaunchKernelGGL((k_block_inverse<double>), dim3(g), dim3(128), 0, c.stream, A.nbr, A.bs_r,
+                           A.vals.as<double>(), diag, (double*)invD, d_flag);
+    else if (dp == Prec::F32)
+        hipLaunchKernelGGL((k_block_inverse<float>), dim3(g), dim3(128), 0, c.stream, A.nbr, A.bs_r,
+                           A.vals.as<double>(), diag, (float*)invD, d_flag);
+    else
+        throw Error(-3, "block inverses are stored in FP64 or FP32");
+    FH_HIP(hipGetLastError());
+}
+
+void launch_convert(Ctx& c, int64_t n, const void* src, Prec sp, void* dst, Prec dp) {
+    if (n <= 0) return;
+    int g = grid_for(n);
+#define FH_CONV(TS, TD)                                                                                  \
+    hipLaunchKernelGGL((k_convert<TS, TD>), dim3(g), dim3(256), 0, c.stream, n, (const TS*)src, (TD*)dst); \
+    break
+    if (sp == Prec::F64) {
+        switch (dp) {
+            case Prec::F64: FH_CONV(double, double);
+            case Prec::F32: FH_CONV(double, float);
+            case Prec::BF16: FH_CONV(double, bf16_t);
+        }
+    } else if (sp == Prec::F32) {
+        switch (dp) {
+            case Prec::F64: FH_CONV(float, double);
+            case Prec::F32: FH_CONV(float, float);
+            case Prec::BF16: FH_CONV(float, bf16_t);
+        }
+    } else {
+        switch (dp) {
+            case Prec::F64: FH_CONV(bf16_t, double);
+            case Prec::F32: FH_CONV(bf16_t, float);
+            case Prec::BF16: FH_CONV(bf16_t, bf16_t);
+        }
+    }
+#undef FH_CONV
+    FH_HIP(hipGetLastError());
+}
+
+void launch_fill(Ctx& c, int64_t n, void* x, Prec p, double v) {
+    if (n <= 0) return;
+    int g = grid_for(n);
+    if (p == Prec::F64) hipLaunchKernelGGL((k_fill<double>), dim3(g), dim3(256), 0, c.stream, n, (double*)x, v);
+    else if (p == Prec::F32) hipLaunchKernelGGL((k_fill<float>), dim3(g), dim3(256), 0, c.stream, n, (float*)x, v);
+    else hipLaunchKernelGGL((k_fill<bf16_t>), dim3(g), dim3(256), 0, c.stream, n, (bf16_t*)x, v);
+    FH_HIP(hipGetLastError());
+}
+
+void launch_axpby(Ctx& c, int64_t n, Prec p, const void* x, void* y, double ha, int ia, int ib, double hb, int ic,
+                  int id) {
+    if (n <= 0) return;
+    int g = grid_for(n);
+    if (p == Prec::F64)
+        hipLaunchKernelGGL((k_axpby<double>), dim3(g), dim3(256), 0, c.stream, n, (const double*)x, (double*)y,
+                           c.d_scalars, ha, ia, ib, hb, ic, id);
+    else
+        hipLaunchKernelGGL((k_axpby<float>), dim3(g), dim3(256), 0, c.stream, n, (const float*)x, (float*)y,
+                           c.d_scalars, ha, ia, ib, hb, ic, id);
+    FH_HIP(hipGetLastError());
+}
+
+void launch_cg_update(Ctx& c, int64_t n, Prec p, void* x, const void* d, void* r, const void* q, int inum, int iden,
+                      double sign) {
+    if (n <= 0) return;
+    int g = grid_for(n);
+    if (p == Prec::F64)
+        hipLaunchKernelGGL((k_cg_update<double>), dim3(g), dim3(256), 0, c.stream, n, (double*)x, (const double*)d,
+                           (double*)r, (const double*)q, c.d_scalars, inum, iden, sign);
+    else
+        hipLaunchKernelGGL((k_cg_update<float>), dim3(g), dim3(256), 0, c.stream, n, (float*)x, (const float*)d,
+                           (float*)r, (const float*)q, c.d_scalars, inum, iden, sign);
+    FH_HIP(hipGetLastError());
+}
+
+void launch_dots(Ctx& c, int64_t n, Prec p, int nd, const void* const* xs, const void* const* ys, const int* slots) {
+    FH_CHECK(nd >= 1 && nd <= 3, -3, "1 to 3 dot products per launch");
+    const void* X[3] = {xs[0], nd > 1 ? xs[1] : xs[0], nd > 2 ? xs[2] : xs[0]};
+    const void* Y[3] = {ys[0], nd > 1 ? ys[1] : ys[0], nd > 2 ? ys[2] : ys[0]};
+#define FH_DOTS(T, ND)                                                                                            \
+    hipLaunchKernelGGL((k_dots<T, ND>), dim3(kDotBlocks), dim3(kWG), 0, c.stream, n, (const T*)X[0], (const T*)Y[0], \
+                       (const T*)X[1], (const T*)Y[1], (const T*)X[2], (const T*)Y[2], c.d_partials)
+    if (p == Prec::F64) {
+        if (nd == 1) FH_DOTS(double, 1); else if (nd == 2) FH_DOTS(double, 2); else FH_DOTS(double, 3);
+    } else {
+        if (nd == 1) FH_DOTS(float, 1); else if (nd == 2) FH_DOTS(float, 2); else FH_DOTS(float, 3);
+    }
+#undef FH_DOTS
+    int s0 = slots[0], s1 = nd > 1 ? slots[1] : 0, s2 = nd > 2 ? slots[2] : 0;
+    if (nd == 1) hipLaunchKernelGGL((k_dots_final<1>), dim3(1), dim3(kDotBlocks), 0, c.stream, c.d_partials, c.d_scalars, s0, s1, s2);
+    else if (nd == 2) hipLaunchKernelGGL((k_dots_final<2>), dim3(1), dim3(kDotBlocks), 0, c.stream, c.d_partials, c.d_scalars, s0, s1, s2);
+    else hipLaunchKernelGGL((k_dots_final<3>), dim3(1), dim3(kDotBlocks), 0, c.stream, c.d_partials, c.d_scalars, s0, s1, s2);
+    FH_HIP(hipGetLastError());
+}
+
+void launch_scalar_op(Ctx& c, int op, int dst, int a, int b) {
+    hipLaunchKernelGGL(k_scalar_op, dim3(1), dim3(1), 0, c.stream, c.d_scalars, op, dst, a, b);
+    FH_HIP(hipGetLastError());
+}
+
+void launch_agg_restrict(Ctx& c, int nagg, int nf, Prec p, const int* mptr, const int* members, const void* r,
+                         void* rc) {
+    int64_t nt = (int64_t)nagg * nf;
+    if (nt <= 0) return;
+    int g = (int)((nt + 255) / 256);
+    if (p == Prec::F64)
+        hipLaunchKernelGGL((k_agg_restrict<double>), dim3(g), dim3(256), 0, c.stream, nagg, nf, mptr, members,
+                           (const double*)r, (double*)rc);
+    else
+        hipLaunchKernelGGL((k_agg_restrict<float>), dim3(g), dim3(256), 0, c.stream, nagg, nf, mptr, members,
+                           (const float*)r, (float*)rc);
+    FH_HIP(hipGetLastError());
+}
+
+void launch_agg_prolong_add(Ctx& c, int nb, int nf, Prec p, const int* agg, const void* ec, void* x) {
+    int64_t nt = (int64_t)nb * nf;
+    if (nt <= 0) return;
+    int g = (int)((nt + 255) / 256);
+    if (p == Prec::F64)
+        hipLaunchKernelGGL((k_agg_prolong_add<double>), dim3(g), dim3(256), 0, c.stream, nb, nf, agg,
+                           (const double*)ec, (double*)x);
+    else
+        hipLaunchKernelGGL((k_agg_prolong_add<float>), dim3(g), dim3(256), 0, c.stream, nb, nf, agg,
+                           (const float*)ec, (float*)x);
+    FH_HIP(hipGetLastError());
+}
+
+void launch_dense_symv(Ctx& c, int n, const void* M, Prec mp, Prec vp, const void* x, void* y) {
+    if (n <= 0) return;
+    int g = (n + 3) / 4;
+    if (mp == Prec::F64 && vp == Prec::F64)
+        hipLaunchKernelGGL((k_dense_symv<double, double>), dim3(g), dim3(kWG), 0, c.stream, n, (const double*)M,
+                           (const double*)x, (double*)y);
+    else if (mp == Prec::F64 && vp == Prec::F32)
+        hipLaunchKernelGGL((k_dense_symv<double, float>), dim3(g), dim3(kWG), 0, c.stream, n, (const double*)M,
+                           (const float*)x, (float*)y);
+    else
+        throw Error(-3, "coarse matrix must be FP64");
+    FH_HIP(hipGetLastError());
+}
+
+}  // namespace fh

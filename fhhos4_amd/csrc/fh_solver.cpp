@@ -1,0 +1,588 @@
+// Host-side Solver API: Krylov drivers, multigrid hierarchy and cycles.
+#include "fh_solver.h"
+
+#include <chrono>
+#include <cmath>
+#include <utility>
+
+#include "fh_kernels.h"
+
+namespace fh {
+
+// Device scalar slots.
+enum : int {
+    S_RZ = 0, S_RZOLD = 1, S_DAD = 2, S_RR = 3, S_RD = 4, S_ZADK = 5, S_DKADK = 6, S_BB = 7,
+    S_KBASE = 32,  // K-cycle: 8 slots per level
+};
+
+Ctx::Ctx(int dev) : device(dev) {
+    if (dev < 0) {
+        host_only = true;
+        return;
+    }
+    FH_HIP(hipSetDevice(dev));
+    FH_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    FH_HIP(hipMalloc(&d_partials, sizeof(double) * kMaxPartials * kPartialSlots));
+    FH_HIP(hipMalloc(&d_scalars, sizeof(double) * kScalars));
+    FH_HIP(hipHostMalloc(&h_scalars, sizeof(double) * kScalars, hipHostMallocDefault));
+    FH_HIP(hipMemsetAsync(d_scalars, 0, sizeof(double) * kScalars, stream));
+    FH_HIP(hipStreamSynchronize(stream));
+}
+
+Ctx::~Ctx() {
+    if (host_only) return;
+    if (stream) (void)hipStreamSynchronize(stream);
+    if (d_partials) (void)hipFree(d_partials);
+    if (d_scalars) (void)hipFree(d_scalars);
+    if (h_scalars) (void)hipHostFree(h_scalars);
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+static void read_scalars(Ctx& c, int first, int count) {
+    FH_HIP(hipMemcpyAsync(c.h_scalars + first, c.d_scalars + first, sizeof(double) * count, hipMemcpyDeviceToHost,
+                          c.stream));
+    FH_HIP(hipStreamSynchronize(c.stream));
+}
+
+static void dot1(Ctx& c, int64_t n, Prec p, const void* x, const void* y, int slot) {
+    const void* xs[1] = {x};
+    const void* ys[1] = {y};
+    int sl[1] = {slot};
+    launch_dots(c, n, p, 1, xs, ys, sl);
+}
+static void dot2(Ctx& c, int64_t n, Prec p, const void* x0, const void* y0, int s0, const void* x1, const void* y1,
+                 int s1) {
+    const void* xs[2] = {x0, x1};
+    const void* ys[2] = {y0, y1};
+    int sl[2] = {s0, s1};
+    launch_dots(c, n, p, 2, xs, ys, sl);
+}
+static void dot3(Ctx& c, int64_t n, Prec p, const void* x0, const void* y0, int s0, const void* x1, const void* y1,
+                 int s1, const void* x2, const void* y2, int s2) {
+    const void* xs[3] = {x0, x1, x2};
+    const void* ys[3] = {y0, y1, y2};
+    int sl[3] = {s0, s1, s2};
+    launch_dots(c, n, p, 3, xs, ys, sl);
+}
+static void copy(Ctx& c, int64_t n, Prec p, const void* s, void* d) {
+    FH_HIP(hipMemcpyAsync(d, s, (size_t)n * prec_bytes(p), hipMemcpyDeviceToDevice, c.stream));
+}
+
+// --------------------------------------------------------------- smoothers
+SmootherSpec SmootherSpec::parse(const std::string& code, int iterations, double relax) {
+    SmootherSpec s;
+    s.iterations = iterations;
+    if (code == "bj" || code == "j") { s.kind = SmootherKind::BlockJacobi; s.omega = relax; }
+    else if (code == "bj23" || code == "j23") { s.kind = SmootherKind::BlockJacobi; s.omega = 2.0 / 3.0; }
+    else if (code == "bgs" || code == "gs" || code == "bsor" || code == "sor") { s.kind = SmootherKind::BlockGS; s.sweep = Sweep::Forward; }
+    else if (code == "rbgs" || code == "rgs" || code == "rbsor" || code == "rsor") { s.kind = SmootherKind::BlockGS; s.sweep = Sweep::Backward; }
+    else if (code == "sbgs" || code == "sgs") { s.kind = SmootherKind::BlockGS; s.sweep = Sweep::Symmetric; }
+    else throw Error(-2, "Unmanaged smoother code: '" + code + "'");
+    if (s.kind == SmootherKind::BlockGS) {
+        FH_CHECK(relax == 1.0, -2, "block SOR (relaxation != 1) is not available; use bgs/rbgs with -relax 1");
+        s.omega = 1.0;
+    }
+    return s;
+}
+
+void smoother_setup(Ctx& c, const Bcsr& A64, LevelSmoother& S, Prec dprec) {
+    S.dprec = dprec;
+    const int nf = A64.bs_r;
+    std::vector<int> diag = diag_blocks(A64);
+    DevBuf d_diag(sizeof(int) * diag.size());
+    FH_HIP(hipMemcpyAsync(d_diag.p, diag.data(), sizeof(int) * diag.size(), hipMemcpyHostToDevice, c.stream));
+    S.invD.alloc((size_t)A64.nbr * nf * nf * prec_bytes(dprec));
+    DevBuf flag(sizeof(int));
+    FH_HIP(hipMemsetAsync(flag.p, 0, sizeof(int), c.stream));
+    launch_block_inverse(c, A64, d_diag.as<int>(), S.invD.p, dprec, flag.as<int>());
+    int h_flag = 0;
+    FH_HIP(hipMemcpyAsync(&h_flag, flag.p, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    c.sync();
+    FH_CHECK(h_flag == 0, -6, "singular diagonal block in block row " + std::to_string(h_flag - 1));
+    if (S.spec.kind == SmootherKind::BlockGS) {
+        std::vector<int> col;
+        S.ncolours = greedy_colouring(A64, col);
+        S.colour_ptr.assign(S.ncolours + 1, 0);
+        for (int cc : col) S.colour_ptr[cc + 1]++;
+        for (int k = 0; k < S.ncolours; k++) S.colour_ptr[k + 1] += S.colour_ptr[k];
+        std::vector<int> rows(A64.nbr), pos(S.colour_ptr.begin(), S.colour_ptr.end() - 1);
+        for (int i = 0; i < A64.nbr; i++) rows[pos[col[i]]++] = i;
+        S.colour_rows.alloc(sizeof(int) * rows.size());
+        FH_HIP(hipMemcpyAsync(S.colour_rows.p, rows.data(), sizeof(int) * rows.size(), hipMemcpyHostToDevice, c.stream));
+        c.sync();
+    }
+}
+
+static void gs_sweep(Ctx& c, const Bcsr& A, LevelSmoother& S, Prec vp, const void* b, DVec& x, bool backward) {
+    for (int k = 0; k < S.ncolours; k++) {
+        int cc = backward ? S.ncolours - 1 - k : k;
+        int cnt = S.colour_ptr[cc + 1] - S.colour_ptr[cc];
+        const int* rows = S.colour_rows.as<int>() + S.colour_ptr[cc];
+        launch_bcsr(c, A, OP_SMOOTH, vp, x.data(), b, x.data(), 1.0, 0.0, 1.0, S.invD.p, S.dprec, rows, cnt);
+    }
+}
+
+void smoother_apply(Ctx& c, const Bcsr& A, LevelSmoother& S, Prec vp, const void* b, DVec& x, DVec& work,
+                    bool& x_is_zero) {
+    const int nf = A.bs_r;
+    for (int it = 0; it < S.spec.iterations; it++) {
+        if (S.spec.kind == SmootherKind::BlockJacobi) {
+            if (x_is_zero)
+                launch_bjac_zero(c, A.nbr, nf, vp, b, x.data(), S.spec.omega, S.invD.p, S.dprec, nullptr, 0);
+            else {
+                launch_bcsr(c, A, OP_SMOOTH, vp, x.data(), b, work.data(), 1.0, 0.0, S.spec.omega, S.invD.p, S.dprec,
+                            nullptr, 0);
+                std::swap(x.buf, work.buf);
+            }
+        } else {
+            if (x_is_zero) launch_fill(c, x.n, x.data(), vp, 0.0);
+            if (S.spec.sweep == Sweep::Forward || S.spec.sweep == Sweep::Symmetric) gs_sweep(c, A, S, vp, b, x, false);
+            if (S.spec.sweep == Sweep::Backward || S.spec.sweep == Sweep::Symmetric) gs_sweep(c, A, S, vp, b, x, true);
+        }
+        x_is_zero = false;
+    }
+}
+
+// --------------------------------------------------------------- multigrid
+struct MgLevel {
+    const Bcsr* A64 = nullptr;
+    std::unique_ptr<Bcsr> A64_own, Aw_own;
+    const Bcsr* Aw = nullptr;
+    LevelSmoother pre, post;
+    // transfer to the next coarser level
+    bool agg = false;
+    const Bcsr* P64 = nullptr;
+    std::unique_ptr<Bcsr> P_own, R64_own, Pw_own, Rw_own;
+    const Bcsr* Pw = nullptr;
+    const Bcsr* Rw = nullptr;
+    DevBuf agg_map, agg_ptr, agg_members;
+    int nagg = 0;
+    // work vectors (working precision)
+    DVec b, x, r, work, kc, kv, kd, kw;
+    int64_t n = 0;
+};
+
+class Multigrid {
+public:
+    Multigrid(Ctx& c, const SolverArgs& a) : c_(c), a_(a), vp_(a.mg_prec) {}
+    std::vector<MgLevel> L;
+    DevBuf coarse_inv;
+    int coarse_n = 0;
+    double op_complexity = 0;
+
+    void setup(const Bcsr& A, const std::vector<std::pair<const Bcsr*, const Bcsr*>>& given, bool amg) {
+        L.clear();
+        L.emplace_back();
+        L[0].A64 = &A;
+        if (amg) build_amg();
+        else {
+            for (size_t l = 1; l < given.size(); l++) {
+                FH_CHECK(given[l].first != nullptr, -2, "missing operator of level " + std::to_string(l));
+                L.emplace_back();
+                L[l].A64 = given[l].first;
+            }
+            for (size_t l = 0; l + 1 < given.size(); l++) {
+                FH_CHECK(given[l].second != nullptr, -2, "missing prolongation of level " + std::to_string(l));
+                L[l].P64 = given[l].second;
+                FH_CHECK(L[l].P64->nbr == L[l].A64->nbr && L[l].P64->nbc == L[l + 1].A64->nbr, -2,
+                         "prolongation shape does not match the level operators");
+                L[l].R64_own = bcsr_transpose(c_, *L[l].P64);  // R = P^T (LevelForHHO.h:535-543)
+            }
+        }
+        finish();
+    }
+
+    // Preconditioner application on FP64 vectors: z = M r (one cycle from 0).
+    void apply(const double* r, double* z) {
+        MgLevel& F = L[0];
+        if (vp_ == Prec::F64) {
+            copy(c_, F.n, Prec::F64, r, F.b.data());
+            cycle(0, F.b.data(), F.x, true, nullptr);
+            copy(c_, F.n, Prec::F64, F.x.data(), z);
+        } else {
+            launch_convert(c_, F.n, r, Prec::F64, F.b.data(), vp_);
+            cycle(0, F.b.data(), F.x, true, nullptr);
+            launch_convert(c_, F.n, F.x.data(), vp_, z, Prec::F64);
+        }
+    }
+    // Stand-alone cycle on FP64 vectors (x updated in place).
+    void iterate(const double* b, double* x) {
+        MgLevel& F = L[0];
+        FH_CHECK(vp_ == Prec::F64, -2, "stand-alone multigrid runs in FP64");
+        copy(c_, F.n, Prec::F64, x, F.x.data());
+        cycle(0, b, F.x, false, nullptr);
+        copy(c_, F.n, Prec::F64, F.x.data(), x);
+    }
+
+private:
+    Ctx& c_;
+    SolverArgs a_;
+    Prec vp_;
+
+    void build_amg() {
+        // AggregAMG with double pairwise aggregation (AggregAMG.h:32-139,
+        // Multigrid.h:121-283): coarsen while the level has more than
+        // max_coarse unknowns and the coarsening ratio stays above 1.01.
+        while (L.back().A64->rows() > a_.max_coarse && L.size() < 40) {
+            MgLevel& F = L.back();
+            const Bcsr& A = *F.A64;
+            std::vector<int> agg1, agg2;
+            std::vector<std::vector<int>> mem1, mem2;
+            bool coarsest = false;
+            int n1 = pairwise_aggregate(A, a_.theta, agg1, mem1, coarsest);
+            if (coarsest) break;
+            auto A1 = galerkin_aggregate(c_, A, agg1, n1);
+            int n2 = pairwise_aggregate(*A1, a_.theta, agg2, mem2, coarsest);
+            if (coarsest) break;
+            // final aggregates = members of members, in order (AggregAMG.h:142-154)
+            std::vector<int> ptr(n2 + 1, 0), mem;
+            mem.reserve(A.nbr);
+            for (int J = 0; J < n2; J++) {
+                for (int e : mem2[J])
+                    for (int k : mem1[e]) mem.push_back(k);
+                ptr[J + 1] = (int)mem.size();
+            }
+            std::vector<int> agg(A.nbr);
+            for (int i = 0; i < A.nbr; i++) agg[i] = agg2[agg1[i]];
+            auto Ac = galerkin_aggregate(c_, *A1, agg2, n2);
+            F.agg = true;
+            F.nagg = n2;
+            F.agg_map.alloc(sizeof(int) * agg.size());
+            F.agg_ptr.alloc(sizeof(int) * ptr.size());
+            F.agg_members.alloc(sizeof(int) * mem.size());
+            FH_HIP(hipMemcpyAsync(F.agg_map.p, agg.data(), sizeof(int) * agg.size(), hipMemcpyHostToDevice, c_.stream));
+            FH_HIP(hipMemcpyAsync(F.agg_ptr.p, ptr.data(), sizeof(int) * ptr.size(), hipMemcpyHostToDevice, c_.stream));
+            FH_HIP(hipMemcpyAsync(F.agg_members.p, mem.data(), sizeof(int) * mem.size(), hipMemcpyHostToDevice, c_.stream));
+            c_.sync();
+            MgLevel C;
+            C.A64_own = std::move(Ac);
+            C.A64 = C.A64_own.get();
+            L.push_back(std::move(C));
+        }
+    }
+
+    void finish() {
+        const Prec dprec = vp_ == Prec::F64 ? Prec::F64 : Prec::F32;
+        double fine_nnz = (double)L[0].A64->nnzb;
+        op_complexity = 0;
+        for (size_t l = 0; l < L.size(); l++) {
+            MgLevel& V = L[l];
+            V.n = V.A64->rows();
+            op_complexity += (double)V.A64->nnzb / fine_nnz;
+            if (a_.mg_store != Prec::F64) {
+                V.Aw_own = bcsr_convert(c_, *V.A64, a_.mg_store);
+                V.Aw = V.Aw_own.get();
+            } else
+                V.Aw = V.A64;
+            bool coarsest = l + 1 == L.size();
+            V.b.resize(V.n, vp_);
+            V.x.resize(V.n, vp_);
+            V.r.resize(V.n, vp_);
+            V.work.resize(V.n, vp_);
+            if (a_.cycle == 'K' && l > 0 && !coarsest) {
+                V.kc.resize(V.n, vp_); V.kv.resize(V.n, vp_); V.kd.resize(V.n, vp_); V.kw.resize(V.n, vp_);
+            }
+            if (!coarsest) {
+                V.pre.spec = SmootherSpec::parse(a_.pre_smoother, a_.nu1, a_.relax);
+                V.post.spec = SmootherSpec::parse(a_.post_smoother, a_.nu2, a_.relax);
+                smoother_setup(c_, *V.A64, V.pre, dprec);
+                smoother_setup(c_, *V.A64, V.post, dprec);
+                if (!V.agg) {
+                    Prec tp = vp_ == Prec::F64 ? Prec::F64 : Prec::F32;
+                    if (tp != Prec::F64) {
+                        V.Pw_own = bcsr_convert(c_, *V.P64, tp);
+                        V.Rw_own = bcsr_convert(c_, *V.R64_own, tp);
+                        V.Pw = V.Pw_own.get();
+                        V.Rw = V.Rw_own.get();
+                    } else {
+                        V.Pw = V.P64;
+                        V.Rw = V.R64_own.get();
+                    }
+                }
+            } else {
+                std::vector<double> inv = dense_spd_inverse(*V.A64);
+                coarse_n = (int)V.n;
+                coarse_inv.alloc(sizeof(double) * inv.size());
+                FH_HIP(hipMemcpyAsync(coarse_inv.p, inv.data(), sizeof(double) * inv.size(), hipMemcpyHostToDevice,
+                                      c_.stream));
+                c_.sync();
+            }
+        }
+    }
+
+    void restrict_to(int l, const DVec& r, DVec& rc) {
+        MgLevel& V = L[l];
+        if (V.agg)
+            launch_agg_restrict(c_, V.nagg, V.A64->bs_r, vp_, V.agg_ptr.as<int>(), V.agg_members.as<int>(), r.data(),
+                                rc.data());
+        else
+            launch_bcsr(c_, *V.Rw, OP_SPMV, vp_, r.data(), nullptr, rc.data(), 1.0, 0.0, 0.0, nullptr, vp_, nullptr, 0);
+    }
+    void prolong_add(int l, const DVec& ec, DVec& x) {
+        MgLevel& V = L[l];
+        if (V.agg)
+            launch_agg_prolong_add(c_, V.A64->nbr, V.A64->bs_r, vp_, V.agg_map.as<int>(), ec.data(), x.data());
+        else
+            launch_bcsr(c_, *V.Pw, OP_SPMV, vp_, ec.data(), nullptr, x.data(), 1.0, 1.0, 0.0, nullptr, vp_, nullptr, 0);
+    }
+
+public:
+    // Multigrid::MultigridCycle (Multigrid.h:438-608).
+    void cycle(int l, const void* b, DVec& x, bool x_is_zero, DVec* Ax) {
+        MgLevel& V = L[l];
+        if (l + 1 == (int)L.size()) {
+            launch_dense_symv(c_, coarse_n, coarse_inv.p, Prec::F64, vp_, b, x.data());
+            if (Ax) launch_bcsr(c_, *V.Aw, OP_SPMV, vp_, x.data(), nullptr, Ax->data(), 1.0, 0.0, 0.0, nullptr, vp_, nullptr, 0);
+            return;
+        }
+        bool xz = x_is_zero;
+        smoother_apply(c_, *V.Aw, V.pre, vp_, b, x, V.work, xz);
+        if (xz) copy(c_, V.n, vp_, b, V.r.data());
+        else launch_bcsr(c_, *V.Aw, OP_RESID, vp_, x.data(), b, V.r.data(), 1.0, 0.0, 0.0, nullptr, vp_, nullptr, 0);
+        MgLevel& C = L[l + 1];
+        restrict_to(l, V.r, C.b);
+        bool coarse_is_last = l + 2 == (int)L.size();
+        if (a_.cycle == 'K' && !coarse_is_last) {
+            kcycle(l + 1, C.b, C.x);
+        } else {
+            int loops = a_.cycle == 'W' ? std::max(2, a_.wloops) : 1;
+            bool ez = true;
+            for (int w = 0; w < loops; w++) {
+                cycle(l + 1, C.b.data(), C.x, ez, nullptr);
+                ez = false;
+                if (coarse_is_last) break;
+            }
+        }
+        if (xz) { launch_fill(c_, V.n, x.data(), vp_, 0.0); xz = false; }
+        prolong_add(l, C.x, x);
+        smoother_apply(c_, *V.Aw, V.post, vp_, b, x, V.work, xz);
+        if (Ax) launch_bcsr(c_, *V.Aw, OP_SPMV, vp_, x.data(), nullptr, Ax->data(), 1.0, 0.0, 0.0, nullptr, vp_, nullptr, 0);
+    }
+
+    // Multigrid::FCGForKCycle (Multigrid.h:611-656); r is overwritten.
+    void kcycle(int l, DVec& r, DVec& out) {
+        MgLevel& V = L[l];
+        const int s = S_KBASE + 8 * l;
+        const double t = 0.25;
+        cycle(l, r.data(), V.kc, true, &V.kv);
+        // rho1 = c.v, alpha1 = c.r, |r|^2
+        dot3(c_, V.n, vp_, V.kc.data(), V.kv.data(), s + 0, V.kc.data(), r.data(), s + 1, r.data(), r.data(), s + 2);
+        read_scalars(c_, s, 3);
+        double rho1 = c_.h_scalars[s + 0], alpha1 = c_.h_scalars[s + 1];
+        double rold = std::sqrt(c_.h_scalars[s + 2]);
+        launch_axpby(c_, V.n, vp_, V.kv.data(), r.data(), -(alpha1 / rho1), -1, -1, 1.0, -1, -1);
+        dot1(c_, V.n, vp_, r.data(), r.data(), s + 3);
+        read_scalars(c_, s + 3, 1);
+        double rnew = std::sqrt(c_.h_scalars[s + 3]);
+        if (rnew <= t * rold) {
+            launch_axpby(c_, V.n, vp_, V.kc.data(), out.data(), alpha1 / rho1, -1, -1, 0.0, -1, -1);
+            return;
+        }
+        cycle(l, r.data(), V.kd, true, &V.kw);
+        dot3(c_, V.n, vp_, V.kd.data(), V.kv.data(), s + 4, V.kd.data(), V.kw.data(), s + 5, V.kd.data(), r.data(), s + 6);
+        read_scalars(c_, s + 4, 3);
+        double gamma = c_.h_scalars[s + 4], beta = c_.h_scalars[s + 5], alpha2 = c_.h_scalars[s + 6];
+        double rho2 = beta - gamma * gamma / rho1;
+        double ca = alpha1 / rho1 - gamma * alpha2 / (rho1 * rho2), cb = alpha2 / rho2;
+        launch_axpby(c_, V.n, vp_, V.kc.data(), out.data(), ca, -1, -1, 0.0, -1, -1);
+        launch_axpby(c_, V.n, vp_, V.kd.data(), out.data(), cb, -1, -1, 1.0, -1, -1);
+    }
+};
+
+// ------------------------------------------------------------------ Solver
+Solver::Solver(Ctx& c, const SolverArgs& a) : c_(c), args_(a) {}
+Solver::~Solver() = default;
+
+void Solver::SetLevel(int l, const Bcsr* A, const Bcsr* P) {
+    if ((int)given_levels_.size() <= l) given_levels_.resize(l + 1, {nullptr, nullptr});
+    given_levels_[l] = {A, P};
+}
+
+int Solver::NumLevels() const { return mg_ ? (int)mg_->L.size() : 1; }
+int Solver::LevelUnknowns(int l) const {
+    if (!mg_) return A_ ? (int)A_->rows() : 0;
+    return (int)mg_->L.at(l).A64->rows();
+}
+
+void Solver::Setup(const Bcsr& A) {
+    auto t0 = std::chrono::steady_clock::now();
+    FH_CHECK(A.bs_r == A.bs_c && A.nbr == A.nbc, -2, "the operator must be square with square blocks");
+    A_ = &A;
+    const std::string& p = args_.solver == "cg" || args_.solver == "fcg" ? args_.prec : args_.solver;
+    mg_.reset();
+    prec_sm_.reset();
+    if (p == "mg" || p == "aggregamg") {
+        mg_ = std::make_unique<Multigrid>(c_, args_);
+        std::vector<std::pair<const Bcsr*, const Bcsr*>> g = given_levels_;
+        if (p == "mg") {
+            if (g.empty()) g.push_back({&A, nullptr});
+            g[0].first = &A;
+        }
+        mg_->setup(A, g, p == "aggregamg");
+    } else if (p == "bj" || p == "bj23" || p == "bgs") {
+        prec_sm_ = std::make_unique<LevelSmoother>();
+        prec_sm_->spec = SmootherSpec::parse(p, 1, args_.relax);
+        smoother_setup(c_, A, *prec_sm_, Prec::F64);
+    } else if (!(p.empty() || p == "none" || p == "default"))
+        throw Error(-2, "Unknown solver or preconditioner '" + p + "'");
+    // Multigrid::CanOptimizeResidualComputation (Multigrid.h:74-77): the
+    // reference obtains A z from a point Gauss-Seidel post-smoother.
+    fused_ = mg_ && mg_->L.size() > 1 && A.bs_r == 1 && args_.nu2 > 0 &&
+             SmootherSpec::parse(args_.post_smoother, 1, 1.0).kind == SmootherKind::BlockGS;
+    const int64_t n = A.rows();
+    r_.resize(n, Prec::F64); z_.resize(n, Prec::F64); d_.resize(n, Prec::F64); Ad_.resize(n, Prec::F64);
+    Az_.resize(n, Prec::F64); dk_.resize(n, Prec::F64); Adk_.resize(n, Prec::F64); t_.resize(n, Prec::F64);
+    c_.sync();
+    stats_ = SolveStats();
+    stats_.levels = NumLevels();
+    stats_.op_complexity = mg_ ? mg_->op_complexity : 1.0;
+    stats_.setup_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+double Solver::read_scalar(int slot) {
+    read_scalars(c_, slot, 1);
+    return c_.h_scalars[slot];
+}
+
+void Solver::apply_prec(const DVec& r, DVec& z, DVec* Az) {
+    const int64_t n = A_->rows();
+    if (mg_) mg_->apply((const double*)r.data(), (double*)z.data());
+    else if (prec_sm_) {
+        bool xz = true;
+        smoother_apply(c_, *A_, *prec_sm_, Prec::F64, r.data(), z, t_, xz);
+    } else
+        copy(c_, n, Prec::F64, r.data(), z.data());
+    if (Az) launch_bcsr(c_, *A_, OP_SPMV, Prec::F64, z.data(), nullptr, Az->data(), 1.0, 0.0, 0.0, nullptr, Prec::F64, nullptr, 0);
+}
+
+void Solver::Solve(const double* b, double* x) {
+    FH_CHECK(A_ != nullptr, -2, "Setup() must be called before Solve()");
+    c_.sync();
+    auto t0 = std::chrono::steady_clock::now();
+    hist_.clear();
+    if (args_.solver == "cg") solve_cg(b, x);
+    else if (args_.solver == "fcg") solve_fcg(b, x);
+    else solve_stationary(b, x);
+    c_.sync();
+    stats_.solve_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// IterativeSolver::StoppingCriteriaReached (IterativeSolver.h:224-247)
+#define FH_STOP_TEST()                                                                        \
+    [&]() -> bool {                                                                           \
+        if (it == 0) return false;                                                            \
+        FH_CHECK(!std::isinf(res), -7, "divergence of the solver");                           \
+        FH_CHECK(!std::isnan(res), -8, "the residual is NaN.");                               \
+        if (res < args_.tol) { stats_.converged = 1; return true; }                           \
+        if (it >= args_.max_iter) return true;                                                \
+        return false;                                                                         \
+    }
+
+void Solver::solve_cg(const double* b, double* x) {
+    // ConjugateGradient::Solve (ConjugateGradient.h:27-104), x0 = 0
+    const int64_t n = A_->rows();
+    const Prec P = Prec::F64;
+    launch_fill(c_, n, x, P, 0.0);
+    copy(c_, n, P, b, r_.data());
+    dot1(c_, n, P, b, b, S_BB);
+    double bn = std::sqrt(read_scalar(S_BB));
+    apply_prec(r_, z_, nullptr);
+    copy(c_, n, P, z_.data(), d_.data());
+    dot1(c_, n, P, r_.data(), z_.data(), S_RZ);
+    int it = 0;
+    double res = 1.0;
+    hist_.push_back(res);
+    stats_.converged = 0;
+    auto stop = FH_STOP_TEST();
+    while (!stop()) {
+        if (it > 0) launch_axpby(c_, n, P, z_.data(), d_.data(), 1.0, -1, -1, 1.0, S_RZ, S_RZOLD);  // d = z + beta d
+        launch_bcsr(c_, *A_, OP_SPMV, P, d_.data(), nullptr, Ad_.data(), 1.0, 0.0, 0.0, nullptr, P, nullptr, 0);
+        dot1(c_, n, P, d_.data(), Ad_.data(), S_DAD);
+        launch_cg_update(c_, n, P, x, d_.data(), r_.data(), Ad_.data(), S_RZ, S_DAD, 1.0);
+        launch_scalar_op(c_, 4, S_RZOLD, S_RZ, 0);
+        apply_prec(r_, z_, nullptr);
+        dot2(c_, n, P, r_.data(), z_.data(), S_RZ, r_.data(), r_.data(), S_RR);
+        it++;
+        double rr = read_scalar(S_RR);
+        res = bn > 0 ? std::sqrt(rr) / bn : std::sqrt(rr);
+        hist_.push_back(res);
+    }
+    stats_.iterations = it;
+    stats_.final_res = res;
+}
+
+void Solver::solve_fcg(const double* b, double* x) {
+    // FlexibleConjugateGradient::Solve, truncation 1 (FlexibleConjugateGradient.h:47-171), x0 = 0
+    const int64_t n = A_->rows();
+    const Prec P = Prec::F64;
+    launch_fill(c_, n, x, P, 0.0);
+    copy(c_, n, P, b, r_.data());
+    dot1(c_, n, P, b, b, S_BB);
+    double bn = std::sqrt(read_scalar(S_BB));
+    int it = 0;
+    double res = 1.0;
+    hist_.push_back(res);
+    stats_.converged = 0;
+    bool have = false;
+    auto stop = FH_STOP_TEST();
+    while (!stop()) {
+        apply_prec(r_, z_, fused_ ? &Az_ : nullptr);
+        copy(c_, n, P, z_.data(), d_.data());
+        if (fused_) copy(c_, n, P, Az_.data(), Ad_.data());
+        if (have) {
+            dot1(c_, n, P, z_.data(), Adk_.data(), S_ZADK);
+            // d -= (z.Adk / dk.Adk) dk
+            launch_axpby(c_, n, P, dk_.data(), d_.data(), -1.0, S_ZADK, S_DKADK, 1.0, -1, -1);
+            if (fused_) launch_axpby(c_, n, P, Adk_.data(), Ad_.data(), -1.0, S_ZADK, S_DKADK, 1.0, -1, -1);
+        }
+        if (!fused_)
+            launch_bcsr(c_, *A_, OP_SPMV, P, d_.data(), nullptr, Ad_.data(), 1.0, 0.0, 0.0, nullptr, P, nullptr, 0);
+        dot2(c_, n, P, d_.data(), Ad_.data(), S_DAD, r_.data(), d_.data(), S_RD);
+        launch_cg_update(c_, n, P, x, d_.data(), r_.data(), Ad_.data(), S_RD, S_DAD, 1.0);
+        std::swap(dk_.buf, d_.buf);
+        std::swap(Adk_.buf, Ad_.buf);
+        launch_scalar_op(c_, 4, S_DKADK, S_DAD, 0);
+        have = true;
+        dot1(c_, n, P, r_.data(), r_.data(), S_RR);
+        it++;
+        double rr = read_scalar(S_RR);
+        res = bn > 0 ? std::sqrt(rr) / bn : std::sqrt(rr);
+        hist_.push_back(res);
+    }
+    stats_.iterations = it;
+    stats_.final_res = res;
+}
+
+void Solver::solve_stationary(const double* b, double* x) {
+    // IterativeSolver::Solve (IterativeSolver.h:82-191) of a multigrid or a
+    // smoother used as the solver, x0 = 0.
+    const int64_t n = A_->rows();
+    const Prec P = Prec::F64;
+    launch_fill(c_, n, x, P, 0.0);
+    dot1(c_, n, P, b, b, S_BB);
+    double bn = std::sqrt(read_scalar(S_BB));
+    int it = 0;
+    double res = 1.0;
+    hist_.push_back(res);
+    stats_.converged = 0;
+    auto stop = FH_STOP_TEST();
+    bool xz = true;
+    while (!stop()) {
+        if (mg_) mg_->iterate(b, x);
+        else {
+            copy(c_, n, P, x, z_.data());
+            smoother_apply(c_, *A_, *prec_sm_, P, b, z_, t_, xz);
+            copy(c_, n, P, z_.data(), x);
+        }
+        launch_bcsr(c_, *A_, OP_RESID, P, x, b, r_.data(), 1.0, 0.0, 0.0, nullptr, P, nullptr, 0);
+        dot1(c_, n, P, r_.data(), r_.data(), S_RR);
+        it++;
+        double rr = read_scalar(S_RR);
+        res = bn > 0 ? std::sqrt(rr) / bn : std::sqrt(rr);
+        hist_.push_back(res);
+    }
+    stats_.iterations = it;
+    stats_.final_res = res;
+}
+
+}  // namespace fh

@@ -1,0 +1,324 @@
+"""ctypes binding of libfhgpu.so (the C ABI declared in include/fhgpu.h).
+
+Device vectors are passed as raw HBM pointers; this module accepts torch
+tensors on a ROCm device (``tensor.data_ptr()``) or integers.  The library
+must have been built (``fhhos4_amd.build.build()``): there is no fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libfhgpu.so")
+
+F64, F32, BF16 = 0, 1, 2
+_PREC = {"f64": F64, "fp64": F64, F64: F64, "f32": F32, "fp32": F32, F32: F32, "bf16": BF16, BF16: BF16}
+
+
+class FhError(RuntimeError):
+    pass
+
+
+class SolverArgs(C.Structure):
+    _fields_ = [("solver", C.c_char_p), ("prec", C.c_char_p), ("cycle", C.c_char), ("wloops", C.c_int),
+                ("nu1", C.c_int), ("nu2", C.c_int), ("pre_smoother", C.c_char_p), ("post_smoother", C.c_char_p),
+                ("relax", C.c_double), ("tol", C.c_double), ("max_iter", C.c_int), ("max_coarse", C.c_int),
+                ("theta", C.c_double), ("mg_prec", C.c_int), ("mg_store", C.c_int)]
+
+
+class SolveStats(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("converged", C.c_int), ("levels", C.c_int), ("pad", C.c_int),
+                ("final_res", C.c_double), ("setup_s", C.c_double), ("solve_s", C.c_double),
+                ("op_complexity", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
+
+
+_lib = None
+_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_vp = C.c_void_p
+_pp = C.POINTER(C.c_void_p)
+
+# name -> (restype, argtypes)
+_SIGS = {
+    "fh_ctx_create": [C.c_int, _pp],
+    "fh_ctx_destroy": [_vp],
+    "fh_ctx_sync": [_vp],
+    "fh_mat_from_csr": [_vp, C.c_int, C.c_int, C.c_int, _i32p, _i32p, _f64p, C.c_int, _pp],
+    "fh_mat_from_blocks": [_vp, C.c_int, C.c_int, C.c_int, _i32p, _i32p, _f64p, C.c_int, _pp],
+    "fh_mat_convert": [_vp, _vp, C.c_int, _pp],
+    "fh_mat_destroy": [_vp],
+    "fh_mat_info": [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int64),
+                    C.POINTER(C.c_int)],
+    "fh_mat_device_arrays": [_vp, _pp, _pp, _pp],
+    "fh_spmv": [_vp, _vp, C.c_double, _vp, C.c_double, _vp, C.c_int],
+    "fh_residual": [_vp, _vp, _vp, _vp, _vp, C.c_int],
+    "fh_dot": [_vp, C.c_int64, _vp, _vp, C.c_int, C.POINTER(C.c_double)],
+    "fh_axpby": [_vp, C.c_int64, C.c_double, _vp, C.c_double, _vp, C.c_int],
+    "fh_convert": [_vp, C.c_int64, _vp, C.c_int, _vp, C.c_int],
+    "fh_bjac_create": [_vp, _vp, C.c_double, C.c_int, _pp],
+    "fh_bjac_destroy": [_vp],
+    "fh_bjac_inverses": [_vp, _vp, _f64p],
+    "fh_bjac_sweep": [_vp, _vp, _vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int],
+    "fh_coloring_create": [_vp, _vp, _pp],
+    "fh_coloring_destroy": [_vp],
+    "fh_coloring_get": [_vp, C.POINTER(C.c_int), _i32p],
+    "fh_mcgs_sweep": [_vp, _vp, _vp, _vp, C.c_int, _vp, _vp, C.c_int],
+    "fh_aggregate_pairwise": [_vp, _vp, C.c_double, _i32p, C.POINTER(C.c_int), C.POINTER(C.c_int)],
+    "fh_galerkin_aggregate": [_vp, _vp, _i32p, C.c_int, _pp],
+    "fh_solver_create": [_vp, C.POINTER(SolverArgs), _pp],
+    "fh_solver_destroy": [_vp],
+    "fh_solver_set_level": [_vp, C.c_int, _vp, _vp],
+    "fh_solver_setup": [_vp, _vp],
+    "fh_solver_solve": [_vp, _vp, _vp, C.POINTER(SolveStats)],
+    "fh_solver_history": [_vp, _f64p, C.c_int],
+    "fh_solver_level_size": [_vp, C.c_int],
+}
+
+
+def lib():
+    """Loads libfhgpu.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FhError("libfhgpu.so is missing: build it with fhhos4_amd.build.build()")
+        L = C.CDLL(LIB_PATH)
+        for name, args in _SIGS.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = C.c_int
+        L.fh_last_error.restype = C.c_char_p
+        L.fh_last_error.argtypes = []
+        _lib = L
+    return _lib
+
+
+def exported_symbols():
+    return sorted(_SIGS) + ["fh_last_error"]
+
+
+def _check(rc):
+    if rc < 0:
+        raise FhError("fhgpu error %d: %s" % (rc, lib().fh_last_error().decode()))
+    return rc
+
+
+def _ptr(t):
+    if t is None:
+        return None
+    if isinstance(t, int):
+        return t
+    return t.data_ptr()
+
+
+def _vprec(t):
+    try:
+        import torch
+        if t.dtype == torch.float64:
+            return F64
+        if t.dtype == torch.float32:
+            return F32
+    except Exception:
+        pass
+    raise FhError("device vectors must be float64 or float32 tensors")
+
+
+class Context:
+    """Execution context on one device (device < 0: host set-up only)."""
+
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        _check(lib().fh_ctx_create(device, C.byref(h)))
+        self.h = h
+        self.device = device
+
+    def sync(self):
+        _check(lib().fh_ctx_sync(self.h))
+
+    def close(self):
+        if self.h:
+            lib().fh_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Matrix:
+    """Block-CSR operator (dense bs x bs blocks) on the device."""
+
+    def __init__(self, ctx, handle, keep=None):
+        self.ctx, self.h, self._keep = ctx, handle, keep
+
+    @classmethod
+    def from_csr(cls, ctx, A, bs, keep_host=True):
+        A = A.tocsr()
+        A.sort_indices()
+        rp = np.ascontiguousarray(A.indptr, dtype=np.int32)
+        ci = np.ascontiguousarray(A.indices, dtype=np.int32)
+        v = np.ascontiguousarray(A.data, dtype=np.float64)
+        h = C.c_void_p()
+        _check(lib().fh_mat_from_csr(ctx.h, A.shape[0], A.shape[1], bs, rp, ci, v, int(keep_host), C.byref(h)))
+        return cls(ctx, h)
+
+    @classmethod
+    def from_blocks(cls, ctx, nbc, bs, rowptr, colidx, vals, keep_host=True):
+        rp = np.ascontiguousarray(rowptr, dtype=np.int32)
+        ci = np.ascontiguousarray(colidx, dtype=np.int32)
+        v = np.ascontiguousarray(vals, dtype=np.float64).reshape(-1)
+        h = C.c_void_p()
+        _check(lib().fh_mat_from_blocks(ctx.h, len(rp) - 1, nbc, bs, rp, ci, v, int(keep_host), C.byref(h)))
+        return cls(ctx, h)
+
+    def convert(self, prec):
+        h = C.c_void_p()
+        _check(lib().fh_mat_convert(self.ctx.h, self.h, _PREC[prec], C.byref(h)))
+        return Matrix(self.ctx, h)
+
+    def info(self):
+        nbr, nbc, bs, pr = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        nnzb = C.c_int64()
+        _check(lib().fh_mat_info(self.h, C.byref(nbr), C.byref(nbc), C.byref(bs), C.byref(nnzb), C.byref(pr)))
+        return {"nbr": nbr.value, "nbc": nbc.value, "bs": bs.value, "nnzb": nnzb.value, "prec": pr.value}
+
+    def close(self):
+        if self.h:
+            lib().fh_mat_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def spmv(ctx, A, x, y, alpha=1.0, beta=0.0):
+    _check(lib().fh_spmv(ctx.h, A.h, alpha, _ptr(x), beta, _ptr(y), _vprec(x)))
+
+
+def residual(ctx, A, b, x, r):
+    _check(lib().fh_residual(ctx.h, A.h, _ptr(b), _ptr(x), _ptr(r), _vprec(x)))
+
+
+def dot(ctx, x, y):
+    out = C.c_double()
+    _check(lib().fh_dot(ctx.h, x.numel(), _ptr(x), _ptr(y), _vprec(x), C.byref(out)))
+    return out.value
+
+
+def axpby(ctx, a, x, b, y):
+    _check(lib().fh_axpby(ctx.h, x.numel(), a, _ptr(x), b, _ptr(y), _vprec(x)))
+
+
+class BlockJacobi:
+    def __init__(self, ctx, A, omega=1.0, dprec=F64):
+        h = C.c_void_p()
+        _check(lib().fh_bjac_create(ctx.h, A.h, omega, _PREC[dprec], C.byref(h)))
+        self.ctx, self.h, self.A = ctx, h, A
+
+    def inverses(self, nbr, bs):
+        out = np.zeros(nbr * bs * bs)
+        _check(lib().fh_bjac_inverses(self.ctx.h, self.h, out))
+        return out
+
+    def sweep(self, b, x, work, nsweeps=1, x_is_zero=False, A=None):
+        _check(lib().fh_bjac_sweep(self.ctx.h, self.h, (A or self.A).h, _ptr(b), _ptr(x), _ptr(work), nsweeps,
+                                   int(x_is_zero), _vprec(x)))
+
+    def __del__(self):
+        try:
+            lib().fh_bjac_destroy(self.h)
+        except Exception:
+            pass
+
+
+class Colouring:
+    def __init__(self, ctx, A):
+        h = C.c_void_p()
+        _check(lib().fh_coloring_create(ctx.h, A.h, C.byref(h)))
+        self.ctx, self.h, self.nbr = ctx, h, A.info()["nbr"]
+
+    def get(self):
+        nc = C.c_int()
+        col = np.zeros(self.nbr, dtype=np.int32)
+        _check(lib().fh_coloring_get(self.h, C.byref(nc), col))
+        return nc.value, col
+
+    def __del__(self):
+        try:
+            lib().fh_coloring_destroy(self.h)
+        except Exception:
+            pass
+
+
+def mcgs_sweep(ctx, A, bj, col, direction, b, x):
+    _check(lib().fh_mcgs_sweep(ctx.h, A.h, bj.h, col.h, direction, _ptr(b), _ptr(x), _vprec(x)))
+
+
+def aggregate_pairwise(ctx, A, theta=0.25):
+    nbr = A.info()["nbr"]
+    agg = np.zeros(nbr, dtype=np.int32)
+    na, co = C.c_int(), C.c_int()
+    _check(lib().fh_aggregate_pairwise(ctx.h, A.h, theta, agg, C.byref(na), C.byref(co)))
+    return na.value, agg, bool(co.value)
+
+
+def galerkin_aggregate(ctx, A, agg, nagg):
+    h = C.c_void_p()
+    _check(lib().fh_galerkin_aggregate(ctx.h, A.h, np.ascontiguousarray(agg, dtype=np.int32), nagg, C.byref(h)))
+    return Matrix(ctx, h)
+
+
+class Solver:
+    """Solver created from the reference's string codes (SolverFactory)."""
+
+    def __init__(self, ctx, solver="cg", prec="mg", cycle="V", nu=(1, 1), smoothers=("bj", "bj"), relax=1.0,
+                 tol=1e-8, max_iter=200, max_coarse=1000, theta=0.25, mg_prec="f64", mg_store=None, wloops=0):
+        a = SolverArgs()
+        self._strs = [s.encode() for s in (solver, prec or "", smoothers[0], smoothers[1])]
+        a.solver, a.prec, a.pre_smoother, a.post_smoother = self._strs
+        a.cycle = cycle.encode()
+        a.wloops = wloops
+        a.nu1, a.nu2 = nu
+        a.relax, a.tol, a.max_iter = relax, tol, max_iter
+        a.max_coarse, a.theta = max_coarse, theta
+        a.mg_prec = _PREC[mg_prec]
+        a.mg_store = _PREC[mg_store if mg_store is not None else mg_prec]
+        h = C.c_void_p()
+        _check(lib().fh_solver_create(ctx.h, C.byref(a), C.byref(h)))
+        self.ctx, self.h, self.max_iter = ctx, h, max_iter
+        self._levels = []
+
+    def set_level(self, level, A=None, P=None):
+        self._levels.append((A, P))
+        _check(lib().fh_solver_set_level(self.h, level, A.h if A is not None else None, P.h if P is not None else None))
+
+    def setup(self, A):
+        self.A = A
+        _check(lib().fh_solver_setup(self.h, A.h))
+
+    def solve(self, b, x):
+        st = SolveStats()
+        _check(lib().fh_solver_solve(self.h, _ptr(b), _ptr(x), C.byref(st)))
+        return st.as_dict()
+
+    def history(self):
+        out = np.zeros(self.max_iter + 2)
+        n = _check(lib().fh_solver_history(self.h, out, len(out)))
+        return out[:n]
+
+    def level_size(self, l):
+        return _check(lib().fh_solver_level_size(self.h, l))
+
+    def __del__(self):
+        try:
+            lib().fh_solver_destroy(self.h)
+        except Exception:
+            pass

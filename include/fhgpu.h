@@ -1,0 +1,159 @@
+/* fhgpu -- MI355X (gfx950) solve path of the statically condensed HHO face
+ * system.  C ABI: plain pointers and sizes, int status (0 = OK, < 0 = error;
+ * the message of the last error of the calling thread is fh_last_error()).
+ *
+ * Device vectors are raw HBM pointers owned by the caller (any allocator:
+ * hipMalloc, a torch tensor, ...).  Matrices are created from HOST arrays
+ * (copied; the library owns their device memory).  All work of one context
+ * runs on its own HIP stream; functions that return a host scalar or solver
+ * statistics synchronise that stream.  Handles are not thread-safe.
+ *
+ * Citations are files of the reference code (pmatalon/fhhos4, src/...).
+ */
+#ifndef FHGPU_H
+#define FHGPU_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fh_ctx fh_ctx;
+typedef struct fh_mat fh_mat;
+typedef struct fh_bjac fh_bjac;
+typedef struct fh_coloring fh_coloring;
+typedef struct fh_solver fh_solver;
+
+/* storage / vector precision codes */
+enum { FH_F64 = 0, FH_F32 = 1, FH_BF16 = 2 };
+
+/* ------------------------------------------------------------ context */
+/* Replaces the reference's implicit host execution context
+ * (src/Utils/ParallelLoop.h:34-57, BaseParallelLoop thread pool). */
+int fh_ctx_create(int device, fh_ctx** out);
+int fh_ctx_destroy(fh_ctx* ctx);
+int fh_ctx_sync(fh_ctx* ctx);
+/* Error reporting; replaces Utils::FatalError (src/Utils/Utils.h:258-263). */
+const char* fh_last_error(void);
+
+/* ------------------------------------------------------------ matrices */
+/* Condensed operator / transfer matrix from the reference's storage: scalar
+ * CSR (RowMajor, int32, src/Utils/Types.h:11-22), converted to dense bs x bs
+ * blocks.  Replaces NonZeroCoefficients::Fill into SparseMatrix
+ * (src/Utils/NonZeroCoefficients.h:94-96).  keep_host keeps FP64 values on
+ * the host for set-up algorithms (aggregation, Galerkin, transpose). */
+int fh_mat_from_csr(fh_ctx* ctx, int nrows, int ncols, int bs, const int* rowptr, const int* colidx,
+                    const double* vals, int keep_host, fh_mat** out);
+/* Same, from block arrays (nbr+1 row pointers, nnzb block columns, nnzb*bs*bs
+ * row-major block values). */
+int fh_mat_from_blocks(fh_ctx* ctx, int nbr, int nbc, int bs, const int* rowptr, const int* colidx,
+                       const double* vals, int keep_host, fh_mat** out);
+/* Copy with values stored in FH_F64 / FH_F32 / FH_BF16. */
+int fh_mat_convert(fh_ctx* ctx, const fh_mat* A, int prec, fh_mat** out);
+int fh_mat_destroy(fh_mat* A);
+int fh_mat_info(const fh_mat* A, int* nbr, int* nbc, int* bs, int64_t* nnzb, int* prec);
+/* Device pointers of the block structure and values (for profiling tools). */
+int fh_mat_device_arrays(const fh_mat* A, const int** rowptr, const int** colidx, const void** vals);
+
+/* ------------------------------------------------------------ kernels */
+/* y = alpha A x + beta y.  Replaces `A * x` of Eigen's SparseMatrix in the
+ * solvers (src/Solver/Krylov/ConjugateGradient.h:70,
+ * Multigrid/Level.h:139-159).  vprec: FH_F64 (matrix FP64) or FH_F32 (matrix
+ * FP32 or BF16, FP32 accumulation). */
+int fh_spmv(fh_ctx* ctx, const fh_mat* A, double alpha, const void* x, double beta, void* y, int vprec);
+/* r = b - A x  (src/Solver/Multigrid/Multigrid.h:478). */
+int fh_residual(fh_ctx* ctx, const fh_mat* A, const void* b, const void* x, void* r, int vprec);
+/* Deterministic dot product, FP64 accumulation (Eigen dot in
+ * src/Solver/Krylov/ConjugateGradient.h:73,86). */
+int fh_dot(fh_ctx* ctx, int64_t n, const void* x, const void* y, int vprec, double* out);
+/* y = a x + b y  (vector updates, ConjugateGradient.h:67,77,82). */
+int fh_axpby(fh_ctx* ctx, int64_t n, double a, const void* x, double b, void* y, int vprec);
+/* Precision conversion of a vector. */
+int fh_convert(fh_ctx* ctx, int64_t n, const void* src, int sprec, void* dst, int dprec);
+
+/* ------------------------------------------------------------ smoothers */
+/* Block Jacobi set-up: inverse of every diagonal block, stored in dprec
+ * (FH_F64 / FH_F32).  Replaces BlockJacobi::Setup
+ * (src/Solver/FixedPoint/BlockJacobi.h:57-74). */
+int fh_bjac_create(fh_ctx* ctx, const fh_mat* A, double omega, int dprec, fh_bjac** out);
+int fh_bjac_destroy(fh_bjac* bj);
+/* Download of the block inverses (nbr*bs*bs doubles). */
+int fh_bjac_inverses(fh_ctx* ctx, const fh_bjac* bj, double* host_out);
+/* nsweeps damped block-Jacobi sweeps x <- x + w D^-1 (b - A x); work is a
+ * device vector of the same length (x_is_zero: x is treated as 0).
+ * Replaces BlockJacobi::ExecuteOneIteration (BlockJacobi.h:77-127). */
+int fh_bjac_sweep(fh_ctx* ctx, const fh_bjac* bj, const fh_mat* A, const void* b, void* x, void* work, int nsweeps,
+                  int x_is_zero, int vprec);
+/* Greedy colouring of the block graph (block-row order). */
+int fh_coloring_create(fh_ctx* ctx, const fh_mat* A, fh_coloring** out);
+int fh_coloring_destroy(fh_coloring* col);
+int fh_coloring_get(const fh_coloring* col, int* ncolours, int* colour_of_row);
+/* One multicolour block Gauss-Seidel sweep, dir 0 forward / 1 backward /
+ * 2 symmetric; uses the block inverses of bj.  Replaces the lexicographic
+ * BlockGaussSeidel::ExecuteOneIteration (BlockGaussSeidel.h:75-191) with a
+ * colour ordering (same fixed point, different sweep order). */
+int fh_mcgs_sweep(fh_ctx* ctx, const fh_mat* A, const fh_bjac* bj, const fh_coloring* col, int dir, const void* b,
+                  void* x, int vprec);
+
+/* ------------------------------------------------------------ AMG set-up */
+/* One pairwise aggregation pass on block rows (host algorithm, bit-exact with
+ * AlgebraicMesh::Build + PairwiseAggregation::Perform,
+ * src/Solver/Multigrid/AggregAMG/AlgebraicMesh.h:54-153,
+ * PairwiseAggregation.h:9-205).  agg_out: nbr ints. */
+int fh_aggregate_pairwise(fh_ctx* ctx, const fh_mat* A, double theta, int* agg_out, int* nagg, int* coarsest);
+/* Galerkin operator of a piecewise-constant block prolongation
+ * (AggregLevel::GalerkinOperator, AggregAMG.h:156-180, extended to blocks). */
+int fh_galerkin_aggregate(fh_ctx* ctx, const fh_mat* A, const int* agg, int nagg, fh_mat** out);
+
+/* ------------------------------------------------------------ solvers */
+typedef struct fh_solver_args {
+    const char* solver;        /* "cg" | "fcg" | "mg" | "aggregamg" | "bj" | "bgs"  (SolverFactory.h:126-233) */
+    const char* prec;          /* "" | "bj" | "bj23" | "bgs" | "mg" | "aggregamg" */
+    char cycle;                /* 'V' | 'W' | 'K' (Multigrid.h:18-19) */
+    int wloops;
+    int nu1, nu2;              /* pre/post smoothing iterations */
+    const char* pre_smoother;  /* SmootherFactory codes: bj bj23 bgs rbgs sbgs gs rgs sgs j j23 */
+    const char* post_smoother;
+    double relax;              /* RelaxationParameter */
+    double tol;                /* relative residual ||r||/||b|| */
+    int max_iter;
+    int max_coarse;            /* MatrixMaxSizeForCoarsestLevel (ProgramArguments.h:61) */
+    double theta;              /* strong coupling threshold (SolverFactory.h:95) */
+    int mg_prec;               /* working precision of the multigrid vectors: FH_F64 | FH_F32 */
+    int mg_store;              /* storage of the level matrices: FH_F64 | FH_F32 | FH_BF16 */
+} fh_solver_args;
+
+typedef struct fh_solve_stats {
+    int iterations;
+    int converged;
+    int levels;
+    int pad;
+    double final_res;
+    double setup_s;
+    double solve_s;
+    double op_complexity;
+} fh_solve_stats;
+
+/* Creation from string codes (SolverFactory<Dim>::CreateSolver,
+ * src/Solver/SolverFactory.h:126-310). */
+int fh_solver_create(fh_ctx* ctx, const fh_solver_args* args, fh_solver** out);
+int fh_solver_destroy(fh_solver* s);
+/* Geometric multigrid hierarchy input: operator of level l (l >= 1) and the
+ * prolongation from level l+1 to level l (Level::OperatorMatrix / P,
+ * src/Solver/Multigrid/Level.h:11,32; LevelForHHO.h:182-213).  Matrices must
+ * outlive the solver (Solver.h:9,25-28). */
+int fh_solver_set_level(fh_solver* s, int level, const fh_mat* A_level, const fh_mat* P_level);
+/* Solver::Setup (Solver.h:25-28; Multigrid::Setup, Multigrid.h:88-313). */
+int fh_solver_setup(fh_solver* s, const fh_mat* A);
+/* IterativeSolver::Solve(b, "0") (IterativeSolver.h:55-73): x (device, FP64)
+ * receives the solution from a zero initial guess. */
+int fh_solver_solve(fh_solver* s, const double* b, double* x, fh_solve_stats* stats);
+/* Normalised residual at each stopping test (IterationResult.h:121-139);
+ * returns the number of entries written (<= cap). */
+int fh_solver_history(const fh_solver* s, double* out, int cap);
+/* Unknowns of multigrid level l. */
+int fh_solver_level_size(const fh_solver* s, int level);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FHGPU_H */

@@ -321,7 +321,9 @@ void galerkin(const Csr& A, int nf, const std::vector<int>& agg, int nagg, Csr& 
 }
 
 // ----------------------------------------------------------------- multigrid
-enum { SM_BJ = 1, SM_BGS_F = 2, SM_BGS_B = 3, SM_BGS_S = 4 };
+enum { SM_BJ = 1, SM_BGS_F = 2, SM_BGS_B = 3, SM_BGS_S = 4, SM_MC_F = 5, SM_MC_B = 6, SM_MC_S = 7 };
+
+int greedy_colour_blocks(const Csr& A, int nf, std::vector<int>& cptr, std::vector<int>& rows);
 
 struct Level {
     Csr A;
@@ -329,6 +331,7 @@ struct Level {
     std::vector<int> agg; int nagg = 0;  // AMG transfer
     bool amg = false;
     std::vector<double> invD;
+    std::vector<int> cptr, crows;   // multicolour ordering (greedy, block-row order)
 };
 
 struct MG {
@@ -340,7 +343,14 @@ struct MG {
     void smooth(Level& L, int kind, double omega, int nu, const double* b, double* x) {
         for (int s = 0; s < nu; s++) {
             if (kind == SM_BJ) bj_sweep(L.A, nf, L.invD.data(), omega, b, x);
-            else bgs_sweep(L.A, nf, L.invD.data(), kind - SM_BGS_F, b, x);
+            else if (kind >= SM_MC_F) {
+                if (L.cptr.empty()) greedy_colour_blocks(L.A, nf, L.cptr, L.crows);
+                int nc = (int)L.cptr.size() - 1;
+                if (kind != SM_MC_B)
+                    or_mcbgs_sweep(L.A.n, nf, L.A.prp, L.A.pci, L.A.pv, L.invD.data(), nc, L.cptr.data(), L.crows.data(), 0, b, x);
+                if (kind != SM_MC_F)
+                    or_mcbgs_sweep(L.A.n, nf, L.A.prp, L.A.pci, L.A.pv, L.invD.data(), nc, L.cptr.data(), L.crows.data(), 1, b, x);
+            } else bgs_sweep(L.A, nf, L.invD.data(), kind - SM_BGS_F, b, x);
         }
     }
     void restrict_(int l, const std::vector<double>& r, std::vector<double>& rc) {
@@ -426,6 +436,35 @@ struct MG {
 };
 
 double g_setup_s = 0, g_solve_s = 0;
+
+// Greedy colouring of the block graph of A (block rows in increasing order
+// take the smallest colour unused by lower-numbered neighbours); rows grouped
+// by colour in increasing order.
+int greedy_colour_blocks(const Csr& A, int nf, std::vector<int>& cptr, std::vector<int>& rows) {
+    int nb = A.n / nf;
+    std::vector<int> brp(nb + 1, 0), bci;
+    std::vector<int> mark(nb, -1);
+    for (int i = 0; i < nb; i++) {
+        std::vector<int> cols;
+        for (int k = 0; k < nf; k++)
+            for (int p = A.prp[i * nf + k]; p < A.prp[i * nf + k + 1]; p++) {
+                int j = A.pci[p] / nf;
+                if (mark[j] != i) { mark[j] = i; cols.push_back(j); }
+            }
+        std::sort(cols.begin(), cols.end());
+        bci.insert(bci.end(), cols.begin(), cols.end());
+        brp[i + 1] = (int)bci.size();
+    }
+    std::vector<int> col(nb);
+    int nc = or_greedy_colouring(nb, brp.data(), bci.data(), col.data());
+    cptr.assign(nc + 1, 0);
+    for (int c : col) cptr[c + 1]++;
+    for (int c = 0; c < nc; c++) cptr[c + 1] += cptr[c];
+    rows.assign(nb, 0);
+    std::vector<int> pos(cptr.begin(), cptr.end() - 1);
+    for (int i = 0; i < nb; i++) rows[pos[col[i]]++] = i;
+    return nc;
+}
 
 int build_amg(MG& mg, const Csr& A0, int nf, double theta, int max_coarse, int max_levels) {
     mg.lv.clear();

@@ -73,7 +73,8 @@ typedef struct or_solver_cfg {
     int prec;          /* 0 identity, 1 block Jacobi, 2 block GS, 3 GMG (given levels), 4 aggregation AMG */
     int nf;            /* block size */
     double omega;      /* relaxation of block Jacobi */
-    int pre_kind, post_kind;   /* 1 bj, 2 bgs forward, 3 bgs backward, 4 bgs symmetric */
+    int pre_kind, post_kind;   /* 1 bj, 2 bgs forward, 3 bgs backward, 4 bgs symmetric,
+                                  5/6/7 multicolour bgs forward/backward/symmetric */
     double pre_omega, post_omega;
     int nu1, nu2;
     char cycle;        /* 'V', 'W' or 'K' */

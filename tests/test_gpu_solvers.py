@@ -1,0 +1,138 @@
+"""End-to-end parity of the GPU solvers with the reference's own solver on
+the same exported systems (golden fixtures) and with the CPU oracle.
+
+Block-Jacobi-smoothed cycles must reproduce the reference's iteration count
+exactly and its FP64 solution to 1e-10 relative.  Gauss-Seidel runs as a
+multicolour sweep on the GPU, which changes the sweep order: those cases are
+checked against the oracle run with the same multicolour ordering
+(iterations equal, solution to 1e-10), and the reference's lexicographic
+count is stated beside it.
+"""
+import numpy as np
+import pytest
+
+from conftest import csr, gmg_levels, load
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+import fhhos4_amd as F  # noqa: E402
+
+G = F.fhgpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    assert torch.cuda.is_available(), "no ROCm device visible"
+    return G.Context(0)
+
+
+def dev(a):
+    t = torch.from_numpy(np.ascontiguousarray(a)).to(device="cuda", dtype=torch.float64)
+    torch.cuda.synchronize()
+    return t
+
+
+def run_gpu(ctx, d, nf, levels=None, **kw):
+    A = csr(d, "A")
+    M = G.Matrix.from_csr(ctx, A, nf, keep_host=True)
+    s = G.Solver(ctx, **kw)
+    keep = [M]
+    if levels:
+        for l, (Al, Pl) in enumerate(levels):
+            Ml = M if l == 0 else G.Matrix.from_csr(ctx, Al, nf)
+            Pm = G.Matrix.from_csr(ctx, Pl, nf) if Pl is not None else None
+            keep += [Ml, Pm]
+            s.set_level(l, Ml if l > 0 else None, Pm)
+    s.setup(M)
+    b = dev(d["b"])
+    x = torch.zeros_like(b)
+    st = s.solve(b, x)
+    ctx.sync()
+    return x.cpu().numpy(), s.history(), st
+
+
+BJ_CASES = [
+    # fixture (reference run), system fixture, kwargs, nf
+    ("c1_n64_cg_mg_bj", "c1_n64_cg_mg_bj", dict(solver="cg", prec="mg", smoothers=("bj", "bj")), 2),
+    ("c1_n64_cg_mg_bj23", "c1_n64_cg_mg_bj", dict(solver="cg", prec="mg", smoothers=("bj23", "bj23")), 2),
+    ("c1_n32_cg_mg_bj23", "c1_n32_cg_mg_bj23", dict(solver="cg", prec="mg", smoothers=("bj23", "bj23")), 2),
+    ("c1_n32_cg_mg_bj", "c1_n32_cg_mg_bj23", dict(solver="cg", prec="mg", smoothers=("bj", "bj"), max_iter=15), 2),
+    ("c1_n32_cg_bj", "c1_n32_cg_mg_bj23", dict(solver="cg", prec="bj"), 2),
+    ("c1_n32_cg_none", "c1_n32_cg_mg_bj23", dict(solver="cg", prec=""), 2),
+    ("c1_n32_fcg_mg_bj23", "c1_n32_cg_mg_bj23", dict(solver="fcg", prec="mg", smoothers=("bj23", "bj23")), 2),
+    ("c1_n32_cg_mg_w_bj23", "c1_n32_cg_mg_bj23", dict(solver="cg", prec="mg", cycle="W", smoothers=("bj23", "bj23")), 2),
+    ("cart8_k2_cg_mg_bj23", "cart8_k2_cg_mg_bj23", dict(solver="cg", prec="mg", smoothers=("bj23", "bj23")), 6),
+    ("tet4_k2", "tet4_k2", dict(solver="cg", prec="bj"), 6),
+]
+
+
+@pytest.mark.parametrize("case", BJ_CASES, ids=[c[0] for c in BJ_CASES])
+def test_solver_matches_reference(ctx, case):
+    name, sysname, kw, nf = case
+    print("case", name, flush=True)
+    f, d = load(name), load(sysname)
+    levels = gmg_levels(d) if kw.get("prec") == "mg" else None
+    x, hist, st = run_gpu(ctx, d, nf, levels=levels, **kw)
+    assert st["iterations"] == int(f["info_iterations"])
+    assert np.linalg.norm(x - f["x"]) <= 1e-10 * np.linalg.norm(f["x"])
+    k = min(6, len(hist))
+    np.testing.assert_allclose(hist[:k], f["hist"][:k], rtol=1e-8)
+
+
+GS_CASES = [
+    ("c1_n32_cg_mg_bgs", "c1_n32_cg_mg_bj23", "cg", "mg", "V", 2),
+    ("c1_n32_fcg_mg_k_bgs", "c1_n32_cg_mg_bj23", "fcg", "mg", "K", 2),
+    ("tet8_k0_fcg_amg", "tet8_k0_fcg_amg", "fcg", "aggregamg", "K", 1),
+    ("tet8_k0_cg_amg_v", "tet8_k0_fcg_amg", "cg", "aggregamg", "V", 1),
+]
+
+
+@pytest.mark.parametrize("case", GS_CASES, ids=[c[0] for c in GS_CASES])
+def test_gauss_seidel_solver_matches_oracle_multicolour(ctx, case):
+    name, sysname, solver, prec, cycle, nf = case
+    print("case", name, flush=True)
+    f, d = load(name), load(sysname)
+    A = csr(d, "A")
+    levels = gmg_levels(d) if prec == "mg" else None
+    maxit = 25 if name == "tet8_k0_cg_amg_v" else 200
+    x, hist, st = run_gpu(ctx, d, nf, levels=levels, solver=solver, prec=prec, cycle=cycle,
+                          smoothers=("bgs", "rbgs"), max_iter=maxit)
+    xo, ho, oo = O.solve(A, d["b"], solver, prec, nf=nf, cycle=cycle, smoothers=("mcbgs", "mcrbgs"),
+                         levels=levels, max_iter=maxit)
+    assert st["iterations"] == oo.iterations
+    assert np.linalg.norm(x - xo) <= 1e-10 * np.linalg.norm(xo)
+    # stated: reference (lexicographic) count and distance to its solution
+    print("reference lexicographic iterations", int(f["info_iterations"]), "multicolour", st["iterations"],
+          "rel. distance to reference solution", np.linalg.norm(x - f["x"]) / np.linalg.norm(f["x"]))
+    assert abs(st["iterations"] - int(f["info_iterations"])) <= 3
+
+
+def test_amg_hierarchy_matches_reference_levels(ctx):
+    d = load("tet8_k0_fcg_amg")
+    A = csr(d, "A")
+    M = G.Matrix.from_csr(ctx, A, 1, keep_host=True)
+    s = G.Solver(ctx, solver="fcg", prec="aggregamg", cycle="K", smoothers=("bgs", "rbgs"))
+    s.setup(M)
+    sizes = [s.level_size(l) for l in range(int(d["nlevels"]))]
+    ref = [A.shape[0]] + [csr(d, "level%d_A" % l).shape[0] for l in range(1, int(d["nlevels"]))]
+    assert sizes == ref
+
+
+@pytest.mark.parametrize("mg_prec,mg_store", [("f32", "f32"), ("f32", "bf16")])
+def test_mixed_precision_fcg_reaches_fp64_tolerance(ctx, mg_prec, mg_store):
+    """FP64 FCG with an FP32 (or BF16-stored) multigrid preconditioner
+    converges to the same 1e-8 relative residual; its solution is compared
+    with the FP64 oracle solve."""
+    d = load("c1_n64_cg_mg_bj")
+    A = csr(d, "A")
+    x, hist, st = run_gpu(ctx, d, 2, levels=gmg_levels(d), solver="fcg", prec="mg", smoothers=("bj23", "bj23"),
+                          mg_prec=mg_prec, mg_store=mg_store)
+    assert st["converged"] == 1 and hist[-1] < 1e-8
+    xo, _, oo = O.solve(A, d["b"], "fcg", "mg", nf=2, smoothers=("bj23", "bj23"), levels=gmg_levels(d))
+    err = np.linalg.norm(x - xo) / np.linalg.norm(xo)
+    print("mixed precision", mg_prec, mg_store, "iterations", st["iterations"], "vs FP64", oo.iterations,
+          "rel. error vs FP64 oracle", err)
+    assert err < 1e-7
+    assert st["iterations"] <= oo.iterations + 3
