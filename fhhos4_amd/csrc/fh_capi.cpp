@@ -6,6 +6,7 @@
 #include "fh_kernels.h"
 #include "fh_matrix.h"
 #include "fh_solver.h"
+#include "fh_hho.h"
 
 struct fh_ctx {
     fh::Ctx c;
@@ -361,6 +362,100 @@ int fh_solver_history(const fh_solver* s, double* out, int cap) {
     });
     return rc ? rc : n;
 }
+struct fh_hho {
+    std::unique_ptr<fh::HhoProblem> p;
+};
+
+int fh_hho_create(const fh_hho_args* a, fh_hho** out) {
+    return guard([&] {
+        NEED(a); NEED(out);
+        fh::HhoArgs h;
+        h.dim = a->dim;
+        if (a->mesh) h.mesh = a->mesh;
+        h.n = a->n;
+        h.k = a->k;
+        h.perturb = a->perturb;
+        h.seed = a->seed;
+        if (a->source) h.source = a->source;
+        if (a->kappa) h.kappa = a->kappa;
+        h.sigma = a->sigma;
+        if (a->subdomain > 0) h.subdomain = a->subdomain;
+        h.aniso = a->aniso > 0 ? a->aniso : 1e-2;
+        FH_CHECK(h.dim == 2 || h.dim == 3, -2, "dim must be 2 or 3");
+        auto o = new fh_hho;
+        try {
+            o->p = std::make_unique<fh::HhoProblem>(h);
+        } catch (...) {
+            delete o;
+            throw;
+        }
+        *out = o;
+    });
+}
+int fh_hho_destroy(fh_hho* h) {
+    return guard([&] { delete h; });
+}
+int fh_hho_get_info(const fh_hho* h, fh_hho_info* info) {
+    return guard([&] {
+        NEED(h); NEED(info);
+        const auto& p = *h->p;
+        info->nb = p.nb;
+        info->nf = p.nf;
+        info->ncells = p.ncells();
+        info->nfaces = p.nfaces();
+        info->nnzb = p.rowptr.empty() ? 0 : p.rowptr.back();
+        info->mesh_s = p.times.mesh_s;
+        info->local_s = p.times.local_s;
+        info->assemble_s = p.times.assemble_s;
+    });
+}
+int fh_hho_matrix(fh_ctx* ctx, const fh_hho* h, int keep_host, fh_mat** out) {
+    return guard([&] {
+        NEED(ctx); NEED(h); NEED(out);
+        const auto& p = *h->p;
+        auto m = new fh_mat;
+        m->A = fh::bcsr_from_blocks(ctx->c, p.nb, p.nb, p.nf, p.nf, p.rowptr.data(), p.colidx.data(), p.vals.data(),
+                                    keep_host != 0);
+        *out = m;
+    });
+}
+int fh_hho_rhs(const fh_hho* h, double* b) {
+    return guard([&] {
+        NEED(h); NEED(b);
+        std::memcpy(b, h->p->b.data(), sizeof(double) * h->p->b.size());
+    });
+}
+int fh_hho_blocks(const fh_hho* h, int* rowptr, int* colidx, double* vals) {
+    return guard([&] {
+        NEED(h);
+        const auto& p = *h->p;
+        if (rowptr) std::memcpy(rowptr, p.rowptr.data(), sizeof(int) * p.rowptr.size());
+        if (colidx) std::memcpy(colidx, p.colidx.data(), sizeof(int) * p.colidx.size());
+        if (vals) std::memcpy(vals, p.vals.data(), sizeof(double) * p.vals.size());
+    });
+}
+int fh_hho_l2_error(const fh_hho* h, const double* x, double* err) {
+    return guard([&] {
+        NEED(h); NEED(x); NEED(err);
+        *err = h->p->L2Error(x);
+    });
+}
+int fh_hho_face_cells(const fh_hho* h, int* out) {
+    return guard([&] {
+        NEED(h); NEED(out);
+        std::vector<int> v;
+        h->p->FaceCells(v);
+        std::memcpy(out, v.data(), sizeof(int) * v.size());
+    });
+}
+int fh_hho_cell_kappa(const fh_hho* h, double* out) {
+    return guard([&] {
+        NEED(h); NEED(out);
+        auto k = h->p->CellKappa();
+        std::memcpy(out, k.data(), sizeof(double) * k.size());
+    });
+}
+
 int fh_solver_level_size(const fh_solver* s, int level) {
     int n = 0;
     int rc = guard([&] {

@@ -1,0 +1,883 @@
+// HHO discretisation on structured affine meshes + static condensation.
+#include "fh_hho.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <random>
+#include <unordered_map>
+
+#include "fh_common.h"
+
+namespace fh {
+namespace {
+
+using V3 = std::array<double, 3>;
+using I3 = std::array<int, 3>;
+
+// ------------------------------------------------------------- quadrature
+// Gauss-Legendre nodes/weights on [0,1]: Newton iteration on the Legendre
+// three-term recurrence (Golub & Welsch / textbook method).
+void gauss_legendre01(int n, std::vector<double>& x, std::vector<double>& w) {
+    x.assign(n, 0.0);
+    w.assign(n, 0.0);
+    for (int i = 0; i < n; i++) {
+        double z = std::cos(M_PI * (i + 0.75) / (n + 0.5)), pp = 1;
+        for (int it = 0; it < 100; it++) {
+            double p1 = 1, p2 = 0;
+            for (int j = 1; j <= n; j++) {
+                double p3 = p2;
+                p2 = p1;
+                p1 = ((2.0 * j - 1) * z * p2 - (j - 1.0) * p3) / j;
+            }
+            pp = n * (z * p1 - p2) / (z * z - 1);
+            double z1 = z;
+            z = z1 - p1 / pp;
+            if (std::fabs(z - z1) < 1e-16) break;
+        }
+        x[i] = 0.5 * (1 - z);
+        w[i] = 1.0 / ((1 - z * z) * pp * pp);
+    }
+}
+
+struct Quad {
+    std::vector<V3> p;
+    std::vector<double> w;
+};
+
+// Reference shapes: simplex (kind 0) with vertices 0, e_1..e_d, box (kind 1) [0,1]^d.
+// Simplex rules: collapsed (Duffy / Stroud conical product) Gauss-Legendre.
+Quad ref_quad(int dim, int kind, int n) {
+    std::vector<double> x, w;
+    gauss_legendre01(n, x, w);
+    Quad q;
+    if (dim == 1) {
+        for (int i = 0; i < n; i++) { q.p.push_back({x[i], 0, 0}); q.w.push_back(w[i]); }
+    } else if (dim == 2) {
+        for (int i = 0; i < n; i++)
+            for (int j = 0; j < n; j++) {
+                if (kind == 1) { q.p.push_back({x[i], x[j], 0}); q.w.push_back(w[i] * w[j]); }
+                else { q.p.push_back({x[i], x[j] * (1 - x[i]), 0}); q.w.push_back(w[i] * w[j] * (1 - x[i])); }
+            }
+    } else {
+        for (int i = 0; i < n; i++)
+            for (int j = 0; j < n; j++)
+                for (int l = 0; l < n; l++) {
+                    if (kind == 1) { q.p.push_back({x[i], x[j], x[l]}); q.w.push_back(w[i] * w[j] * w[l]); }
+                    else {
+                        double u = x[i], v = x[j], t = x[l];
+                        q.p.push_back({u, v * (1 - u), t * (1 - u) * (1 - v)});
+                        q.w.push_back(w[i] * w[j] * w[l] * (1 - u) * (1 - u) * (1 - v));
+                    }
+                }
+    }
+    return q;
+}
+
+// ------------------------------------------------------ orthonormal bases
+// Monomials of total degree <= p in graded order, orthonormalised in L2 of
+// the reference shape (coefficients C = L^-1, M = L L^T).
+struct RefBasis {
+    int dim = 0, p = 0, n = 0;
+    std::vector<I3> e;
+    std::vector<double> C;  // n x n, lower triangular
+    void monos(const V3& x, double* m, double (*g)[3]) const {
+        double pw[3][12], dpw[3][12];
+        for (int a = 0; a < 3; a++) {
+            pw[a][0] = 1; dpw[a][0] = 0;
+            for (int q = 1; q <= p; q++) { pw[a][q] = pw[a][q - 1] * x[a]; dpw[a][q] = q * pw[a][q - 1]; }
+        }
+        for (int i = 0; i < n; i++) {
+            const I3& E = e[i];
+            m[i] = pw[0][E[0]] * pw[1][E[1]] * pw[2][E[2]];
+            if (g) {
+                g[i][0] = dpw[0][E[0]] * pw[1][E[1]] * pw[2][E[2]];
+                g[i][1] = pw[0][E[0]] * dpw[1][E[1]] * pw[2][E[2]];
+                g[i][2] = pw[0][E[0]] * pw[1][E[1]] * dpw[2][E[2]];
+            }
+        }
+    }
+    void eval(const V3& x, double* v, double (*g)[3]) const {
+        double m[64], gm[64][3];
+        monos(x, m, g ? gm : nullptr);
+        for (int i = 0; i < n; i++) {
+            double s = 0, g0 = 0, g1 = 0, g2 = 0;
+            for (int j = 0; j <= i; j++) {
+                double c = C[i * n + j];
+                s += c * m[j];
+                if (g) { g0 += c * gm[j][0]; g1 += c * gm[j][1]; g2 += c * gm[j][2]; }
+            }
+            v[i] = s;
+            if (g) { g[i][0] = g0; g[i][1] = g1; g[i][2] = g2; }
+        }
+    }
+};
+
+RefBasis make_basis(int dim, int p, const Quad& q) {
+    RefBasis B;
+    B.dim = dim; B.p = p;
+    for (int d = 0; d <= p; d++) {
+        if (dim == 1) B.e.push_back({d, 0, 0});
+        else if (dim == 2)
+            for (int a = d; a >= 0; a--) B.e.push_back({a, d - a, 0});
+        else
+            for (int a = d; a >= 0; a--)
+                for (int b = d - a; b >= 0; b--) B.e.push_back({a, b, d - a - b});
+    }
+    B.n = (int)B.e.size();
+    const int n = B.n;
+    FH_CHECK(n <= 64, -2, "polynomial degree too high");
+    std::vector<double> M(n * n, 0.0), m(n);
+    for (size_t k = 0; k < q.p.size(); k++) {
+        B.monos(q.p[k], m.data(), nullptr);
+        for (int i = 0; i < n; i++)
+            for (int j = 0; j < n; j++) M[i * n + j] += q.w[k] * m[i] * m[j];
+    }
+    // Cholesky M = L L^T
+    std::vector<double> L(n * n, 0.0);
+    for (int j = 0; j < n; j++) {
+        double d = M[j * n + j];
+        for (int k = 0; k < j; k++) d -= L[j * n + k] * L[j * n + k];
+        FH_CHECK(d > 0, -2, "reference mass matrix not SPD");
+        L[j * n + j] = std::sqrt(d);
+        for (int i = j + 1; i < n; i++) {
+            double s = M[i * n + j];
+            for (int k = 0; k < j; k++) s -= L[i * n + k] * L[j * n + k];
+            L[i * n + j] = s / L[j * n + j];
+        }
+    }
+    // C = L^-1 (lower triangular)
+    B.C.assign(n * n, 0.0);
+    for (int c = 0; c < n; c++) {
+        for (int i = c; i < n; i++) {
+            double s = i == c ? 1.0 : 0.0;
+            for (int k = c; k < i; k++) s -= L[i * n + k] * B.C[k * n + c];
+            B.C[i * n + c] = s / L[i * n + i];
+        }
+    }
+    return B;
+}
+
+// ------------------------------------------------------------ small dense
+// In-place Cholesky of an SPD n x n (row-major, lower part used).
+bool chol(int n, double* A) {
+    for (int j = 0; j < n; j++) {
+        double d = A[j * n + j];
+        for (int k = 0; k < j; k++) d -= A[j * n + k] * A[j * n + k];
+        if (!(d > 0)) return false;
+        d = std::sqrt(d);
+        A[j * n + j] = d;
+        for (int i = j + 1; i < n; i++) {
+            double s = A[i * n + j];
+            for (int k = 0; k < j; k++) s -= A[i * n + k] * A[j * n + k];
+            A[i * n + j] = s / d;
+        }
+    }
+    return true;
+}
+// Solve L L^T X = B for X (n x m, row-major, ldb = m) in place.
+void chol_solve(int n, const double* L, double* B, int m) {
+    for (int c = 0; c < m; c++) {
+        for (int i = 0; i < n; i++) {
+            double s = B[i * m + c];
+            for (int k = 0; k < i; k++) s -= L[i * n + k] * B[k * m + c];
+            B[i * m + c] = s / L[i * n + i];
+        }
+        for (int i = n - 1; i >= 0; i--) {
+            double s = B[i * m + c];
+            for (int k = i + 1; k < n; k++) s -= L[k * n + i] * B[k * m + c];
+            B[i * m + c] = s / L[i * n + i];
+        }
+    }
+}
+
+inline V3 sub(const V3& a, const V3& b) { return {a[0] - b[0], a[1] - b[1], a[2] - b[2]}; }
+inline double dot3(const V3& a, const V3& b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+inline double nrm(const V3& a) { return std::sqrt(dot3(a, a)); }
+inline V3 cross(const V3& a, const V3& b) {
+    return {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------- Impl
+struct HhoProblem::Impl {
+    HhoArgs a;
+    int dim = 3, kind = 0;   // cell kind: 0 simplex, 1 box
+    int nvc = 0, nfc = 0, nvf = 0;
+    std::vector<V3> V;
+    std::vector<int> cv;     // cell vertices (nvc per cell), local order of the mesher
+    std::vector<int> cf;     // cell faces (nfc per cell)
+    std::vector<int> fv;     // face vertices (nvf per face; parametrisation uses 0, 1 and 2 for squares: a, b, d)
+    std::vector<int> fcells; // 2 per face
+    std::vector<int> funk;   // block row of an interior face, -1 for a Dirichlet face
+    std::vector<double> K;   // 9 per cell
+    int ncell = 0, nface = 0;
+    // reference machinery
+    int nc = 0, nr = 0, nfb = 0;
+    RefBasis rb, fb;         // reconstruction (cell) and face bases
+    Quad cq, fq;             // cell and face rules
+    std::vector<double> Shat;  // 9 x nr x nr : int d_a psi_i d_b psi_j
+    std::vector<double> cellPsi;  // nq x nr: psi at cell quadrature points
+    struct FaceTable {
+        std::vector<double> Phi;   // 3 x nr x nfb
+        std::vector<double> Psi;   // 3 x nr x nc
+        std::vector<double> MFT;   // nfb x nr
+    };
+    std::map<std::array<int, 9>, int> table_index;
+    std::vector<FaceTable> tables;
+    std::vector<int> cell_face_table;  // nfc per cell
+
+    V3 ref_vertex(int local) const {
+        if (kind == 0) {
+            V3 r{0, 0, 0};
+            if (local > 0) r[local - 1] = 1;
+            return r;
+        }
+        return {double(local & 1), double((local >> 1) & 1), double((local >> 2) & 1)};
+    }
+
+    double source(const V3& x) const {
+        double w = a.source == "sine" ? 4 * M_PI : M_PI;
+        double s = 1;
+        for (int d = 0; d < dim; d++) s *= std::sin(w * x[d]);
+        return dim * w * w * s;
+    }
+    double exact(const V3& x) const {
+        double w = a.source == "sine" ? 4 * M_PI : M_PI;
+        double s = 1;
+        for (int d = 0; d < dim; d++) s *= std::sin(w * x[d]);
+        return s;
+    }
+
+    void build_mesh();
+    void build_tables();
+    void geometry(int c, V3& v0, double J[3][3], double Ji[3][3], double& det) const;
+    // Local condensation; outputs (optional): condensed matrix (m nfb)^2 and rhs;
+    // or, for post-processing, the reconstruction coefficients from face values.
+    void local(int c, double* Acond, double* bcond, const double* xF, double* recon) const;
+};
+
+// ------------------------------------------------------------------- mesh
+void HhoProblem::Impl::build_mesh() {
+    const int n = a.n;
+    dim = a.dim;
+    const double h = 1.0 / n;
+    FH_CHECK(n >= 1 && n <= 1000, -2, "mesh size n out of range");
+    const int nv1 = n + 1;
+    // vertices (x fastest, then y, then z), perturbed inside the domain
+    int nverts = dim == 3 ? nv1 * nv1 * nv1 : nv1 * nv1;
+    V.resize(nverts);
+    std::mt19937_64 rng(a.seed);
+    std::uniform_real_distribution<double> U(-1.0, 1.0);
+    for (int iz = 0; iz < (dim == 3 ? nv1 : 1); iz++)
+        for (int iy = 0; iy < nv1; iy++)
+            for (int ix = 0; ix < nv1; ix++) {
+                int id = (iz * nv1 + iy) * nv1 + ix;
+                V3 p{ix * h, iy * h, dim == 3 ? iz * h : 0.0};
+                bool interior = ix > 0 && ix < n && iy > 0 && iy < n && (dim == 2 || (iz > 0 && iz < n));
+                if (a.perturb > 0 && interior)
+                    for (int d = 0; d < dim; d++) p[d] += a.perturb * h * U(rng);
+                V[id] = p;
+            }
+    auto vid = [&](int ix, int iy, int iz) { return (iz * nv1 + iy) * nv1 + ix; };
+
+    std::unordered_map<uint64_t, int> fmap;
+    std::vector<int> interior_list;
+    fcells.clear();
+    fv.clear();
+    auto face_key = [](std::vector<int> v) {
+        std::sort(v.begin(), v.end());
+        uint64_t k = 0;
+        for (int x : v) k = k * 2097152ull + (uint64_t)x;
+        return k;
+    };
+    // Creates the face if it does not exist yet (then a boundary face),
+    // otherwise marks it interior; returns the face id.
+    auto touch = [&](int cell, const std::vector<int>& vs, uint64_t key) {
+        auto it = fmap.find(key);
+        if (it == fmap.end()) {
+            int id = (int)(fcells.size() / 2);
+            fmap.emplace(key, id);
+            for (int x : vs) fv.push_back(x);
+            fcells.push_back(cell);
+            fcells.push_back(-1);
+            return id;
+        }
+        int id = it->second;
+        fcells[2 * id + 1] = cell;
+        interior_list.push_back(id);
+        return id;
+    };
+
+    if (a.mesh == "stetra") {
+        FH_CHECK(dim == 3, -2, "stetra meshes are 3-D");
+        kind = 0; nvc = 4; nfc = 4; nvf = 3;
+        ncell = 6 * n * n * n;
+        cv.resize((size_t)ncell * 4);
+        cf.assign((size_t)ncell * 4, -1);
+        std::vector<int> nadded(ncell, 0);
+        // Cube split and face creation order of Cube_CartesianTetrahedralMesh::Build
+        // (src/Mesher/InHouse/Cube_CartesianTetrahedralMesh.h:44-99).
+        static const int T[6][4] = {{2, 3, 7, 1}, {7, 2, 1, 0}, {0, 1, 7, 5}, {7, 2, 0, 6}, {5, 7, 0, 6}, {5, 0, 4, 6}};
+        // corner code = x + 2y + 4z: v000=0 v100=1 v010=2 v110=3 v001=4 v101=5 v011=6 v111=7
+        struct FC { int t0, t1, a, b, c; };
+        static const FC FCALLS[18] = {
+            {0, -1, 2, 7, 3}, {0, -1, 2, 3, 1}, {0, 1, 2, 1, 7}, {0, -1, 1, 3, 7}, {1, 3, 7, 2, 0}, {1, 2, 7, 0, 1},
+            {1, -1, 0, 2, 1}, {2, -1, 0, 1, 5}, {2, 4, 0, 5, 7}, {2, -1, 5, 1, 7}, {3, -1, 7, 2, 6}, {3, 4, 7, 6, 0},
+            {3, -1, 6, 2, 0}, {4, -1, 5, 7, 6}, {4, 5, 5, 6, 0}, {5, -1, 5, 4, 0}, {5, -1, 5, 6, 4}, {5, -1, 6, 0, 4}};
+        int cube = 0;
+        for (int iz = 0; iz < n; iz++)
+            for (int iy = 0; iy < n; iy++)
+                for (int ix = 0; ix < n; ix++, cube++) {
+                    int corner[8];
+                    for (int cc = 0; cc < 8; cc++) corner[cc] = vid(ix + (cc & 1), iy + ((cc >> 1) & 1), iz + ((cc >> 2) & 1));
+                    for (int t = 0; t < 6; t++)
+                        for (int q = 0; q < 4; q++) cv[(size_t)(6 * cube + t) * 4 + q] = corner[T[t][q]];
+                    for (const FC& f : FCALLS) {
+                        std::vector<int> vs{corner[f.a], corner[f.b], corner[f.c]};
+                        int c0 = 6 * cube + f.t0;
+                        int id;
+                        if (f.t1 < 0) id = touch(c0, vs, face_key(vs));
+                        else {
+                            id = (int)(fcells.size() / 2);
+                            for (int x : vs) fv.push_back(x);
+                            fcells.push_back(c0);
+                            fcells.push_back(6 * cube + f.t1);
+                            interior_list.push_back(id);
+                            fmap.emplace(face_key(vs), id);
+                        }
+                        int cs[2] = {c0, f.t1 < 0 ? -1 : 6 * cube + f.t1};
+                        if (f.t1 < 0 && fcells[2 * id] != c0) cs[0] = c0;  // second touch: this cell
+                        for (int s = 0; s < 2; s++) {
+                            int cell = cs[s];
+                            if (cell < 0) continue;
+                            FH_CHECK(nadded[cell] < 4, -2, "tetrahedron with more than 4 faces");
+                            cf[(size_t)cell * 4 + nadded[cell]++] = id;
+                        }
+                    }
+                }
+    } else if (a.mesh == "cart") {
+        FH_CHECK(a.perturb == 0, -2, "perturbed Cartesian cells are not affine; use stetra");
+        kind = 1;
+        nvc = dim == 3 ? 8 : 4;
+        nfc = 2 * dim;
+        nvf = dim == 3 ? 4 : 2;
+        ncell = dim == 3 ? n * n * n : n * n;
+        cv.resize((size_t)ncell * nvc);
+        cf.assign((size_t)ncell * nfc, -1);
+        int cell = 0;
+        for (int iz = 0; iz < (dim == 3 ? n : 1); iz++)
+            for (int iy = 0; iy < n; iy++)
+                for (int ix = 0; ix < n; ix++, cell++) {
+                    for (int cc = 0; cc < nvc; cc++)
+                        cv[(size_t)cell * nvc + cc] = vid(ix + (cc & 1), iy + ((cc >> 1) & 1), dim == 3 ? iz + ((cc >> 2) & 1) : 0);
+                    // faces: -x, +x, -y, +y (, -z, +z); vertex order a, b, (c), d with
+                    // parametrisation a + s (b - a) + t (d - a)
+                    for (int f = 0; f < nfc; f++) {
+                        int axis = f / 2, side = f % 2;
+                        std::vector<int> vs;
+                        if (dim == 2) {
+                            int o[2] = {0, 0};
+                            o[axis] = side;
+                            int t1 = 1 - axis;
+                            int p0[2] = {o[0], o[1]}, p1[2] = {o[0], o[1]};
+                            p1[t1] = 1;
+                            vs = {vid(ix + p0[0], iy + p0[1], 0), vid(ix + p1[0], iy + p1[1], 0)};
+                        } else {
+                            int t1 = (axis + 1) % 3, t2 = (axis + 2) % 3;
+                            int lo = std::min(t1, t2), hi = std::max(t1, t2);
+                            int pa[3] = {0, 0, 0};
+                            pa[axis] = side;
+                            int pb[3] = {pa[0], pa[1], pa[2]}, pd[3] = {pa[0], pa[1], pa[2]}, pc[3] = {pa[0], pa[1], pa[2]};
+                            pb[lo] = 1; pd[hi] = 1; pc[lo] = 1; pc[hi] = 1;
+                            vs = {vid(ix + pa[0], iy + pa[1], iz + pa[2]), vid(ix + pb[0], iy + pb[1], iz + pb[2]),
+                                  vid(ix + pc[0], iy + pc[1], iz + pc[2]), vid(ix + pd[0], iy + pd[1], iz + pd[2])};
+                        }
+                        uint64_t key = (uint64_t)(*std::min_element(vs.begin(), vs.end())) * 3 + axis;
+                        cf[(size_t)cell * nfc + f] = touch(cell, vs, key);
+                    }
+                }
+        // Interior faces in the order of Square_CartesianMesh / Cube_CartesianMesh::Build
+        // (src/Mesher/InHouse/Square_CartesianMesh.h:173-200, Cube_CartesianMesh.h:170-205):
+        // cell by cell, its +x, +y (, +z) interface.
+        interior_list.clear();
+        for (int c = 0; c < ncell; c++)
+            for (int f = 1; f < nfc; f += 2) {
+                int id = cf[(size_t)c * nfc + f];
+                if (fcells[2 * id + 1] >= 0) interior_list.push_back(id);
+            }
+    } else
+        throw Error(-2, "unknown mesh '" + a.mesh + "' (stetra | cart)");
+    nface = (int)(fcells.size() / 2);
+    funk.assign(nface, -1);
+    for (size_t i = 0; i < interior_list.size(); i++) funk[interior_list[i]] = (int)i;
+
+    // diffusion tensor per cell
+    K.assign((size_t)ncell * 9, 0.0);
+    std::mt19937_64 krng(a.seed ^ 0x9E3779B97F4A7C15ull);
+    std::normal_distribution<double> N01(0.0, 1.0);
+    std::vector<double> sub_kappa;
+    int nsub = 0;
+    if (a.kappa == "subdomain") {
+        int ns = std::max(1, n / std::max(1, a.subdomain));
+        nsub = ns;
+        sub_kappa.resize((size_t)ns * ns * (dim == 3 ? ns : 1));
+        for (auto& s : sub_kappa) s = std::exp(a.sigma * N01(krng));
+    }
+    for (int c = 0; c < ncell; c++) {
+        double kap = 1.0, kz = 1.0;
+        if (a.kappa == "lognormal") kap = std::exp(a.sigma * N01(krng));
+        else if (a.kappa == "subdomain") {
+            // background cell of the cell (tets: 6 per cube)
+            int bc = kind == 0 ? c / 6 : c;
+            int ix = bc % n, iy = (bc / n) % n, iz = dim == 3 ? bc / (n * n) : 0;
+            int sd = std::max(1, a.subdomain);
+            int sx = std::min(ix / sd, nsub - 1), sy = std::min(iy / sd, nsub - 1), sz = std::min(iz / sd, nsub - 1);
+            int s = (dim == 3 ? sz * nsub * nsub : 0) + sy * nsub + sx;
+            kap = sub_kappa[s];
+            if (s % 8 == 7) kz = a.aniso;  // one subdomain in eight is anisotropic
+        } else if (a.kappa != "one")
+            throw Error(-2, "unknown kappa field '" + a.kappa + "'");
+        K[(size_t)c * 9 + 0] = kap;
+        K[(size_t)c * 9 + 4] = kap;
+        K[(size_t)c * 9 + 8] = dim == 3 ? kap * kz : 1.0;
+        if (dim == 2) K[(size_t)c * 9 + 8] = 1.0;
+    }
+}
+
+void HhoProblem::Impl::geometry(int c, V3& v0, double J[3][3], double Ji[3][3], double& det) const {
+    const int* cvv = &cv[(size_t)c * nvc];
+    v0 = V[cvv[0]];
+    int idx[3];
+    if (kind == 0) { idx[0] = 1; idx[1] = 2; idx[2] = 3; }
+    else { idx[0] = 1; idx[1] = 2; idx[2] = 4; }
+    for (int r = 0; r < 3; r++)
+        for (int q = 0; q < 3; q++) J[r][q] = 0;
+    for (int q = 0; q < dim; q++) {
+        V3 e = sub(V[cvv[idx[q]]], v0);
+        for (int r = 0; r < 3; r++) J[r][q] = e[r];
+    }
+    if (dim == 2) J[2][2] = 1;
+    det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) - J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+          J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+    double id = 1.0 / det;
+    Ji[0][0] = (J[1][1] * J[2][2] - J[1][2] * J[2][1]) * id;
+    Ji[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id;
+    Ji[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id;
+    Ji[1][0] = (J[1][2] * J[2][0] - J[1][0] * J[2][2]) * id;
+    Ji[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id;
+    Ji[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id;
+    Ji[2][0] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) * id;
+    Ji[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id;
+    Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
+}
+
+void HhoProblem::Impl::build_tables() {
+    const int k = a.k;
+    FH_CHECK(k >= 0 && k <= 3, -2, "face degree k must be 0..3");
+    cq = ref_quad(dim, kind, k + 4);
+    fq = ref_quad(dim - 1, kind, k + 3);
+    rb = make_basis(dim, k + 1, cq);
+    fb = make_basis(dim - 1, k, fq);
+    nr = rb.n;
+    nfb = fb.n;
+    nc = 0;
+    for (auto& e : rb.e) if (e[0] + e[1] + e[2] <= k) nc++;
+    // stiffness tensors on the reference shape
+    Shat.assign((size_t)9 * nr * nr, 0.0);
+    cellPsi.assign(cq.p.size() * nr, 0.0);
+    std::vector<double> v(nr);
+    std::vector<double> g(nr * 3);
+    for (size_t q = 0; q < cq.p.size(); q++) {
+        rb.eval(cq.p[q], v.data(), (double(*)[3])g.data());
+        for (int i = 0; i < nr; i++) cellPsi[q * nr + i] = v[i];
+        for (int ax = 0; ax < 3; ax++)
+            for (int bx = 0; bx < 3; bx++)
+                for (int i = 0; i < nr; i++)
+                    for (int j = 0; j < nr; j++)
+                        Shat[((size_t)(ax * 3 + bx) * nr + i) * nr + j] += cq.w[q] * g[i * 3 + ax] * g[j * 3 + bx];
+    }
+    // face tables keyed by the reference position of the face parametrisation
+    cell_face_table.assign((size_t)ncell * nfc, -1);
+    for (int c = 0; c < ncell; c++) {
+        const int* cvv = &cv[(size_t)c * nvc];
+        for (int f = 0; f < nfc; f++) {
+            int fid = cf[(size_t)c * nfc + f];
+            const int* fvv = &fv[(size_t)fid * nvf];
+            // a, b, and the second axis vertex (d for squares, c for triangles)
+            int pv[3] = {fvv[0], dim >= 2 ? fvv[1] : -1, dim == 3 ? (kind == 1 ? fvv[3] : fvv[2]) : -1};
+            std::array<int, 9> key{};
+            for (int s = 0; s < dim; s++) {
+                int loc = -1;
+                for (int l = 0; l < nvc; l++) if (cvv[l] == pv[s]) loc = l;
+                FH_CHECK(loc >= 0, -2, "face vertex not in cell");
+                V3 r = ref_vertex(loc);
+                for (int d = 0; d < 3; d++) key[s * 3 + d] = (int)r[d];
+            }
+            auto it = table_index.find(key);
+            int ti;
+            if (it == table_index.end()) {
+                ti = (int)tables.size();
+                table_index.emplace(key, ti);
+                FaceTable T;
+                T.Phi.assign((size_t)3 * nr * nfb, 0.0);
+                T.Psi.assign((size_t)3 * nr * nc, 0.0);
+                T.MFT.assign((size_t)nfb * nr, 0.0);
+                V3 ra{double(key[0]), double(key[1]), double(key[2])};
+                V3 e1{0, 0, 0}, e2{0, 0, 0};
+                if (dim >= 2) e1 = sub(V3{double(key[3]), double(key[4]), double(key[5])}, ra);
+                if (dim == 3) e2 = sub(V3{double(key[6]), double(key[7]), double(key[8])}, ra);
+                std::vector<double> chi(nfb);
+                for (size_t q = 0; q < fq.p.size(); q++) {
+                    const V3& eta = fq.p[q];
+                    V3 xi{ra[0] + eta[0] * e1[0] + eta[1] * e2[0], ra[1] + eta[0] * e1[1] + eta[1] * e2[1],
+                          ra[2] + eta[0] * e1[2] + eta[1] * e2[2]};
+                    rb.eval(xi, v.data(), (double(*)[3])g.data());
+                    fb.eval(eta, chi.data(), nullptr);
+                    double w = fq.w[q];
+                    for (int ax = 0; ax < 3; ax++)
+                        for (int i = 0; i < nr; i++) {
+                            double gi = w * g[i * 3 + ax];
+                            for (int j = 0; j < nfb; j++) T.Phi[((size_t)ax * nr + i) * nfb + j] += gi * chi[j];
+                            for (int j = 0; j < nc; j++) T.Psi[((size_t)ax * nr + i) * nc + j] += gi * v[j];
+                        }
+                    for (int j = 0; j < nfb; j++)
+                        for (int i = 0; i < nr; i++) T.MFT[(size_t)j * nr + i] += w * chi[j] * v[i];
+                }
+                tables.push_back(std::move(T));
+            } else
+                ti = it->second;
+            cell_face_table[(size_t)c * nfc + f] = ti;
+        }
+    }
+}
+
+// Local HHO operators of cell c (Diff_HHOElement::InitHHO,
+// Diff_HHOElement.h:104-150, 312-509) and their static condensation.
+void HhoProblem::Impl::local(int c, double* Acond, double* bcond, const double* xF, double* recon) const {
+    const int m = nfc, nh = nc + m * nfb;
+    V3 v0;
+    double J[3][3], Ji[3][3], det;
+    geometry(c, v0, J, Ji, det);
+    const double adet = std::fabs(det);
+    const double* Kc = &K[(size_t)c * 9];
+    // G = J^-1 K J^-T
+    double G[3][3];
+    for (int r = 0; r < 3; r++)
+        for (int q = 0; q < 3; q++) {
+            double s = 0;
+            for (int u = 0; u < 3; u++)
+                for (int w = 0; w < 3; w++) s += Ji[r][u] * Kc[u * 3 + w] * Ji[q][w];
+            G[r][q] = s;
+        }
+    thread_local std::vector<double> S, B, P, A, Dt, ProjF, DF, tmp, W;
+    S.assign(nr * nr, 0.0);
+    for (int ax = 0; ax < dim; ax++)
+        for (int bx = 0; bx < dim; bx++) {
+            const double gab = G[ax][bx];
+            const double* Sh = &Shat[(size_t)(ax * 3 + bx) * nr * nr];
+            for (int e = 0; e < nr * nr; e++) S[e] += gab * Sh[e];
+        }
+    B.assign((size_t)nr * nh, 0.0);
+    for (int i = 0; i < nr; i++)
+        for (int j = 0; j < nc; j++) B[i * nh + j] = S[i * nr + j];
+    ProjF.assign((size_t)m * nfb * nr, 0.0);
+    std::vector<double> coef(m);
+    // centroid of the cell
+    V3 cc{0, 0, 0};
+    for (int l = 0; l < nvc; l++)
+        for (int d = 0; d < 3; d++) cc[d] += V[cv[(size_t)c * nvc + l]][d] / nvc;
+    for (int f = 0; f < m; f++) {
+        int fid = cf[(size_t)c * nfc + f];
+        const int* fvv = &fv[(size_t)fid * nvf];
+        V3 a0 = V[fvv[0]];
+        V3 E1 = sub(V[fvv[1]], a0);
+        V3 E2 = dim == 3 ? sub(V[kind == 1 ? fvv[3] : fvv[2]], a0) : V3{0, 0, 0};
+        double sF, hF;
+        V3 nF;
+        if (dim == 3) {
+            V3 cr = cross(E1, E2);
+            sF = nrm(cr);
+            nF = {cr[0] / sF, cr[1] / sF, cr[2] / sF};
+            if (kind == 0) hF = std::max({nrm(E1), nrm(E2), nrm(sub(V[fvv[2]], V[fvv[1]]))});
+            else hF = std::max(nrm(E1), nrm(E2));
+        } else {
+            sF = nrm(E1);
+            hF = sF;
+            nF = {E1[1] / sF, -E1[0] / sF, 0};
+        }
+        V3 fc = a0;
+        for (int l = 1; l < nvf; l++)
+            for (int d = 0; d < 3; d++) fc[d] += V[fvv[l]][d];
+        for (int d = 0; d < 3; d++) fc[d] /= nvf;
+        if (dot3(nF, sub(fc, cc)) < 0) nF = {-nF[0], -nF[1], -nF[2]};
+        // t = J^-1 K n
+        V3 Kn{Kc[0] * nF[0] + Kc[1] * nF[1] + Kc[2] * nF[2], Kc[3] * nF[0] + Kc[4] * nF[1] + Kc[5] * nF[2],
+              Kc[6] * nF[0] + Kc[7] * nF[1] + Kc[8] * nF[2]};
+        V3 t{Ji[0][0] * Kn[0] + Ji[0][1] * Kn[1] + Ji[0][2] * Kn[2], Ji[1][0] * Kn[0] + Ji[1][1] * Kn[1] + Ji[1][2] * Kn[2],
+             Ji[2][0] * Kn[0] + Ji[2][1] * Kn[1] + Ji[2][2] * Kn[2]};
+        coef[f] = dot3(Kn, nF) / hF;
+        const FaceTable& T = tables[cell_face_table[(size_t)c * nfc + f]];
+        const double s1 = std::sqrt(sF / adet), s2 = sF / adet;
+        for (int ax = 0; ax < dim; ax++) {
+            const double* Phi = &T.Phi[(size_t)ax * nr * nfb];
+            const double* Psi = &T.Psi[(size_t)ax * nr * nc];
+            const double ta = t[ax];
+            for (int i = 0; i < nr; i++) {
+                for (int j = 0; j < nfb; j++) B[i * nh + nc + f * nfb + j] += s1 * ta * Phi[i * nfb + j];
+                for (int j = 0; j < nc; j++) B[i * nh + j] -= s2 * ta * Psi[i * nc + j];
+            }
+        }
+        for (int e = 0; e < nfb * nr; e++) ProjF[(size_t)f * nfb * nr + e] = s1 * T.MFT[e];
+    }
+    // Reconstruction: row 0 from the mean-value condition, rows 1.. from S'
+    P.assign((size_t)nr * nh, 0.0);
+    P[0] = 1.0;
+    {
+        const int n1 = nr - 1;
+        tmp.assign((size_t)n1 * n1, 0.0);
+        for (int i = 0; i < n1; i++)
+            for (int j = 0; j < n1; j++) tmp[i * n1 + j] = S[(i + 1) * nr + j + 1];
+        FH_CHECK(n1 == 0 || chol(n1, tmp.data()), -6, "reconstruction stiffness not SPD");
+        for (int i = 0; i < n1; i++)
+            for (int j = 0; j < nh; j++) P[(i + 1) * nh + j] = B[(i + 1) * nh + j];
+        if (n1) chol_solve(n1, tmp.data(), &P[nh], nh);
+    }
+    // A = P^T S P + stabilisation
+    W.assign((size_t)nr * nh, 0.0);  // S P
+    for (int i = 0; i < nr; i++)
+        for (int l = 0; l < nr; l++) {
+            double s = S[i * nr + l];
+            if (s == 0.0) continue;
+            for (int j = 0; j < nh; j++) W[i * nh + j] += s * P[l * nh + j];
+        }
+    A.assign((size_t)nh * nh, 0.0);
+    for (int i = 0; i < nr; i++)
+        for (int r = 0; r < nh; r++) {
+            double p = P[i * nh + r];
+            if (p == 0.0) continue;
+            for (int j = 0; j < nh; j++) A[r * nh + j] += p * W[i * nh + j];
+        }
+    Dt.assign((size_t)nc * nh, 0.0);
+    for (int i = 0; i < nc; i++) {
+        for (int j = 0; j < nh; j++) Dt[i * nh + j] = P[i * nh + j];
+        Dt[i * nh + i] -= 1.0;
+    }
+    DF.assign((size_t)nfb * nh, 0.0);
+    for (int f = 0; f < m; f++) {
+        const double* PF = &ProjF[(size_t)f * nfb * nr];
+        std::fill(DF.begin(), DF.end(), 0.0);
+        for (int j = 0; j < nfb; j++) {
+            for (int i = 0; i < nr; i++) {
+                double p = PF[j * nr + i];
+                for (int r = 0; r < nh; r++) DF[j * nh + r] += p * P[i * nh + r];
+            }
+            DF[j * nh + nc + f * nfb + j] -= 1.0;
+            for (int i = 0; i < nc; i++) {
+                double p = PF[j * nr + i];
+                for (int r = 0; r < nh; r++) DF[j * nh + r] -= p * Dt[i * nh + r];
+            }
+        }
+        const double cf_ = coef[f];
+        for (int j = 0; j < nfb; j++)
+            for (int r = 0; r < nh; r++) {
+                double d = cf_ * DF[j * nh + r];
+                if (d == 0.0) continue;
+                for (int s = 0; s < nh; s++) A[r * nh + s] += d * DF[j * nh + s];
+            }
+    }
+    // right-hand side on the cell: B_T = int f psi_i
+    std::vector<double> BT(nc, 0.0);
+    if (bcond || recon) {
+        const double sq = std::sqrt(adet);
+        for (size_t q = 0; q < cq.p.size(); q++) {
+            const V3& xi = cq.p[q];
+            V3 x{v0[0] + J[0][0] * xi[0] + J[0][1] * xi[1] + J[0][2] * xi[2],
+                 v0[1] + J[1][0] * xi[0] + J[1][1] * xi[1] + J[1][2] * xi[2],
+                 v0[2] + J[2][0] * xi[0] + J[2][1] * xi[1] + J[2][2] * xi[2]};
+            double fw = cq.w[q] * source(x) * sq;
+            for (int i = 0; i < nc; i++) BT[i] += fw * cellPsi[q * nr + i];
+        }
+    }
+    // Static condensation (Diffusion_HHO.h:624-652, HHOCellSpace.h:170-203)
+    const int nF = m * nfb;
+    tmp.assign((size_t)nc * nc, 0.0);
+    for (int i = 0; i < nc; i++)
+        for (int j = 0; j < nc; j++) tmp[i * nc + j] = A[i * nh + j];
+    FH_CHECK(chol(nc, tmp.data()), -6, "A_TT not SPD");
+    if (Acond || bcond) {
+        std::vector<double> X((size_t)nc * (nF + 1));
+        for (int i = 0; i < nc; i++) {
+            for (int j = 0; j < nF; j++) X[i * (nF + 1) + j] = A[i * nh + nc + j];
+            X[i * (nF + 1) + nF] = BT[i];
+        }
+        chol_solve(nc, tmp.data(), X.data(), nF + 1);
+        if (Acond)
+            for (int r = 0; r < nF; r++)
+                for (int s = 0; s < nF; s++) {
+                    double v = A[(nc + r) * nh + nc + s];
+                    for (int i = 0; i < nc; i++) v -= A[i * nh + nc + r] * X[i * (nF + 1) + s];
+                    Acond[r * nF + s] = v;
+                }
+        if (bcond)
+            for (int r = 0; r < nF; r++) {
+                double v = 0;
+                for (int i = 0; i < nc; i++) v -= A[i * nh + nc + r] * X[i * (nF + 1) + nF];
+                bcond[r] = v;
+            }
+    }
+    if (recon) {
+        // cell unknowns v_T = A_TT^-1 (B_T - A_TF v_F), then p = P [v_T; v_F]
+        std::vector<double> vT(nc);
+        for (int i = 0; i < nc; i++) {
+            double s = BT[i];
+            for (int j = 0; j < nF; j++) s -= A[i * nh + nc + j] * xF[j];
+            vT[i] = s;
+        }
+        chol_solve(nc, tmp.data(), vT.data(), 1);
+        for (int i = 0; i < nr; i++) {
+            double s = 0;
+            for (int j = 0; j < nc; j++) s += P[i * nh + j] * vT[j];
+            for (int j = 0; j < nF; j++) s += P[i * nh + nc + j] * xF[j];
+            recon[i] = s;
+        }
+    }
+}
+
+// ----------------------------------------------------------------- public
+HhoProblem::HhoProblem(const HhoArgs& args) : impl(new Impl) {
+    Impl& I = *impl;
+    I.a = args;
+    auto t0 = std::chrono::steady_clock::now();
+    I.build_mesh();
+    I.build_tables();
+    auto t1 = std::chrono::steady_clock::now();
+    times.mesh_s = std::chrono::duration<double>(t1 - t0).count();
+    nf = I.nfb;
+    nb = 0;
+    for (int f = 0; f < I.nface; f++) if (I.funk[f] >= 0) nb++;
+    // block pattern: faces sharing a cell
+    std::vector<std::vector<int>> rows(nb);
+    for (int c = 0; c < I.ncell; c++)
+        for (int f = 0; f < I.nfc; f++) {
+            int r = I.funk[I.cf[(size_t)c * I.nfc + f]];
+            if (r < 0) continue;
+            for (int g = 0; g < I.nfc; g++) {
+                int s = I.funk[I.cf[(size_t)c * I.nfc + g]];
+                if (s >= 0) rows[r].push_back(s);
+            }
+        }
+    rowptr.assign(nb + 1, 0);
+    for (int r = 0; r < nb; r++) {
+        auto& v = rows[r];
+        std::sort(v.begin(), v.end());
+        v.erase(std::unique(v.begin(), v.end()), v.end());
+        rowptr[r + 1] = rowptr[r] + (int)v.size();
+    }
+    colidx.resize(rowptr[nb]);
+    for (int r = 0; r < nb; r++) std::copy(rows[r].begin(), rows[r].end(), colidx.begin() + rowptr[r]);
+    rows.clear();
+    rows.shrink_to_fit();
+    const size_t bsz = (size_t)nf * nf;
+    vals.assign((size_t)rowptr[nb] * bsz, 0.0);
+    b.assign((size_t)nb * nf, 0.0);
+    auto t2 = std::chrono::steady_clock::now();
+    const int nF = I.nfc * nf;
+    // Each off-diagonal block receives one cell contribution and each
+    // diagonal block two, so atomic accumulation is order independent.
+#pragma omp parallel
+    {
+        std::vector<double> Ac((size_t)nF * nF), bc(nF);
+        std::vector<int> rr(I.nfc), pos((size_t)I.nfc * I.nfc);
+#pragma omp for schedule(dynamic, 256)
+        for (int c = 0; c < I.ncell; c++) {
+            I.local(c, Ac.data(), bc.data(), nullptr, nullptr);
+            for (int f = 0; f < I.nfc; f++) rr[f] = I.funk[I.cf[(size_t)c * I.nfc + f]];
+            for (int f = 0; f < I.nfc; f++) {
+                if (rr[f] < 0) continue;
+                for (int g = 0; g < I.nfc; g++) {
+                    if (rr[g] < 0) continue;
+                    const int* beg = &colidx[rowptr[rr[f]]];
+                    const int* end = &colidx[rowptr[rr[f] + 1]];
+                    int p = (int)(std::lower_bound(beg, end, rr[g]) - &colidx[0]);
+                    double* dst = &vals[(size_t)p * bsz];
+                    for (int a = 0; a < nf; a++)
+                        for (int bb = 0; bb < nf; bb++) {
+                            double v = Ac[(size_t)(f * nf + a) * nF + g * nf + bb];
+#pragma omp atomic
+                            dst[a * nf + bb] += v;
+                        }
+                }
+                for (int a = 0; a < nf; a++) {
+#pragma omp atomic
+                    b[(size_t)rr[f] * nf + a] += bc[f * nf + a];
+                }
+            }
+        }
+    }
+    auto t3 = std::chrono::steady_clock::now();
+    times.local_s = std::chrono::duration<double>(t3 - t2).count();
+    times.assemble_s = std::chrono::duration<double>(t3 - t1).count();
+}
+
+HhoProblem::~HhoProblem() { delete impl; }
+
+int HhoProblem::ncells() const { return impl->ncell; }
+int HhoProblem::nfaces() const { return impl->nface; }
+
+std::vector<double> HhoProblem::CellKappa() const {
+    std::vector<double> k(impl->ncell);
+    for (int c = 0; c < impl->ncell; c++) k[c] = impl->K[(size_t)c * 9];
+    return k;
+}
+
+void HhoProblem::FaceCells(std::vector<int>& out) const {
+    const Impl& I = *impl;
+    out.assign((size_t)nb * 2, -1);
+    for (int f = 0; f < I.nface; f++) {
+        int r = I.funk[f];
+        if (r < 0) continue;
+        out[2 * r] = I.fcells[2 * f];
+        out[2 * r + 1] = I.fcells[2 * f + 1];
+    }
+}
+
+double HhoProblem::L2Error(const double* x) const {
+    const Impl& I = *impl;
+    double err = 0, nrm2 = 0;
+#pragma omp parallel reduction(+ : err, nrm2)
+    {
+        std::vector<double> xF((size_t)I.nfc * nf), rec(I.nr), psi(I.nr);
+#pragma omp for schedule(dynamic, 256)
+        for (int c = 0; c < I.ncell; c++) {
+            for (int f = 0; f < I.nfc; f++) {
+                int r = I.funk[I.cf[(size_t)c * I.nfc + f]];
+                for (int a = 0; a < nf; a++) xF[f * nf + a] = r >= 0 ? x[(size_t)r * nf + a] : 0.0;
+            }
+            I.local(c, nullptr, nullptr, xF.data(), rec.data());
+            V3 v0;
+            double J[3][3], Ji[3][3], det;
+            I.geometry(c, v0, J, Ji, det);
+            const double adet = std::fabs(det), isq = 1.0 / std::sqrt(adet);
+            for (size_t q = 0; q < I.cq.p.size(); q++) {
+                const V3& xi = I.cq.p[q];
+                V3 xx{v0[0] + J[0][0] * xi[0] + J[0][1] * xi[1] + J[0][2] * xi[2],
+                      v0[1] + J[1][0] * xi[0] + J[1][1] * xi[1] + J[1][2] * xi[2],
+                      v0[2] + J[2][0] * xi[0] + J[2][1] * xi[1] + J[2][2] * xi[2]};
+                double ph = 0;
+                for (int i = 0; i < I.nr; i++) ph += rec[i] * I.cellPsi[q * I.nr + i];
+                ph *= isq;
+                double u = I.exact(xx);
+                double w = I.cq.w[q] * adet;
+                err += w * (u - ph) * (u - ph);
+                nrm2 += w * u * u;
+            }
+        }
+    }
+    return nrm2 > 0 ? std::sqrt(err / nrm2) : std::sqrt(err);
+}
+
+}  // namespace fh

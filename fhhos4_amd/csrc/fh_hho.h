@@ -1,0 +1,62 @@
+// HHO discretisation of -div(K grad u) = f with homogeneous Dirichlet
+// conditions on structured affine meshes, statically condensed onto the face
+// unknowns: the producer of the face system A x_F = b that the solvers take.
+//
+// It follows the reference's HHO operators (src/Discretizations/HHO/
+// Diff_HHOElement.h:104-509, Diffusion_HHO.h:252-704): reconstruction in
+// P^{k+1}(T) with the mean-value condition, consistency P^T S P, the "hho"
+// stabilisation with weight (K n.n)/h_F, static condensation
+// A = A_FF - A_TF^T A_TT^-1 A_TF, b = -A_TF^T A_TT^-1 B_T, and the
+// relative L2 error of the reconstructed solution (Diffusion_HHO.h:120-149).
+// Bases are orthonormal in L2 of each cell / face (built once on the
+// reference shape; the reference uses Legendre or orthogonalised monomials,
+// which span the same spaces, so the discrete solution is the same).
+#pragma once
+#include <array>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace fh {
+
+struct HhoArgs {
+    int dim = 3;
+    std::string mesh = "stetra";   // "stetra" (cube split into 6 tets) | "cart" (squares / cubes) | "stri" (2-D triangles)
+    int n = 8;                     // cells per direction of the background grid
+    int k = 2;                     // face degree
+    double perturb = 0.0;          // interior vertex perturbation, fraction of h (uniform per coordinate)
+    uint64_t seed = 1;
+    std::string source = "pisine"; // "pisine": f = d pi^2 prod sin(pi x) | "sine": 4 pi periodic
+    std::string kappa = "one";     // "one" | "lognormal" (per cell) | "subdomain" (per block of cells)
+    double sigma = 0.0;            // lognormal standard deviation
+    int subdomain = 8;             // cells per subdomain edge ("subdomain")
+    double aniso = 1e-2;           // z-scaling of the anisotropic subdomains ("subdomain")
+};
+
+struct HhoTimes {
+    double mesh_s = 0, local_s = 0, assemble_s = 0;
+};
+
+class HhoProblem {
+public:
+    explicit HhoProblem(const HhoArgs& a);
+    ~HhoProblem();
+    // condensed system: nb block rows (interior faces) of nf x nf blocks
+    int nb = 0, nf = 0;
+    std::vector<int> rowptr, colidx;
+    std::vector<double> vals, b;
+    int ncells() const;
+    int nfaces() const;
+    // relative L2 error of the k+1 reconstruction of the solution x (face
+    // unknowns, length nb*nf) against the exact solution of the source
+    double L2Error(const double* x) const;
+    // cell diffusion coefficient (isotropic part) of every cell
+    std::vector<double> CellKappa() const;
+    // face adjacency: cells of every interior face (2 per face)
+    void FaceCells(std::vector<int>& out) const;
+    HhoTimes times;
+    struct Impl;
+    Impl* impl = nullptr;
+};
+
+}  // namespace fh

@@ -76,7 +76,27 @@ _SIGS = {
     "fh_solver_solve": [_vp, _vp, _vp, C.POINTER(SolveStats)],
     "fh_solver_history": [_vp, _f64p, C.c_int],
     "fh_solver_level_size": [_vp, C.c_int],
+    "fh_hho_create": [_vp, _pp],
+    "fh_hho_destroy": [_vp],
+    "fh_hho_get_info": [_vp, _vp],
+    "fh_hho_matrix": [_vp, _vp, C.c_int, _pp],
+    "fh_hho_rhs": [_vp, _f64p],
+    "fh_hho_blocks": [_vp, _i32p, _i32p, _f64p],
+    "fh_hho_l2_error": [_vp, _f64p, C.POINTER(C.c_double)],
+    "fh_hho_face_cells": [_vp, _i32p],
+    "fh_hho_cell_kappa": [_vp, _f64p],
 }
+
+
+class HhoArgs(C.Structure):
+    _fields_ = [("dim", C.c_int), ("mesh", C.c_char_p), ("n", C.c_int), ("k", C.c_int), ("perturb", C.c_double),
+                ("seed", C.c_uint64), ("source", C.c_char_p), ("kappa", C.c_char_p), ("sigma", C.c_double),
+                ("subdomain", C.c_int), ("aniso", C.c_double)]
+
+
+class HhoInfo(C.Structure):
+    _fields_ = [("nb", C.c_int), ("nf", C.c_int), ("ncells", C.c_int), ("nfaces", C.c_int), ("nnzb", C.c_int64),
+                ("mesh_s", C.c_double), ("local_s", C.c_double), ("assemble_s", C.c_double)]
 
 
 def lib():
@@ -320,5 +340,67 @@ class Solver:
     def __del__(self):
         try:
             lib().fh_solver_destroy(self.h)
+        except Exception:
+            pass
+
+
+class HhoProblem:
+    """Condensed HHO face system of -div(K grad u) = f on a structured mesh."""
+
+    def __init__(self, dim=3, mesh="stetra", n=8, k=2, perturb=0.0, seed=1, source="pisine", kappa="one",
+                 sigma=0.0, subdomain=8, aniso=1e-2):
+        a = HhoArgs()
+        self._strs = [mesh.encode(), source.encode(), kappa.encode()]
+        a.dim, a.mesh, a.n, a.k, a.perturb, a.seed = dim, self._strs[0], n, k, perturb, seed
+        a.source, a.kappa, a.sigma, a.subdomain, a.aniso = self._strs[1], self._strs[2], sigma, subdomain, aniso
+        h = C.c_void_p()
+        _check(lib().fh_hho_create(C.byref(a), C.byref(h)))
+        self.h = h
+        info = HhoInfo()
+        _check(lib().fh_hho_get_info(self.h, C.byref(info)))
+        self.info = {k: getattr(info, k) for k, _ in info._fields_}
+        self.nb, self.nf, self.nnzb = info.nb, info.nf, info.nnzb
+        self.n_unknowns = info.nb * info.nf
+
+    def matrix(self, ctx, keep_host=True):
+        h = C.c_void_p()
+        _check(lib().fh_hho_matrix(ctx.h, self.h, int(keep_host), C.byref(h)))
+        return Matrix(ctx, h)
+
+    def rhs(self):
+        b = np.zeros(self.n_unknowns)
+        _check(lib().fh_hho_rhs(self.h, b))
+        return b
+
+    def blocks(self):
+        rp = np.zeros(self.nb + 1, dtype=np.int32)
+        ci = np.zeros(self.nnzb, dtype=np.int32)
+        v = np.zeros(self.nnzb * self.nf * self.nf)
+        _check(lib().fh_hho_blocks(self.h, rp, ci, v))
+        return rp, ci, v.reshape(-1, self.nf, self.nf)
+
+    def scipy(self):
+        import scipy.sparse as sp
+        rp, ci, v = self.blocks()
+        return sp.bsr_matrix((v, ci, rp), shape=(self.n_unknowns, self.n_unknowns)).tocsr()
+
+    def l2_error(self, x):
+        e = C.c_double()
+        _check(lib().fh_hho_l2_error(self.h, np.ascontiguousarray(x, dtype=np.float64), C.byref(e)))
+        return e.value
+
+    def face_cells(self):
+        out = np.zeros(2 * self.nb, dtype=np.int32)
+        _check(lib().fh_hho_face_cells(self.h, out))
+        return out.reshape(-1, 2)
+
+    def cell_kappa(self):
+        out = np.zeros(self.info["ncells"])
+        _check(lib().fh_hho_cell_kappa(self.h, out))
+        return out
+
+    def __del__(self):
+        try:
+            lib().fh_hho_destroy(self.h)
         except Exception:
             pass

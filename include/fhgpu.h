@@ -153,6 +153,46 @@ int fh_solver_history(const fh_solver* s, double* out, int cap);
 /* Unknowns of multigrid level l. */
 int fh_solver_level_size(const fh_solver* s, int level);
 
+/* ------------------------------------------------------------ HHO system */
+/* Producer of the condensed face system on structured affine meshes:
+ * mesh + local HHO operators + static condensation + assembly
+ * (Program_Diffusion_HHO<Dim>::Execute up to Assemble,
+ * src/Program/Program_Diffusion_HHO.h:16-238; Diffusion_HHO::Assemble,
+ * src/Discretizations/HHO/Diffusion_HHO.h:252-704). */
+typedef struct fh_hho fh_hho;
+typedef struct fh_hho_args {
+    int dim;                /* 2 or 3 */
+    const char* mesh;       /* "stetra" (cube -> 6 tets, Cube_CartesianTetrahedralMesh.h) | "cart" */
+    int n;                  /* background cells per direction */
+    int k;                  /* face polynomial degree 0..3 */
+    double perturb;         /* interior vertex perturbation (fraction of h) */
+    uint64_t seed;
+    const char* source;     /* "pisine" | "sine" (DiffusionTestCase.h:62-131) */
+    const char* kappa;      /* "one" | "lognormal" | "subdomain" */
+    double sigma;           /* lognormal standard deviation */
+    int subdomain;          /* cells per subdomain edge */
+    double aniso;           /* z-scaling of the anisotropic subdomains */
+} fh_hho_args;
+typedef struct fh_hho_info {
+    int nb, nf, ncells, nfaces;
+    int64_t nnzb;
+    double mesh_s, local_s, assemble_s;
+} fh_hho_info;
+int fh_hho_create(const fh_hho_args* args, fh_hho** out);
+int fh_hho_destroy(fh_hho* h);
+int fh_hho_get_info(const fh_hho* h, fh_hho_info* info);
+/* Device BCSR of the condensed operator A (Diffusion_HHO::A). */
+int fh_hho_matrix(fh_ctx* ctx, const fh_hho* h, int keep_host, fh_mat** out);
+/* Host copies: condensed right-hand side b (nb*nf) and block arrays. */
+int fh_hho_rhs(const fh_hho* h, double* b);
+int fh_hho_blocks(const fh_hho* h, int* rowptr, int* colidx, double* vals);
+/* Relative L2 error of the k+1 reconstruction of the face solution x (host)
+ * against the exact solution (Diffusion_HHO::L2Error, Diffusion_HHO.h:120-149). */
+int fh_hho_l2_error(const fh_hho* h, const double* x, double* err);
+/* Cells of every interior face (2 ints per block row) and cell kappa. */
+int fh_hho_face_cells(const fh_hho* h, int* out);
+int fh_hho_cell_kappa(const fh_hho* h, double* out);
+
 #ifdef __cplusplus
 }
 #endif

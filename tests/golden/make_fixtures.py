@@ -122,6 +122,9 @@ def main():
     # Block (n_f = 6) tetrahedral system: SpMV / block smoothers / block aggregation
     tet2 = ["-dim", "3", "-mesh", "stetra", "-k", "2", "-n", "4", "-source", "sine"]
     run("tet4_k2", tet2 + ["-s", "cg", "-prec", "bj"], aggreg=tet2 + ["-nosolve"])
+    # finer tetrahedral k=2 system: L2 error and iteration count only
+    run("tet8_k2_cg_bj", ["-dim", "3", "-mesh", "stetra", "-k", "2", "-n", "8", "-source", "sine", "-s", "cg",
+                          "-prec", "bj"], keep_A=False)
     # 3-D Cartesian k=2 with GMG V(1,1) bj23
     run("cart8_k2_cg_mg_bj23", ["-dim", "3", "-mesh", "cart", "-k", "2", "-n", "8", "-s", "cg", "-prec", "mg",
                                 "-cycle", "V,1,1", "-smoothers", "bj23,bj23"], export_mg=True)

@@ -1,0 +1,135 @@
+"""The condensed HHO face system produced by the library against the
+reference's own exported systems (golden fixtures), closed forms and
+convergence rates.  Host-only: no GPU needed."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+from conftest import csr, load
+from oracle import oracle as O
+
+F = pytest.importorskip("fhhos4_amd")
+G = F.fhgpu
+
+# (fixture, HHO arguments, face dofs)
+CASES = [
+    ("c1_n64_cg_mg_bj", dict(dim=2, mesh="cart", n=64, k=1, source="pisine"), 2),
+    ("cart8_k2_cg_mg_bj23", dict(dim=3, mesh="cart", n=8, k=2, source="pisine"), 6),
+    ("tet8_k0_fcg_amg", dict(dim=3, mesh="stetra", n=8, k=0, source="sine"), 1),
+    ("tet4_k2", dict(dim=3, mesh="stetra", n=4, k=2, source="sine"), 6),
+]
+
+
+def block_pattern(M, nf):
+    M = sp.csr_matrix(M)
+    M.eliminate_zeros()
+    c = M.tocoo()
+    return set(zip((c.row // nf).tolist(), (c.col // nf).tolist()))
+
+
+@pytest.mark.parametrize("name,kw,nf", CASES, ids=[c[0] for c in CASES])
+def test_block_sparsity_matches_reference(name, kw, nf):
+    """Same face numbering (interior faces in the reference's order) and the
+    same coupling structure.  Blocks of the reference's pattern are all
+    present; blocks present only here hold rounding noise the reference's own
+    arithmetic happened to cancel to exactly 0 (it stores noise elsewhere:
+    |a_ij| ~ 1e-17 |a_ii| entries are part of its pattern)."""
+    P = G.HhoProblem(**kw)
+    A = P.scipy()
+    Ar = csr(load(name), "A")
+    assert P.nf == nf and P.n_unknowns == Ar.shape[0]
+    rp, ci, _ = P.blocks()
+    mine = set((i, int(j)) for i in range(P.nb) for j in ci[rp[i]:rp[i + 1]])
+    ref = block_pattern(Ar, nf)
+    assert ref <= mine
+    extra = mine - ref
+    assert len(extra) <= 0.02 * len(ref)
+    d = np.abs(A.diagonal())
+    for (I, J) in list(extra)[:200]:
+        blk = A[I * nf:(I + 1) * nf, J * nf:(J + 1) * nf].toarray()
+        scale = np.sqrt(np.outer(d[I * nf:(I + 1) * nf], d[J * nf:(J + 1) * nf]))
+        assert np.abs(blk).max() <= 1e-13 * scale.max()
+
+
+def test_scalar_values_match_reference_k0():
+    """k = 0: one basis function per face; the reference's is 1, ours is
+    1/sqrt(|F|): A_ours = D A_ref D with D = diag(1/sqrt|F|)."""
+    P = G.HhoProblem(dim=3, mesh="stetra", n=8, k=0, source="sine")
+    A = P.scipy()
+    Ar = csr(load("tet8_k0_fcg_amg"), "A")
+    s = np.sqrt(Ar.diagonal() / A.diagonal())
+    As = sp.diags(s) @ A @ sp.diags(s)
+    assert abs(As - Ar).max() <= 1e-14 * abs(Ar).max()
+    b = P.rhs() * s
+    # right-hand sides differ only by the quadrature of the source
+    assert np.linalg.norm(b - load("tet8_k0_fcg_amg")["b"]) <= 5e-3 * np.linalg.norm(b)
+
+
+L2_CASES = [
+    ("c1_n64_cg_mg_bj", dict(dim=2, mesh="cart", n=64, k=1, source="pisine"), 9),
+    ("cart8_k2_cg_mg_bj23", dict(dim=3, mesh="cart", n=8, k=2, source="pisine"), 8),
+    ("tet8_k0_fcg_amg", dict(dim=3, mesh="stetra", n=8, k=0, source="sine"), 3),
+    ("tet8_k2_cg_bj", dict(dim=3, mesh="stetra", n=8, k=2, source="sine"), 3),
+]
+
+
+@pytest.mark.parametrize("name,kw,digits", L2_CASES, ids=[c[0] for c in L2_CASES])
+def test_l2_error_matches_reference(name, kw, digits):
+    P = G.HhoProblem(**kw)
+    x = spla.spsolve(P.scipy().tocsc(), P.rhs())
+    e = P.l2_error(x)
+    ref = float(load(name)["info_l2"])
+    assert float("%.*e" % (digits - 1, e)) == float("%.*e" % (digits - 1, ref)), (e, ref)
+
+
+def _bj_spectrum(A, nf):
+    """Eigenvalues of D^-1 A (D = block diagonal), invariant under any change
+    of face basis A -> T^T A T with T block diagonal."""
+    Ad = A.toarray()
+    nb = A.shape[0] // nf
+    Linv = np.zeros_like(Ad)
+    for i in range(nb):
+        s = slice(i * nf, (i + 1) * nf)
+        Linv[s, s] = np.linalg.inv(np.linalg.cholesky(Ad[s, s]))
+    return np.linalg.eigvalsh(Linv @ Ad @ Linv.T)
+
+
+def test_block_jacobi_spectrum_matches_reference():
+    """The k = 2 tetrahedral operator equals the reference's up to the face
+    basis: same spectrum of the block-Jacobi-preconditioned operator (which
+    also fixes the CG + block-Jacobi convergence for a given right-hand side)."""
+    P = G.HhoProblem(dim=3, mesh="stetra", n=4, k=2, source="sine")
+    ev = _bj_spectrum(P.scipy(), 6)
+    ev_ref = _bj_spectrum(csr(load("tet4_k2"), "A"), 6)
+    np.testing.assert_allclose(ev, ev_ref, rtol=1e-9, atol=1e-9 * ev_ref.max())
+
+
+def test_symmetric_positive_definite_and_perturbed():
+    for kw in (dict(dim=3, mesh="stetra", n=3, k=2, perturb=0.2, seed=3),
+               dict(dim=3, mesh="stetra", n=3, k=1, kappa="lognormal", sigma=2.0),
+               dict(dim=2, mesh="cart", n=6, k=3, kappa="lognormal", sigma=1.0)):
+        A = G.HhoProblem(**kw).scipy().toarray()
+        assert np.abs(A - A.T).max() <= 1e-13 * np.abs(A).max()
+        np.linalg.cholesky(A)
+
+
+@pytest.mark.parametrize("kw,rate", [(dict(dim=2, mesh="cart", k=1), 3.0), (dict(dim=3, mesh="stetra", k=1), 3.0),
+                                     (dict(dim=3, mesh="stetra", k=1, perturb=0.2, seed=5), 3.0)])
+def test_l2_convergence_rate(kw, rate):
+    """Smooth solution: ||u - p_h|| = O(h^{k+2})."""
+    errs = []
+    ns = (4, 8) if kw["dim"] == 3 else (8, 16)
+    for n in ns:
+        P = G.HhoProblem(n=n, source="pisine", **kw)
+        x = spla.spsolve(P.scipy().tocsc(), P.rhs())
+        errs.append(P.l2_error(x))
+    assert np.log2(errs[0] / errs[1]) > rate - 0.4, errs
+
+
+def test_heterogeneous_subdomain_field():
+    P = G.HhoProblem(dim=3, mesh="cart", n=8, k=1, kappa="subdomain", sigma=3.0, subdomain=4, seed=11)
+    kap = P.cell_kappa()
+    assert len(np.unique(kap)) == 8  # one value per 4^3-cell subdomain
+    fc = P.face_cells()
+    assert fc.shape == (P.nb, 2) and (fc >= 0).all()
