@@ -1,0 +1,250 @@
+#!/usr/bin/env python3
+"""Time to 1e-8 relative residual of the condensed HHO face system on one GPU.
+
+Workload (one "step" = one solve from x = 0 to ||r||/||b|| < 1e-8 with the
+operator, the multigrid hierarchy and b already resident in HBM): 3-D Poisson
+on the unit cube, 64^3 Cartesian cubes split into 6 tetrahedra (1.57 M
+cells), interior vertices perturbed by a seeded uniform +-0.2 h, HHO k = 2
+(6 unknowns per triangular face, 3.1 M faces, 18.7 M unknowns), kappa = 1,
+f = 3 pi^2 prod sin(pi x_i).  The plain method is FP64 CG preconditioned by
+one multigrid V-cycle with block smoothing; beside it: FP64 FCG with an FP32
+(and a BF16-stored) multigrid preconditioner.
+
+Prints one JSON line; progress goes to stderr.  --sweep runs a list of
+solver variants and prints one JSON line per variant instead.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+METRIC = "s to 1e-8 rel. residual & GB/s: FP64 MG-PCG vs mixed/learned FCG, 1 GPU"
+HBM_PEAK_GBS = 8000.0
+T0 = time.time()
+
+
+def log(*a):
+    print("[bench %.1fs]" % (time.time() - T0), *a, file=sys.stderr, flush=True)
+
+
+def spmv_bytes(nb, nf, nnzb, sv=8, sx=8):
+    """Algorithmic bytes of one block-CSR SpMV: values, block column indices,
+    row pointers, x read once and y written once."""
+    return nnzb * nf * nf * sv + 4 * nnzb + 4 * (nb + 1) + 2 * nb * nf * sx
+
+
+def timed_solves(ctx, solver, b, x, steps, warmup):
+    for _ in range(warmup):
+        st = solver.solve(b, x)
+    ctx.sync()
+    ts = []
+    for _ in range(steps):
+        ctx.sync()
+        t0 = time.perf_counter()
+        st = solver.solve(b, x)
+        ctx.sync()
+        ts.append(time.perf_counter() - t0)
+    return ts, st
+
+
+class Variant:
+    def __init__(self, prec="gmg", galerkin=True, smoother="bj", relax=0.5, nu=2, cycle="V"):
+        self.prec, self.galerkin, self.smoother, self.relax, self.nu, self.cycle = \
+            prec, galerkin, smoother, relax, nu, cycle
+
+    def label(self):
+        return "%s%s %s(%d,%d) %s%s" % (self.prec, "-galerkin" if self.prec == "gmg" and self.galerkin else "",
+                                        self.cycle, self.nu, self.nu, self.smoother,
+                                        " w=%g" % self.relax if self.smoother == "bj" else "")
+
+
+def make_solver(ctx, G, hier, A, v, solver="cg", mg_prec="f64", mg_store=None, max_iter=1000):
+    from fhhos4_amd import gmg
+    s = G.Solver(ctx, solver=solver, prec="mg" if v.prec == "gmg" else "aggregamg", cycle=v.cycle, nu=(v.nu, v.nu),
+                 smoothers=(v.smoother, v.smoother), relax=v.relax, max_iter=max_iter, tol=1e-8, mg_prec=mg_prec,
+                 mg_store=mg_store, galerkin=v.galerkin)
+    if v.prec == "gmg":
+        gmg.attach(s, hier)
+    t = time.perf_counter()
+    s.setup(A)
+    return s, time.perf_counter() - t
+
+
+def cpu_baseline(args, G, O, v):
+    """The oracle (CPU restatement of the same algorithm, OpenMP) on a
+    bounded sample of the same workload family (smaller mesh)."""
+    from fhhos4_amd import gmg
+    n = args.cpu_n
+    hctx = G.Context(-1)
+    P = G.HhoProblem(dim=3, mesh="stetra", n=n, k=args.k, perturb=args.perturb, seed=args.seed, source="pisine")
+    H = gmg.build_hierarchy(hctx, P)
+    As = [H[0][1]]
+    if v.galerkin:
+        for l in range(len(H) - 1):
+            As.append(G.galerkin_rap(hctx, As[-1], H[l][2]))
+    else:
+        As = [h[1] for h in H]
+    levels = [(As[l].scipy(), H[l][2].scipy() if H[l][2] is not None else None) for l in range(len(H))]
+    sm = "mc" + v.smoother if "gs" in v.smoother else v.smoother
+    x, hist, out = O.solve(levels[0][0], P.rhs(), "cg", "mg", nf=P.nf, cycle=v.cycle, smoothers=(sm, sm),
+                           omega=v.relax, nu=(v.nu, v.nu), levels=levels, max_iter=args.max_iter)
+    setup_s, solve_s = O.last_times()
+    N = levels[0][0].shape[0]
+    return {"N": N, "iterations": out.iterations, "solve_s": solve_s, "setup_s": setup_s,
+            "dof_it_per_s": N * out.iterations / solve_s, "n": n}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=64)
+    ap.add_argument("--k", type=int, default=2)
+    ap.add_argument("--perturb", type=float, default=0.2)
+    ap.add_argument("--seed", type=int, default=2026)
+    ap.add_argument("--prec", default="gmg", choices=["gmg", "amg"])
+    ap.add_argument("--no-galerkin", action="store_true")
+    ap.add_argument("--smoother", default="bj")
+    ap.add_argument("--relax", type=float, default=0.5)
+    ap.add_argument("--nu", type=int, default=2)
+    ap.add_argument("--cycle", default="V")
+    ap.add_argument("--max-iter", type=int, default=1000)
+    ap.add_argument("--cpu-n", type=int, default=16)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--sweep", action="store_true", help="time a list of solver variants")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import fhhos4_amd as F
+    from fhhos4_amd import gmg
+
+    G = F.fhgpu
+    assert torch.cuda.is_available(), "no ROCm device"
+    log("assembling HHO system n=%d k=%d (perturb %.2f)" % (args.n, args.k, args.perturb))
+    P = G.HhoProblem(dim=3, mesh="stetra", n=args.n, k=args.k, perturb=args.perturb, seed=args.seed,
+                     source="pisine")
+    nb, nf, nnzb = P.nb, P.nf, P.nnzb
+    N = nb * nf
+    log("system: %d faces x %d = %d unknowns, %d blocks, assembly %.1fs" % (nb, nf, N, nnzb, P.info["assemble_s"]))
+    ctx = G.Context(0)
+    t = time.perf_counter()
+    hier = gmg.build_hierarchy(ctx, P)
+    hier_s = time.perf_counter() - t
+    A = hier[0][1]
+    log("geometric hierarchy", [h[0].n_unknowns for h in hier], "%.1fs" % hier_s)
+    b = torch.from_numpy(P.rhs()).cuda()
+    x = torch.zeros_like(b)
+    torch.cuda.synchronize()
+
+    if args.sweep:
+        variants = [Variant("gmg", True, "bj", 0.5, 2), Variant("gmg", True, "bj", 0.5, 1),
+                    Variant("gmg", False, "bj", 0.5, 2), Variant("gmg", True, "sbgs", 1.0, 1),
+                    Variant("gmg", False, "sbgs", 1.0, 1), Variant("amg", False, "sbgs", 1.0, 1)]
+        for v in variants:
+            log("variant", v.label())
+            try:
+                s, su = make_solver(ctx, G, hier, A, v, max_iter=args.max_iter)
+                log("  set-up %.1fs" % su)
+                ts, st = timed_solves(ctx, s, b, x, args.steps, args.warmup)
+                print(json.dumps({"variant": v.label(), "s": float(np.mean(ts)), "iterations": st["iterations"],
+                                  "converged": st["converged"], "setup_s": su, "levels": st["levels"],
+                                  "op_complexity": st["op_complexity"]}), flush=True)
+                del s
+            except Exception as e:  # report and continue with the next variant
+                print(json.dumps({"variant": v.label(), "error": str(e)}), flush=True)
+        return
+
+    v = Variant(args.prec, not args.no_galerkin, args.smoother, args.relax, args.nu, args.cycle)
+    log("set-up FP64 MG-PCG:", v.label())
+    s64, setup64 = make_solver(ctx, G, hier, A, v, max_iter=args.max_iter)
+    levels = [s64.level_size(l) for l in range(s64_levels(s64))]
+    log("levels", levels, "setup %.1fs" % setup64)
+    ts64, st64 = timed_solves(ctx, s64, b, x, args.steps, args.warmup)
+    t64 = float(np.mean(ts64))
+    x64 = x.cpu().numpy()
+    l2_64 = P.l2_error(x64)
+    log("FP64 MG-PCG: %d iterations, %.3f s, res %.2e, L2 %.4e" % (st64["iterations"], t64, st64["final_res"], l2_64))
+
+    # in-situ duration of the fine-level SpMV (dominant kernel family)
+    G.profile_begin(ctx, A, 0)
+    s64.solve(b, x)
+    nl, spmv_ms = G.profile_end(ctx)
+    bytes_fine = spmv_bytes(nb, nf, nnzb)
+    achieved = bytes_fine / (spmv_ms * 1e-3) / 1e9
+    log("fine SpMV: %d launches, %.3f ms, %.0f GB/s" % (nl, spmv_ms, achieved))
+    G.profile_begin(ctx, A, 2)
+    s64.solve(b, x)
+    nls, smooth_ms = G.profile_end(ctx)
+    del s64
+
+    mixed = {}
+    if not args.no_mixed:
+        for store in ("f32", "bf16"):
+            log("set-up FCG + %s multigrid" % store)
+            s, su = make_solver(ctx, G, hier, A, v, solver="fcg", mg_prec="f32", mg_store=store,
+                                max_iter=args.max_iter)
+            ts, st = timed_solves(ctx, s, b, x, args.steps, args.warmup)
+            xm = x.cpu().numpy()
+            mixed[store] = {"s": float(np.mean(ts)), "iterations": st["iterations"], "setup_s": su,
+                            "rel_err_vs_fp64": float(np.linalg.norm(xm - x64) / np.linalg.norm(x64)),
+                            "l2": P.l2_error(xm), "speedup_vs_fp64": t64 / float(np.mean(ts))}
+            log("FCG + %s MG: %d iterations, %.3f s" % (store, st["iterations"], mixed[store]["s"]))
+            del s
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+        log("CPU baseline (oracle, n=%d)" % args.cpu_n)
+        c = cpu_baseline(args, G, O, v)
+        est = N * st64["iterations"] / c["dof_it_per_s"]
+        cpu = {"value": est, "unit": "s", "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count())),
+               "kind": "port",
+               "sample": "oracle CG + GMG %s on %d^3x6 perturbed tets (N=%d, %d it, %.2f s solve, %.3g DOF.it/s); "
+                         "value = full workload N=%d x %d it at that DOF.it/s"
+                         % (v.label(), c["n"], c["N"], c["iterations"], c["solve_s"], c["dof_it_per_s"], N,
+                            st64["iterations"])}
+        log("CPU baseline:", cpu["sample"])
+
+    line = {
+        "metric": METRIC, "value": t64, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": t64 * 1e3, "higher_is_better": False, "scaling": "strong", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "3-D Poisson, %d^3 cubes x 6 tets perturbed +-%.2fh, HHO k=%d (%d unknowns), "
+                               "CG + %s, tol 1e-8" % (args.n, args.perturb, args.k, N, v.label()),
+                   "unknowns": N, "faces": nb, "blocks": nnzb, "levels": levels},
+        "iterations": st64["iterations"], "setup_s": setup64, "hierarchy_s": hier_s, "l2_error": l2_64,
+        "dof_it_per_s": N * st64["iterations"] / t64,
+        "mixed_fcg": mixed,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_bcsr<6,double,double,double,OP_SPMV> (fine level)",
+                     "bytes_per_launch": bytes_fine, "mean_ms": spmv_ms, "launches": nl,
+                     "smooth_mean_ms": smooth_ms, "smooth_launches": nls},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def s64_levels(s):
+    n = 0
+    while True:
+        try:
+            s.level_size(n)
+            n += 1
+        except Exception:
+            return n
+
+
+if __name__ == "__main__":
+    main()

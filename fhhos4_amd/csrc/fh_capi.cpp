@@ -113,6 +113,18 @@ int fh_mat_info(const fh_mat* A, int* nbr, int* nbc, int* bs, int64_t* nnzb, int
         if (prec) *prec = (int)A->A->prec;
     });
 }
+int fh_mat_blocks(const fh_mat* A, int* rowptr, int* colidx, double* vals) {
+    return guard([&] {
+        NEED(A);
+        const fh::Bcsr& M = *A->A;
+        if (rowptr) std::memcpy(rowptr, M.h_rowptr.data(), sizeof(int) * M.h_rowptr.size());
+        if (colidx) std::memcpy(colidx, M.h_colidx.data(), sizeof(int) * M.h_colidx.size());
+        if (vals) {
+            FH_CHECK(!M.h_vals.empty(), -3, "matrix created without host values (keep_host = 0)");
+            std::memcpy(vals, M.h_vals.data(), sizeof(double) * M.h_vals.size());
+        }
+    });
+}
 int fh_mat_device_arrays(const fh_mat* A, const int** rowptr, const int** colidx, const void** vals) {
     return guard([&] {
         NEED(A);
@@ -157,6 +169,36 @@ int fh_convert(fh_ctx* ctx, int64_t n, const void* src, int sprec, void* dst, in
     return guard([&] {
         NEED(ctx); NEED_DEV(ctx); NEED(src); NEED(dst);
         fh::launch_convert(ctx->c, n, src, P(sprec), dst, P(dprec));
+    });
+}
+
+int fh_profile_begin(fh_ctx* ctx, const fh_mat* A, int mode) {
+    return guard([&] {
+        NEED(ctx); NEED_DEV(ctx); NEED(A);
+        FH_CHECK(mode >= 0 && mode <= 2, -2, "mode must be 0, 1 or 2");
+        ctx->c.prof_matrix = A->A.get();
+        ctx->c.prof_mode = mode;
+        ctx->c.prof_events.clear();
+    });
+}
+int fh_profile_end(fh_ctx* ctx, int* launches, double* mean_ms) {
+    return guard([&] {
+        NEED(ctx); NEED_DEV(ctx);
+        ctx->c.sync();
+        double tot = 0;
+        for (auto& e : ctx->c.prof_events) {
+            float ms = 0;
+            FH_HIP(hipEventElapsedTime(&ms, e.first, e.second));
+            tot += ms;
+            (void)hipEventDestroy(e.first);
+            (void)hipEventDestroy(e.second);
+        }
+        int n = (int)ctx->c.prof_events.size();
+        if (launches) *launches = n;
+        if (mean_ms) *mean_ms = n ? tot / n : 0.0;
+        ctx->c.prof_events.clear();
+        ctx->c.prof_matrix = nullptr;
+        ctx->c.prof_mode = -1;
     });
 }
 
@@ -287,6 +329,15 @@ int fh_galerkin_aggregate(fh_ctx* ctx, const fh_mat* A, const int* agg, int nagg
     });
 }
 
+int fh_galerkin_rap(fh_ctx* ctx, const fh_mat* A, const fh_mat* P, fh_mat** out) {
+    return guard([&] {
+        NEED(ctx); NEED(A); NEED(P); NEED(out);
+        auto m = new fh_mat;
+        m->A = fh::galerkin_rap(ctx->c, *A->A, *P->A);
+        *out = m;
+    });
+}
+
 int fh_solver_create(fh_ctx* ctx, const fh_solver_args* a, fh_solver** out) {
     return guard([&] {
         NEED(ctx); NEED(a); NEED(out);
@@ -307,6 +358,7 @@ int fh_solver_create(fh_ctx* ctx, const fh_solver_args* a, fh_solver** out) {
         s.theta = a->theta > 0 ? a->theta : 0.25;
         s.mg_prec = P(a->mg_prec);
         s.mg_store = P(a->mg_store);
+        s.galerkin = a->galerkin != 0;
         FH_CHECK(s.mg_prec != fh::Prec::BF16, -2, "multigrid vectors are FP64 or FP32");
         FH_CHECK(s.mg_prec == fh::Prec::F32 || s.mg_store == fh::Prec::F64, -2,
                  "FP32/BF16 level matrices need FP32 multigrid vectors");
@@ -381,6 +433,7 @@ int fh_hho_create(const fh_hho_args* a, fh_hho** out) {
         h.sigma = a->sigma;
         if (a->subdomain > 0) h.subdomain = a->subdomain;
         h.aniso = a->aniso > 0 ? a->aniso : 1e-2;
+        h.ref_n = a->ref_n;
         FH_CHECK(h.dim == 2 || h.dim == 3, -2, "dim must be 2 or 3");
         auto o = new fh_hho;
         try {
@@ -438,6 +491,18 @@ int fh_hho_l2_error(const fh_hho* h, const double* x, double* err) {
     return guard([&] {
         NEED(h); NEED(x); NEED(err);
         *err = h->p->L2Error(x);
+    });
+}
+int fh_hho_prolongation(fh_ctx* ctx, const fh_hho* fine, const fh_hho* coarse, int keep_host, fh_mat** out) {
+    return guard([&] {
+        NEED(ctx); NEED(fine); NEED(coarse); NEED(out);
+        std::vector<int> rp, ci;
+        std::vector<double> v;
+        fine->p->Prolongation(*coarse->p, rp, ci, v);
+        auto m = new fh_mat;
+        m->A = fh::bcsr_from_blocks(ctx->c, fine->p->nb, coarse->p->nb, fine->p->nf, fine->p->nf, rp.data(), ci.data(),
+                                    v.data(), keep_host != 0);
+        *out = m;
     });
 }
 int fh_hho_face_cells(const fh_hho* h, int* out) {

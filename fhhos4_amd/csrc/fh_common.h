@@ -46,6 +46,11 @@ struct Ctx {
     static constexpr int kMaxPartials = 2048;
     static constexpr int kPartialSlots = 4;
     static constexpr int kScalars = 512;  // K-cycle uses 32 + 8 * level + 7 (levels < 40)
+    // In-situ timing of the block-row kernel on one matrix (HIP events on
+    // this stream around every launch of that operation).
+    const void* prof_matrix = nullptr;
+    int prof_mode = -1;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_events;
     explicit Ctx(int dev);
     ~Ctx();
     void sync() {

@@ -256,8 +256,11 @@ struct HhoProblem::Impl {
     void build_tables();
     void geometry(int c, V3& v0, double J[3][3], double Ji[3][3], double& det) const;
     // Local condensation; outputs (optional): condensed matrix (m nfb)^2 and rhs;
-    // or, for post-processing, the reconstruction coefficients from face values.
-    void local(int c, double* Acond, double* bcond, const double* xF, double* recon) const;
+    // or, for post-processing, the reconstruction coefficients from face values;
+    // or the reconstruction-from-faces operator RF (nr x m nfb) without source,
+    // p = P [-A_TT^-1 A_TF; I] v_F (ReconstructionFromFacesMatrix,
+    // Diff_HHOElement.h:212-232).
+    void local(int c, double* Acond, double* bcond, const double* xF, double* recon, double* RF = nullptr) const;
 };
 
 // ------------------------------------------------------------------- mesh
@@ -270,16 +273,33 @@ void HhoProblem::Impl::build_mesh() {
     // vertices (x fastest, then y, then z), perturbed inside the domain
     int nverts = dim == 3 ? nv1 * nv1 * nv1 : nv1 * nv1;
     V.resize(nverts);
-    std::mt19937_64 rng(a.seed);
-    std::uniform_real_distribution<double> U(-1.0, 1.0);
+    // Perturbation of a vertex = counter-based uniform(-1,1) draws (splitmix64
+    // of the seed and the vertex's index on the reference grid) times
+    // perturb * h_ref: coarsened meshes reuse the positions of the finest.
+    const int rn = a.ref_n > 0 ? a.ref_n : n;
+    FH_CHECK(rn % n == 0, -2, "ref_n must be a multiple of n");
+    const int ratio = rn / n;
+    const double href = 1.0 / rn;
+    auto splitmix = [](uint64_t x) {
+        x += 0x9E3779B97F4A7C15ull;
+        x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+        x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+        return x ^ (x >> 31);
+    };
     for (int iz = 0; iz < (dim == 3 ? nv1 : 1); iz++)
         for (int iy = 0; iy < nv1; iy++)
             for (int ix = 0; ix < nv1; ix++) {
                 int id = (iz * nv1 + iy) * nv1 + ix;
                 V3 p{ix * h, iy * h, dim == 3 ? iz * h : 0.0};
                 bool interior = ix > 0 && ix < n && iy > 0 && iy < n && (dim == 2 || (iz > 0 && iz < n));
-                if (a.perturb > 0 && interior)
-                    for (int d = 0; d < dim; d++) p[d] += a.perturb * h * U(rng);
+                if (a.perturb > 0 && interior) {
+                    uint64_t gid = ((uint64_t)(iz * ratio) * (rn + 1) + (uint64_t)(iy * ratio)) * (rn + 1) + (uint64_t)(ix * ratio);
+                    for (int d = 0; d < dim; d++) {
+                        uint64_t r = splitmix(a.seed * 0x100000001B3ull ^ (gid * 4 + d));
+                        double u = (double)(r >> 11) * (1.0 / 9007199254740992.0) * 2.0 - 1.0;
+                        p[d] += a.perturb * href * u;
+                    }
+                }
                 V[id] = p;
             }
     auto vid = [&](int ix, int iy, int iz) { return (iz * nv1 + iy) * nv1 + ix; };
@@ -422,7 +442,8 @@ void HhoProblem::Impl::build_mesh() {
     std::vector<double> sub_kappa;
     int nsub = 0;
     if (a.kappa == "subdomain") {
-        int ns = std::max(1, n / std::max(1, a.subdomain));
+        FH_CHECK(a.subdomain % ratio == 0, -2, "subdomains must be unions of coarse cells");
+        int ns = std::max(1, rn / std::max(1, a.subdomain));
         nsub = ns;
         sub_kappa.resize((size_t)ns * ns * (dim == 3 ? ns : 1));
         for (auto& s : sub_kappa) s = std::exp(a.sigma * N01(krng));
@@ -433,7 +454,7 @@ void HhoProblem::Impl::build_mesh() {
         else if (a.kappa == "subdomain") {
             // background cell of the cell (tets: 6 per cube)
             int bc = kind == 0 ? c / 6 : c;
-            int ix = bc % n, iy = (bc / n) % n, iz = dim == 3 ? bc / (n * n) : 0;
+            int ix = (bc % n) * ratio, iy = ((bc / n) % n) * ratio, iz = (dim == 3 ? bc / (n * n) : 0) * ratio;
             int sd = std::max(1, a.subdomain);
             int sx = std::min(ix / sd, nsub - 1), sy = std::min(iy / sd, nsub - 1), sz = std::min(iz / sd, nsub - 1);
             int s = (dim == 3 ? sz * nsub * nsub : 0) + sy * nsub + sx;
@@ -557,7 +578,7 @@ void HhoProblem::Impl::build_tables() {
 
 // Local HHO operators of cell c (Diff_HHOElement::InitHHO,
 // Diff_HHOElement.h:104-150, 312-509) and their static condensation.
-void HhoProblem::Impl::local(int c, double* Acond, double* bcond, const double* xF, double* recon) const {
+void HhoProblem::Impl::local(int c, double* Acond, double* bcond, const double* xF, double* recon, double* RF) const {
     const int m = nfc, nh = nc + m * nfb;
     V3 v0;
     double J[3][3], Ji[3][3], det;
@@ -704,6 +725,7 @@ void HhoProblem::Impl::local(int c, double* Acond, double* bcond, const double* 
     }
     // Static condensation (Diffusion_HHO.h:624-652, HHOCellSpace.h:170-203)
     const int nF = m * nfb;
+    if (RF && !bcond && !recon) BT.assign(nc, 0.0);
     tmp.assign((size_t)nc * nc, 0.0);
     for (int i = 0; i < nc; i++)
         for (int j = 0; j < nc; j++) tmp[i * nc + j] = A[i * nh + j];
@@ -727,6 +749,18 @@ void HhoProblem::Impl::local(int c, double* Acond, double* bcond, const double* 
                 double v = 0;
                 for (int i = 0; i < nc; i++) v -= A[i * nh + nc + r] * X[i * (nF + 1) + nF];
                 bcond[r] = v;
+            }
+    }
+    if (RF) {
+        std::vector<double> X((size_t)nc * nF);
+        for (int i = 0; i < nc; i++)
+            for (int j = 0; j < nF; j++) X[i * nF + j] = A[i * nh + nc + j];
+        chol_solve(nc, tmp.data(), X.data(), nF);
+        for (int i = 0; i < nr; i++)
+            for (int j = 0; j < nF; j++) {
+                double s = P[i * nh + nc + j];
+                for (int l = 0; l < nc; l++) s -= P[i * nh + l] * X[l * nF + j];
+                RF[i * nF + j] = s;
             }
     }
     if (recon) {
@@ -824,6 +858,154 @@ HhoProblem::HhoProblem(const HhoArgs& args) : impl(new Impl) {
 }
 
 HhoProblem::~HhoProblem() { delete impl; }
+
+void HhoProblem::Prolongation(const HhoProblem& coarse, std::vector<int>& prp, std::vector<int>& pci,
+                              std::vector<double>& pv) const {
+    const Impl& Fm = *impl;
+    const Impl& Cm = *coarse.impl;
+    FH_CHECK(Fm.dim == Cm.dim && Fm.kind == Cm.kind && Fm.a.k == Cm.a.k && Fm.a.mesh == Cm.a.mesh, -2,
+             "prolongation between problems of different kinds");
+    FH_CHECK(Cm.a.n * 2 == Fm.a.n, -2, "the coarse problem must be on the once-coarsened mesh (n/2)");
+    const int dim = Fm.dim, n = Fm.a.n, nc_ = Cm.a.n;
+    const int nfb = Fm.nfb, nr = Cm.nr, nfc = Cm.nfc, nF = nfc * nfb;
+    // fine cell -> coarse cell
+    std::vector<int> f2c(Fm.ncell);
+    int tet_table[8][6] = {};
+    if (Fm.kind == 0) {
+        // Nested tetrahedra of the cube split: the unperturbed centroid of a
+        // fine tetrahedron lies inside exactly one tetrahedron of its coarse cube.
+        static const int T[6][4] = {{2, 3, 7, 1}, {7, 2, 1, 0}, {0, 1, 7, 5}, {7, 2, 0, 6}, {5, 7, 0, 6}, {5, 0, 4, 6}};
+        auto corner = [](int cc, double s) { return V3{s * (cc & 1), s * ((cc >> 1) & 1), s * ((cc >> 2) & 1)}; };
+        for (int o = 0; o < 8; o++)
+            for (int t = 0; t < 6; t++) {
+                V3 g{0, 0, 0};
+                for (int q = 0; q < 4; q++) {
+                    V3 p = corner(T[t][q], 1.0);
+                    for (int d = 0; d < 3; d++) g[d] += (p[d] + ((o >> d) & 1)) / 4.0;
+                }
+                int found = -1;
+                for (int tc = 0; tc < 6 && found < 0; tc++) {
+                    V3 v0 = corner(T[tc][0], 2.0), e1 = sub(corner(T[tc][1], 2.0), v0), e2 = sub(corner(T[tc][2], 2.0), v0),
+                       e3 = sub(corner(T[tc][3], 2.0), v0), r = sub(g, v0);
+                    double det = dot3(e1, cross(e2, e3));
+                    double l1 = dot3(r, cross(e2, e3)) / det, l2 = dot3(e1, cross(r, e3)) / det, l3 = dot3(e1, cross(e2, r)) / det;
+                    if (l1 > 1e-9 && l2 > 1e-9 && l3 > 1e-9 && l1 + l2 + l3 < 1 - 1e-9) found = tc;
+                }
+                FH_CHECK(found >= 0, -2, "non-nested tetrahedral refinement");
+                tet_table[o][t] = found;
+            }
+    }
+    for (int c = 0; c < Fm.ncell; c++) {
+        int bc = Fm.kind == 0 ? c / 6 : c;
+        int ix = bc % n, iy = (bc / n) % n, iz = dim == 3 ? bc / (n * n) : 0;
+        int cb = ((dim == 3 ? (iz / 2) * nc_ : 0) + iy / 2) * nc_ + ix / 2;
+        if (Fm.kind == 0) {
+            int o = (ix & 1) + 2 * (iy & 1) + 4 * (iz & 1);
+            f2c[c] = 6 * cb + tet_table[o][c % 6];
+        } else
+            f2c[c] = cb;
+    }
+    // reconstruction-from-faces operator of every coarse cell
+    std::vector<double> RFall((size_t)Cm.ncell * nr * nF);
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int c = 0; c < Cm.ncell; c++) Cm.local(c, nullptr, nullptr, nullptr, nullptr, &RFall[(size_t)c * nr * nF]);
+    // rows: fine interior faces
+    std::vector<std::vector<int>> rcols(nb);
+    std::vector<std::vector<double>> rvals(nb);
+    const size_t bsz = (size_t)nfb * nfb;
+#pragma omp parallel
+    {
+        std::vector<double> psi(nr), chi(nfb), Tr((size_t)nfb * nr);
+        std::vector<std::pair<int, int>> ent;  // (coarse row, local face of the coarse cell)
+#pragma omp for schedule(dynamic, 512)
+        for (int f = 0; f < Fm.nface; f++) {
+            int row = Fm.funk[f];
+            if (row < 0) continue;
+            int c1 = Fm.fcells[2 * f], c2 = Fm.fcells[2 * f + 1];
+            int C1 = f2c[c1], C2 = c2 >= 0 ? f2c[c2] : -1;
+            int cs[2] = {C1, C2 != C1 ? C2 : -1};
+            // fine face geometry
+            const int* fvv = &Fm.fv[(size_t)f * Fm.nvf];
+            V3 a0 = Fm.V[fvv[0]];
+            V3 E1 = sub(Fm.V[fvv[1]], a0);
+            V3 E2 = dim == 3 ? sub(Fm.V[Fm.kind == 1 ? fvv[3] : fvv[2]], a0) : V3{0, 0, 0};
+            double sF = dim == 3 ? nrm(cross(E1, E2)) : nrm(E1);
+            V3 nF_ = dim == 3 ? cross(E1, E2) : V3{E1[1], -E1[0], 0};
+            double nn = nrm(nF_);
+            for (int d = 0; d < 3; d++) nF_[d] /= nn;
+            // weights: normal diffusivity (Weight, LevelForHHO.h:743-763)
+            double w[2] = {1.0, 0.0};
+            if (cs[1] >= 0) {
+                double k1 = 0, k2 = 0;
+                for (int r = 0; r < 3; r++)
+                    for (int q = 0; q < 3; q++) {
+                        k1 += nF_[r] * Cm.K[(size_t)cs[0] * 9 + r * 3 + q] * nF_[q];
+                        k2 += nF_[r] * Cm.K[(size_t)cs[1] * 9 + r * 3 + q] * nF_[q];
+                    }
+                w[0] = k1 / (k1 + k2);
+                w[1] = k2 / (k1 + k2);
+            }
+            std::vector<int>& cols = rcols[row];
+            std::vector<double>& vals_ = rvals[row];
+            for (int s = 0; s < 2; s++) {
+                int C = cs[s];
+                if (C < 0) continue;
+                // trace projection of the coarse reconstruction basis on the fine face
+                V3 v0;
+                double J[3][3], Ji[3][3], det;
+                Cm.geometry(C, v0, J, Ji, det);
+                const double isq = 1.0 / std::sqrt(std::fabs(det)), ssq = std::sqrt(sF);
+                std::fill(Tr.begin(), Tr.end(), 0.0);
+                for (size_t q = 0; q < Fm.fq.p.size(); q++) {
+                    const V3& eta = Fm.fq.p[q];
+                    V3 y{a0[0] + eta[0] * E1[0] + eta[1] * E2[0], a0[1] + eta[0] * E1[1] + eta[1] * E2[1],
+                         a0[2] + eta[0] * E1[2] + eta[1] * E2[2]};
+                    V3 r = sub(y, v0);
+                    V3 xi{Ji[0][0] * r[0] + Ji[0][1] * r[1] + Ji[0][2] * r[2], Ji[1][0] * r[0] + Ji[1][1] * r[1] + Ji[1][2] * r[2],
+                          Ji[2][0] * r[0] + Ji[2][1] * r[1] + Ji[2][2] * r[2]};
+                    if (dim == 2) xi[2] = 0;
+                    Cm.rb.eval(xi, psi.data(), nullptr);
+                    Fm.fb.eval(eta, chi.data(), nullptr);
+                    double wq = Fm.fq.w[q] * ssq * isq;
+                    for (int j = 0; j < nfb; j++)
+                        for (int i = 0; i < nr; i++) Tr[(size_t)j * nr + i] += wq * chi[j] * psi[i];
+                }
+                const double* RF = &RFall[(size_t)C * nr * nF];
+                for (int lf = 0; lf < nfc; lf++) {
+                    int crow = Cm.funk[Cm.cf[(size_t)C * nfc + lf]];
+                    if (crow < 0) continue;  // Dirichlet coarse face: zero value
+                    int pos = -1;
+                    for (size_t t = 0; t < cols.size(); t++) if (cols[t] == crow) pos = (int)t;
+                    if (pos < 0) {
+                        pos = (int)cols.size();
+                        cols.push_back(crow);
+                        vals_.resize(vals_.size() + bsz, 0.0);
+                    }
+                    double* blk = &vals_[(size_t)pos * bsz];
+                    for (int j = 0; j < nfb; j++)
+                        for (int b2 = 0; b2 < nfb; b2++) {
+                            double sum = 0;
+                            for (int i = 0; i < nr; i++) sum += Tr[(size_t)j * nr + i] * RF[(size_t)i * nF + lf * nfb + b2];
+                            blk[j * nfb + b2] += w[s] * sum;
+                        }
+                }
+            }
+        }
+    }
+    prp.assign(nb + 1, 0);
+    for (int r = 0; r < nb; r++) prp[r + 1] = prp[r] + (int)rcols[r].size();
+    pci.resize(prp[nb]);
+    pv.assign((size_t)prp[nb] * bsz, 0.0);
+    for (int r = 0; r < nb; r++) {
+        std::vector<int> perm(rcols[r].size());
+        for (size_t t = 0; t < perm.size(); t++) perm[t] = (int)t;
+        std::sort(perm.begin(), perm.end(), [&](int x, int y) { return rcols[r][x] < rcols[r][y]; });
+        for (size_t t = 0; t < perm.size(); t++) {
+            pci[prp[r] + t] = rcols[r][perm[t]];
+            std::memcpy(&pv[(size_t)(prp[r] + t) * bsz], &rvals[r][(size_t)perm[t] * bsz], bsz * sizeof(double));
+        }
+    }
+}
 
 int HhoProblem::ncells() const { return impl->ncell; }
 int HhoProblem::nfaces() const { return impl->nface; }

@@ -29,8 +29,11 @@ struct HhoArgs {
     std::string source = "pisine"; // "pisine": f = d pi^2 prod sin(pi x) | "sine": 4 pi periodic
     std::string kappa = "one";     // "one" | "lognormal" (per cell) | "subdomain" (per block of cells)
     double sigma = 0.0;            // lognormal standard deviation
-    int subdomain = 8;             // cells per subdomain edge ("subdomain")
+    int subdomain = 8;             // cells per subdomain edge ("subdomain"), counted on the reference grid
     double aniso = 1e-2;           // z-scaling of the anisotropic subdomains ("subdomain")
+    int ref_n = 0;                 // finest grid of the hierarchy (0: n).  Vertex perturbations and
+                                   // subdomains are defined on it, so a mesh with n = ref_n / 2^l
+                                   // is the geometric coarsening of the finest one.
 };
 
 struct HhoTimes {
@@ -54,6 +57,13 @@ public:
     std::vector<double> CellKappa() const;
     // face adjacency: cells of every interior face (2 per face)
     void FaceCells(std::vector<int>& out) const;
+    // Prolongation from the problem on the once-coarsened mesh (n/2) to this
+    // one, as nf x nf blocks (fine faces x coarse faces): the coarse cell's
+    // k+1 reconstruction from its face values, L2-projected on the fine
+    // faces, weighted by normal diffusivity across coarse faces
+    // (CellInterp_Trace, LevelForHHO.h:192-213, 743-763, 899-954).
+    void Prolongation(const HhoProblem& coarse, std::vector<int>& prp, std::vector<int>& pci,
+                      std::vector<double>& pv) const;
     HhoTimes times;
     struct Impl;
     Impl* impl = nullptr;

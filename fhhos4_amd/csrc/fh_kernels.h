@@ -54,6 +54,12 @@ void launch_agg_restrict(Ctx& c, int nagg, int nf, Prec p, const int* mptr, cons
 // prolong-add: x[k*nf+a] += ec[agg[k]*nf+a]
 void launch_agg_prolong_add(Ctx& c, int nb, int nf, Prec p, const int* agg, const void* ec, void* x);
 
+// p-transfers between block sizes nf (fine) and nfc <= nf (coarse) for
+// hierarchical bases: restriction keeps the first nfc modes of every block,
+// prolongation injects them (adds into x).
+void launch_p_restrict(Ctx& c, int nb, int nf, int nfc, Prec p, const void* r, void* rc);
+void launch_p_prolong_add(Ctx& c, int nb, int nf, int nfc, Prec p, const void* ec, void* x);
+
 // y = M x with M dense symmetric n x n (precision mp), vectors in vp.
 void launch_dense_symv(Ctx& c, int n, const void* M, Prec mp, Prec vp, const void* x, void* y);
 

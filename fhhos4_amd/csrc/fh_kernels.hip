@@ -279,6 +279,24 @@ __global__ void k_agg_prolong_add(int nb, int nf, const int* __restrict__ agg, c
     x[t] = (T)((AT)x[t] + (AT)ec[(int64_t)agg[k] * nf + a]);
 }
 
+template <class T>
+__global__ void k_p_restrict(int nb, int nf, int nfc, const T* __restrict__ r, T* __restrict__ rc) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)nb * nfc) return;
+    int i = (int)(t / nfc), a = (int)(t - (int64_t)i * nfc);
+    rc[t] = r[(int64_t)i * nf + a];
+}
+
+template <class T>
+__global__ void k_p_prolong_add(int nb, int nf, int nfc, const T* __restrict__ ec, T* __restrict__ x) {
+    using AT = typename Acc<T>::T;
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)nb * nfc) return;
+    int i = (int)(t / nfc), a = (int)(t - (int64_t)i * nfc);
+    int64_t j = (int64_t)i * nf + a;
+    x[j] = (T)((AT)x[j] + (AT)ec[t]);
+}
+
 // One wavefront per row of a dense symmetric matrix.
 template <class TM, class TV>
 __global__ __launch_bounds__(kWG) void k_dense_symv(int n, const TM* __restrict__ M, const TV* __restrict__ x,
@@ -355,9 +373,30 @@ void bcsr_dispatch_types(Ctx& c, const Bcsr& A, BcsrMode mode, Prec vp, const vo
 
 }  // namespace
 
+static void launch_bcsr_impl(Ctx& c, const Bcsr& A, BcsrMode mode, Prec vprec, const void* x, const void* b, void* y,
+                             double alpha, double beta, double omega, const void* invD, Prec dprec, const int* rows,
+                             int nrows);
+
 void launch_bcsr(Ctx& c, const Bcsr& A, BcsrMode mode, Prec vprec, const void* x, const void* b, void* y,
                  double alpha, double beta, double omega, const void* invD, Prec dprec, const int* rows,
                  int nrows) {
+    const bool prof = c.prof_matrix == (const void*)&A && c.prof_mode == (int)mode && !rows;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (prof) {
+        FH_HIP(hipEventCreate(&e0));
+        FH_HIP(hipEventCreate(&e1));
+        FH_HIP(hipEventRecord(e0, c.stream));
+    }
+    launch_bcsr_impl(c, A, mode, vprec, x, b, y, alpha, beta, omega, invD, dprec, rows, nrows);
+    if (prof) {
+        FH_HIP(hipEventRecord(e1, c.stream));
+        c.prof_events.push_back({e0, e1});
+    }
+}
+
+static void launch_bcsr_impl(Ctx& c, const Bcsr& A, BcsrMode mode, Prec vprec, const void* x, const void* b, void* y,
+                             double alpha, double beta, double omega, const void* invD, Prec dprec, const int* rows,
+                             int nrows) {
     FH_CHECK(A.bs_r == A.bs_c, -3, "block-row kernel needs square blocks");
     switch (A.bs_r) {
         case 1: bcsr_dispatch_types<1>(c, A, mode, vprec, x, b, y, alpha, beta, omega, invD, dprec, rows, nrows); break;
@@ -516,6 +555,32 @@ void launch_agg_prolong_add(Ctx& c, int nb, int nf, Prec p, const int* agg, cons
     else
         hipLaunchKernelGGL((k_agg_prolong_add<float>), dim3(g), dim3(256), 0, c.stream, nb, nf, agg,
                            (const float*)ec, (float*)x);
+    FH_HIP(hipGetLastError());
+}
+
+void launch_p_restrict(Ctx& c, int nb, int nf, int nfc, Prec p, const void* r, void* rc) {
+    int64_t nt = (int64_t)nb * nfc;
+    if (nt <= 0) return;
+    int g = (int)((nt + 255) / 256);
+    if (p == Prec::F64)
+        hipLaunchKernelGGL((k_p_restrict<double>), dim3(g), dim3(256), 0, c.stream, nb, nf, nfc, (const double*)r,
+                           (double*)rc);
+    else
+        hipLaunchKernelGGL((k_p_restrict<float>), dim3(g), dim3(256), 0, c.stream, nb, nf, nfc, (const float*)r,
+                           (float*)rc);
+    FH_HIP(hipGetLastError());
+}
+
+void launch_p_prolong_add(Ctx& c, int nb, int nf, int nfc, Prec p, const void* ec, void* x) {
+    int64_t nt = (int64_t)nb * nfc;
+    if (nt <= 0) return;
+    int g = (int)((nt + 255) / 256);
+    if (p == Prec::F64)
+        hipLaunchKernelGGL((k_p_prolong_add<double>), dim3(g), dim3(256), 0, c.stream, nb, nf, nfc,
+                           (const double*)ec, (double*)x);
+    else
+        hipLaunchKernelGGL((k_p_prolong_add<float>), dim3(g), dim3(256), 0, c.stream, nb, nf, nfc, (const float*)ec,
+                           (float*)x);
     FH_HIP(hipGetLastError());
 }
 

@@ -330,6 +330,127 @@ std::unique_ptr<Bcsr> galerkin_aggregate(Ctx& c, const Bcsr& A, const std::vecto
     return Ac;
 }
 
+std::unique_ptr<Bcsr> galerkin_rap(Ctx& c, const Bcsr& A, const Bcsr& P) {
+    FH_CHECK(!A.h_vals.empty() && !P.h_vals.empty(), -3, "Galerkin product needs host values");
+    FH_CHECK(A.bs_r == A.bs_c && P.bs_r == A.bs_r && P.bs_c == A.bs_r && P.nbr == A.nbr, -2,
+             "Galerkin product: incompatible block shapes");
+    const int bs = A.bs_r, nc = P.nbc;
+    const size_t bsz = (size_t)bs * bs;
+    // R = P^T by block rows of the coarse space
+    std::vector<int> rrp(nc + 1, 0), rci(P.nnzb), rpos(P.nnzb);
+    for (int64_t p = 0; p < P.nnzb; p++) rrp[P.h_colidx[p] + 1]++;
+    for (int J = 0; J < nc; J++) rrp[J + 1] += rrp[J];
+    {
+        std::vector<int> fill(rrp.begin(), rrp.end() - 1);
+        for (int i = 0; i < P.nbr; i++)
+            for (int p = P.h_rowptr[i]; p < P.h_rowptr[i + 1]; p++) {
+                int q = fill[P.h_colidx[p]]++;
+                rci[q] = i;
+                rpos[q] = p;  // block P_iJ; R_Ji = P_iJ^T
+            }
+    }
+    std::vector<std::vector<int>> ccols(nc);
+    std::vector<std::vector<double>> cvals(nc);
+#pragma omp parallel
+    {
+        std::vector<int> slot(nc, -1);
+        std::vector<double> T(bsz), U(bsz);
+#pragma omp for schedule(dynamic, 64)
+        for (int I = 0; I < nc; I++) {
+            std::vector<int>& cols = ccols[I];
+            std::vector<double>& acc = cvals[I];
+            for (int q = rrp[I]; q < rrp[I + 1]; q++) {
+                const int i = rci[q];
+                const double* Pi = &P.h_vals[(size_t)rpos[q] * bsz];  // P_iI (bs x bs); R_Ii = Pi^T
+                for (int pa = A.h_rowptr[i]; pa < A.h_rowptr[i + 1]; pa++) {
+                    const int j = A.h_colidx[pa];
+                    const double* Aij = &A.h_vals[(size_t)pa * bsz];
+                    // T = Pi^T Aij
+                    for (int r = 0; r < bs; r++)
+                        for (int s = 0; s < bs; s++) {
+                            double v = 0;
+                            for (int t = 0; t < bs; t++) v += Pi[t * bs + r] * Aij[t * bs + s];
+                            T[r * bs + s] = v;
+                        }
+                    for (int pp = P.h_rowptr[j]; pp < P.h_rowptr[j + 1]; pp++) {
+                        const int J = P.h_colidx[pp];
+                        const double* Pj = &P.h_vals[(size_t)pp * bsz];
+                        int sl = slot[J];
+                        if (sl < 0) {
+                            sl = slot[J] = (int)cols.size();
+                            cols.push_back(J);
+                            acc.resize(acc.size() + bsz, 0.0);
+                        }
+                        double* dst = &acc[(size_t)sl * bsz];
+                        for (int r = 0; r < bs; r++)
+                            for (int s = 0; s < bs; s++) {
+                                double v = 0;
+                                for (int t = 0; t < bs; t++) v += T[r * bs + t] * Pj[t * bs + s];
+                                dst[r * bs + s] += v;
+                            }
+                    }
+                }
+            }
+            // sort the row by column
+            std::vector<int> perm(cols.size());
+            std::iota(perm.begin(), perm.end(), 0);
+            std::sort(perm.begin(), perm.end(), [&cols](int a, int b) { return cols[a] < cols[b]; });
+            std::vector<int> sc(cols.size());
+            std::vector<double> sv(acc.size());
+            for (size_t t = 0; t < perm.size(); t++) {
+                sc[t] = cols[perm[t]];
+                std::memcpy(&sv[t * bsz], &acc[(size_t)perm[t] * bsz], bsz * sizeof(double));
+            }
+            for (int J : cols) slot[J] = -1;
+            cols.swap(sc);
+            acc.swap(sv);
+        }
+    }
+    std::vector<int> rp(nc + 1, 0);
+    for (int I = 0; I < nc; I++) rp[I + 1] = rp[I] + (int)ccols[I].size();
+    std::vector<int> ci(rp[nc]);
+    std::vector<double> v((size_t)rp[nc] * bsz);
+    for (int I = 0; I < nc; I++) {
+        std::copy(ccols[I].begin(), ccols[I].end(), ci.begin() + rp[I]);
+        std::copy(cvals[I].begin(), cvals[I].end(), v.begin() + (size_t)rp[I] * bsz);
+    }
+    // exact symmetry (the product is symmetric up to rounding)
+    for (int I = 0; I < nc; I++)
+        for (int p = rp[I]; p < rp[I + 1]; p++) {
+            int J = ci[p];
+            if (J <= I) continue;
+            int q = (int)(std::lower_bound(ci.begin() + rp[J], ci.begin() + rp[J + 1], I) - ci.begin());
+            if (q >= rp[J + 1] || ci[q] != I) continue;
+            for (int r = 0; r < bs; r++)
+                for (int s = 0; s < bs; s++) {
+                    double m = 0.5 * (v[(size_t)p * bsz + r * bs + s] + v[(size_t)q * bsz + s * bs + r]);
+                    v[(size_t)p * bsz + r * bs + s] = m;
+                    v[(size_t)q * bsz + s * bs + r] = m;
+                }
+        }
+    return bcsr_from_blocks(c, nc, nc, bs, bs, rp.data(), ci.data(), v.data(), true);
+}
+
+std::unique_ptr<Bcsr> bcsr_mode_submatrix(Ctx& c, const Bcsr& A, int nfc) {
+    FH_CHECK(!A.h_vals.empty(), -3, "p-coarsening needs host values");
+    FH_CHECK(nfc >= 1 && nfc <= A.bs_r && A.bs_r == A.bs_c, -2, "invalid number of kept modes");
+    const int bs = A.bs_r;
+    std::vector<double> v((size_t)A.nnzb * nfc * nfc);
+    for (int64_t p = 0; p < A.nnzb; p++)
+        for (int a = 0; a < nfc; a++)
+            for (int b = 0; b < nfc; b++)
+                v[(size_t)p * nfc * nfc + a * nfc + b] = A.h_vals[(size_t)p * bs * bs + a * bs + b];
+    auto S = bcsr_from_blocks(c, A.nbr, A.nbc, nfc, nfc, A.h_rowptr.data(), A.h_colidx.data(), v.data(), true);
+    if (!A.h_mask.empty()) {
+        S->h_mask.resize(v.size());
+        for (int64_t p = 0; p < A.nnzb; p++)
+            for (int a = 0; a < nfc; a++)
+                for (int b = 0; b < nfc; b++)
+                    S->h_mask[(size_t)p * nfc * nfc + a * nfc + b] = A.h_mask[(size_t)p * bs * bs + a * bs + b];
+    }
+    return S;
+}
+
 std::vector<double> dense_spd_inverse(const Bcsr& A) {
     FH_CHECK(!A.h_vals.empty(), -3, "coarse solver needs host values");
     const int bs = A.bs_r;

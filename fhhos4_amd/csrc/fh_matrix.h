@@ -44,6 +44,14 @@ int pairwise_aggregate(const Bcsr& A, double theta, std::vector<int>& agg, std::
 // result (structure + FP64 values), device copy uploaded.
 std::unique_ptr<Bcsr> galerkin_aggregate(Ctx& c, const Bcsr& A, const std::vector<int>& agg, int nagg);
 
+// Galerkin operator P^T A P of a block prolongation P (host FP64 values of A
+// and P; Level::ComputeGalerkinOperator, Level.h:133-137).
+std::unique_ptr<Bcsr> galerkin_rap(Ctx& c, const Bcsr& A, const Bcsr& P);
+
+// Galerkin operator of the p-injection keeping the first nfc modes of every
+// block (hierarchical bases): the leading nfc x nfc sub-block of every block.
+std::unique_ptr<Bcsr> bcsr_mode_submatrix(Ctx& c, const Bcsr& A, int nfc);
+
 // Dense inverse of a small SPD matrix given as BCSR (host values), via
 // Cholesky.  Throws on a non-positive pivot.
 std::vector<double> dense_spd_inverse(const Bcsr& A);

@@ -157,6 +157,8 @@ struct MgLevel {
     const Bcsr* Rw = nullptr;
     DevBuf agg_map, agg_ptr, agg_members;
     int nagg = 0;
+    bool ptrunc = false;  // p-transfer keeping the first nfc modes of every block
+    int nfc = 1;
     // work vectors (working precision)
     DVec b, x, r, work, kc, kv, kd, kw;
     int64_t n = 0;
@@ -177,13 +179,18 @@ public:
         if (amg) build_amg();
         else {
             for (size_t l = 1; l < given.size(); l++) {
-                FH_CHECK(given[l].first != nullptr, -2, "missing operator of level " + std::to_string(l));
+                FH_CHECK(a_.galerkin || given[l].first != nullptr, -2, "missing operator of level " + std::to_string(l));
                 L.emplace_back();
                 L[l].A64 = given[l].first;
             }
             for (size_t l = 0; l + 1 < given.size(); l++) {
                 FH_CHECK(given[l].second != nullptr, -2, "missing prolongation of level " + std::to_string(l));
                 L[l].P64 = given[l].second;
+                if (a_.galerkin) {
+                    // coarse operator R A P (Level::ComputeGalerkinOperator, Level.h:133-137)
+                    L[l + 1].A64_own = galerkin_rap(c_, *L[l].A64, *L[l].P64);
+                    L[l + 1].A64 = L[l + 1].A64_own.get();
+                }
                 FH_CHECK(L[l].P64->nbr == L[l].A64->nbr && L[l].P64->nbc == L[l + 1].A64->nbr, -2,
                          "prolongation shape does not match the level operators");
                 L[l].R64_own = bcsr_transpose(c_, *L[l].P64);  // R = P^T (LevelForHHO.h:535-543)
@@ -220,6 +227,18 @@ private:
     Prec vp_;
 
     void build_amg() {
+        // Blocks of hierarchical face bases: the first coarsening keeps the
+        // lowest mode of every face (p-injection, LevelForHHO.h:586-620, as
+        // HighOrderAggregAMG coarsens blocks to their first component,
+        // HighOrderAggregAMG.h:28-45); scalar pairwise aggregation follows.
+        if (L[0].A64->bs_r > 1 && L[0].A64->rows() > a_.max_coarse) {
+            L[0].ptrunc = true;
+            L[0].nfc = 1;
+            MgLevel C;
+            C.A64_own = bcsr_mode_submatrix(c_, *L[0].A64, 1);
+            C.A64 = C.A64_own.get();
+            L.push_back(std::move(C));
+        }
         // AggregAMG with double pairwise aggregation (AggregAMG.h:32-139,
         // Multigrid.h:121-283): coarsen while the level has more than
         // max_coarse unknowns and the coarsening ratio stays above 1.01.
@@ -287,7 +306,7 @@ private:
                 V.post.spec = SmootherSpec::parse(a_.post_smoother, a_.nu2, a_.relax);
                 smoother_setup(c_, *V.A64, V.pre, dprec);
                 smoother_setup(c_, *V.A64, V.post, dprec);
-                if (!V.agg) {
+                if (!V.agg && !V.ptrunc) {
                     Prec tp = vp_ == Prec::F64 ? Prec::F64 : Prec::F32;
                     if (tp != Prec::F64) {
                         V.Pw_own = bcsr_convert(c_, *V.P64, tp);
@@ -312,7 +331,9 @@ private:
 
     void restrict_to(int l, const DVec& r, DVec& rc) {
         MgLevel& V = L[l];
-        if (V.agg)
+        if (V.ptrunc)
+            launch_p_restrict(c_, V.A64->nbr, V.A64->bs_r, V.nfc, vp_, r.data(), rc.data());
+        else if (V.agg)
             launch_agg_restrict(c_, V.nagg, V.A64->bs_r, vp_, V.agg_ptr.as<int>(), V.agg_members.as<int>(), r.data(),
                                 rc.data());
         else
@@ -320,7 +341,9 @@ private:
     }
     void prolong_add(int l, const DVec& ec, DVec& x) {
         MgLevel& V = L[l];
-        if (V.agg)
+        if (V.ptrunc)
+            launch_p_prolong_add(c_, V.A64->nbr, V.A64->bs_r, V.nfc, vp_, ec.data(), x.data());
+        else if (V.agg)
             launch_agg_prolong_add(c_, V.A64->nbr, V.A64->bs_r, vp_, V.agg_map.as<int>(), ec.data(), x.data());
         else
             launch_bcsr(c_, *V.Pw, OP_SPMV, vp_, ec.data(), nullptr, x.data(), 1.0, 1.0, 0.0, nullptr, vp_, nullptr, 0);

@@ -72,6 +72,7 @@ struct SolverArgs {
     double theta = 0.25;             // strong coupling threshold
     Prec mg_prec = Prec::F64;        // working precision of the preconditioner vectors
     Prec mg_store = Prec::F64;       // storage of level matrices (F64 / F32 / BF16)
+    bool galerkin = false;           // geometric multigrid: coarse operators P^T A P (UseGalerkinOperator)
 };
 
 class Solver {

@@ -24,7 +24,7 @@ class SolverArgs(C.Structure):
     _fields_ = [("solver", C.c_char_p), ("prec", C.c_char_p), ("cycle", C.c_char), ("wloops", C.c_int),
                 ("nu1", C.c_int), ("nu2", C.c_int), ("pre_smoother", C.c_char_p), ("post_smoother", C.c_char_p),
                 ("relax", C.c_double), ("tol", C.c_double), ("max_iter", C.c_int), ("max_coarse", C.c_int),
-                ("theta", C.c_double), ("mg_prec", C.c_int), ("mg_store", C.c_int)]
+                ("theta", C.c_double), ("mg_prec", C.c_int), ("mg_store", C.c_int), ("galerkin", C.c_int)]
 
 
 class SolveStats(C.Structure):
@@ -54,6 +54,7 @@ _SIGS = {
     "fh_mat_info": [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int64),
                     C.POINTER(C.c_int)],
     "fh_mat_device_arrays": [_vp, _pp, _pp, _pp],
+    "fh_mat_blocks": [_vp, _i32p, _i32p, _f64p],
     "fh_spmv": [_vp, _vp, C.c_double, _vp, C.c_double, _vp, C.c_int],
     "fh_residual": [_vp, _vp, _vp, _vp, _vp, C.c_int],
     "fh_dot": [_vp, C.c_int64, _vp, _vp, C.c_int, C.POINTER(C.c_double)],
@@ -69,6 +70,7 @@ _SIGS = {
     "fh_mcgs_sweep": [_vp, _vp, _vp, _vp, C.c_int, _vp, _vp, C.c_int],
     "fh_aggregate_pairwise": [_vp, _vp, C.c_double, _i32p, C.POINTER(C.c_int), C.POINTER(C.c_int)],
     "fh_galerkin_aggregate": [_vp, _vp, _i32p, C.c_int, _pp],
+    "fh_galerkin_rap": [_vp, _vp, _vp, _pp],
     "fh_solver_create": [_vp, C.POINTER(SolverArgs), _pp],
     "fh_solver_destroy": [_vp],
     "fh_solver_set_level": [_vp, C.c_int, _vp, _vp],
@@ -76,6 +78,8 @@ _SIGS = {
     "fh_solver_solve": [_vp, _vp, _vp, C.POINTER(SolveStats)],
     "fh_solver_history": [_vp, _f64p, C.c_int],
     "fh_solver_level_size": [_vp, C.c_int],
+    "fh_profile_begin": [_vp, _vp, C.c_int],
+    "fh_profile_end": [_vp, C.POINTER(C.c_int), C.POINTER(C.c_double)],
     "fh_hho_create": [_vp, _pp],
     "fh_hho_destroy": [_vp],
     "fh_hho_get_info": [_vp, _vp],
@@ -83,6 +87,7 @@ _SIGS = {
     "fh_hho_rhs": [_vp, _f64p],
     "fh_hho_blocks": [_vp, _i32p, _i32p, _f64p],
     "fh_hho_l2_error": [_vp, _f64p, C.POINTER(C.c_double)],
+    "fh_hho_prolongation": [_vp, _vp, _vp, C.c_int, _pp],
     "fh_hho_face_cells": [_vp, _i32p],
     "fh_hho_cell_kappa": [_vp, _f64p],
 }
@@ -91,7 +96,7 @@ _SIGS = {
 class HhoArgs(C.Structure):
     _fields_ = [("dim", C.c_int), ("mesh", C.c_char_p), ("n", C.c_int), ("k", C.c_int), ("perturb", C.c_double),
                 ("seed", C.c_uint64), ("source", C.c_char_p), ("kappa", C.c_char_p), ("sigma", C.c_double),
-                ("subdomain", C.c_int), ("aniso", C.c_double)]
+                ("subdomain", C.c_int), ("aniso", C.c_double), ("ref_n", C.c_int)]
 
 
 class HhoInfo(C.Structure):
@@ -196,6 +201,17 @@ class Matrix:
         _check(lib().fh_mat_from_blocks(ctx.h, len(rp) - 1, nbc, bs, rp, ci, v, int(keep_host), C.byref(h)))
         return cls(ctx, h)
 
+    def scipy(self):
+        """Host copy as a scipy CSR matrix (matrix created with keep_host)."""
+        import scipy.sparse as sp
+        i = self.info()
+        rp = np.zeros(i["nbr"] + 1, dtype=np.int32)
+        ci = np.zeros(i["nnzb"], dtype=np.int32)
+        v = np.zeros(i["nnzb"] * i["bs"] * i["bs"])
+        _check(lib().fh_mat_blocks(self.h, rp, ci, v))
+        bs = i["bs"]
+        return sp.bsr_matrix((v.reshape(-1, bs, bs), ci, rp), shape=(i["nbr"] * bs, i["nbc"] * bs)).tocsr()
+
     def convert(self, prec):
         h = C.c_void_p()
         _check(lib().fh_mat_convert(self.ctx.h, self.h, _PREC[prec], C.byref(h)))
@@ -296,11 +312,18 @@ def galerkin_aggregate(ctx, A, agg, nagg):
     return Matrix(ctx, h)
 
 
+def galerkin_rap(ctx, A, P):
+    h = C.c_void_p()
+    _check(lib().fh_galerkin_rap(ctx.h, A.h, P.h, C.byref(h)))
+    return Matrix(ctx, h)
+
+
 class Solver:
     """Solver created from the reference's string codes (SolverFactory)."""
 
     def __init__(self, ctx, solver="cg", prec="mg", cycle="V", nu=(1, 1), smoothers=("bj", "bj"), relax=1.0,
-                 tol=1e-8, max_iter=200, max_coarse=1000, theta=0.25, mg_prec="f64", mg_store=None, wloops=0):
+                 tol=1e-8, max_iter=200, max_coarse=1000, theta=0.25, mg_prec="f64", mg_store=None, wloops=0,
+                 galerkin=False):
         a = SolverArgs()
         self._strs = [s.encode() for s in (solver, prec or "", smoothers[0], smoothers[1])]
         a.solver, a.prec, a.pre_smoother, a.post_smoother = self._strs
@@ -311,6 +334,7 @@ class Solver:
         a.max_coarse, a.theta = max_coarse, theta
         a.mg_prec = _PREC[mg_prec]
         a.mg_store = _PREC[mg_store if mg_store is not None else mg_prec]
+        a.galerkin = int(galerkin)
         h = C.c_void_p()
         _check(lib().fh_solver_create(ctx.h, C.byref(a), C.byref(h)))
         self.ctx, self.h, self.max_iter = ctx, h, max_iter
@@ -348,11 +372,14 @@ class HhoProblem:
     """Condensed HHO face system of -div(K grad u) = f on a structured mesh."""
 
     def __init__(self, dim=3, mesh="stetra", n=8, k=2, perturb=0.0, seed=1, source="pisine", kappa="one",
-                 sigma=0.0, subdomain=8, aniso=1e-2):
+                 sigma=0.0, subdomain=8, aniso=1e-2, ref_n=0):
         a = HhoArgs()
         self._strs = [mesh.encode(), source.encode(), kappa.encode()]
         a.dim, a.mesh, a.n, a.k, a.perturb, a.seed = dim, self._strs[0], n, k, perturb, seed
         a.source, a.kappa, a.sigma, a.subdomain, a.aniso = self._strs[1], self._strs[2], sigma, subdomain, aniso
+        a.ref_n = ref_n
+        self.args = dict(dim=dim, mesh=mesh, n=n, k=k, perturb=perturb, seed=seed, source=source, kappa=kappa,
+                         sigma=sigma, subdomain=subdomain, aniso=aniso, ref_n=ref_n)
         h = C.c_void_p()
         _check(lib().fh_hho_create(C.byref(a), C.byref(h)))
         self.h = h
@@ -389,6 +416,19 @@ class HhoProblem:
         _check(lib().fh_hho_l2_error(self.h, np.ascontiguousarray(x, dtype=np.float64), C.byref(e)))
         return e.value
 
+    def prolongation(self, ctx, coarse, keep_host=True):
+        """HHO prolongation from `coarse` (mesh n/2) to this problem."""
+        h = C.c_void_p()
+        _check(lib().fh_hho_prolongation(ctx.h, self.h, coarse.h, int(keep_host), C.byref(h)))
+        return Matrix(ctx, h)
+
+    def coarsened(self):
+        """The same problem on the once-coarsened mesh (same finest grid)."""
+        kw = dict(self.args)
+        kw["ref_n"] = kw["ref_n"] or kw["n"]
+        kw["n"] = kw["n"] // 2
+        return HhoProblem(**kw)
+
     def face_cells(self):
         out = np.zeros(2 * self.nb, dtype=np.int32)
         _check(lib().fh_hho_face_cells(self.h, out))
@@ -404,3 +444,14 @@ class HhoProblem:
             lib().fh_hho_destroy(self.h)
         except Exception:
             pass
+
+
+def profile_begin(ctx, A, mode=0):
+    """Time every launch of `mode` (0 spmv, 1 residual, 2 smoothing) on A."""
+    _check(lib().fh_profile_begin(ctx.h, A.h, mode))
+
+
+def profile_end(ctx):
+    n, ms = C.c_int(), C.c_double()
+    _check(lib().fh_profile_end(ctx.h, C.byref(n), C.byref(ms)))
+    return n.value, ms.value

@@ -51,6 +51,9 @@ int fh_mat_from_blocks(fh_ctx* ctx, int nbr, int nbc, int bs, const int* rowptr,
 int fh_mat_convert(fh_ctx* ctx, const fh_mat* A, int prec, fh_mat** out);
 int fh_mat_destroy(fh_mat* A);
 int fh_mat_info(const fh_mat* A, int* nbr, int* nbc, int* bs, int64_t* nnzb, int* prec);
+/* Host copy of the block arrays of a matrix created with keep_host (rowptr:
+ * nbr+1, colidx: nnzb, vals: nnzb*bs*bs FP64); NULL pointers are skipped. */
+int fh_mat_blocks(const fh_mat* A, int* rowptr, int* colidx, double* vals);
 /* Device pointers of the block structure and values (for profiling tools). */
 int fh_mat_device_arrays(const fh_mat* A, const int** rowptr, const int** colidx, const void** vals);
 
@@ -69,6 +72,14 @@ int fh_dot(fh_ctx* ctx, int64_t n, const void* x, const void* y, int vprec, doub
 int fh_axpby(fh_ctx* ctx, int64_t n, double a, const void* x, double b, void* y, int vprec);
 /* Precision conversion of a vector. */
 int fh_convert(fh_ctx* ctx, int64_t n, const void* src, int sprec, void* dst, int dprec);
+
+/* In-situ timing of the block-row kernel: every launch of operation `mode`
+ * (0 spmv, 1 residual, 2 smoothing) on matrix A is bracketed by HIP events on
+ * the context's stream until fh_profile_end, which returns the number of
+ * launches and their mean duration in ms.  Replaces the reference's Timer
+ * accounting of matrix-vector products (src/Utils/Timer.h:60-123). */
+int fh_profile_begin(fh_ctx* ctx, const fh_mat* A, int mode);
+int fh_profile_end(fh_ctx* ctx, int* launches, double* mean_ms);
 
 /* ------------------------------------------------------------ smoothers */
 /* Block Jacobi set-up: inverse of every diagonal block, stored in dprec
@@ -104,6 +115,10 @@ int fh_aggregate_pairwise(fh_ctx* ctx, const fh_mat* A, double theta, int* agg_o
  * (AggregLevel::GalerkinOperator, AggregAMG.h:156-180, extended to blocks). */
 int fh_galerkin_aggregate(fh_ctx* ctx, const fh_mat* A, const int* agg, int nagg, fh_mat** out);
 
+/* Galerkin operator P^T A P of a block prolongation (host; matrices with
+ * keep_host).  Level::ComputeGalerkinOperator (src/Solver/Multigrid/Level.h:133-137). */
+int fh_galerkin_rap(fh_ctx* ctx, const fh_mat* A, const fh_mat* P, fh_mat** out);
+
 /* ------------------------------------------------------------ solvers */
 typedef struct fh_solver_args {
     const char* solver;        /* "cg" | "fcg" | "mg" | "aggregamg" | "bj" | "bgs"  (SolverFactory.h:126-233) */
@@ -120,6 +135,7 @@ typedef struct fh_solver_args {
     double theta;              /* strong coupling threshold (SolverFactory.h:95) */
     int mg_prec;               /* working precision of the multigrid vectors: FH_F64 | FH_F32 */
     int mg_store;              /* storage of the level matrices: FH_F64 | FH_F32 | FH_BF16 */
+    int galerkin;              /* geometric multigrid: coarse operators R A P (-g 1, Multigrid.h:21) */
 } fh_solver_args;
 
 typedef struct fh_solve_stats {
@@ -170,8 +186,9 @@ typedef struct fh_hho_args {
     const char* source;     /* "pisine" | "sine" (DiffusionTestCase.h:62-131) */
     const char* kappa;      /* "one" | "lognormal" | "subdomain" */
     double sigma;           /* lognormal standard deviation */
-    int subdomain;          /* cells per subdomain edge */
+    int subdomain;          /* cells per subdomain edge (on the reference grid) */
     double aniso;           /* z-scaling of the anisotropic subdomains */
+    int ref_n;              /* finest grid of a hierarchy (0: n); perturbation and subdomains live on it */
 } fh_hho_args;
 typedef struct fh_hho_info {
     int nb, nf, ncells, nfaces;
@@ -189,6 +206,11 @@ int fh_hho_blocks(const fh_hho* h, int* rowptr, int* colidx, double* vals);
 /* Relative L2 error of the k+1 reconstruction of the face solution x (host)
  * against the exact solution (Diffusion_HHO::L2Error, Diffusion_HHO.h:120-149). */
 int fh_hho_l2_error(const fh_hho* h, const double* x, double* err);
+/* Geometric multigrid prolongation from the problem on the once-coarsened
+ * mesh (coarse->n = fine->n / 2, same ref_n) to the fine one: nf x nf block
+ * matrix (fine faces x coarse faces).  Replaces LevelForHHO::SetupProlongation_H
+ * with CellInterp_Trace (src/Solver/Multigrid/MultigridForHHO/LevelForHHO.h:182-213). */
+int fh_hho_prolongation(fh_ctx* ctx, const fh_hho* fine, const fh_hho* coarse, int keep_host, fh_mat** out);
 /* Cells of every interior face (2 ints per block row) and cell kappa. */
 int fh_hho_face_cells(const fh_hho* h, int* out);
 int fh_hho_cell_kappa(const fh_hho* h, double* out);

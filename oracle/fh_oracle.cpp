@@ -327,9 +327,12 @@ int greedy_colour_blocks(const Csr& A, int nf, std::vector<int>& cptr, std::vect
 
 struct Level {
     Csr A;
+    int nf = 1;          // block size of this level
     Csr P, R;            // GMG transfer (level -> coarser)
     std::vector<int> agg; int nagg = 0;  // AMG transfer
     bool amg = false;
+    bool ptrunc = false; // p-transfer: keep the first nfc modes of every block (hierarchical bases)
+    int nfc = 1;
     std::vector<double> invD;
     std::vector<int> cptr, crows;   // multicolour ordering (greedy, block-row order)
 };
@@ -338,9 +341,9 @@ struct MG {
     std::vector<Level> lv;
     Dense coarse;
     const or_solver_cfg* cfg = nullptr;
-    int nf = 1;
 
     void smooth(Level& L, int kind, double omega, int nu, const double* b, double* x) {
+        const int nf = L.nf;
         for (int s = 0; s < nu; s++) {
             if (kind == SM_BJ) bj_sweep(L.A, nf, L.invD.data(), omega, b, x);
             else if (kind >= SM_MC_F) {
@@ -355,6 +358,15 @@ struct MG {
     }
     void restrict_(int l, const std::vector<double>& r, std::vector<double>& rc) {
         Level& L = lv[l];
+        const int nf = L.nf;
+        if (L.ptrunc) {
+            // p-restriction = truncation to the first nfc modes (LevelForHHO.h:586-620)
+            int nb = L.A.n / nf;
+            rc.assign((size_t)nb * L.nfc, 0.0);
+            for (int i = 0; i < nb; i++)
+                for (int a = 0; a < L.nfc; a++) rc[(size_t)i * L.nfc + a] = r[(size_t)i * nf + a];
+            return;
+        }
         if (!L.amg) {
             rc.assign(L.R.n, 0.0);
             spmv(L.R, r.data(), rc.data());
@@ -369,6 +381,14 @@ struct MG {
     }
     void prolong_add(int l, const std::vector<double>& ec, double* x) {
         Level& L = lv[l];
+        const int nf = L.nf;
+        if (L.ptrunc) {
+            // p-prolongation = injection of the low modes (LevelForHHO.h:586-620)
+            int nb = L.A.n / nf;
+            for (int i = 0; i < nb; i++)
+                for (int a = 0; a < L.nfc; a++) x[(size_t)i * nf + a] += ec[(size_t)i * L.nfc + a];
+            return;
+        }
         if (!L.amg) {
             std::vector<double> t(L.P.n);
             spmv(L.P, ec.data(), t.data());
@@ -466,10 +486,35 @@ int greedy_colour_blocks(const Csr& A, int nf, std::vector<int>& cptr, std::vect
     return nc;
 }
 
-int build_amg(MG& mg, const Csr& A0, int nf, double theta, int max_coarse, int max_levels) {
+// Aggregation hierarchy.  For blocks (nf > 1) with pfirst, the first
+// coarsening is a p-coarsening to the lowest mode of every block
+// (hierarchical face bases; HighOrderAggregAMG, HighOrderAggregAMG.h:28-45,
+// with unsmoothed injection) and scalar pairwise aggregation follows.
+int build_amg(MG& mg, const Csr& A0, int nf, double theta, int max_coarse, int max_levels, bool pfirst) {
     mg.lv.clear();
     mg.lv.emplace_back();
     mg.lv[0].A = A0;
+    mg.lv[0].nf = nf;
+    if (nf > 1 && pfirst && A0.n > max_coarse) {
+        Level& L = mg.lv[0];
+        L.ptrunc = true;
+        L.nfc = 1;
+        Level C;
+        int nb = A0.n / nf;
+        C.A.n = C.A.m = nb;
+        C.A.rp.assign(nb + 1, 0);
+        for (int i = 0; i < nb; i++) {
+            int row = i * nf;
+            for (int p = A0.prp[row]; p < A0.prp[row + 1]; p++)
+                if (A0.pci[p] % nf == 0) { C.A.ci.push_back(A0.pci[p] / nf); C.A.v.push_back(A0.pv[p]); }
+            C.A.rp[i + 1] = (int)C.A.ci.size();
+        }
+        C.A.own();
+        C.nf = 1;
+        mg.lv.push_back(std::move(C));
+        for (auto& q : mg.lv) if (!q.A.rp.empty()) q.A.own();
+        nf = 1;
+    }
     while ((int)mg.lv.size() < max_levels && mg.lv.back().A.n > max_coarse) {
         Level& L = mg.lv.back();
         // DoublePairwiseAggregation (AggregAMG.h:40-100)
@@ -494,6 +539,7 @@ int build_amg(MG& mg, const Csr& A0, int nf, double theta, int max_coarse, int m
         L.nagg = n2;
         Level C;
         galerkin(A1, nf, a2, n2, C.A);
+        C.nf = nf;
         mg.lv.push_back(std::move(C));
         // re-own pointers after the move of the vector storage
         for (auto& q : mg.lv) if (!q.A.rp.empty()) q.A.own();
@@ -536,7 +582,9 @@ void or_mcbgs_sweep(int n, int nf, const int* rp, const int* ci, const double* v
     Csr A; A.view(n, n, rp, ci, v);
     for (int cc = 0; cc < ncol; cc++) {
         int c = dir == 0 ? cc : ncol - 1 - cc;
-        std::vector<double> xs(x, x + n);  // values of the previous colours
+        // rows of one colour are not coupled: their updates read only the
+        // values of the other colours and may run in any order
+#pragma omp parallel for schedule(static)
         for (int q = colour_ptr[c]; q < colour_ptr[c + 1]; q++) {
             int i = colour_rows[q];
             double tmp[64];
@@ -544,7 +592,7 @@ void or_mcbgs_sweep(int n, int nf, const int* rp, const int* ci, const double* v
                 int row = i * nf + k;
                 double t = b[row];
                 for (int p = rp[row]; p < rp[row + 1]; p++)
-                    if (ci[p] / nf != i) t -= v[p] * xs[ci[p]];
+                    if (ci[p] / nf != i) t -= v[p] * x[ci[p]];
                 tmp[k] = t;
             }
             const double* Di = invD + (size_t)i * nf * nf;
@@ -590,7 +638,7 @@ int or_amg_hierarchy(int n, int nf, const int* rp, const int* ci, const double* 
                      int max_coarse, int* agg0, int* level_n, int max_levels) {
     Csr A; A.view(n, n, rp, ci, v);
     MG mg;
-    int nl = build_amg(mg, A, nf, theta, max_coarse, max_levels);
+    int nl = build_amg(mg, A, nf, theta, max_coarse, max_levels, true);
     for (int l = 0; l < nl; l++) level_n[l] = mg.lv[l].A.n;
     if (nl > 1) std::memcpy(agg0, mg.lv[0].agg.data(), mg.lv[0].agg.size() * sizeof(int));
     return nl;
@@ -609,7 +657,6 @@ int or_solve(const or_solver_cfg* cfg, const or_level_desc* levels, const double
     const int n = A.n;
     MG mg;
     mg.cfg = cfg;
-    mg.nf = nf;
     std::vector<double> invD;
     if (cfg->prec == 1 || cfg->prec == 2) {
         if (block_inverse(A, nf, invD)) return -2;
@@ -618,6 +665,7 @@ int or_solve(const or_solver_cfg* cfg, const or_level_desc* levels, const double
         for (int l = 0; l < cfg->nlevels; l++) {
             const or_level_desc& d = levels[l];
             mg.lv[l].A.view(d.n, d.n, d.rp, d.ci, d.v);
+            mg.lv[l].nf = nf;
             if (l + 1 < cfg->nlevels) {
                 Level& L = mg.lv[l];
                 L.P.view(d.n, d.nc, d.prp, d.pci, d.pv);
@@ -637,11 +685,11 @@ int or_solve(const or_solver_cfg* cfg, const or_level_desc* levels, const double
             }
         }
     } else if (cfg->prec == 4) {
-        build_amg(mg, A, nf, cfg->theta, cfg->max_coarse, 32);
+        build_amg(mg, A, nf, cfg->theta, cfg->max_coarse, 32, true);
     }
     if (cfg->prec >= 3) {
         for (size_t l = 0; l + 1 < mg.lv.size(); l++)
-            if (block_inverse(mg.lv[l].A, nf, mg.lv[l].invD)) return -2;
+            if (block_inverse(mg.lv[l].A, mg.lv[l].nf, mg.lv[l].invD)) return -2;
         if (!mg.coarse.factor(mg.lv.back().A)) return -3;
         out->nlevels = (int)mg.lv.size();
         for (size_t l = 0; l < mg.lv.size() && l < 32; l++) out->level_n[l] = mg.lv[l].A.n;
