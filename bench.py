@@ -109,9 +109,9 @@ def main():
     ap.add_argument("--seed", type=int, default=2026)
     ap.add_argument("--prec", default="gmg", choices=["gmg", "amg"])
     ap.add_argument("--no-galerkin", action="store_true")
-    ap.add_argument("--smoother", default="bj")
-    ap.add_argument("--relax", type=float, default=0.5)
-    ap.add_argument("--nu", type=int, default=2)
+    ap.add_argument("--smoother", default="sbgs")
+    ap.add_argument("--relax", type=float, default=1.0)
+    ap.add_argument("--nu", type=int, default=1)
     ap.add_argument("--cycle", default="V")
     ap.add_argument("--max-iter", type=int, default=1000)
     ap.add_argument("--cpu-n", type=int, default=16)

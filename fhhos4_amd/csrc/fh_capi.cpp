@@ -3,6 +3,7 @@
 #include <string>
 
 #include "../../include/fhgpu.h"
+#include "fh_io.h"
 #include "fh_kernels.h"
 #include "fh_matrix.h"
 #include "fh_solver.h"
@@ -131,6 +132,48 @@ int fh_mat_device_arrays(const fh_mat* A, const int** rowptr, const int** colidx
         if (rowptr) *rowptr = A->A->rowptr.as<int>();
         if (colidx) *colidx = A->A->colidx.as<int>();
         if (vals) *vals = A->A->vals.p;
+    });
+}
+
+int fh_mat_export_market(fh_ctx* ctx, const fh_mat* A, const char* path) {
+    return guard([&] {
+        NEED(ctx); NEED(A); NEED(path);
+        fh::market_write_matrix(ctx->c, *A->A, path);
+    });
+}
+int fh_mat_import_market(fh_ctx* ctx, const char* path, int bs, int keep_host, fh_mat** out) {
+    return guard([&] {
+        NEED(ctx); NEED(path); NEED(out);
+        int n = 0, m = 0;
+        std::vector<int> rp, ci;
+        std::vector<double> v;
+        fh::market_read_csr(path, n, m, rp, ci, v);
+        auto mat = new fh_mat;
+        try {
+            mat->A = fh::bcsr_from_csr(ctx->c, n, m, bs, rp.data(), ci.data(), v.data(), keep_host != 0);
+        } catch (...) {
+            delete mat;
+            throw;
+        }
+        *out = mat;
+    });
+}
+int fh_vec_export_market(int64_t n, const double* v, const char* path) {
+    return guard([&] {
+        NEED(v); NEED(path);
+        FH_CHECK(n > 0, -2, "vector length must be positive");
+        fh::market_write_vector(n, v, path);
+    });
+}
+int fh_vec_import_market(const char* path, int64_t cap, double* v, int64_t* n) {
+    return guard([&] {
+        NEED(path); NEED(n);
+        std::vector<double> x = fh::market_read_vector(path);
+        *n = (int64_t)x.size();
+        if (v) {
+            FH_CHECK(cap >= (int64_t)x.size(), -2, "output buffer too small");
+            std::memcpy(v, x.data(), sizeof(double) * x.size());
+        }
     });
 }
 

@@ -55,6 +55,10 @@ _SIGS = {
                     C.POINTER(C.c_int)],
     "fh_mat_device_arrays": [_vp, _pp, _pp, _pp],
     "fh_mat_blocks": [_vp, _i32p, _i32p, _f64p],
+    "fh_mat_export_market": [_vp, _vp, C.c_char_p],
+    "fh_mat_import_market": [_vp, C.c_char_p, C.c_int, C.c_int, _pp],
+    "fh_vec_export_market": [C.c_int64, _f64p, C.c_char_p],
+    "fh_vec_import_market": [C.c_char_p, C.c_int64, _vp, C.POINTER(C.c_int64)],
     "fh_spmv": [_vp, _vp, C.c_double, _vp, C.c_double, _vp, C.c_int],
     "fh_residual": [_vp, _vp, _vp, _vp, _vp, C.c_int],
     "fh_dot": [_vp, C.c_int64, _vp, _vp, C.c_int, C.POINTER(C.c_double)],
@@ -201,6 +205,17 @@ class Matrix:
         _check(lib().fh_mat_from_blocks(ctx.h, len(rp) - 1, nbc, bs, rp, ci, v, int(keep_host), C.byref(h)))
         return cls(ctx, h)
 
+    @classmethod
+    def read_market(cls, ctx, path, bs, keep_host=True):
+        """Matrix Market file (the reference's A.dat format) -> block matrix."""
+        h = C.c_void_p()
+        _check(lib().fh_mat_import_market(ctx.h, os.fsencode(path), bs, int(keep_host), C.byref(h)))
+        return cls(ctx, h)
+
+    def write_market(self, path):
+        """Writes the scalar operator in the reference's A.dat format."""
+        _check(lib().fh_mat_export_market(self.ctx.h, self.h, os.fsencode(path)))
+
     def scipy(self):
         """Host copy as a scipy CSR matrix (matrix created with keep_host)."""
         import scipy.sparse as sp
@@ -233,6 +248,20 @@ class Matrix:
             self.close()
         except Exception:
             pass
+
+
+def write_market_vector(v, path):
+    """Host vector -> the reference's b.dat / solutionFaces.dat format."""
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    _check(lib().fh_vec_export_market(v.size, v, os.fsencode(path)))
+
+
+def read_market_vector(path):
+    n = C.c_int64()
+    _check(lib().fh_vec_import_market(os.fsencode(path), 0, None, C.byref(n)))
+    v = np.zeros(n.value)
+    _check(lib().fh_vec_import_market(os.fsencode(path), v.size, v.ctypes.data, C.byref(n)))
+    return v
 
 
 def spmv(ctx, A, x, y, alpha=1.0, beta=0.0):

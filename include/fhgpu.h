@@ -57,6 +57,24 @@ int fh_mat_blocks(const fh_mat* A, int* rowptr, int* colidx, double* vals);
 /* Device pointers of the block structure and values (for profiling tools). */
 int fh_mat_device_arrays(const fh_mat* A, const int** rowptr, const int** colidx, const void** vals);
 
+/* -------------------------------------------- exported linear system */
+/* Writes the scalar operator as Matrix Market text, byte-compatible with the
+ * reference's export of A.dat / level<l>_A.dat: ExportModule::ExportMatrix
+ * (src/Utils/ExportModule.h:68-73) -> Eigen::saveMarket (header
+ * "%%MatrixMarket matrix coordinate  real general", "rows cols nnz", 1-based
+ * "i j v" in row-major order, "%.64e" values).  Stored entries: the scalar
+ * pattern the matrix was created from (keep_host) or every non-zero entry. */
+int fh_mat_export_market(fh_ctx* ctx, const fh_mat* A, const char* path);
+/* Reads such a file (Eigen::loadMarket semantics: duplicates summed) into a
+ * block matrix with bs x bs blocks (rows and columns multiples of bs). */
+int fh_mat_import_market(fh_ctx* ctx, const char* path, int bs, int keep_host, fh_mat** out);
+/* Host vector <-> "%%MatrixMarket matrix array real general" text, as
+ * ExportModule::ExportVector (ExportModule.h:75-80, Eigen::saveMarketVector)
+ * writes b.dat and solutionFaces.dat.  Import: *n receives the length; v
+ * (capacity cap) is filled when not NULL. */
+int fh_vec_export_market(int64_t n, const double* v, const char* path);
+int fh_vec_import_market(const char* path, int64_t cap, double* v, int64_t* n);
+
 /* ------------------------------------------------------------ kernels */
 /* y = alpha A x + beta y.  Replaces `A * x` of Eigen's SparseMatrix in the
  * solvers (src/Solver/Krylov/ConjugateGradient.h:70,

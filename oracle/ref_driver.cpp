@@ -95,6 +95,7 @@ struct Opts
 	int maxIter = 200;
 	string out = "./out";
 	bool exportMG = false;
+	bool exportDat = false;
 	bool aggregOnly = false;
 	int aggregBlock = -1;
 	bool solve = true;
@@ -182,6 +183,14 @@ static int Run(const Opts& o)
 	auto t1 = chrono::steady_clock::now();
 	write_csr(o.out + "/A.csr", problem->A);
 	write_vec(o.out + "/b.vec", problem->b);
+	if (o.exportDat)
+	{
+		// The reference's own Matrix Market export of the linear system
+		// (Diffusion_HHO.h:682-687 -> ExportModule::ExportMatrix/ExportVector).
+		ExportModule dat(o.out);
+		dat.ExportMatrix(problem->A, "A");
+		dat.ExportVector(problem->b, "b");
+	}
 
 	FILE* info = fopen((o.out + "/info.txt").c_str(), "w");
 	fprintf(info, "rows %ld\nnnz %ld\nnf %d\nassemble_ms %.3f\n", (long)problem->A.rows(), (long)problem->A.nonZeros(), faceBasis->Size(),
@@ -224,6 +233,7 @@ static int Run(const Opts& o)
 	Vector x = it->Solve(problem->b, "0");
 	auto t4 = chrono::steady_clock::now();
 	write_vec(o.out + "/x.vec", x);
+	if (o.exportDat) out.ExportVector(x, "solutionFaces");
 	Vector h(hist.size());
 	for (size_t i = 0; i < hist.size(); i++) h[i] = hist[i];
 	write_vec(o.out + "/hist.vec", h);
@@ -265,6 +275,7 @@ int main(int argc, char** argv)
 		else if (a == "-max-iter") o.maxIter = stoi(next());
 		else if (a == "-out") o.out = next();
 		else if (a == "-export-mg") o.exportMG = true;
+		else if (a == "-export-dat") o.exportDat = true;
 		else if (a == "-aggreg") o.aggregOnly = true;
 		else if (a == "-aggreg-block") o.aggregBlock = stoi(next());
 		else if (a == "-nosolve") o.solve = false;

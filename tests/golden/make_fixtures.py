@@ -13,7 +13,11 @@ repository compiled against the reference sources in place, see
 * ``agg1``     : first-pass pairwise aggregate of every block row
                  (AlgebraicMesh.h:54-125, PairwiseAggregation.h:9-54).
 
-Run: ``python tests/golden/make_fixtures.py`` (needs the reference tree).
+* ``dat_*``    : the reference's Matrix Market export (``A.dat``, ``b.dat``,
+                 ``solutionFaces.dat``; ExportModule.h:68-87) as raw bytes.
+
+Run: ``python tests/golden/make_fixtures.py [name ...]`` (needs the reference
+tree; with names, only those fixtures are regenerated).
 """
 import os
 import subprocess
@@ -27,6 +31,7 @@ import scipy.sparse as sp
 HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(os.path.dirname(HERE))
 REFDRV = os.path.join(REPO, "oracle", "_ref", "refdrv")
+ONLY = set(sys.argv[1:])  # optional fixture names to (re)generate
 
 
 def read_csr(path):
@@ -61,11 +66,15 @@ def pack_csr(prefix, M, d):
     d[prefix + "_shape"] = np.array(M.shape, dtype=np.int64)
 
 
-def run(name, args, export_mg=False, keep_A=True, aggreg=None):
+def run(name, args, export_mg=False, keep_A=True, aggreg=None, export_dat=False):
+    if ONLY and name not in ONLY:
+        return
     with tempfile.TemporaryDirectory() as tmp:
         cmd = [REFDRV] + args + ["-out", tmp]
         if export_mg:
             cmd.append("-export-mg")
+        if export_dat:
+            cmd.append("-export-dat")
         subprocess.run(cmd, check=True, stdout=subprocess.DEVNULL)
         info = read_info(os.path.join(tmp, "info.txt"))
         d = {"cmd": np.array(" ".join(["refdrv"] + args))}
@@ -77,6 +86,11 @@ def run(name, args, export_mg=False, keep_A=True, aggreg=None):
             d["hist"] = read_vec(os.path.join(tmp, "hist.vec"))
         for k, v in info.items():
             d["info_" + k] = np.array(v)
+        if export_dat:
+            # the reference's Matrix Market files, byte for byte
+            for f in ("A", "b", "solutionFaces"):
+                with open(os.path.join(tmp, f + ".dat"), "rb") as fh:
+                    d["dat_" + f] = np.frombuffer(fh.read(), dtype=np.uint8)
         if export_mg:
             lvl = 0
             while os.path.exists(os.path.join(tmp, "level%d_A.dat" % lvl)):
@@ -128,6 +142,10 @@ def main():
     # 3-D Cartesian k=2 with GMG V(1,1) bj23
     run("cart8_k2_cg_mg_bj23", ["-dim", "3", "-mesh", "cart", "-k", "2", "-n", "8", "-s", "cg", "-prec", "mg",
                                 "-cycle", "V,1,1", "-smoothers", "bj23,bj23"], export_mg=True)
+    # Exported linear systems (Matrix Market text) of tiny 2-D and 3-D problems
+    run("mm_c1_n4", c1s + ["-n", "4", "-s", "cg", "-prec", "mg", "-smoothers", "bj,bj"], export_dat=True)
+    run("mm_tet2_k1", ["-dim", "3", "-mesh", "stetra", "-k", "1", "-n", "2", "-source", "sine", "-s", "cg",
+                       "-prec", "bj"], export_dat=True)
 
 
 if __name__ == "__main__":
