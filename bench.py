@@ -98,6 +98,52 @@ def cpu_baseline(args, G, O, v):
             "dof_it_per_s": N * out.iterations / solve_s, "n": n}
 
 
+def kernel_bench(args, G, np, torch):
+    """Block-row SpMV of the fine operator alone (FP64, FP32, BF16-stored),
+    timed with HIP events on the library stream, plus a device copy of known
+    size (bandwidth reference and calibration of the HBM byte counters)."""
+    P = G.HhoProblem(dim=3, mesh="stetra", n=args.n, k=args.k, perturb=args.perturb, seed=args.seed,
+                     source="pisine")
+    nb, nf, nnzb = P.nb, P.nf, P.nnzb
+    N = nb * nf
+    ctx = G.Context(0)
+    A64 = P.matrix(ctx, keep_host=False)
+    del P
+    out = {"kernel": "bcsr_spmv", "n": args.n, "unknowns": N, "blocks": nnzb, "reps": args.reps}
+    x = torch.rand(N, dtype=torch.float64, device="cuda")
+    for name, A, xv, sv, sx in (("f64", A64, x, 8, 8), ("f32", A64.convert("f32"), x.float(), 4, 4),
+                                ("bf16", A64.convert("bf16"), x.float(), 2, 4)):
+        y = torch.zeros_like(xv)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            G.spmv(ctx, A, xv, y)
+        ctx.sync()
+        G.profile_begin(ctx, A, 0)
+        for _ in range(args.reps):
+            G.spmv(ctx, A, xv, y)
+        nl, ms = G.profile_end(ctx)
+        byt = spmv_bytes(nb, nf, nnzb, sv, sx)
+        out[name] = {"mean_ms": ms, "launches": nl, "bytes": byt, "GBs": byt / (ms * 1e-3) / 1e9}
+        log("SpMV %s: %.3f ms, %.0f GB/s" % (name, ms, out[name]["GBs"]))
+        if A is not A64:
+            A.close()
+    # device-to-device copy of 4 GiB on torch's stream (the events below are on that stream)
+    src = torch.empty(1 << 29, dtype=torch.float64, device="cuda").fill_(1.0)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(args.reps):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.reps
+    cb = 2 * src.numel() * 8
+    out["copy"] = {"bytes": cb, "mean_ms": ms, "GBs": cb / (ms * 1e-3) / 1e9}
+    log("copy 2 x %d MiB: %.3f ms, %.0f GB/s" % (src.numel() * 8 >> 20, ms, out["copy"]["GBs"]))
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -118,6 +164,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--sweep", action="store_true", help="time a list of solver variants")
+    ap.add_argument("--kernel", choices=["spmv"], help="time one kernel family alone")
+    ap.add_argument("--reps", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -131,6 +179,8 @@ def main():
 
     G = F.fhgpu
     assert torch.cuda.is_available(), "no ROCm device"
+    if args.kernel == "spmv":
+        return kernel_bench(args, G, np, torch)
     log("assembling HHO system n=%d k=%d (perturb %.2f)" % (args.n, args.k, args.perturb))
     P = G.HhoProblem(dim=3, mesh="stetra", n=args.n, k=args.k, perturb=args.perturb, seed=args.seed,
                      source="pisine")

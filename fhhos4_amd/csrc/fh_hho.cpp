@@ -379,6 +379,51 @@ void HhoProblem::Impl::build_mesh() {
                         }
                     }
                 }
+    } else if (a.mesh == "stri") {
+        // Square_TriangularMesh::Build (src/Mesher/InHouse/Square_TriangularMesh.h:121-239):
+        // square (ix, iy) -> lower triangle 2q = (v00, v10, v01) and upper
+        // triangle 2q+1 = (v01, v10, v11); interior edges square by square:
+        // the oblique edge, then the east and the north interface.
+        FH_CHECK(dim == 2, -2, "stri meshes are 2-D");
+        kind = 0; nvc = 3; nfc = 3; nvf = 2;
+        ncell = 2 * n * n;
+        cv.resize((size_t)ncell * 3);
+        cf.assign((size_t)ncell * 3, -1);
+        std::vector<int> nadded(ncell, 0);
+        auto lower = [&](int ix, int iy) { return 2 * (iy * n + ix); };
+        for (int iy = 0; iy < n; iy++)
+            for (int ix = 0; ix < n; ix++) {
+                int c = lower(ix, iy);
+                int v00 = vid(ix, iy, 0), v10 = vid(ix + 1, iy, 0), v01 = vid(ix, iy + 1, 0), v11 = vid(ix + 1, iy + 1, 0);
+                int lv[3] = {v00, v10, v01}, uv[3] = {v01, v10, v11};
+                for (int q = 0; q < 3; q++) { cv[(size_t)c * 3 + q] = lv[q]; cv[(size_t)(c + 1) * 3 + q] = uv[q]; }
+            }
+        auto add_face = [&](int c0, int c1, int va, int vb) {
+            int id = (int)(fcells.size() / 2);
+            fv.push_back(va); fv.push_back(vb);
+            fcells.push_back(c0); fcells.push_back(c1);
+            for (int cc : {c0, c1}) {
+                if (cc < 0) continue;
+                FH_CHECK(nadded[cc] < 3, -2, "triangle with more than 3 faces");
+                cf[(size_t)cc * 3 + nadded[cc]++] = id;
+            }
+            if (c1 >= 0) interior_list.push_back(id);
+        };
+        for (int ix = 0; ix < n; ix++) {
+            add_face(lower(ix, 0), -1, vid(ix, 0, 0), vid(ix + 1, 0, 0));            // south
+            add_face(lower(ix, n - 1) + 1, -1, vid(ix, n, 0), vid(ix + 1, n, 0));    // north
+        }
+        for (int iy = 0; iy < n; iy++) {
+            add_face(lower(0, iy), -1, vid(0, iy + 1, 0), vid(0, iy, 0));            // west
+            add_face(lower(n - 1, iy) + 1, -1, vid(n, iy, 0), vid(n, iy + 1, 0));    // east
+        }
+        for (int iy = 0; iy < n; iy++)
+            for (int ix = 0; ix < n; ix++) {
+                int lo = lower(ix, iy), up = lo + 1;
+                add_face(lo, up, vid(ix, iy + 1, 0), vid(ix + 1, iy, 0));            // oblique
+                if (ix != n - 1) add_face(up, lower(ix + 1, iy), vid(ix + 1, iy + 1, 0), vid(ix + 1, iy, 0));
+                if (iy != n - 1) add_face(up, lower(ix, iy + 1), vid(ix, iy + 1, 0), vid(ix + 1, iy + 1, 0));
+            }
     } else if (a.mesh == "cart") {
         FH_CHECK(a.perturb == 0, -2, "perturbed Cartesian cells are not affine; use stetra");
         kind = 1;
@@ -430,7 +475,7 @@ void HhoProblem::Impl::build_mesh() {
                 if (fcells[2 * id + 1] >= 0) interior_list.push_back(id);
             }
     } else
-        throw Error(-2, "unknown mesh '" + a.mesh + "' (stetra | cart)");
+        throw Error(-2, "unknown mesh '" + a.mesh + "' (stetra | stri | cart)");
     nface = (int)(fcells.size() / 2);
     funk.assign(nface, -1);
     for (size_t i = 0; i < interior_list.size(); i++) funk[interior_list[i]] = (int)i;
@@ -453,7 +498,7 @@ void HhoProblem::Impl::build_mesh() {
         if (a.kappa == "lognormal") kap = std::exp(a.sigma * N01(krng));
         else if (a.kappa == "subdomain") {
             // background cell of the cell (tets: 6 per cube)
-            int bc = kind == 0 ? c / 6 : c;
+            int bc = kind == 0 ? c / (dim == 3 ? 6 : 2) : c;
             int ix = (bc % n) * ratio, iy = ((bc / n) % n) * ratio, iz = (dim == 3 ? bc / (n * n) : 0) * ratio;
             int sd = std::max(1, a.subdomain);
             int sx = std::min(ix / sd, nsub - 1), sy = std::min(iy / sd, nsub - 1), sz = std::min(iz / sd, nsub - 1);
@@ -871,7 +916,18 @@ void HhoProblem::Prolongation(const HhoProblem& coarse, std::vector<int>& prp, s
     // fine cell -> coarse cell
     std::vector<int> f2c(Fm.ncell);
     int tet_table[8][6] = {};
-    if (Fm.kind == 0) {
+    if (Fm.kind == 0 && dim == 2) {
+        // Nested triangles of the square split: lower (v00, v10, v01) lies
+        // below the anti-diagonal x + y = 1 of its square, upper above it; a
+        // fine triangle belongs to the coarse triangle containing its centroid.
+        for (int o = 0; o < 4; o++)
+            for (int t = 0; t < 2; t++) {
+                // centroid of the fine triangle in the coarse square [0,2]^2
+                double gx = (o & 1) + (t == 0 ? 1.0 / 3.0 : 2.0 / 3.0);
+                double gy = ((o >> 1) & 1) + (t == 0 ? 1.0 / 3.0 : 2.0 / 3.0);
+                tet_table[o][t] = gx + gy < 2.0 ? 0 : 1;
+            }
+    } else if (Fm.kind == 0) {
         // Nested tetrahedra of the cube split: the unperturbed centroid of a
         // fine tetrahedron lies inside exactly one tetrahedron of its coarse cube.
         static const int T[6][4] = {{2, 3, 7, 1}, {7, 2, 1, 0}, {0, 1, 7, 5}, {7, 2, 0, 6}, {5, 7, 0, 6}, {5, 0, 4, 6}};
@@ -896,12 +952,13 @@ void HhoProblem::Prolongation(const HhoProblem& coarse, std::vector<int>& prp, s
             }
     }
     for (int c = 0; c < Fm.ncell; c++) {
-        int bc = Fm.kind == 0 ? c / 6 : c;
+        const int per = Fm.kind == 0 ? (dim == 3 ? 6 : 2) : 1;
+        int bc = c / per;
         int ix = bc % n, iy = (bc / n) % n, iz = dim == 3 ? bc / (n * n) : 0;
         int cb = ((dim == 3 ? (iz / 2) * nc_ : 0) + iy / 2) * nc_ + ix / 2;
         if (Fm.kind == 0) {
             int o = (ix & 1) + 2 * (iy & 1) + 4 * (iz & 1);
-            f2c[c] = 6 * cb + tet_table[o][c % 6];
+            f2c[c] = per * cb + tet_table[o][c % per];
         } else
             f2c[c] = cb;
     }

@@ -15,7 +15,7 @@ struct File {
     FILE* f = nullptr;
     File(const std::string& path, const char* mode) {
         f = std::fopen(path.c_str(), mode);
-        FH_CHECK(f != nullptr, -7, "cannot open " + path);
+        FH_CHECK(f != nullptr, -11, "cannot open " + path);
     }
     ~File() {
         if (f) std::fclose(f);
@@ -101,7 +101,7 @@ void market_write_matrix(Ctx& c, const Bcsr& A, const std::string& path) {
                     }
         }
         for (int t = 0; t < nb; t++)
-            FH_CHECK(std::fwrite(text[t].data(), 1, text[t].size(), out.f) == text[t].size(), -7,
+            FH_CHECK(std::fwrite(text[t].data(), 1, text[t].size(), out.f) == text[t].size(), -11,
                      "write error on " + path);
     }
 }
@@ -110,18 +110,18 @@ void market_read_csr(const std::string& path, int& nrows, int& ncols, std::vecto
                      std::vector<int>& colidx, std::vector<double>& vals) {
     File in(path, "rb");
     std::string line;
-    FH_CHECK(get_line(in.f, line), -7, path + ": empty file");
+    FH_CHECK(get_line(in.f, line), -11, path + ": empty file");
     FH_CHECK(line.rfind("%%MatrixMarket matrix coordinate", 0) == 0 && line.find("real") != std::string::npos &&
                  line.find("general") != std::string::npos,
-             -7, path + ": not a 'coordinate real general' Matrix Market file");
+             -11, path + ": not a 'coordinate real general' Matrix Market file");
     long long m = -1, n = -1, nnz = -1;
     while (get_line(in.f, line)) {
         if (line.empty() || line[0] == '%') continue;
-        FH_CHECK(std::sscanf(line.c_str(), "%lld %lld %lld", &m, &n, &nnz) == 3 && m > 0 && n > 0 && nnz >= 0, -7,
+        FH_CHECK(std::sscanf(line.c_str(), "%lld %lld %lld", &m, &n, &nnz) == 3 && m > 0 && n > 0 && nnz >= 0, -11,
                  path + ": bad size line");
         break;
     }
-    FH_CHECK(m > 0 && m < (1ll << 31) && n < (1ll << 31), -7, path + ": missing or too large size line");
+    FH_CHECK(m > 0 && m < (1ll << 31) && n < (1ll << 31), -11, path + ": missing or too large size line");
     std::vector<int> ri, ci;
     std::vector<double> v;
     ri.reserve(nnz); ci.reserve(nnz); v.reserve(nnz);
@@ -130,17 +130,17 @@ void market_read_csr(const std::string& path, int& nrows, int& ncols, std::vecto
         char* p = &line[0];
         char* q;
         long long i = std::strtoll(p, &q, 10);
-        FH_CHECK(q != p, -7, path + ": bad entry line");
+        FH_CHECK(q != p, -11, path + ": bad entry line");
         p = q;
         long long j = std::strtoll(p, &q, 10);
-        FH_CHECK(q != p, -7, path + ": bad entry line");
+        FH_CHECK(q != p, -11, path + ": bad entry line");
         p = q;
         double x = std::strtod(p, &q);
-        FH_CHECK(q != p, -7, path + ": bad entry line");
-        FH_CHECK(i >= 1 && i <= m && j >= 1 && j <= n, -7, path + ": entry index out of range");
+        FH_CHECK(q != p, -11, path + ": bad entry line");
+        FH_CHECK(i >= 1 && i <= m && j >= 1 && j <= n, -11, path + ": entry index out of range");
         ri.push_back((int)(i - 1)); ci.push_back((int)(j - 1)); v.push_back(x);
     }
-    FH_CHECK((long long)v.size() == nnz, -7, path + ": entry count differs from the size line");
+    FH_CHECK((long long)v.size() == nnz, -11, path + ": entry count differs from the size line");
     nrows = (int)m; ncols = (int)n;
     // counting sort by row, stable sort by column, duplicates summed
     std::vector<int64_t> cnt(nrows + 1, 0);
@@ -174,33 +174,33 @@ void market_write_vector(int64_t n, const double* v, const std::string& path) {
     for (int64_t i = 0; i < n; i++) {
         int k = put_value(buf, sizeof buf, v[i]);
         buf[k++] = '\n';
-        FH_CHECK(std::fwrite(buf, 1, k, out.f) == (size_t)k, -7, "write error on " + path);
+        FH_CHECK(std::fwrite(buf, 1, k, out.f) == (size_t)k, -11, "write error on " + path);
     }
 }
 
 std::vector<double> market_read_vector(const std::string& path) {
     File in(path, "rb");
     std::string line;
-    FH_CHECK(get_line(in.f, line), -7, path + ": empty file");
-    FH_CHECK(line.rfind("%%MatrixMarket matrix array real", 0) == 0, -7,
+    FH_CHECK(get_line(in.f, line), -11, path + ": empty file");
+    FH_CHECK(line.rfind("%%MatrixMarket matrix array real", 0) == 0, -11,
              path + ": not an 'array real' Matrix Market file");
     long long n = -1, cols = -1;
     while (get_line(in.f, line)) {
         if (line.empty() || line[0] == '%') continue;
-        FH_CHECK(std::sscanf(line.c_str(), "%lld %lld", &n, &cols) == 2 && n > 0 && cols == 1, -7,
+        FH_CHECK(std::sscanf(line.c_str(), "%lld %lld", &n, &cols) == 2 && n > 0 && cols == 1, -11,
                  path + ": bad size line (expected 'n 1')");
         break;
     }
-    FH_CHECK(n > 0, -7, path + ": missing size line");
+    FH_CHECK(n > 0, -11, path + ": missing size line");
     std::vector<double> v;
     v.reserve(n);
     while ((long long)v.size() < n && get_line(in.f, line)) {
         char* q;
         double x = std::strtod(line.c_str(), &q);
-        FH_CHECK(q != line.c_str(), -7, path + ": bad value line");
+        FH_CHECK(q != line.c_str(), -11, path + ": bad value line");
         v.push_back(x);
     }
-    FH_CHECK((long long)v.size() == n, -7, path + ": fewer values than the size line states");
+    FH_CHECK((long long)v.size() == n, -11, path + ": fewer values than the size line states");
     return v;
 }
 

@@ -1,13 +1,16 @@
 // gfx950 kernels of the condensed-HHO solve path.
 //
 // All kernels are written for CDNA wave64.  The block-row kernel is the
-// workhorse: one wavefront per block row, lane = (block slot, row in block),
-// so a wavefront streams SLOTS consecutive n_f x n_f blocks of the row as one
-// contiguous span of HBM (values are stored block row after block row), and
-// the per-row partial sums are reduced across slots with DPP/shuffles.
-// Workgroup ids are remapped so that each XCD walks a contiguous range of
-// block rows (better reuse of x in that XCD's L2).
+// workhorse.  Values are stored block row after block row, so the blocks of
+// one row are one contiguous span of HBM.  For even n_f (2, 4, 6, 10), 16
+// lanes stream a row's span in contiguous 16-byte chunks (k_bcsr_grp, four
+// rows per wavefront).  For odd n_f, one wavefront takes a row with
+// lane = (block slot, row in block) (k_bcsr).  Partial sums are reduced with
+// cross-lane shuffles.  Workgroup ids are remapped so that each XCD walks a
+// contiguous range of block rows (better reuse of x in that XCD's L2).
 #include "fh_kernels.h"
+
+#include <cstdlib>
 
 namespace fh {
 namespace {
@@ -97,6 +100,100 @@ __global__ __launch_bounds__(kWG) void k_bcsr(int nbr, const int* __restrict__ r
             if (lane < NF) z += (AT)ldv(D + j) * rj;
         }
         if (lane < NF) y[row] = (TV)((AT)x[row] + (AT)omega * z);
+    }
+}
+
+// Two consecutive values (same block row of the block: NF is even).
+__device__ __forceinline__ void ld2(const double* p, double& a, double& b) {
+    const double2 v = *(const double2*)p;
+    a = v.x; b = v.y;
+}
+__device__ __forceinline__ void ld2(const float* p, float& a, float& b) {
+    const float2 v = *(const float2*)p;
+    a = v.x; b = v.y;
+}
+__device__ __forceinline__ void ld2(const bf16_t* p, float& a, float& b) {
+    const uint32_t u = *(const uint32_t*)p;
+    a = __uint_as_float(u << 16);
+    b = __uint_as_float(u & 0xffff0000u);
+}
+
+// Block-row kernel for even NF with 16 lanes per block row (4 rows per
+// wavefront).  The blocks of a row are one contiguous span of NF*NF*nblk
+// values; lane g of the group streams the 2-value chunks g, g+16, g+32, ...
+// of that span, so every load instruction of the wavefront reads four
+// contiguous 16-lane segments.  A chunk (q) lies in block p = q / (NF*NF/2),
+// row i, columns j and j+1 of that block; the lane keeps one partial sum per
+// block row i and the group reduces them with xor-shuffles.
+template <int NF, class TA, class TV, class TD, int MODE>
+__global__ __launch_bounds__(kWG) void k_bcsr_grp(int nbr, const int* __restrict__ rp, const int* __restrict__ ci,
+                                                   const TA* __restrict__ val, const TV* x,
+                                                   const TV* __restrict__ b, TV* y, double alpha, double beta,
+                                                   double omega, const TD* __restrict__ invD,
+                                                   const int* __restrict__ rows, int nrows) {
+    static_assert(NF % 2 == 0, "even block size");
+    using AT = typename Acc<TV>::T;
+    constexpr int G = 16, U = 4;
+    constexpr int PAIRS = NF * NF / 2;
+    const int wg = xcd_remap(blockIdx.x, gridDim.x);
+    const int t = wg * kWG + (int)threadIdx.x;
+    const int w = t / G, g = t & (G - 1);
+    const int count = rows ? nrows : nbr;
+    if ((t & ~63) / G >= count) return;  // whole wavefront past the end (wave-uniform)
+    const bool active = w < count;
+    const int br = active ? (rows ? rows[w] : w) : 0;
+    const int s = active ? rp[br] : 0;
+    const int npairs = active ? (rp[br + 1] - s) * PAIRS : 0;
+    const TA* vb = val + (int64_t)s * (2 * PAIRS);
+    AT acc[NF];
+#pragma unroll
+    for (int k = 0; k < NF; k++) acc[k] = 0;
+    for (int q0 = 0; q0 < npairs; q0 += G * U) {
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int q = q0 + u * G + g;
+            const bool ok = q < npairs;
+            const int qq = ok ? q : npairs - 1;
+            const int p = qq / PAIRS;
+            const int r = (qq - p * PAIRS) * 2;
+            const int i = r / NF, j = r - i * NF;
+            typename Acc<TV>::T a0, a1;
+            {
+                decltype(ldv(vb)) v0, v1;
+                ld2(vb + (int64_t)qq * 2, v0, v1);
+                a0 = (AT)v0; a1 = (AT)v1;
+            }
+            const int c = ci[s + p];
+            TV x0, x1;
+            ld2(x + (int64_t)c * NF + j, x0, x1);
+            const AT prod = ok ? a0 * (AT)x0 + a1 * (AT)x1 : (AT)0;
+#pragma unroll
+            for (int k = 0; k < NF; k++) acc[k] += (i == k) ? prod : (AT)0;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < NF; k++) {
+#pragma unroll
+        for (int m = G / 2; m > 0; m >>= 1) acc[k] += __shfl_xor(acc[k], m, 64);
+    }
+    if (!active || g >= NF) return;
+    AT mine = acc[0];
+#pragma unroll
+    for (int k = 1; k < NF; k++) mine = (g == k) ? acc[k] : mine;
+    const int64_t row = (int64_t)br * NF + g;
+    if (MODE == OP_SPMV) {
+        AT v = (AT)alpha * mine;
+        if (beta != 0.0) v += (AT)beta * (AT)y[row];
+        y[row] = (TV)v;
+    } else if (MODE == OP_RESID) {
+        y[row] = (TV)((AT)b[row] - mine);
+    } else {
+        const TD* D = invD + row * NF;
+        const TV* bb = b + (int64_t)br * NF;
+        AT z = 0;
+#pragma unroll
+        for (int k = 0; k < NF; k++) z += (AT)ldv(D + k) * ((AT)bb[k] - acc[k]);
+        y[row] = (TV)((AT)x[row] + (AT)omega * z);
     }
 }
 
@@ -321,12 +418,22 @@ inline int grid_for(int64_t n, int bs = 256, int cap = 65536) {
 }
 
 // ---------------------------------------------------------------- dispatch
+// Block-row kernel variant for even block sizes: 1 (default) 16 lanes per
+// row with contiguous 2-value chunks, 0 one wavefront per row.  Read once
+// from FHGPU_BCSR.
+int bcsr_variant() {
+    static const int v = [] {
+        const char* e = std::getenv("FHGPU_BCSR");
+        return e ? std::atoi(e) : 1;
+    }();
+    return v;
+}
+
 template <int NF, class TA, class TV, class TD>
 void bcsr_dispatch_mode(Ctx& c, const Bcsr& A, BcsrMode mode, const void* x, const void* b, void* y, double alpha,
                         double beta, double omega, const void* invD, const int* rows, int nrows) {
     int count = rows ? nrows : A.nbr;
     if (count <= 0) return;
-    int nwg = (count + 3) / 4;
     auto rp = A.rowptr.as<int>();
     auto ci = A.colidx.as<int>();
     auto val = A.vals.as<TA>();
@@ -334,6 +441,29 @@ void bcsr_dispatch_mode(Ctx& c, const Bcsr& A, BcsrMode mode, const void* x, con
     auto bb = (const TV*)b;
     auto yy = (TV*)y;
     auto D = (const TD*)invD;
+    if constexpr (NF % 2 == 0) {
+        const bool aligned = ((uintptr_t)x % (2 * sizeof(TV))) == 0;
+        if (bcsr_variant() == 1 && aligned) {
+            const int nwg = (int)(((int64_t)count * 16 + kWG - 1) / kWG);
+            switch (mode) {
+                case OP_SPMV:
+                    hipLaunchKernelGGL((k_bcsr_grp<NF, TA, TV, TD, OP_SPMV>), dim3(nwg), dim3(kWG), 0, c.stream, A.nbr,
+                                       rp, ci, val, xx, bb, yy, alpha, beta, omega, D, rows, nrows);
+                    break;
+                case OP_RESID:
+                    hipLaunchKernelGGL((k_bcsr_grp<NF, TA, TV, TD, OP_RESID>), dim3(nwg), dim3(kWG), 0, c.stream,
+                                       A.nbr, rp, ci, val, xx, bb, yy, alpha, beta, omega, D, rows, nrows);
+                    break;
+                default:
+                    hipLaunchKernelGGL((k_bcsr_grp<NF, TA, TV, TD, OP_SMOOTH>), dim3(nwg), dim3(kWG), 0, c.stream,
+                                       A.nbr, rp, ci, val, xx, bb, yy, alpha, beta, omega, D, rows, nrows);
+                    break;
+            }
+            FH_HIP(hipGetLastError());
+            return;
+        }
+    }
+    const int nwg = (count + 3) / 4;
     switch (mode) {
         case OP_SPMV:
             hipLaunchKernelGGL((k_bcsr<NF, TA, TV, TD, OP_SPMV>), dim3(nwg), dim3(kWG), 0, c.stream, A.nbr, rp, ci, val,

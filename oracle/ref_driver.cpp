@@ -30,6 +30,7 @@ using namespace std;
 #include "Discretizations/HHO/Diffusion_HHO.h"
 #include "TestCases/Diffusion/DiffTestCaseFactory.h"
 #include "Mesher/InHouse/Square_CartesianMesh.h"
+#include "Mesher/InHouse/Square_TriangularMesh.h"
 #include "Mesher/InHouse/Cube_CartesianMesh.h"
 #include "Mesher/InHouse/Cube_CartesianTetrahedralMesh.h"
 #include "FunctionalBasis/FunctionalBasisFactory.h"
@@ -104,7 +105,11 @@ struct Opts
 // In-house structured meshes, as MeshFactory<Dim>::BuildMesh builds them for
 // these codes (src/Mesher/MeshFactory.h:142-143,381-399).
 template <int Dim> Mesh<Dim>* BuildMesh(const Opts& o);
-template <> Mesh<2>* BuildMesh<2>(const Opts& o) { return new Square_CartesianMesh(o.n, o.n); }
+template <> Mesh<2>* BuildMesh<2>(const Opts& o)
+{
+	if (o.mesh == "stri") return new Square_TriangularMesh(o.n, o.n);
+	return new Square_CartesianMesh(o.n, o.n);
+}
 template <> Mesh<3>* BuildMesh<3>(const Opts& o)
 {
 	if (o.mesh == "stetra") return new Cube_CartesianTetrahedralMesh(o.n);

@@ -142,6 +142,9 @@ def main():
     # 3-D Cartesian k=2 with GMG V(1,1) bj23
     run("cart8_k2_cg_mg_bj23", ["-dim", "3", "-mesh", "cart", "-k", "2", "-n", "8", "-s", "cg", "-prec", "mg",
                                 "-cycle", "V,1,1", "-smoothers", "bj23,bj23"], export_mg=True)
+    # 2-D triangles (Square_TriangularMesh), k = 2: the learned-smoother family
+    run("stri8_k2_cg_bj", ["-dim", "2", "-mesh", "stri", "-k", "2", "-n", "8", "-source", "pisine", "-s", "cg",
+                           "-prec", "bj"])
     # Exported linear systems (Matrix Market text) of tiny 2-D and 3-D problems
     run("mm_c1_n4", c1s + ["-n", "4", "-s", "cg", "-prec", "mg", "-smoothers", "bj,bj"], export_dat=True)
     run("mm_tet2_k1", ["-dim", "3", "-mesh", "stetra", "-k", "1", "-n", "2", "-source", "sine", "-s", "cg",

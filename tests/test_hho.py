@@ -18,6 +18,7 @@ CASES = [
     ("cart8_k2_cg_mg_bj23", dict(dim=3, mesh="cart", n=8, k=2, source="pisine"), 6),
     ("tet8_k0_fcg_amg", dict(dim=3, mesh="stetra", n=8, k=0, source="sine"), 1),
     ("tet4_k2", dict(dim=3, mesh="stetra", n=4, k=2, source="sine"), 6),
+    ("stri8_k2_cg_bj", dict(dim=2, mesh="stri", n=8, k=2, source="pisine"), 3),
 ]
 
 
@@ -71,6 +72,7 @@ L2_CASES = [
     ("cart8_k2_cg_mg_bj23", dict(dim=3, mesh="cart", n=8, k=2, source="pisine"), 8),
     ("tet8_k0_fcg_amg", dict(dim=3, mesh="stetra", n=8, k=0, source="sine"), 3),
     ("tet8_k2_cg_bj", dict(dim=3, mesh="stetra", n=8, k=2, source="sine"), 3),
+    ("stri8_k2_cg_bj", dict(dim=2, mesh="stri", n=8, k=2, source="pisine"), 7),
 ]
 
 
@@ -95,13 +97,15 @@ def _bj_spectrum(A, nf):
     return np.linalg.eigvalsh(Linv @ Ad @ Linv.T)
 
 
-def test_block_jacobi_spectrum_matches_reference():
-    """The k = 2 tetrahedral operator equals the reference's up to the face
+@pytest.mark.parametrize("name,kw,nf", [("tet4_k2", dict(dim=3, mesh="stetra", n=4, k=2, source="sine"), 6),
+                                        ("stri8_k2_cg_bj", dict(dim=2, mesh="stri", n=8, k=2, source="pisine"), 3)])
+def test_block_jacobi_spectrum_matches_reference(name, kw, nf):
+    """The k = 2 simplicial operators equal the reference's up to the face
     basis: same spectrum of the block-Jacobi-preconditioned operator (which
     also fixes the CG + block-Jacobi convergence for a given right-hand side)."""
-    P = G.HhoProblem(dim=3, mesh="stetra", n=4, k=2, source="sine")
-    ev = _bj_spectrum(P.scipy(), 6)
-    ev_ref = _bj_spectrum(csr(load("tet4_k2"), "A"), 6)
+    P = G.HhoProblem(**kw)
+    ev = _bj_spectrum(P.scipy(), nf)
+    ev_ref = _bj_spectrum(csr(load(name), "A"), nf)
     np.testing.assert_allclose(ev, ev_ref, rtol=1e-9, atol=1e-9 * ev_ref.max())
 
 
