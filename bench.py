@@ -52,21 +52,26 @@ def timed_solves(ctx, solver, b, x, steps, warmup):
 
 
 class Variant:
-    def __init__(self, prec="gmg", galerkin=True, smoother="bj", relax=0.5, nu=2, cycle="V"):
+    def __init__(self, prec="gmg", galerkin=True, smoother="bj", relax=0.5, nu=2, cycle="V", coarse=None,
+                 coarse_relax=0.0):
         self.prec, self.galerkin, self.smoother, self.relax, self.nu, self.cycle = \
             prec, galerkin, smoother, relax, nu, cycle
+        self.coarse, self.coarse_relax = coarse, coarse_relax
 
     def label(self):
-        return "%s%s %s(%d,%d) %s%s" % (self.prec, "-galerkin" if self.prec == "gmg" and self.galerkin else "",
-                                        self.cycle, self.nu, self.nu, self.smoother,
-                                        " w=%g" % self.relax if self.smoother == "bj" else "")
+        s = "%s%s %s(%d,%d) %s%s" % (self.prec, "-galerkin" if self.prec == "gmg" and self.galerkin else "",
+                                     self.cycle, self.nu, self.nu, self.smoother,
+                                     " w=%g" % self.relax if self.smoother == "bj" else "")
+        if self.coarse:
+            s += ", coarse levels %s%s" % (self.coarse, " w=%g" % self.coarse_relax if self.coarse == "bj" else "")
+        return s
 
 
 def make_solver(ctx, G, hier, A, v, solver="cg", mg_prec="f64", mg_store=None, max_iter=1000):
     from fhhos4_amd import gmg
     s = G.Solver(ctx, solver=solver, prec="mg" if v.prec == "gmg" else "aggregamg", cycle=v.cycle, nu=(v.nu, v.nu),
                  smoothers=(v.smoother, v.smoother), relax=v.relax, max_iter=max_iter, tol=1e-8, mg_prec=mg_prec,
-                 mg_store=mg_store, galerkin=v.galerkin)
+                 mg_store=mg_store, galerkin=v.galerkin, coarse_smoother=v.coarse, coarse_relax=v.coarse_relax)
     if v.prec == "gmg":
         gmg.attach(s, hier)
     t = time.perf_counter()
@@ -89,9 +94,11 @@ def cpu_baseline(args, G, O, v):
     else:
         As = [h[1] for h in H]
     levels = [(As[l].scipy(), H[l][2].scipy() if H[l][2] is not None else None) for l in range(len(H))]
-    sm = "mc" + v.smoother if "gs" in v.smoother else v.smoother
+    mc = lambda k: "mc" + k if k and "gs" in k else k  # the GPU's multicolour ordering
+    sm = mc(v.smoother)
     x, hist, out = O.solve(levels[0][0], P.rhs(), "cg", "mg", nf=P.nf, cycle=v.cycle, smoothers=(sm, sm),
-                           omega=v.relax, nu=(v.nu, v.nu), levels=levels, max_iter=args.max_iter)
+                           omega=v.relax, nu=(v.nu, v.nu), levels=levels, max_iter=args.max_iter,
+                           coarse_smoother=mc(v.coarse), coarse_omega=v.coarse_relax)
     setup_s, solve_s = O.last_times()
     N = levels[0][0].shape[0]
     return {"N": N, "iterations": out.iterations, "solve_s": solve_s, "setup_s": setup_s,
@@ -159,6 +166,8 @@ def main():
     ap.add_argument("--relax", type=float, default=1.0)
     ap.add_argument("--nu", type=int, default=1)
     ap.add_argument("--cycle", default="V")
+    ap.add_argument("--coarse-smoother", default=None, help="smoother of the levels below the finest")
+    ap.add_argument("--coarse-relax", type=float, default=0.0)
     ap.add_argument("--max-iter", type=int, default=1000)
     ap.add_argument("--cpu-n", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -198,9 +207,10 @@ def main():
     torch.cuda.synchronize()
 
     if args.sweep:
-        variants = [Variant("gmg", True, "bj", 0.5, 2), Variant("gmg", True, "bj", 0.5, 1),
-                    Variant("gmg", False, "bj", 0.5, 2), Variant("gmg", True, "sbgs", 1.0, 1),
-                    Variant("gmg", False, "sbgs", 1.0, 1), Variant("amg", False, "sbgs", 1.0, 1)]
+        variants = [Variant("gmg", True, "sbgs", 1.0, 1), Variant("gmg", True, "sbgs", 1.0, 1, "V", "cheb2"),
+                    Variant("gmg", True, "cheb3", 1.0, 1), Variant("gmg", True, "cheb2", 1.0, 1),
+                    Variant("gmg", True, "cheb4", 1.0, 1), Variant("gmg", False, "sbgs", 1.0, 1),
+                    Variant("gmg", False, "cheb3", 1.0, 1)]
         for v in variants:
             log("variant", v.label())
             try:
@@ -215,7 +225,8 @@ def main():
                 print(json.dumps({"variant": v.label(), "error": str(e)}), flush=True)
         return
 
-    v = Variant(args.prec, not args.no_galerkin, args.smoother, args.relax, args.nu, args.cycle)
+    v = Variant(args.prec, not args.no_galerkin, args.smoother, args.relax, args.nu, args.cycle, args.coarse_smoother,
+                args.coarse_relax)
     log("set-up FP64 MG-PCG:", v.label())
     s64, setup64 = make_solver(ctx, G, hier, A, v, max_iter=args.max_iter)
     levels = [s64.level_size(l) for l in range(s64_levels(s64))]

@@ -6,6 +6,7 @@
 #include "fh_io.h"
 #include "fh_kernels.h"
 #include "fh_matrix.h"
+#include "fh_nn.h"
 #include "fh_solver.h"
 #include "fh_hho.h"
 
@@ -27,6 +28,12 @@ struct fh_coloring {
 struct fh_solver {
     fh_ctx* ctx;
     std::unique_ptr<fh::Solver> s;
+};
+struct fh_nn {
+    std::unique_ptr<fh::Mpnn> m;
+};
+struct fh_nngraph {
+    std::unique_ptr<fh::NnGraph> g;
 };
 
 namespace {
@@ -402,6 +409,10 @@ int fh_solver_create(fh_ctx* ctx, const fh_solver_args* a, fh_solver** out) {
         s.mg_prec = P(a->mg_prec);
         s.mg_store = P(a->mg_store);
         s.galerkin = a->galerkin != 0;
+        if (a->coarse_smoother) s.coarse_smoother = a->coarse_smoother;
+        s.coarse_relax = a->coarse_relax;
+        if (!s.coarse_smoother.empty())
+            (void)fh::SmootherSpec::parse(s.coarse_smoother, 1, s.coarse_relax > 0 ? s.coarse_relax : s.relax);
         FH_CHECK(s.mg_prec != fh::Prec::BF16, -2, "multigrid vectors are FP64 or FP32");
         FH_CHECK(s.mg_prec == fh::Prec::F32 || s.mg_store == fh::Prec::F64, -2,
                  "FP32/BF16 level matrices need FP32 multigrid vectors");
@@ -444,7 +455,91 @@ int fh_solver_solve(fh_solver* s, const double* b, double* x, fh_solve_stats* st
             st->setup_s = t.setup_s;
             st->solve_s = t.solve_s;
             st->op_complexity = t.op_complexity;
+            st->nn_used = t.nn_used;
+            st->nn_fallback = t.nn_fallback;
         }
+    });
+}
+
+int fh_nn_create(fh_ctx* ctx, int nf, int layers, uint64_t seed, fh_nn** out) {
+    return guard([&] {
+        NEED(ctx); NEED_DEV(ctx); NEED(out);
+        auto n = new fh_nn;
+        try {
+            n->m = std::make_unique<fh::Mpnn>(ctx->c, nf, layers, seed);
+        } catch (...) {
+            delete n;
+            throw;
+        }
+        *out = n;
+    });
+}
+int fh_nn_destroy(fh_nn* nn) {
+    return guard([&] { delete nn; });
+}
+int fh_nn_num_params(const fh_nn* nn, int64_t* n) {
+    return guard([&] { NEED(nn); NEED(n); *n = nn->m->shape().total(); });
+}
+int fh_nn_set_params(fh_nn* nn, const float* params) {
+    return guard([&] { NEED(nn); NEED(params); nn->m->set_params(params); });
+}
+int fh_nn_get_params(const fh_nn* nn, float* params) {
+    return guard([&] { NEED(nn); NEED(params); nn->m->get_params(params); });
+}
+int fh_nn_graph_create(fh_ctx* ctx, const fh_mat* A, const double* face_features, fh_nngraph** out) {
+    return guard([&] {
+        NEED(ctx); NEED_DEV(ctx); NEED(A); NEED(out);
+        auto g = new fh_nngraph;
+        try {
+            g->g = fh::nn_graph_create(ctx->c, *A->A, face_features);
+        } catch (...) {
+            delete g;
+            throw;
+        }
+        *out = g;
+    });
+}
+int fh_nn_graph_destroy(fh_nngraph* g) {
+    return guard([&] { delete g; });
+}
+int fh_nn_forward(fh_ctx* ctx, fh_nn* nn, const fh_nngraph* g, const double* r1, double* delta, int bf16) {
+    return guard([&] {
+        NEED(ctx); NEED_DEV(ctx); NEED(nn); NEED(g); NEED(r1); NEED(delta);
+        nn->m->reserve(g->g->n);
+        nn->m->forward(*g->g, r1, delta, bf16 != 0);
+    });
+}
+int fh_nn_loss_grad(fh_ctx* ctx, fh_nn* nn, const fh_nngraph* g, const double* r1, const double* e1, double* loss,
+                    float* grad) {
+    return guard([&] {
+        NEED(ctx); NEED_DEV(ctx); NEED(nn); NEED(g); NEED(r1); NEED(e1);
+        double l = nn->m->loss_grad(*g->g, r1, e1, grad);
+        if (loss) *loss = l;
+    });
+}
+int fh_nn_train_step(fh_ctx* ctx, fh_nn* nn, const fh_nngraph* g, const double* r1, const double* e1, float lr,
+                     double* loss) {
+    return guard([&] {
+        NEED(ctx); NEED_DEV(ctx); NEED(nn); NEED(g); NEED(r1); NEED(e1);
+        FH_CHECK(lr > 0, -2, "learning rate must be positive");
+        double l = nn->m->train_step(*g->g, r1, e1, lr);
+        if (loss) *loss = l;
+    });
+}
+int fh_solver_set_nn(fh_solver* s, fh_nn* nn, const fh_nngraph* g, double omega) {
+    return guard([&] {
+        NEED(s);
+        FH_CHECK((nn == nullptr) == (g == nullptr), -2, "network and graph go together");
+        s->s->SetLearnedSmoother(nn ? nn->m.get() : nullptr, g ? g->g.get() : nullptr, omega > 0 ? omega : 1.0);
+    });
+}
+int fh_solver_set_stop(fh_solver* s, double tol, int max_iter) {
+    return guard([&] { NEED(s); s->s->SetStop(tol, max_iter); });
+}
+int fh_solver_apply_prec(fh_solver* s, const double* r, double* z) {
+    return guard([&] {
+        NEED(s); NEED(r); NEED(z);
+        s->s->ApplyPreconditioner(r, z);
     });
 }
 int fh_solver_history(const fh_solver* s, double* out, int cap) {
@@ -477,6 +572,8 @@ int fh_hho_create(const fh_hho_args* a, fh_hho** out) {
         if (a->subdomain > 0) h.subdomain = a->subdomain;
         h.aniso = a->aniso > 0 ? a->aniso : 1e-2;
         h.ref_n = a->ref_n;
+        h.source_seed = a->source_seed;
+        h.source_modes = a->source_modes > 0 ? a->source_modes : 8;
         FH_CHECK(h.dim == 2 || h.dim == 3, -2, "dim must be 2 or 3");
         auto o = new fh_hho;
         try {

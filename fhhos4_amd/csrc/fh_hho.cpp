@@ -239,13 +239,52 @@ struct HhoProblem::Impl {
         return {double(local & 1), double((local >> 1) & 1), double((local >> 2) & 1)};
     }
 
+    // "modes": u = sum_m a_m prod_d sin(p_md pi x_d), f = -Lap u.
+    struct Mode { double a; int p[3]; };
+    std::vector<Mode> modes;
+    void make_modes() {
+        modes.clear();
+        if (a.source != "modes") return;
+        FH_CHECK(a.source_modes >= 1 && a.source_modes <= 64, -2, "source_modes must be 1..64");
+        uint64_t st = a.source_seed * 0x2545F4914F6CDD1Dull + 0x1234567ull;
+        auto nxt = [&st]() {
+            uint64_t x = (st += 0x9E3779B97F4A7C15ull);
+            x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+            x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+            return x ^ (x >> 31);
+        };
+        for (int m = 0; m < a.source_modes; m++) {
+            Mode md;
+            md.a = (double)(nxt() >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+            for (int d = 0; d < 3; d++) md.p[d] = 1 + (int)(nxt() % 4);
+            modes.push_back(md);
+        }
+    }
     double source(const V3& x) const {
+        if (!modes.empty()) {
+            double f = 0;
+            for (const Mode& md : modes) {
+                double s = 1, k2 = 0;
+                for (int d = 0; d < dim; d++) { s *= std::sin(md.p[d] * M_PI * x[d]); k2 += md.p[d] * md.p[d]; }
+                f += md.a * M_PI * M_PI * k2 * s;
+            }
+            return f;
+        }
         double w = a.source == "sine" ? 4 * M_PI : M_PI;
         double s = 1;
         for (int d = 0; d < dim; d++) s *= std::sin(w * x[d]);
         return dim * w * w * s;
     }
     double exact(const V3& x) const {
+        if (!modes.empty()) {
+            double u = 0;
+            for (const Mode& md : modes) {
+                double s = 1;
+                for (int d = 0; d < dim; d++) s *= std::sin(md.p[d] * M_PI * x[d]);
+                u += md.a * s;
+            }
+            return u;
+        }
         double w = a.source == "sine" ? 4 * M_PI : M_PI;
         double s = 1;
         for (int d = 0; d < dim; d++) s *= std::sin(w * x[d]);
@@ -830,6 +869,9 @@ void HhoProblem::Impl::local(int c, double* Acond, double* bcond, const double* 
 HhoProblem::HhoProblem(const HhoArgs& args) : impl(new Impl) {
     Impl& I = *impl;
     I.a = args;
+    FH_CHECK(args.source == "pisine" || args.source == "sine" || args.source == "modes", -2,
+             "unknown source '" + args.source + "' (pisine | sine | modes)");
+    I.make_modes();
     auto t0 = std::chrono::steady_clock::now();
     I.build_mesh();
     I.build_tables();

@@ -26,7 +26,10 @@ struct HhoArgs {
     int k = 2;                     // face degree
     double perturb = 0.0;          // interior vertex perturbation, fraction of h (uniform per coordinate)
     uint64_t seed = 1;
-    std::string source = "pisine"; // "pisine": f = d pi^2 prod sin(pi x) | "sine": 4 pi periodic
+    std::string source = "pisine"; // "pisine": f = d pi^2 prod sin(pi x) | "sine": 4 pi periodic |
+                                   // "modes": sum of source_modes random modes a prod sin(p_d pi x_d)
+    uint64_t source_seed = 1;
+    int source_modes = 8;
     std::string kappa = "one";     // "one" | "lognormal" (per cell) | "subdomain" (per block of cells)
     double sigma = 0.0;            // lognormal standard deviation
     int subdomain = 8;             // cells per subdomain edge ("subdomain"), counted on the reference grid

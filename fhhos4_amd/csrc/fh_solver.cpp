@@ -3,9 +3,11 @@
 
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <utility>
 
 #include "fh_kernels.h"
+#include "fh_nn.h"
 
 namespace fh {
 
@@ -77,12 +79,43 @@ SmootherSpec SmootherSpec::parse(const std::string& code, int iterations, double
     else if (code == "bgs" || code == "gs" || code == "bsor" || code == "sor") { s.kind = SmootherKind::BlockGS; s.sweep = Sweep::Forward; }
     else if (code == "rbgs" || code == "rgs" || code == "rbsor" || code == "rsor") { s.kind = SmootherKind::BlockGS; s.sweep = Sweep::Backward; }
     else if (code == "sbgs" || code == "sgs") { s.kind = SmootherKind::BlockGS; s.sweep = Sweep::Symmetric; }
+    else if (code == "cheb2" || code == "cheb3" || code == "cheb4") {
+        s.kind = SmootherKind::Chebyshev;
+        s.degree = code[4] - '0';
+        return s;
+    }
     else throw Error(-2, "Unmanaged smoother code: '" + code + "'");
     if (s.kind == SmootherKind::BlockGS) {
         FH_CHECK(relax == 1.0, -2, "block SOR (relaxation != 1) is not available; use bgs/rbgs with -relax 1");
         s.omega = 1.0;
     }
     return s;
+}
+
+double estimate_lmax(Ctx& c, const Bcsr& A64, const void* invD64) {
+    const int64_t n = A64.rows();
+    std::vector<double> h(n);
+    for (int64_t i = 0; i < n; i++) h[i] = 1.0 + (double)((i * 7919) % 13) / 13.0;
+    DVec v(n, Prec::F64), t(n, Prec::F64), w(n, Prec::F64), zero(n, Prec::F64);
+    FH_HIP(hipMemcpyAsync(v.data(), h.data(), sizeof(double) * n, hipMemcpyHostToDevice, c.stream));
+    launch_fill(c, n, zero.data(), Prec::F64, 0.0);
+    double lam = 0;
+    const int slot = Ctx::kScalars - 8;
+    for (int it = 0; it < kChebPowerIts; it++) {
+        // t = v + D^-1 (0 - A v) ; w = v - t = D^-1 A v
+        launch_bcsr(c, A64, OP_SMOOTH, Prec::F64, v.data(), zero.data(), t.data(), 1.0, 0.0, 1.0, invD64, Prec::F64,
+                    nullptr, 0);
+        FH_HIP(hipMemcpyAsync(w.data(), v.data(), sizeof(double) * n, hipMemcpyDeviceToDevice, c.stream));
+        launch_axpby(c, n, Prec::F64, t.data(), w.data(), -1.0, -1, -1, 1.0, -1, -1);
+        dot2(c, n, Prec::F64, w.data(), w.data(), slot, v.data(), v.data(), slot + 1);
+        read_scalars(c, slot, 2);
+        const double nw = std::sqrt(c.h_scalars[slot]), nv = std::sqrt(c.h_scalars[slot + 1]);
+        FH_CHECK(nw > 0 && nv > 0, -6, "power iteration met a zero vector");
+        lam = nw / nv;
+        launch_axpby(c, n, Prec::F64, w.data(), v.data(), 1.0 / nw, -1, -1, 0.0, -1, -1);
+    }
+    c.sync();
+    return lam;
 }
 
 void smoother_setup(Ctx& c, const Bcsr& A64, LevelSmoother& S, Prec dprec) {
@@ -99,6 +132,16 @@ void smoother_setup(Ctx& c, const Bcsr& A64, LevelSmoother& S, Prec dprec) {
     FH_HIP(hipMemcpyAsync(&h_flag, flag.p, sizeof(int), hipMemcpyDeviceToHost, c.stream));
     c.sync();
     FH_CHECK(h_flag == 0, -6, "singular diagonal block in block row " + std::to_string(h_flag - 1));
+    if (S.spec.kind == SmootherKind::Chebyshev) {
+        if (dprec == Prec::F64)
+            S.lmax = estimate_lmax(c, A64, S.invD.p);
+        else {
+            LevelSmoother S64;
+            smoother_setup(c, A64, S64, Prec::F64);
+            S.lmax = estimate_lmax(c, A64, S64.invD.p);
+        }
+        S.d.resize(A64.rows(), dprec);
+    }
     if (S.spec.kind == SmootherKind::BlockGS) {
         std::vector<int> col;
         S.ncolours = greedy_colouring(A64, col);
@@ -133,6 +176,27 @@ void smoother_apply(Ctx& c, const Bcsr& A, LevelSmoother& S, Prec vp, const void
                 launch_bcsr(c, A, OP_SMOOTH, vp, x.data(), b, work.data(), 1.0, 0.0, S.spec.omega, S.invD.p, S.dprec,
                             nullptr, 0);
                 std::swap(x.buf, work.buf);
+            }
+        } else if (S.spec.kind == SmootherKind::Chebyshev) {
+            // Chebyshev polynomial of D^-1 A on [lo, hi] (three-term recurrence)
+            const double hi = kChebHigh * S.lmax, lo = kChebLow * S.lmax;
+            const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sigma = theta / delta;
+            double rho = 1.0 / sigma;
+            for (int k = 0; k < S.spec.degree; k++) {
+                if (k == 0 && x_is_zero) {
+                    launch_bjac_zero(c, A.nbr, nf, vp, b, work.data(), 1.0, S.invD.p, S.dprec, nullptr, 0);
+                    launch_fill(c, x.n, x.data(), vp, 0.0);
+                } else
+                    launch_bcsr(c, A, OP_SMOOTH, vp, x.data(), b, work.data(), 1.0, 0.0, 1.0, S.invD.p, S.dprec,
+                                nullptr, 0);
+                if (k == 0)
+                    launch_cheb_update(c, x.n, vp, work.data(), x.data(), S.d.data(), 0.0, 1.0 / theta);
+                else {
+                    const double rn = 1.0 / (2.0 * sigma - rho);
+                    launch_cheb_update(c, x.n, vp, work.data(), x.data(), S.d.data(), rn * rho, 2.0 * rn / delta);
+                    rho = rn;
+                }
+                x_is_zero = false;
             }
         } else {
             if (x_is_zero) launch_fill(c, x.n, x.data(), vp, 0.0);
@@ -199,8 +263,75 @@ public:
         finish();
     }
 
+    ~Multigrid() {
+        if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+    }
+
     // Preconditioner application on FP64 vectors: z = M r (one cycle from 0).
+    // V- and W-cycles have no host decision inside, so the whole cycle can be
+    // captured once per (r, z) pair into a HIP graph and replayed.  Whether
+    // replay beats eager launching depends on the cycle (measured on MI355X:
+    // faster for ~250 launches per cycle, slower for ~1000), so the first
+    // application of a pair times one eager cycle and one replay and keeps
+    // the faster.  The K-cycle reads scalars on the host and always runs
+    // eagerly, as does every cycle while launches are being timed.  The
+    // graph holds no memset node (vectors are cleared by kernels).
     void apply(const double* r, double* z) {
+        if (a_.cycle == 'K' || !graphs_enabled() || c_.prof_matrix != nullptr) {
+            apply_eager(r, z);
+            return;
+        }
+        if (graph_r_ == r && graph_z_ == z && graph_mode_ != 0) {
+            if (graph_mode_ == 2) FH_HIP(hipGraphLaunch(graph_exec_, c_.stream));
+            else apply_eager(r, z);
+            return;
+        }
+        if (graph_exec_) {
+            FH_HIP(hipGraphExecDestroy(graph_exec_));
+            graph_exec_ = nullptr;
+        }
+        graph_mode_ = 0;
+        // eager reference time (this application's result stays in z)
+        c_.sync();
+        auto t0 = std::chrono::steady_clock::now();
+        apply_eager(r, z);
+        c_.sync();
+        const double t_eager = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        hipGraph_t graph = nullptr;
+        FH_HIP(hipStreamBeginCapture(c_.stream, hipStreamCaptureModeThreadLocal));
+        try {
+            apply_eager(r, z);
+        } catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(c_.stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        FH_HIP(hipStreamEndCapture(c_.stream, &graph));
+        hipError_t e = hipGraphInstantiate(&graph_exec_, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        FH_HIP(e);
+        graph_r_ = r;
+        graph_z_ = z;
+        FH_HIP(hipGraphLaunch(graph_exec_, c_.stream));  // first replay (uploads the graph)
+        c_.sync();
+        t0 = std::chrono::steady_clock::now();
+        FH_HIP(hipGraphLaunch(graph_exec_, c_.stream));
+        c_.sync();
+        const double t_graph = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        graph_mode_ = t_graph < t_eager ? 2 : 1;
+        if (graph_mode_ == 1) {
+            FH_HIP(hipGraphExecDestroy(graph_exec_));
+            graph_exec_ = nullptr;
+        }
+        t_eager_ = t_eager;
+        t_graph_ = t_graph;
+    }
+    // 0 undecided, 1 eager, 2 graph replay; timings of the decision (s)
+    int graph_mode_ = 0;
+    double t_eager_ = 0, t_graph_ = 0;
+
+    void apply_eager(const double* r, double* z) {
         MgLevel& F = L[0];
         if (vp_ == Prec::F64) {
             copy(c_, F.n, Prec::F64, r, F.b.data());
@@ -225,6 +356,18 @@ private:
     Ctx& c_;
     SolverArgs a_;
     Prec vp_;
+    hipGraphExec_t graph_exec_ = nullptr;
+    const double* graph_r_ = nullptr;
+    double* graph_z_ = nullptr;
+
+    // FHGPU_GRAPH=0 runs every cycle eagerly (for timing comparisons).
+    static bool graphs_enabled() {
+        static const bool on = [] {
+            const char* e = std::getenv("FHGPU_GRAPH");
+            return !(e && std::atoi(e) == 0);
+        }();
+        return on;
+    }
 
     void build_amg() {
         // Blocks of hierarchical face bases: the first coarsening keeps the
@@ -302,8 +445,14 @@ private:
                 V.kc.resize(V.n, vp_); V.kv.resize(V.n, vp_); V.kd.resize(V.n, vp_); V.kw.resize(V.n, vp_);
             }
             if (!coarsest) {
-                V.pre.spec = SmootherSpec::parse(a_.pre_smoother, a_.nu1, a_.relax);
-                V.post.spec = SmootherSpec::parse(a_.post_smoother, a_.nu2, a_.relax);
+                if (l > 0 && !a_.coarse_smoother.empty()) {
+                    const double w = a_.coarse_relax > 0 ? a_.coarse_relax : a_.relax;
+                    V.pre.spec = SmootherSpec::parse(a_.coarse_smoother, a_.nu1, w);
+                    V.post.spec = SmootherSpec::parse(a_.coarse_smoother, a_.nu2, w);
+                } else {
+                    V.pre.spec = SmootherSpec::parse(a_.pre_smoother, a_.nu1, a_.relax);
+                    V.post.spec = SmootherSpec::parse(a_.post_smoother, a_.nu2, a_.relax);
+                }
                 smoother_setup(c_, *V.A64, V.pre, dprec);
                 smoother_setup(c_, *V.A64, V.post, dprec);
                 if (!V.agg && !V.ptrunc) {
@@ -421,6 +570,13 @@ void Solver::SetLevel(int l, const Bcsr* A, const Bcsr* P) {
     given_levels_[l] = {A, P};
 }
 
+void Solver::SetLearnedSmoother(Mpnn* nn, const NnGraph* g, double omega) {
+    FH_CHECK((nn == nullptr) == (g == nullptr), -2, "network and graph go together");
+    nn_ = nn;
+    nng_ = g;
+    nn_omega_ = omega;
+}
+
 int Solver::NumLevels() const { return mg_ ? (int)mg_->L.size() : 1; }
 int Solver::LevelUnknowns(int l) const {
     if (!mg_) return A_ ? (int)A_->rows() : 0;
@@ -453,6 +609,13 @@ void Solver::Setup(const Bcsr& A) {
     fused_ = mg_ && mg_->L.size() > 1 && A.bs_r == 1 && args_.nu2 > 0 &&
              SmootherSpec::parse(args_.post_smoother, 1, 1.0).kind == SmootherKind::BlockGS;
     const int64_t n = A.rows();
+    if (nn_) {
+        FH_CHECK(args_.solver == "fcg", -2, "the learned post-smoother runs inside FCG (flexible preconditioner)");
+        FH_CHECK(nng_->A == &A, -2, "the network graph was built for another operator");
+        nn_->reserve(nng_->n);
+        r1_.resize(n, Prec::F64); delta_.resize(n, Prec::F64); Adelta_.resize(n, Prec::F64);
+        nn_counters_.alloc(2 * sizeof(int));
+    }
     r_.resize(n, Prec::F64); z_.resize(n, Prec::F64); d_.resize(n, Prec::F64); Ad_.resize(n, Prec::F64);
     Az_.resize(n, Prec::F64); dk_.resize(n, Prec::F64); Adk_.resize(n, Prec::F64); t_.resize(n, Prec::F64);
     c_.sync();
@@ -460,6 +623,25 @@ void Solver::Setup(const Bcsr& A) {
     stats_.levels = NumLevels();
     stats_.op_complexity = mg_ ? mg_->op_complexity : 1.0;
     stats_.setup_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+void Solver::SetStop(double tol, int max_iter) {
+    FH_CHECK(tol > 0 && max_iter >= 0, -2, "tolerance must be positive and max_iter >= 0");
+    args_.tol = tol;
+    args_.max_iter = max_iter;
+}
+
+void Solver::ApplyPreconditioner(const double* r, double* z) {
+    FH_CHECK(A_ != nullptr, -2, "Setup() must be called before ApplyPreconditioner()");
+    const int64_t n = A_->rows();
+    // through the solver's own work vectors, so that a captured cycle is reused
+    copy(c_, n, Prec::F64, r, r_.data());
+    Mpnn* keep = nn_;
+    nn_ = nullptr;  // the plain preconditioner, without the learned post-smoother
+    apply_prec(r_, z_, nullptr);
+    nn_ = keep;
+    copy(c_, n, Prec::F64, z_.data(), z);
+    c_.sync();
 }
 
 double Solver::read_scalar(int slot) {
@@ -475,6 +657,18 @@ void Solver::apply_prec(const DVec& r, DVec& z, DVec* Az) {
         smoother_apply(c_, *A_, *prec_sm_, Prec::F64, r.data(), z, t_, xz);
     } else
         copy(c_, n, Prec::F64, r.data(), z.data());
+    if (nn_) {
+        const Prec P = Prec::F64;
+        launch_bcsr(c_, *A_, OP_RESID, P, z.data(), r.data(), r1_.data(), 1.0, 0.0, 0.0, nullptr, P, nullptr, 0);
+        nn_->forward(*nng_, (const double*)r1_.data(), (double*)delta_.data(), true);
+        launch_bcsr(c_, *A_, OP_SPMV, P, delta_.data(), nullptr, Adelta_.data(), 1.0, 0.0, 0.0, nullptr, P, nullptr, 0);
+        const void* xs[2] = {delta_.data(), delta_.data()};
+        const void* ys[2] = {r1_.data(), Adelta_.data()};
+        int sl[2] = {kNnSlotS1, kNnSlotS2};
+        launch_dots(c_, n, P, 2, xs, ys, sl);
+        nn_select(c_, n, A_->bs_r, kNnSlotS1, kNnSlotS2, (const double*)delta_.data(), (const double*)r1_.data(),
+                  nng_->invD64.as<double>(), nn_omega_, (double*)z.data(), nn_counters_.as<int>());
+    }
     if (Az) launch_bcsr(c_, *A_, OP_SPMV, Prec::F64, z.data(), nullptr, Az->data(), 1.0, 0.0, 0.0, nullptr, Prec::F64, nullptr, 0);
 }
 
@@ -483,10 +677,17 @@ void Solver::Solve(const double* b, double* x) {
     c_.sync();
     auto t0 = std::chrono::steady_clock::now();
     hist_.clear();
+    if (nn_) launch_fill(c_, 2, nn_counters_.p, Prec::F32, 0.0);  // two int zeros (same bit pattern as 0.0f)
     if (args_.solver == "cg") solve_cg(b, x);
     else if (args_.solver == "fcg") solve_fcg(b, x);
     else solve_stationary(b, x);
     c_.sync();
+    if (nn_) {
+        int cnt[2];
+        FH_HIP(hipMemcpy(cnt, nn_counters_.p, sizeof(cnt), hipMemcpyDeviceToHost));
+        stats_.nn_used = cnt[0];
+        stats_.nn_fallback = cnt[1];
+    }
     stats_.solve_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 

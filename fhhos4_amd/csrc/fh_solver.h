@@ -20,7 +20,7 @@
 
 namespace fh {
 
-enum class SmootherKind { BlockJacobi, BlockGS };
+enum class SmootherKind { BlockJacobi, BlockGS, Chebyshev };
 enum class Sweep { Forward = 0, Backward = 1, Symmetric = 2 };
 
 // Smoother codes of SmootherFactory::Create (SmootherFactory.h:9-104)
@@ -31,8 +31,15 @@ struct SmootherSpec {
     Sweep sweep = Sweep::Forward;
     double omega = 1.0;
     int iterations = 1;
+    int degree = 2;  // Chebyshev: polynomial degree per smoothing iteration
     static SmootherSpec parse(const std::string& code, int iterations, double relax);
 };
+
+// Chebyshev smoothing targets [kChebLow, kChebHigh] x the power-iteration
+// estimate of lambda_max(D^-1 A) (kChebPowerIts iterations from a fixed
+// start vector).
+constexpr double kChebLow = 0.25, kChebHigh = 1.1;
+constexpr int kChebPowerIts = 20;
 
 // Per-level smoother: block inverses of the diagonal (FP64 -> dprec) and,
 // for Gauss-Seidel, a colouring of the block graph.
@@ -43,9 +50,15 @@ struct LevelSmoother {
     int ncolours = 0;
     std::vector<int> colour_ptr;   // host: colour c = rows[colour_ptr[c] .. colour_ptr[c+1])
     DevBuf colour_rows;            // device row lists, grouped by colour
+    double lmax = 0;               // Chebyshev: estimate of lambda_max(D^-1 A)
+    DVec d;                        // Chebyshev direction vector
 };
 
+// Power-iteration estimate of lambda_max(D^-1 A) in FP64.
+double estimate_lmax(Ctx& c, const Bcsr& A64, const void* invD64);
+
 struct SolveStats {
+    int nn_used = 0, nn_fallback = 0;  // learned corrections accepted / block-Jacobi fallbacks
     int iterations = 0;
     double final_res = 0;
     double setup_s = 0;
@@ -56,6 +69,8 @@ struct SolveStats {
 };
 
 class Multigrid;
+class Mpnn;
+struct NnGraph;
 
 // Krylov / fixed-point solver created by string codes (SolverFactory.h:126-233).
 struct SolverArgs {
@@ -73,6 +88,12 @@ struct SolverArgs {
     Prec mg_prec = Prec::F64;        // working precision of the preconditioner vectors
     Prec mg_store = Prec::F64;       // storage of level matrices (F64 / F32 / BF16)
     bool galerkin = false;           // geometric multigrid: coarse operators P^T A P (UseGalerkinOperator)
+    // Smoother of the levels below the finest ("" = the same as the finest)
+    // and its relaxation (<= 0: relax).  Galerkin coarse levels are dense
+    // (50-100 blocks per row), so a multicolour sweep there costs 50-100
+    // launches; block Jacobi costs one.
+    std::string coarse_smoother;
+    double coarse_relax = 0.0;
 };
 
 class Solver {
@@ -82,9 +103,19 @@ public:
     // GMG hierarchy input: operator and prolongation of level l (l >= 1 for
     // the operator; P of level l maps level l+1 to level l).
     void SetLevel(int l, const Bcsr* A, const Bcsr* P);
+    // Learned post-smoother inside the preconditioner of FCG: after the
+    // multigrid cycle z0 = M r, r1 = r - A z0, delta = net(r1); z = z0 +
+    // (delta.r1 / delta.A delta) delta when delta.r1 > 0, else the block-Jacobi
+    // step z0 + omega D^-1 r1.  The network and graph must outlive the solver.
+    void SetLearnedSmoother(Mpnn* nn, const NnGraph* g, double omega);
     void Setup(const Bcsr& A);
     // Solves A x = b from x = 0 (FP64 device vectors of length A.rows()).
     void Solve(const double* b, double* x);
+    // IterativeSolver::Tolerance / MaxIterations (IterativeSolver.h:11-23).
+    void SetStop(double tol, int max_iter);
+    // The configured preconditioner alone: z = M r (Preconditioner::Apply,
+    // Preconditioner.h:100-118; one cycle from 0 for a multigrid).
+    void ApplyPreconditioner(const double* r, double* z);
     const SolveStats& Stats() const { return stats_; }
     const std::vector<double>& History() const { return hist_; }
     int NumLevels() const;
@@ -101,6 +132,11 @@ private:
     std::vector<double> hist_;
     DVec r_, z_, d_, Ad_, Az_, dk_, Adk_, t_;
     bool fused_ = false;
+    Mpnn* nn_ = nullptr;
+    const NnGraph* nng_ = nullptr;
+    double nn_omega_ = 1.0;
+    DVec r1_, delta_, Adelta_;
+    DevBuf nn_counters_;
     void apply_prec(const DVec& r, DVec& z, DVec* Az);
     double read_scalar(int slot);
     void solve_cg(const double* b, double* x);

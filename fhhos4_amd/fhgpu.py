@@ -24,13 +24,14 @@ class SolverArgs(C.Structure):
     _fields_ = [("solver", C.c_char_p), ("prec", C.c_char_p), ("cycle", C.c_char), ("wloops", C.c_int),
                 ("nu1", C.c_int), ("nu2", C.c_int), ("pre_smoother", C.c_char_p), ("post_smoother", C.c_char_p),
                 ("relax", C.c_double), ("tol", C.c_double), ("max_iter", C.c_int), ("max_coarse", C.c_int),
-                ("theta", C.c_double), ("mg_prec", C.c_int), ("mg_store", C.c_int), ("galerkin", C.c_int)]
+                ("theta", C.c_double), ("mg_prec", C.c_int), ("mg_store", C.c_int), ("galerkin", C.c_int),
+                ("coarse_smoother", C.c_char_p), ("coarse_relax", C.c_double)]
 
 
 class SolveStats(C.Structure):
     _fields_ = [("iterations", C.c_int), ("converged", C.c_int), ("levels", C.c_int), ("pad", C.c_int),
                 ("final_res", C.c_double), ("setup_s", C.c_double), ("solve_s", C.c_double),
-                ("op_complexity", C.c_double)]
+                ("op_complexity", C.c_double), ("nn_used", C.c_int), ("nn_fallback", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
@@ -39,6 +40,7 @@ class SolveStats(C.Structure):
 _lib = None
 _i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 _f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
 _vp = C.c_void_p
 _pp = C.POINTER(C.c_void_p)
 
@@ -55,6 +57,19 @@ _SIGS = {
                     C.POINTER(C.c_int)],
     "fh_mat_device_arrays": [_vp, _pp, _pp, _pp],
     "fh_mat_blocks": [_vp, _i32p, _i32p, _f64p],
+    "fh_nn_create": [_vp, C.c_int, C.c_int, C.c_uint64, _pp],
+    "fh_nn_destroy": [_vp],
+    "fh_nn_num_params": [_vp, C.POINTER(C.c_int64)],
+    "fh_nn_set_params": [_vp, _f32p],
+    "fh_nn_get_params": [_vp, _f32p],
+    "fh_nn_graph_create": [_vp, _vp, _vp, _pp],
+    "fh_nn_graph_destroy": [_vp],
+    "fh_nn_forward": [_vp, _vp, _vp, _vp, _vp, C.c_int],
+    "fh_nn_loss_grad": [_vp, _vp, _vp, _vp, _vp, C.POINTER(C.c_double), _vp],
+    "fh_nn_train_step": [_vp, _vp, _vp, _vp, _vp, C.c_float, C.POINTER(C.c_double)],
+    "fh_solver_set_nn": [_vp, _vp, _vp, C.c_double],
+    "fh_solver_set_stop": [_vp, C.c_double, C.c_int],
+    "fh_solver_apply_prec": [_vp, _vp, _vp],
     "fh_mat_export_market": [_vp, _vp, C.c_char_p],
     "fh_mat_import_market": [_vp, C.c_char_p, C.c_int, C.c_int, _pp],
     "fh_vec_export_market": [C.c_int64, _f64p, C.c_char_p],
@@ -100,7 +115,8 @@ _SIGS = {
 class HhoArgs(C.Structure):
     _fields_ = [("dim", C.c_int), ("mesh", C.c_char_p), ("n", C.c_int), ("k", C.c_int), ("perturb", C.c_double),
                 ("seed", C.c_uint64), ("source", C.c_char_p), ("kappa", C.c_char_p), ("sigma", C.c_double),
-                ("subdomain", C.c_int), ("aniso", C.c_double), ("ref_n", C.c_int)]
+                ("subdomain", C.c_int), ("aniso", C.c_double), ("ref_n", C.c_int), ("source_seed", C.c_uint64),
+                ("source_modes", C.c_int)]
 
 
 class HhoInfo(C.Structure):
@@ -352,10 +368,11 @@ class Solver:
 
     def __init__(self, ctx, solver="cg", prec="mg", cycle="V", nu=(1, 1), smoothers=("bj", "bj"), relax=1.0,
                  tol=1e-8, max_iter=200, max_coarse=1000, theta=0.25, mg_prec="f64", mg_store=None, wloops=0,
-                 galerkin=False):
+                 galerkin=False, coarse_smoother=None, coarse_relax=0.0):
         a = SolverArgs()
-        self._strs = [s.encode() for s in (solver, prec or "", smoothers[0], smoothers[1])]
-        a.solver, a.prec, a.pre_smoother, a.post_smoother = self._strs
+        self._strs = [s.encode() for s in (solver, prec or "", smoothers[0], smoothers[1], coarse_smoother or "")]
+        a.solver, a.prec, a.pre_smoother, a.post_smoother, a.coarse_smoother = self._strs
+        a.coarse_relax = coarse_relax
         a.cycle = cycle.encode()
         a.wloops = wloops
         a.nu1, a.nu2 = nu
@@ -368,6 +385,19 @@ class Solver:
         _check(lib().fh_solver_create(ctx.h, C.byref(a), C.byref(h)))
         self.ctx, self.h, self.max_iter = ctx, h, max_iter
         self._levels = []
+
+    def set_stop(self, tol, max_iter):
+        self.max_iter = max_iter
+        _check(lib().fh_solver_set_stop(self.h, tol, max_iter))
+
+    def apply_prec(self, r, z):
+        """z = M r with the preconditioner alone (device FP64 vectors)."""
+        _check(lib().fh_solver_apply_prec(self.h, _ptr(r), _ptr(z)))
+
+    def set_learned_smoother(self, nn, graph, omega=1.0):
+        """Learned post-smoother inside FCG (before setup)."""
+        self._nn = (nn, graph)
+        _check(lib().fh_solver_set_nn(self.h, nn.h, graph.h, omega))
 
     def set_level(self, level, A=None, P=None):
         self._levels.append((A, P))
@@ -401,14 +431,15 @@ class HhoProblem:
     """Condensed HHO face system of -div(K grad u) = f on a structured mesh."""
 
     def __init__(self, dim=3, mesh="stetra", n=8, k=2, perturb=0.0, seed=1, source="pisine", kappa="one",
-                 sigma=0.0, subdomain=8, aniso=1e-2, ref_n=0):
+                 sigma=0.0, subdomain=8, aniso=1e-2, ref_n=0, source_seed=1, source_modes=8):
         a = HhoArgs()
         self._strs = [mesh.encode(), source.encode(), kappa.encode()]
         a.dim, a.mesh, a.n, a.k, a.perturb, a.seed = dim, self._strs[0], n, k, perturb, seed
         a.source, a.kappa, a.sigma, a.subdomain, a.aniso = self._strs[1], self._strs[2], sigma, subdomain, aniso
-        a.ref_n = ref_n
+        a.ref_n, a.source_seed, a.source_modes = ref_n, source_seed, source_modes
         self.args = dict(dim=dim, mesh=mesh, n=n, k=k, perturb=perturb, seed=seed, source=source, kappa=kappa,
-                         sigma=sigma, subdomain=subdomain, aniso=aniso, ref_n=ref_n)
+                         sigma=sigma, subdomain=subdomain, aniso=aniso, ref_n=ref_n, source_seed=source_seed,
+                         source_modes=source_modes)
         h = C.c_void_p()
         _check(lib().fh_hho_create(C.byref(a), C.byref(h)))
         self.h = h
@@ -473,6 +504,80 @@ class HhoProblem:
             lib().fh_hho_destroy(self.h)
         except Exception:
             pass
+
+
+class Mpnn:
+    """Learned post-smoother network (message passing over the face graph)."""
+
+    def __init__(self, ctx, nf, layers=3, seed=1):
+        h = C.c_void_p()
+        _check(lib().fh_nn_create(ctx.h, nf, layers, seed, C.byref(h)))
+        self.ctx, self.h, self.nf, self.layers = ctx, h, nf, layers
+
+    def num_params(self):
+        n = C.c_int64()
+        _check(lib().fh_nn_num_params(self.h, C.byref(n)))
+        return n.value
+
+    def get_params(self):
+        p = np.zeros(self.num_params(), dtype=np.float32)
+        _check(lib().fh_nn_get_params(self.h, p))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == self.num_params()
+        _check(lib().fh_nn_set_params(self.h, p))
+
+    def forward(self, graph, r1, delta, bf16=True):
+        _check(lib().fh_nn_forward(self.ctx.h, self.h, graph.h, _ptr(r1), _ptr(delta), int(bf16)))
+
+    def loss_grad(self, graph, r1, e1, want_grad=True):
+        loss = C.c_double()
+        g = np.zeros(self.num_params(), dtype=np.float32) if want_grad else None
+        _check(lib().fh_nn_loss_grad(self.ctx.h, self.h, graph.h, _ptr(r1), _ptr(e1), C.byref(loss),
+                                      g.ctypes.data if g is not None else None))
+        return loss.value, g
+
+    def train_step(self, graph, r1, e1, lr):
+        loss = C.c_double()
+        _check(lib().fh_nn_train_step(self.ctx.h, self.h, graph.h, _ptr(r1), _ptr(e1), lr, C.byref(loss)))
+        return loss.value
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().fh_nn_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+class NnGraph:
+    """Face graph of an FP64 operator with 2 features per block row."""
+
+    def __init__(self, ctx, A, features):
+        f = np.ascontiguousarray(features, dtype=np.float64)
+        h = C.c_void_p()
+        _check(lib().fh_nn_graph_create(ctx.h, A.h, f.ctypes.data, C.byref(h)))
+        self.ctx, self.h, self.A, self._f = ctx, h, A, f
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().fh_nn_graph_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+def face_features(problem):
+    """Mean and half-difference of log kappa of the two cells of every
+    block row (face unknown) of an HHO problem."""
+    fc = problem.face_cells()
+    lk = np.log(problem.cell_kappa())
+    c0, c1 = fc[:, 0], np.where(fc[:, 1] >= 0, fc[:, 1], fc[:, 0])
+    return np.stack([(lk[c0] + lk[c1]) / 2, np.abs(lk[c0] - lk[c1]) / 2], axis=1)
 
 
 def profile_begin(ctx, A, mode=0):

@@ -154,6 +154,11 @@ typedef struct fh_solver_args {
     int mg_prec;               /* working precision of the multigrid vectors: FH_F64 | FH_F32 */
     int mg_store;              /* storage of the level matrices: FH_F64 | FH_F32 | FH_BF16 */
     int galerkin;              /* geometric multigrid: coarse operators R A P (-g 1, Multigrid.h:21) */
+    /* Not in the reference (one smoother for all levels there): smoother code
+     * and relaxation of the levels below the finest.  NULL or "" keeps the
+     * finest level's smoothers; coarse_relax <= 0 keeps relax. */
+    const char* coarse_smoother;
+    double coarse_relax;
 } fh_solver_args;
 
 typedef struct fh_solve_stats {
@@ -165,6 +170,8 @@ typedef struct fh_solve_stats {
     double setup_s;
     double solve_s;
     double op_complexity;
+    int nn_used;               /* learned post-smoother: corrections accepted */
+    int nn_fallback;           /* learned post-smoother: block-Jacobi fallbacks */
 } fh_solve_stats;
 
 /* Creation from string codes (SolverFactory<Dim>::CreateSolver,
@@ -181,6 +188,46 @@ int fh_solver_setup(fh_solver* s, const fh_mat* A);
 /* IterativeSolver::Solve(b, "0") (IterativeSolver.h:55-73): x (device, FP64)
  * receives the solution from a zero initial guess. */
 int fh_solver_solve(fh_solver* s, const double* b, double* x, fh_solve_stats* stats);
+/* ------------------------------------------------- learned post-smoother */
+/* No counterpart in the reference.  A message-passing network over the face
+ * graph (hidden width 32, `layers` layers; a few thousand parameters)
+ * predicts a correction of the preconditioned vector from the residual left
+ * by the multigrid cycle and two diffusion features per face.  It runs inside
+ * FCG (FlexibleConjugateGradient.h:47-171 accepts a varying preconditioner)
+ * after the multigrid cycle of SolverPreconditioner (Preconditioner.h:100-199);
+ * block Jacobi (BlockJacobi.h:77-127) replaces it whenever it would not lower
+ * the energy norm of the error. */
+typedef struct fh_nn fh_nn;
+typedef struct fh_nngraph fh_nngraph;
+int fh_nn_create(fh_ctx* ctx, int nf, int layers, uint64_t seed, fh_nn** out);
+int fh_nn_destroy(fh_nn* nn);
+int fh_nn_num_params(const fh_nn* nn, int64_t* n);
+int fh_nn_set_params(fh_nn* nn, const float* params);
+int fh_nn_get_params(const fh_nn* nn, float* params);
+/* Face graph of an FP64 operator: edge weights |tr A_ij| / sum_k |tr A_ik|,
+ * block inverses, and 2 features per block row (host, nbr x 2: mean and
+ * half-difference of log kappa of the face's cells).  A must outlive it. */
+int fh_nn_graph_create(fh_ctx* ctx, const fh_mat* A, const double* face_features, fh_nngraph** out);
+int fh_nn_graph_destroy(fh_nngraph* g);
+/* delta = network correction of r1 (device FP64 vectors); bf16 != 0 uses the
+ * BF16 weights (inference), else the FP32 training weights. */
+int fh_nn_forward(fh_ctx* ctx, fh_nn* nn, const fh_nngraph* g, const double* r1, double* delta, int bf16);
+/* Relative energy error (e1-delta)^T A (e1-delta) / e1^T r1 of one training
+ * pair (r1 = A e1, device FP64) and its gradient (host FP32, may be NULL). */
+int fh_nn_loss_grad(fh_ctx* ctx, fh_nn* nn, const fh_nngraph* g, const double* r1, const double* e1, double* loss,
+                    float* grad);
+/* One Adam step (lr) on one pair; *loss = the loss before the step. */
+int fh_nn_train_step(fh_ctx* ctx, fh_nn* nn, const fh_nngraph* g, const double* r1, const double* e1, float lr,
+                     double* loss);
+/* Enables the learned post-smoother in an FCG solver (before setup; the graph
+ * must be built on the solver's operator); omega: fallback relaxation. */
+int fh_solver_set_nn(fh_solver* s, fh_nn* nn, const fh_nngraph* g, double omega);
+
+/* IterativeSolver::Tolerance and MaxIterations (IterativeSolver.h:11-23). */
+int fh_solver_set_stop(fh_solver* s, double tol, int max_iter);
+/* z = M r with the solver's preconditioner alone (Preconditioner::Apply,
+ * Preconditioner.h:100-118: one multigrid cycle from 0); device FP64. */
+int fh_solver_apply_prec(fh_solver* s, const double* r, double* z);
 /* Normalised residual at each stopping test (IterationResult.h:121-139);
  * returns the number of entries written (<= cap). */
 int fh_solver_history(const fh_solver* s, double* out, int cap);
@@ -196,17 +243,20 @@ int fh_solver_level_size(const fh_solver* s, int level);
 typedef struct fh_hho fh_hho;
 typedef struct fh_hho_args {
     int dim;                /* 2 or 3 */
-    const char* mesh;       /* "stetra" (cube -> 6 tets, Cube_CartesianTetrahedralMesh.h) | "cart" */
+    const char* mesh;       /* "stetra" (cube -> 6 tets, Cube_CartesianTetrahedralMesh.h) | "stri" (square ->
+                               2 triangles, Square_TriangularMesh.h) | "cart" */
     int n;                  /* background cells per direction */
     int k;                  /* face polynomial degree 0..3 */
     double perturb;         /* interior vertex perturbation (fraction of h) */
     uint64_t seed;
-    const char* source;     /* "pisine" | "sine" (DiffusionTestCase.h:62-131) */
+    const char* source;     /* "pisine" | "sine" (DiffusionTestCase.h:62-131) | "modes" */
     const char* kappa;      /* "one" | "lognormal" | "subdomain" */
     double sigma;           /* lognormal standard deviation */
     int subdomain;          /* cells per subdomain edge (on the reference grid) */
     double aniso;           /* z-scaling of the anisotropic subdomains */
     int ref_n;              /* finest grid of a hierarchy (0: n); perturbation and subdomains live on it */
+    uint64_t source_seed;   /* "modes" source: seed and number (1..64) of random sinusoid modes */
+    int source_modes;
 } fh_hho_args;
 typedef struct fh_hho_info {
     int nb, nf, ncells, nfaces;

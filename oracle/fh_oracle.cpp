@@ -321,7 +321,46 @@ void galerkin(const Csr& A, int nf, const std::vector<int>& agg, int nagg, Csr& 
 }
 
 // ----------------------------------------------------------------- multigrid
-enum { SM_BJ = 1, SM_BGS_F = 2, SM_BGS_B = 3, SM_BGS_S = 4, SM_MC_F = 5, SM_MC_B = 6, SM_MC_S = 7 };
+enum { SM_BJ = 1, SM_BGS_F = 2, SM_BGS_B = 3, SM_BGS_S = 4, SM_MC_F = 5, SM_MC_B = 6, SM_MC_S = 7,
+       SM_CHEB2 = 8, SM_CHEB3 = 9, SM_CHEB4 = 10 };
+
+// out = D^-1 (b - A x), block by block
+void dinv_resid(const Csr& A, int nf, const double* invD, const double* b, const double* x, double* out) {
+    int nb = A.n / nf;
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < nb; i++) {
+        double tmp[64];
+        for (int k = 0; k < nf; k++) {
+            int row = i * nf + k;
+            double t = b ? b[row] : 0.0;
+            for (int p = A.prp[row]; p < A.prp[row + 1]; p++) t -= A.pv[p] * x[A.pci[p]];
+            tmp[k] = t;
+        }
+        const double* Di = invD + (size_t)i * nf * nf;
+        for (int k = 0; k < nf; k++) {
+            double s = 0;
+            for (int l = 0; l < nf; l++) s += Di[k * nf + l] * tmp[l];
+            out[i * nf + k] = s;
+        }
+    }
+}
+
+// Power iteration for lambda_max(D^-1 A): 20 steps from v_i = 1 + (7919 i mod 13) / 13.
+double cheb_lmax(const Csr& A, int nf, const double* invD) {
+    const int n = A.n;
+    std::vector<double> v(n), w(n);
+    for (int i = 0; i < n; i++) v[i] = 1.0 + (double)(((int64_t)i * 7919) % 13) / 13.0;
+    double lam = 0;
+    for (int it = 0; it < 20; it++) {
+        dinv_resid(A, nf, invD, nullptr, v.data(), w.data());  // w = -D^-1 A v
+        double nw = 0, nv = 0;
+        for (int i = 0; i < n; i++) { w[i] = -w[i]; nw += w[i] * w[i]; nv += v[i] * v[i]; }
+        nw = std::sqrt(nw); nv = std::sqrt(nv);
+        lam = nw / nv;
+        for (int i = 0; i < n; i++) v[i] = w[i] / nw;
+    }
+    return lam;
+}
 
 int greedy_colour_blocks(const Csr& A, int nf, std::vector<int>& cptr, std::vector<int>& rows);
 
@@ -335,6 +374,7 @@ struct Level {
     int nfc = 1;
     std::vector<double> invD;
     std::vector<int> cptr, crows;   // multicolour ordering (greedy, block-row order)
+    double lmax = 0;                 // Chebyshev: lambda_max(D^-1 A) estimate
 };
 
 struct MG {
@@ -345,6 +385,27 @@ struct MG {
     void smooth(Level& L, int kind, double omega, int nu, const double* b, double* x) {
         const int nf = L.nf;
         for (int s = 0; s < nu; s++) {
+            if (kind >= SM_CHEB2) {
+                // Chebyshev polynomial of D^-1 A on [0.25, 1.1] x lambda_max estimate
+                if (L.lmax == 0) L.lmax = cheb_lmax(L.A, nf, L.invD.data());
+                const int deg = 2 + (kind - SM_CHEB2);
+                const double hi = 1.1 * L.lmax, lo = 0.25 * L.lmax;
+                const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sigma = theta / delta;
+                double rho = 1.0 / sigma;
+                const int n = L.A.n;
+                std::vector<double> r(n), d(n, 0.0);
+                for (int k = 0; k < deg; k++) {
+                    dinv_resid(L.A, nf, L.invD.data(), b, x, r.data());
+                    if (k == 0)
+                        for (int i = 0; i < n; i++) { d[i] = r[i] / theta; x[i] += d[i]; }
+                    else {
+                        const double rn = 1.0 / (2.0 * sigma - rho);
+                        for (int i = 0; i < n; i++) { d[i] = rn * rho * d[i] + 2.0 * rn / delta * r[i]; x[i] += d[i]; }
+                        rho = rn;
+                    }
+                }
+                continue;
+            }
             if (kind == SM_BJ) bj_sweep(L.A, nf, L.invD.data(), omega, b, x);
             else if (kind >= SM_MC_F) {
                 if (L.cptr.empty()) greedy_colour_blocks(L.A, nf, L.cptr, L.crows);
@@ -408,7 +469,8 @@ struct MG {
             if (Ax) spmv(L.A, x, Ax);
             return;
         }
-        smooth(L, cfg->pre_kind, cfg->pre_omega, cfg->nu1, b, x);
+        const bool cs = l > 0 && cfg->coarse_kind > 0;
+        smooth(L, cs ? cfg->coarse_kind : cfg->pre_kind, cs ? cfg->coarse_omega : cfg->pre_omega, cfg->nu1, b, x);
         std::vector<double> r(n);
         spmv(L.A, x, r.data());
         for (int i = 0; i < n; i++) r[i] = b[i] - r[i];
@@ -426,7 +488,7 @@ struct MG {
             }
         }
         prolong_add(l, ec, x);
-        smooth(L, cfg->post_kind, cfg->post_omega, cfg->nu2, b, x);
+        smooth(L, cs ? cfg->coarse_kind : cfg->post_kind, cs ? cfg->coarse_omega : cfg->post_omega, cfg->nu2, b, x);
         if (Ax) spmv(L.A, x, Ax);
     }
     // Multigrid::FCGForKCycle (Multigrid.h:611-656); r is overwritten
@@ -789,3 +851,212 @@ int or_solve(const or_solver_cfg* cfg, const or_level_desc* levels, const double
 }
 
 }  // extern "C"
+
+// ================================================================ learned post-smoother oracle
+namespace {
+const int NH = 32;
+struct MpShape {
+    int nf, L;
+    int din() const { return nf + 2; }
+    long We() const { return 0; }
+    long be() const { return (long)NH * din(); }
+    long S(int l) const { return be() + NH + (long)l * (2 * NH * NH + NH); }
+    long N(int l) const { return S(l) + NH * NH; }
+    long c(int l) const { return N(l) + NH * NH; }
+    long Wo() const { return S(L); }
+    long bo() const { return Wo() + (long)nf * NH; }
+    long total() const { return bo() + nf; }
+};
+
+struct MpState {
+    int nb, nf, L;
+    std::vector<double> w, invD, v, u, x0, h, m;  // h: (L+1) x nb x NH, m: L x nb x NH
+    double sigma;
+};
+
+void mp_forward(const MpShape& S, int nb, const int* brp, const int* bci, const double* bval, const double* feat,
+                const double* P, const double* r1, double* delta, MpState& st) {
+    const int nf = S.nf, L = S.L, din = S.din();
+    const size_t bs = (size_t)nf * nf;
+    st.nb = nb; st.nf = nf; st.L = L;
+    // edge weights
+    st.w.assign(brp[nb], 0.0);
+    for (int i = 0; i < nb; i++) {
+        double sum = 0;
+        for (int p = brp[i]; p < brp[i + 1]; p++) {
+            if (bci[p] == i) continue;
+            double tr = 0;
+            for (int a = 0; a < nf; a++) tr += bval[p * bs + a * nf + a];
+            sum += std::fabs(tr);
+        }
+        for (int p = brp[i]; p < brp[i + 1]; p++) {
+            double tr = 0;
+            for (int a = 0; a < nf; a++) tr += bval[p * bs + a * nf + a];
+            st.w[p] = (bci[p] == i || sum == 0) ? 0.0 : std::fabs(tr) / sum;
+        }
+    }
+    // block inverses (Gauss-Jordan, partial pivoting)
+    st.invD.assign((size_t)nb * bs, 0.0);
+    for (int i = 0; i < nb; i++) {
+        int pd = -1;
+        for (int p = brp[i]; p < brp[i + 1]; p++) if (bci[p] == i) pd = p;
+        std::vector<double> a(bval + pd * bs, bval + (pd + 1) * bs), inv(bs, 0.0);
+        for (int r = 0; r < nf; r++) inv[r * nf + r] = 1;
+        for (int c = 0; c < nf; c++) {
+            int piv = c;
+            for (int r = c + 1; r < nf; r++) if (std::fabs(a[r * nf + c]) > std::fabs(a[piv * nf + c])) piv = r;
+            for (int j = 0; j < nf; j++) { std::swap(a[piv * nf + j], a[c * nf + j]); std::swap(inv[piv * nf + j], inv[c * nf + j]); }
+            double d = 1.0 / a[c * nf + c];
+            for (int j = 0; j < nf; j++) { a[c * nf + j] *= d; inv[c * nf + j] *= d; }
+            for (int r = 0; r < nf; r++) {
+                if (r == c) continue;
+                double f = a[r * nf + c];
+                for (int j = 0; j < nf; j++) { a[r * nf + j] -= f * a[c * nf + j]; inv[r * nf + j] -= f * inv[c * nf + j]; }
+            }
+        }
+        std::copy(inv.begin(), inv.end(), st.invD.begin() + (size_t)i * bs);
+    }
+    const size_t N = (size_t)nb * nf;
+    st.v.assign(N, 0.0);
+    double vv = 0;
+    for (int i = 0; i < nb; i++)
+        for (int a = 0; a < nf; a++) {
+            double s = 0;
+            for (int b = 0; b < nf; b++) s += st.invD[i * bs + a * nf + b] * r1[i * nf + b];
+            st.v[i * nf + a] = s;
+            vv += s * s;
+        }
+    st.sigma = std::sqrt(vv / (double)N);
+    const double inv = st.sigma > 0 ? 1.0 / st.sigma : 0.0;
+    st.u.assign(N, 0.0);
+    st.x0.assign((size_t)nb * din, 0.0);
+    for (int i = 0; i < nb; i++) {
+        for (int a = 0; a < nf; a++) st.u[i * nf + a] = st.x0[(size_t)i * din + a] = st.v[i * nf + a] * inv;
+        st.x0[(size_t)i * din + nf] = feat[2 * i];
+        st.x0[(size_t)i * din + nf + 1] = feat[2 * i + 1];
+    }
+    const size_t nH = (size_t)nb * NH;
+    st.h.assign((L + 1) * nH, 0.0);
+    st.m.assign(L * nH, 0.0);
+    for (int i = 0; i < nb; i++)
+        for (int c = 0; c < NH; c++) {
+            double acc = P[S.be() + c];
+            for (int k = 0; k < din; k++) acc += P[S.We() + c * din + k] * st.x0[(size_t)i * din + k];
+            st.h[(size_t)i * NH + c] = acc > 0 ? acc : 0;
+        }
+    for (int l = 0; l < L; l++) {
+        const double* hin = &st.h[l * nH];
+        double* hout = &st.h[(l + 1) * nH];
+        double* m = &st.m[l * nH];
+        for (int i = 0; i < nb; i++) {
+            for (int c = 0; c < NH; c++) {
+                double s = 0;
+                for (int p = brp[i]; p < brp[i + 1]; p++) s += st.w[p] * hin[(size_t)bci[p] * NH + c];
+                m[(size_t)i * NH + c] = s;
+            }
+            for (int c = 0; c < NH; c++) {
+                double acc = P[S.c(l) + c];
+                for (int k = 0; k < NH; k++)
+                    acc += P[S.S(l) + c * NH + k] * hin[(size_t)i * NH + k] + P[S.N(l) + c * NH + k] * m[(size_t)i * NH + k];
+                hout[(size_t)i * NH + c] = acc > 0 ? acc : 0;
+            }
+        }
+    }
+    const double* hL = &st.h[L * nH];
+    for (int i = 0; i < nb; i++)
+        for (int a = 0; a < nf; a++) {
+            double y = P[S.bo() + a];
+            for (int k = 0; k < NH; k++) y += P[S.Wo() + a * NH + k] * hL[(size_t)i * NH + k];
+            delta[i * nf + a] = st.sigma * (st.u[i * nf + a] + y);
+        }
+}
+
+void bspmv(int nb, int nf, const int* brp, const int* bci, const double* bval, const double* x, double* y) {
+    const size_t bs = (size_t)nf * nf;
+    for (int i = 0; i < nb; i++)
+        for (int a = 0; a < nf; a++) {
+            double s = 0;
+            for (int p = brp[i]; p < brp[i + 1]; p++)
+                for (int b = 0; b < nf; b++) s += bval[p * bs + a * nf + b] * x[(size_t)bci[p] * nf + b];
+            y[i * nf + a] = s;
+        }
+}
+}  // namespace
+
+extern "C" int or_mpnn_num_params(int nf, int layers) { return (int)MpShape{nf, layers}.total(); }
+
+extern "C" void or_mpnn_forward(int nb, int nf, const int* brp, const int* bci, const double* bval, const double* feat,
+                                int layers, const double* params, const double* r1, double* delta) {
+    MpState st;
+    mp_forward(MpShape{nf, layers}, nb, brp, bci, bval, feat, params, r1, delta, st);
+}
+
+extern "C" double or_mpnn_loss_grad(int nb, int nf, const int* brp, const int* bci, const double* bval,
+                                    const double* feat, int layers, const double* params, const double* r1,
+                                    const double* e1, double* grad) {
+    const MpShape S{nf, layers};
+    const int L = layers, din = S.din();
+    const size_t N = (size_t)nb * nf, nH = (size_t)nb * NH;
+    MpState st;
+    std::vector<double> delta(N), diff(N), Ad(N);
+    mp_forward(S, nb, brp, bci, bval, feat, params, r1, delta.data(), st);
+    double E0 = 0, E = 0;
+    for (size_t k = 0; k < N; k++) { E0 += e1[k] * r1[k]; diff[k] = e1[k] - delta[k]; }
+    bspmv(nb, nf, brp, bci, bval, diff.data(), Ad.data());
+    for (size_t k = 0; k < N; k++) E += diff[k] * Ad[k];
+    if (!grad) return E / E0;
+    std::fill(grad, grad + S.total(), 0.0);
+    // d loss / d delta = -2 A diff / E0 ; dy = sigma * that
+    std::vector<double> gy(N), dh(nH, 0.0), dprev(nH), t(nH);
+    for (size_t k = 0; k < N; k++) gy[k] = st.sigma * (-2.0 * Ad[k] / E0);
+    const double* hL = &st.h[L * nH];
+    for (int i = 0; i < nb; i++)
+        for (int a = 0; a < nf; a++) {
+            const double g = gy[i * nf + a];
+            grad[S.bo() + a] += g;
+            for (int k = 0; k < NH; k++) {
+                grad[S.Wo() + a * NH + k] += g * hL[(size_t)i * NH + k];
+                dh[(size_t)i * NH + k] += params[S.Wo() + a * NH + k] * g;
+            }
+        }
+    for (int l = L - 1; l >= 0; l--) {
+        const double* hout = &st.h[(l + 1) * nH];
+        const double* hin = &st.h[l * nH];
+        const double* m = &st.m[l * nH];
+        for (size_t k = 0; k < nH; k++) if (!(hout[k] > 0)) dh[k] = 0;  // dpre
+        std::fill(dprev.begin(), dprev.end(), 0.0);
+        for (int i = 0; i < nb; i++)
+            for (int c = 0; c < NH; c++) {
+                const double d = dh[(size_t)i * NH + c];
+                grad[S.c(l) + c] += d;
+                for (int k = 0; k < NH; k++) {
+                    grad[S.S(l) + c * NH + k] += d * hin[(size_t)i * NH + k];
+                    grad[S.N(l) + c * NH + k] += d * m[(size_t)i * NH + k];
+                }
+            }
+        for (int i = 0; i < nb; i++)
+            for (int k = 0; k < NH; k++) {
+                double a = 0, b = 0;
+                for (int c = 0; c < NH; c++) {
+                    a += params[S.S(l) + c * NH + k] * dh[(size_t)i * NH + c];
+                    b += params[S.N(l) + c * NH + k] * dh[(size_t)i * NH + c];
+                }
+                dprev[(size_t)i * NH + k] = a;
+                t[(size_t)i * NH + k] = b;
+            }
+        // m_i = sum_p w_p h_{col p}  ->  dh_j += sum_{i, p=(i,j)} w_p t_i
+        for (int i = 0; i < nb; i++)
+            for (int p = brp[i]; p < brp[i + 1]; p++)
+                for (int k = 0; k < NH; k++) dprev[(size_t)bci[p] * NH + k] += st.w[p] * t[(size_t)i * NH + k];
+        dh = dprev;
+    }
+    const double* h0 = &st.h[0];
+    for (size_t k = 0; k < nH; k++) if (!(h0[k] > 0)) dh[k] = 0;
+    for (int i = 0; i < nb; i++)
+        for (int c = 0; c < NH; c++) {
+            const double d = dh[(size_t)i * NH + c];
+            grad[S.be() + c] += d;
+            for (int k = 0; k < din; k++) grad[S.We() + c * din + k] += d * st.x0[(size_t)i * din + k];
+        }
+    return E / E0;
+}

@@ -74,7 +74,9 @@ typedef struct or_solver_cfg {
     int nf;            /* block size */
     double omega;      /* relaxation of block Jacobi */
     int pre_kind, post_kind;   /* 1 bj, 2 bgs forward, 3 bgs backward, 4 bgs symmetric,
-                                  5/6/7 multicolour bgs forward/backward/symmetric */
+                                  5/6/7 multicolour bgs forward/backward/symmetric,
+                                  8/9/10 Chebyshev of degree 2/3/4 of D^-1 A on
+                                  [0.25, 1.1] x its power-iteration lambda_max */
     double pre_omega, post_omega;
     int nu1, nu2;
     char cycle;        /* 'V', 'W' or 'K' */
@@ -84,6 +86,8 @@ typedef struct or_solver_cfg {
     double tol;
     int max_iter;
     int nlevels;       /* GMG: number of levels given in `levels` */
+    int coarse_kind;   /* > 0: smoother kind of the levels below the finest (both sweeps) */
+    double coarse_omega;
 } or_solver_cfg;
 
 typedef struct or_solver_out {
@@ -108,6 +112,25 @@ int or_amg_hierarchy(int n, int nf, const int* rp, const int* ci, const double* 
 /* Seconds spent by the last or_solve in its setup and in its iterations
  * (wall clock), for the CPU baseline. */
 void or_last_times(double* setup_s, double* solve_s);
+
+/* ------------------------------------------------ learned post-smoother
+ * Oracle of the message-passing network of the HIP library (FP64): the
+ * operator is given in block-CSR form (nb block rows of nf x nf row-major
+ * blocks); feat holds 2 features per block row; params are laid out as
+ * We (32 x (nf+2)), be (32), then per layer S (32 x 32), N (32 x 32), c (32),
+ * then Wo (nf x 32), bo (nf).
+ *   v = D^-1 r1, sigma = RMS(v), u = v / sigma, x0 = [u, feat]
+ *   h = relu(We x0 + be); per layer m_i = sum_j w_ij h_j (w_ij =
+ *   |tr A_ij| / sum_{k != i} |tr A_ik|, w_ii = 0), h = relu(S h + N m + c)
+ *   delta = sigma (u + Wo h + bo)
+ * Loss of a pair (r1 = A e1, e1): (e1 - delta)^T A (e1 - delta) / (e1^T r1).
+ * No reference counterpart: pinned by central finite differences of the
+ * loss (tests/test_oracle.py). */
+int or_mpnn_num_params(int nf, int layers);
+void or_mpnn_forward(int nb, int nf, const int* brp, const int* bci, const double* bval, const double* feat,
+                     int layers, const double* params, const double* r1, double* delta);
+double or_mpnn_loss_grad(int nb, int nf, const int* brp, const int* bci, const double* bval, const double* feat,
+                         int layers, const double* params, const double* r1, const double* e1, double* grad);
 
 #ifdef __cplusplus
 }

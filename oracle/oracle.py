@@ -26,7 +26,7 @@ class SolverCfg(C.Structure):
                 ("pre_kind", C.c_int), ("post_kind", C.c_int), ("pre_omega", C.c_double),
                 ("post_omega", C.c_double), ("nu1", C.c_int), ("nu2", C.c_int), ("cycle", C.c_char),
                 ("wloops", C.c_int), ("max_coarse", C.c_int), ("theta", C.c_double), ("tol", C.c_double),
-                ("max_iter", C.c_int), ("nlevels", C.c_int)]
+                ("max_iter", C.c_int), ("nlevels", C.c_int), ("coarse_kind", C.c_int), ("coarse_omega", C.c_double)]
 
 
 class SolverOut(C.Structure):
@@ -61,6 +61,11 @@ def lib():
         L.or_solve.argtypes = [C.POINTER(SolverCfg), C.POINTER(LevelDesc), _f64p, _f64p, _f64p,
                                C.POINTER(SolverOut)]
         L.or_last_times.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.or_mpnn_num_params.argtypes = [C.c_int, C.c_int]
+        L.or_mpnn_forward.argtypes = [C.c_int, C.c_int, _i32p, _i32p, _f64p, _f64p, C.c_int, _f64p, _f64p, _f64p]
+        L.or_mpnn_loss_grad.argtypes = [C.c_int, C.c_int, _i32p, _i32p, _f64p, _f64p, C.c_int, _f64p, _f64p, _f64p,
+                                        C.c_void_p]
+        L.or_mpnn_loss_grad.restype = C.c_double
         _lib = L
     return _lib
 
@@ -138,12 +143,12 @@ def amg_hierarchy(A, nf, theta=0.25, max_coarse=1000):
 
 SMOOTHER_KIND = {"bj": 1, "bj23": 1, "bgs": 2, "rbgs": 3, "sbgs": 4, "gs": 2, "rgs": 3, "sgs": 4,
                  # multicolour block Gauss-Seidel (greedy colouring), the ordering of the HIP path
-                 "mcbgs": 5, "mcrbgs": 6, "mcsbgs": 7}
+                 "mcbgs": 5, "mcrbgs": 6, "mcsbgs": 7, "cheb2": 8, "cheb3": 9, "cheb4": 10}
 PREC_KIND = {"": 0, "none": 0, "bj": 1, "bgs": 2, "mg": 3, "aggregamg": 4}
 
 
 def solve(A, b, solver="cg", prec="mg", nf=1, cycle="V", nu=(1, 1), smoothers=("bj", "bj"), omega=1.0,
-          levels=None, tol=1e-8, max_iter=200, max_coarse=1000, theta=0.25):
+          levels=None, tol=1e-8, max_iter=200, max_coarse=1000, theta=0.25, coarse_smoother=None, coarse_omega=0.0):
     """Runs the oracle solver; `levels` = [(A_l, P_l), ...] for GMG (P of the
     coarsest level is None).  Returns (x, hist, out)."""
     keep = []
@@ -170,6 +175,9 @@ def solve(A, b, solver="cg", prec="mg", nf=1, cycle="V", nu=(1, 1), smoothers=("
     cfg.pre_omega = 2.0 / 3.0 if smoothers[0] == "bj23" else omega
     cfg.post_omega = 2.0 / 3.0 if smoothers[1] == "bj23" else omega
     cfg.nu1, cfg.nu2 = nu
+    if coarse_smoother:
+        cfg.coarse_kind = SMOOTHER_KIND[coarse_smoother]
+        cfg.coarse_omega = 2.0 / 3.0 if coarse_smoother == "bj23" else (coarse_omega if coarse_omega > 0 else omega)
     cfg.cycle = cycle.encode()
     cfg.wloops = 2 if cycle == "W" else 1
     cfg.max_coarse = max_coarse
@@ -190,3 +198,37 @@ def last_times():
     s, t = C.c_double(), C.c_double()
     lib().or_last_times(C.byref(s), C.byref(t))
     return s.value, t.value
+
+
+def _bsr(A, nf):
+    import scipy.sparse as sp
+    B = sp.bsr_matrix(A, blocksize=(nf, nf))
+    B.sort_indices()
+    return (np.ascontiguousarray(B.indptr, dtype=np.int32), np.ascontiguousarray(B.indices, dtype=np.int32),
+            np.ascontiguousarray(B.data, dtype=np.float64).reshape(-1))
+
+
+def mpnn_num_params(nf, layers=3):
+    return lib().or_mpnn_num_params(nf, layers)
+
+
+def mpnn_forward(A, nf, feat, params, r1, layers=3):
+    """Learned post-smoother correction delta of r1 (FP64)."""
+    rp, ci, v = _bsr(A, nf)
+    delta = np.zeros(A.shape[0])
+    lib().or_mpnn_forward(len(rp) - 1, nf, rp, ci, v, np.ascontiguousarray(feat, dtype=np.float64).reshape(-1),
+                          layers, np.ascontiguousarray(params, dtype=np.float64), np.ascontiguousarray(r1, dtype=np.float64),
+                          delta)
+    return delta
+
+
+def mpnn_loss_grad(A, nf, feat, params, r1, e1, layers=3, want_grad=True):
+    """Relative energy error of the corrected vector and its gradient."""
+    rp, ci, v = _bsr(A, nf)
+    g = np.zeros(len(params)) if want_grad else None
+    loss = lib().or_mpnn_loss_grad(len(rp) - 1, nf, rp, ci, v,
+                                   np.ascontiguousarray(feat, dtype=np.float64).reshape(-1), layers,
+                                   np.ascontiguousarray(params, dtype=np.float64),
+                                   np.ascontiguousarray(r1, dtype=np.float64), np.ascontiguousarray(e1, dtype=np.float64),
+                                   g.ctypes.data if g is not None else None)
+    return loss, g

@@ -1,0 +1,142 @@
+"""Learned post-smoother on the GPU: forward and gradient of the HIP kernels
+against the FP64 oracle (itself pinned by finite differences in
+test_oracle.py), BF16 inference, training, and the gain of the trained
+network inside FCG over the same solver without it."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+
+F = pytest.importorskip("fhhos4_amd")
+G = F.fhgpu
+pytestmark = pytest.mark.gpu
+
+SOLVER = dict(prec="aggregamg", cycle="V", nu=(1, 1), smoothers=("sbgs", "sbgs"))
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    return G.Context(0)
+
+
+def _problem(n=6, seed=3, sigma=1.0, source="pisine", source_seed=1):
+    return G.HhoProblem(dim=2, mesh="stri", n=n, k=2, perturb=0.2, seed=seed, kappa="lognormal", sigma=sigma,
+                        source=source, source_seed=source_seed)
+
+
+def _random_params(nn, scale=0.1, seed=1):
+    p = nn.get_params()
+    rng = np.random.default_rng(seed)
+    return (p + scale * rng.standard_normal(p.size)).astype(np.float32)
+
+
+@pytest.mark.parametrize("layers", [1, 3])
+def test_forward_matches_oracle(ctx, layers):
+    P = _problem()
+    A = P.matrix(ctx, keep_host=True)
+    feat = G.face_features(P)
+    g = G.NnGraph(ctx, A, feat)
+    nn = G.Mpnn(ctx, P.nf, layers=layers, seed=7)
+    params = _random_params(nn)
+    nn.set_params(params)
+    r1 = np.random.default_rng(2).standard_normal(P.n_unknowns)
+    ref = O.mpnn_forward(P.scipy(), P.nf, feat, params.astype(np.float64), r1, layers=layers)
+    d = torch.zeros(P.n_unknowns, dtype=torch.float64, device="cuda")
+    nn.forward(g, torch.from_numpy(r1).cuda(), d, bf16=False)
+    err32 = np.linalg.norm(d.cpu().numpy() - ref) / np.linalg.norm(ref)
+    assert err32 < 1e-5, err32
+    nn.forward(g, torch.from_numpy(r1).cuda(), d, bf16=True)
+    err16 = np.linalg.norm(d.cpu().numpy() - ref) / np.linalg.norm(ref)
+    assert err16 < 3e-2, err16
+    # homogeneity: the correction scales with the residual
+    nn.forward(g, torch.from_numpy(-2.5 * r1).cuda(), d, bf16=False)
+    assert np.linalg.norm(d.cpu().numpy() + 2.5 * ref) <= 1e-5 * 2.5 * np.linalg.norm(ref)
+
+
+def test_untrained_network_is_block_jacobi(ctx):
+    P = _problem()
+    A = P.matrix(ctx, keep_host=True)
+    g = G.NnGraph(ctx, A, G.face_features(P))
+    nn = G.Mpnn(ctx, P.nf, layers=3, seed=7)
+    r1 = np.random.default_rng(4).standard_normal(P.n_unknowns)
+    d = torch.zeros(P.n_unknowns, dtype=torch.float64, device="cuda")
+    nn.forward(g, torch.from_numpy(r1).cuda(), d, bf16=False)
+    S = P.scipy().toarray()
+    nf = P.nf
+    bj = np.zeros_like(r1)
+    for i in range(P.nb):
+        s = slice(i * nf, (i + 1) * nf)
+        bj[s] = np.linalg.solve(S[s, s], r1[s])
+    assert np.linalg.norm(d.cpu().numpy() - bj) <= 1e-5 * np.linalg.norm(bj)
+
+
+@pytest.mark.parametrize("layers", [1, 3])
+def test_gradient_matches_oracle(ctx, layers):
+    P = _problem(n=5)
+    A = P.matrix(ctx, keep_host=True)
+    feat = G.face_features(P)
+    g = G.NnGraph(ctx, A, feat)
+    nn = G.Mpnn(ctx, P.nf, layers=layers, seed=11)
+    params = _random_params(nn, 0.2, seed=5)
+    nn.set_params(params)
+    rng = np.random.default_rng(6)
+    e1 = rng.standard_normal(P.n_unknowns)
+    S = P.scipy()
+    r1 = S @ e1
+    loss_ref, grad_ref = O.mpnn_loss_grad(S, P.nf, feat, params.astype(np.float64), r1, e1, layers=layers)
+    loss, grad = nn.loss_grad(g, torch.from_numpy(r1).cuda(), torch.from_numpy(e1).cuda())
+    assert abs(loss - loss_ref) <= 1e-5 * abs(loss_ref)
+    scale = np.abs(grad_ref).max()
+    assert np.abs(grad - grad_ref).max() <= 2e-4 * scale, np.abs(grad - grad_ref).max() / scale
+    # deterministic: a second evaluation gives the same bits
+    loss2, grad2 = nn.loss_grad(g, torch.from_numpy(r1).cuda(), torch.from_numpy(e1).cuda())
+    assert loss2 == loss and np.array_equal(grad2, grad)
+
+
+@pytest.fixture(scope="module")
+def trained(ctx):
+    """A network trained on pairs of two operators (seeded lognormal kappa,
+    sigma = 2) and 8 random-mode right-hand sides each."""
+    from fhhos4_amd import learned as Lr
+    systems, pairs = [], []
+    for ks in (11, 12):
+        sy = Lr.System(ctx, _problem(n=24, seed=ks, sigma=2.0), SOLVER)
+        rhs = [_problem(n=24, seed=ks, sigma=2.0, source="modes", source_seed=100 * ks + j).rhs() for j in range(8)]
+        systems.append(sy)
+        pairs += Lr.make_pairs(ctx, sy, rhs, iters=(1, 3, 6))
+    nn = G.Mpnn(ctx, 3, layers=3, seed=1)
+    before = Lr.mean_loss(nn, pairs)
+    hist = Lr.train(nn, pairs, epochs=6, lr=2e-3)
+    return nn, pairs, before, hist, systems
+
+
+def test_training_lowers_the_energy_error(trained):
+    nn, pairs, before, hist, _ = trained
+    print("relative energy error of the corrected error: untrained %.4f, epochs %s" % (before, np.round(hist, 4)))
+    assert len(pairs) >= 40
+    assert hist[-1] < 0.9 * before
+
+
+def test_learned_fcg_gain(ctx, trained):
+    """Unseen operator and right-hand side: FCG + AMG with the trained
+    post-smoother reaches 1e-8 in fewer iterations than with the untrained one
+    (which is block Jacobi with an energy-optimal step), and both give the
+    solution of the plain solver."""
+    from fhhos4_amd import learned as Lr
+    nn, _, _, _, _ = trained
+    sy = Lr.System(ctx, _problem(n=24, seed=13, sigma=2.0), SOLVER)
+    b = _problem(n=24, seed=13, sigma=2.0, source="modes", source_seed=9999).rhs()
+    st0, x0, _ = Lr.solve_with(ctx, sy, b, None, solver_kw=SOLVER)
+    untrained = G.Mpnn(ctx, 3, layers=3, seed=1)
+    st1, x1, _ = Lr.solve_with(ctx, sy, b, untrained, solver_kw=SOLVER)
+    st2, x2, _ = Lr.solve_with(ctx, sy, b, nn, solver_kw=SOLVER)
+    print("iterations: plain %d, block-Jacobi post-step %d, learned %d (accepted %d, fallback %d)"
+          % (st0["iterations"], st1["iterations"], st2["iterations"], st2["nn_used"], st2["nn_fallback"]))
+    for st in (st0, st1, st2):
+        assert st["converged"] == 1
+    assert st2["nn_used"] + st2["nn_fallback"] == st2["iterations"]
+    xr = x0.cpu().numpy()
+    for x in (x1, x2):
+        assert np.linalg.norm(x.cpu().numpy() - xr) <= 1e-6 * np.linalg.norm(xr)
+    assert st2["iterations"] < st1["iterations"]

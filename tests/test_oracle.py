@@ -198,3 +198,61 @@ def test_cg_solution_matches_direct_solve():
     x, _, _ = O.solve(A, d["b"], "cg", "none", nf=2, tol=1e-12, max_iter=1000)
     xd = spla.spsolve(A.tocsc(), d["b"])
     assert np.linalg.norm(x - xd) <= 1e-9 * np.linalg.norm(xd)
+
+
+def _mpnn_case(n=4, seed=3):
+    """Small 2-D perturbed-triangle k=2 operator with lognormal kappa and its
+    face features (built by the library's host-only assembly)."""
+    G = pytest.importorskip("fhhos4_amd").fhgpu
+    P = G.HhoProblem(dim=2, mesh="stri", n=n, k=2, perturb=0.2, seed=seed, kappa="lognormal", sigma=1.0)
+    return P.scipy(), P.nf, G.face_features(P)
+
+
+@pytest.mark.parametrize("layers", [1, 2])
+def test_mpnn_oracle_gradient_matches_finite_differences(layers):
+    """The oracle's backward pass against central differences of its own
+    loss, on random parameters (every parameter block is probed)."""
+    A, nf, feat = _mpnn_case()
+    rng = np.random.default_rng(0)
+    npar = O.mpnn_num_params(nf, layers)
+    params = 0.3 * rng.standard_normal(npar)
+    e1 = rng.standard_normal(A.shape[0])
+    r1 = A @ e1
+    loss, g = O.mpnn_loss_grad(A, nf, feat, params, r1, e1, layers=layers)
+    assert loss > 0
+    idx = np.unique(np.concatenate([rng.choice(npar, 60, replace=False), [0, npar - 1, npar - 1 - nf]]))
+    worst = 0.0
+    for k in idx:
+        h = 1e-6 * max(1.0, abs(params[k]))
+        pp, pm = params.copy(), params.copy()
+        pp[k] += h
+        pm[k] -= h
+        fd = (O.mpnn_loss_grad(A, nf, feat, pp, r1, e1, layers, False)[0] -
+              O.mpnn_loss_grad(A, nf, feat, pm, r1, e1, layers, False)[0]) / (2 * h)
+        worst = max(worst, abs(fd - g[k]))
+    assert worst <= 1e-7 * np.abs(g).max()
+
+
+def test_mpnn_oracle_structure():
+    """Zero output layer: the correction is the block-Jacobi step D^-1 r1 and
+    the loss is its relative energy error; the correction is homogeneous of
+    degree 1 in r1; a perfect correction (delta = e1) has loss 0."""
+    A, nf, feat = _mpnn_case()
+    rng = np.random.default_rng(1)
+    npar = O.mpnn_num_params(nf, 3)
+    params = 0.3 * rng.standard_normal(npar)
+    wo = npar - nf - 32 * nf
+    params[wo:] = 0.0
+    r1 = rng.standard_normal(A.shape[0])
+    d = O.mpnn_forward(A, nf, feat, params, r1)
+    Ad = A.toarray()
+    bj = np.concatenate([np.linalg.solve(Ad[i:i + nf, i:i + nf], r1[i:i + nf]) for i in range(0, len(r1), nf)])
+    np.testing.assert_allclose(d, bj, rtol=1e-12, atol=1e-12 * np.abs(bj).max())
+    params = 0.3 * rng.standard_normal(npar)
+    d1 = O.mpnn_forward(A, nf, feat, params, r1)
+    d2 = O.mpnn_forward(A, nf, feat, params, 3.0 * r1)
+    np.testing.assert_allclose(d2, 3.0 * d1, rtol=1e-12, atol=1e-12 * np.abs(d2).max())
+    e1 = spla.spsolve(A.tocsc(), r1)
+    loss, _ = O.mpnn_loss_grad(A, nf, feat, params, r1, e1, 3, False)
+    diff = e1 - d1
+    assert abs(loss - diff @ (A @ diff) / (e1 @ r1)) <= 1e-12 * loss
