@@ -37,6 +37,25 @@ def spmv_bytes(nb, nf, nnzb, sv=8, sx=8):
     return nnzb * nf * nf * sv + 4 * nnzb + 4 * (nb + 1) + 2 * nb * nf * sx
 
 
+TRAFFIC_SUMMARY = os.path.join(REPO, "profiles", "bcsr_spmv_traffic_tet64_k2.md")
+
+
+def measured_traffic(workload_key):
+    """HBM bytes per launch of the fine-level SpMV from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary of the same workload (scaled
+    by the counters' reading for a copy of known size), or None."""
+    import re
+    try:
+        text = open(TRAFFIC_SUMMARY).read()
+    except OSError:
+        return None
+    m = re.search(r"<!-- traffic: (\{.*?\}) -->", text)
+    if not m:
+        return None
+    d = json.loads(m.group(1))
+    return d.get("f64_bytes") if d.get("workload") == workload_key else None
+
+
 def timed_solves(ctx, solver, b, x, steps, warmup):
     for _ in range(warmup):
         st = solver.solve(b, x)
@@ -151,6 +170,66 @@ def kernel_bench(args, G, np, torch):
     print(json.dumps(out), flush=True)
 
 
+def learned_leg(args, G, ctx, hier, A, P, b, x, x64, t64, v):
+    """FCG with the FP32 multigrid and the learned post-smoother.  The
+    network is trained on the GPU (library kernels) on pairs from plain
+    FCG solves of the same family on a smaller mesh (n = --train-n, random
+    sinusoid right-hand sides), then runs on the full system."""
+    import numpy as np
+    import torch
+    from fhhos4_amd import gmg
+    from fhhos4_amd import learned as Lr
+
+    t0 = time.perf_counter()
+    base = dict(dim=3, mesh="stetra", n=args.train_n, k=args.k, perturb=args.perturb, seed=args.seed)
+    Pt = G.HhoProblem(source="pisine", **base)
+    Ht = gmg.build_hierarchy(ctx, Pt)
+    kw = dict(prec="mg", cycle=v.cycle, nu=(v.nu, v.nu), smoothers=(v.smoother, v.smoother), relax=v.relax,
+              galerkin=v.galerkin, coarse_smoother=v.coarse, coarse_relax=v.coarse_relax)
+
+    class Sys(Lr.System):
+        def __init__(self):
+            self.problem, self.A, self.n = Pt, Ht[0][1], Pt.n_unknowns
+            self.features = G.face_features(Pt)
+            self.graph = G.NnGraph(ctx, self.A, self.features)
+            self.solver = G.Solver(ctx, solver="fcg", tol=1e-12, max_iter=args.max_iter, **kw)
+            gmg.attach(self.solver, Ht)
+            self.solver.setup(self.A)
+            self.tol, self.max_iter = 1e-12, args.max_iter
+
+    sy = Sys()
+    rhs = [G.HhoProblem(source="modes", source_seed=1000 + j, **base).rhs() for j in range(args.train_rhs)]
+    pairs = Lr.make_pairs(ctx, sy, rhs, iters=(1, 3, 6))
+    t_pairs = time.perf_counter() - t0
+    nn = G.Mpnn(ctx, P.nf, layers=3, seed=1)
+    before = Lr.mean_loss(nn, pairs)
+    t1 = time.perf_counter()
+    hist = Lr.train(nn, pairs, epochs=args.train_epochs, lr=2e-3, log=log)
+    t_train = time.perf_counter() - t1
+    log("learned smoother: %d pairs (%.1fs), training %.1fs, energy error %.3f -> %.3f"
+        % (len(pairs), t_pairs, t_train, before, hist[-1]))
+    feats = G.face_features(P)
+    graph = G.NnGraph(ctx, A, feats)
+    s = G.Solver(ctx, solver="fcg", tol=1e-8, max_iter=args.max_iter, mg_prec="f32", mg_store="f32", **kw)
+    gmg.attach(s, hier)
+    s.set_learned_smoother(nn, graph, 1.0)
+    t2 = time.perf_counter()
+    s.setup(A)
+    su = time.perf_counter() - t2
+    ts, st = timed_solves(ctx, s, b, x, args.steps, args.warmup)
+    xm = x.cpu().numpy()
+    out = {"s": float(np.mean(ts)), "iterations": st["iterations"], "setup_s": su, "mg": "f32",
+           "accepted": st["nn_used"], "fallback": st["nn_fallback"], "params": nn.num_params(),
+           "train": {"n": args.train_n, "pairs": len(pairs), "pairs_s": t_pairs, "train_s": t_train,
+                     "epochs": args.train_epochs, "energy_error_untrained": before, "energy_error_trained": hist[-1]},
+           "rel_err_vs_fp64": float(np.linalg.norm(xm - x64) / np.linalg.norm(x64)),
+           "l2": P.l2_error(xm), "speedup_vs_fp64": t64 / float(np.mean(ts))}
+    log("FCG + f32 MG + learned post-smoother: %d iterations (%d accepted, %d fallback), %.3f s"
+        % (st["iterations"], st["nn_used"], st["nn_fallback"], out["s"]))
+    del s, graph
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -172,6 +251,10 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--train-n", type=int, default=16, help="mesh of the learned smoother's training problems")
+    ap.add_argument("--train-rhs", type=int, default=8)
+    ap.add_argument("--train-epochs", type=int, default=6)
     ap.add_argument("--sweep", action="store_true", help="time a list of solver variants")
     ap.add_argument("--kernel", choices=["spmv"], help="time one kernel family alone")
     ap.add_argument("--reps", type=int, default=20)
@@ -263,6 +346,10 @@ def main():
             log("FCG + %s MG: %d iterations, %.3f s" % (store, st["iterations"], mixed[store]["s"]))
             del s
 
+    learned = None
+    if not args.no_learned:
+        learned = learned_leg(args, G, ctx, hier, A, P, b, x, x64, t64, v)
+
     cpu = None
     if not args.no_cpu_baseline:
         from oracle import oracle as O
@@ -287,8 +374,11 @@ def main():
         "iterations": st64["iterations"], "setup_s": setup64, "hierarchy_s": hier_s, "l2_error": l2_64,
         "dof_it_per_s": N * st64["iterations"] / t64,
         "mixed_fcg": mixed,
+        "learned_fcg": learned,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic("stetra n=%d k=%d perturb %g seed %d" % (args.n, args.k, args.perturb,
+                                                                                         args.seed)),
                      "kernel": "k_bcsr<6,double,double,double,OP_SPMV> (fine level)",
                      "bytes_per_launch": bytes_fine, "mean_ms": spmv_ms, "launches": nl,
                      "smooth_mean_ms": smooth_ms, "smooth_launches": nls},

@@ -574,6 +574,7 @@ int fh_hho_create(const fh_hho_args* a, fh_hho** out) {
         h.ref_n = a->ref_n;
         h.source_seed = a->source_seed;
         h.source_modes = a->source_modes > 0 ? a->source_modes : 8;
+        h.host_assembly = a->defer_assembly == 0;
         FH_CHECK(h.dim == 2 || h.dim == 3, -2, "dim must be 2 or 3");
         auto o = new fh_hho;
         try {
@@ -606,15 +607,30 @@ int fh_hho_matrix(fh_ctx* ctx, const fh_hho* h, int keep_host, fh_mat** out) {
     return guard([&] {
         NEED(ctx); NEED(h); NEED(out);
         const auto& p = *h->p;
+        FH_CHECK(p.host_assembled, -2, "problem created with deferred assembly: use fh_hho_assemble_gpu");
         auto m = new fh_mat;
         m->A = fh::bcsr_from_blocks(ctx->c, p.nb, p.nb, p.nf, p.nf, p.rowptr.data(), p.colidx.data(), p.vals.data(),
                                     keep_host != 0);
         *out = m;
     });
 }
+int fh_hho_assemble_gpu(fh_ctx* ctx, const fh_hho* h, int keep_host, fh_mat** out, double* b_dev) {
+    return guard([&] {
+        NEED(ctx); NEED_DEV(ctx); NEED(h); NEED(out); NEED(b_dev);
+        auto m = new fh_mat;
+        try {
+            m->A = h->p->AssembleDevice(ctx->c, b_dev, keep_host != 0);
+        } catch (...) {
+            delete m;
+            throw;
+        }
+        *out = m;
+    });
+}
 int fh_hho_rhs(const fh_hho* h, double* b) {
     return guard([&] {
         NEED(h); NEED(b);
+        FH_CHECK(h->p->host_assembled, -2, "problem created with deferred assembly: the right-hand side is on the device");
         std::memcpy(b, h->p->b.data(), sizeof(double) * h->p->b.size());
     });
 }
@@ -624,7 +640,10 @@ int fh_hho_blocks(const fh_hho* h, int* rowptr, int* colidx, double* vals) {
         const auto& p = *h->p;
         if (rowptr) std::memcpy(rowptr, p.rowptr.data(), sizeof(int) * p.rowptr.size());
         if (colidx) std::memcpy(colidx, p.colidx.data(), sizeof(int) * p.colidx.size());
-        if (vals) std::memcpy(vals, p.vals.data(), sizeof(double) * p.vals.size());
+        if (vals) {
+            FH_CHECK(p.host_assembled, -2, "problem created with deferred assembly: no host values");
+            std::memcpy(vals, p.vals.data(), sizeof(double) * p.vals.size());
+        }
     });
 }
 int fh_hho_l2_error(const fh_hho* h, const double* x, double* err) {

@@ -10,6 +10,7 @@
 #include <unordered_map>
 
 #include "fh_common.h"
+#include "fh_hho_dev.h"
 
 namespace fh {
 namespace {
@@ -903,6 +904,13 @@ HhoProblem::HhoProblem(const HhoArgs& args) : impl(new Impl) {
     rows.clear();
     rows.shrink_to_fit();
     const size_t bsz = (size_t)nf * nf;
+    if (!args.host_assembly) {
+        // values come from AssembleDevice
+        auto t3 = std::chrono::steady_clock::now();
+        times.assemble_s = std::chrono::duration<double>(t3 - t1).count();
+        return;
+    }
+    host_assembled = true;
     vals.assign((size_t)rowptr[nb] * bsz, 0.0);
     b.assign((size_t)nb * nf, 0.0);
     auto t2 = std::chrono::steady_clock::now();
@@ -1159,6 +1167,91 @@ double HhoProblem::L2Error(const double* x) const {
         }
     }
     return nrm2 > 0 ? std::sqrt(err / nrm2) : std::sqrt(err);
+}
+
+std::unique_ptr<Bcsr> HhoProblem::AssembleDevice(Ctx& c, double* b_dev, bool keep_host) const {
+    FH_CHECK(!c.host_only, -9, "device assembly needs a device context");
+    const Impl& I = *impl;
+    const int m = I.nfc;
+    auto t0 = std::chrono::steady_clock::now();
+    // per-cell maps: block of every (face, face) pair and block row of every face
+    std::vector<int> blk((size_t)I.ncell * m * m, -1), frow((size_t)I.ncell * m, -1);
+#pragma omp parallel for schedule(static)
+    for (int cc = 0; cc < I.ncell; cc++) {
+        int rr[8];
+        for (int f = 0; f < m; f++) rr[f] = frow[(size_t)cc * m + f] = I.funk[I.cf[(size_t)cc * m + f]];
+        for (int f = 0; f < m; f++) {
+            if (rr[f] < 0) continue;
+            const int* beg = &colidx[rowptr[rr[f]]];
+            const int* end = &colidx[rowptr[rr[f] + 1]];
+            for (int g = 0; g < m; g++)
+                if (rr[g] >= 0) blk[((size_t)cc * m + f) * m + g] = (int)(std::lower_bound(beg, end, rr[g]) - &colidx[0]);
+        }
+    }
+    std::vector<double> Vf(3 * I.V.size()), cqp(3 * I.cq.p.size()), modes;
+    for (size_t v = 0; v < I.V.size(); v++)
+        for (int a = 0; a < 3; a++) Vf[3 * v + a] = I.V[v][a];
+    for (size_t q = 0; q < I.cq.p.size(); q++)
+        for (int a = 0; a < 3; a++) cqp[3 * q + a] = I.cq.p[q][a];
+    for (const auto& md : I.modes) modes.insert(modes.end(), {md.a, (double)md.p[0], (double)md.p[1], (double)md.p[2]});
+    std::vector<double> Phi, Psi, MFT;
+    for (const auto& T : I.tables) {
+        Phi.insert(Phi.end(), T.Phi.begin(), T.Phi.end());
+        Psi.insert(Psi.end(), T.Psi.begin(), T.Psi.end());
+        MFT.insert(MFT.end(), T.MFT.begin(), T.MFT.end());
+    }
+    std::vector<DevBuf> keep;
+    auto up = [&](const void* h, size_t bytes) {
+        keep.emplace_back(std::max<size_t>(bytes, 8));
+        if (bytes) FH_HIP(hipMemcpyAsync(keep.back().p, h, bytes, hipMemcpyHostToDevice, c.stream));
+        return keep.back().p;
+    };
+    HhoDev d{};
+    d.dim = I.dim; d.kind = I.kind; d.nvc = I.nvc; d.nfc = m; d.nvf = I.nvf;
+    d.nc = I.nc; d.nr = I.nr; d.nfb = I.nfb; d.nq = (int)I.cq.p.size(); d.ncell = I.ncell;
+    d.V = (const double*)up(Vf.data(), sizeof(double) * Vf.size());
+    d.cv = (const int*)up(I.cv.data(), sizeof(int) * I.cv.size());
+    d.cf = (const int*)up(I.cf.data(), sizeof(int) * I.cf.size());
+    d.fv = (const int*)up(I.fv.data(), sizeof(int) * I.fv.size());
+    d.K = (const double*)up(I.K.data(), sizeof(double) * I.K.size());
+    d.cft = (const int*)up(I.cell_face_table.data(), sizeof(int) * I.cell_face_table.size());
+    d.Shat = (const double*)up(I.Shat.data(), sizeof(double) * I.Shat.size());
+    d.cellPsi = (const double*)up(I.cellPsi.data(), sizeof(double) * I.cellPsi.size());
+    d.cqp = (const double*)up(cqp.data(), sizeof(double) * cqp.size());
+    d.cqw = (const double*)up(I.cq.w.data(), sizeof(double) * I.cq.w.size());
+    d.Phi = (const double*)up(Phi.data(), sizeof(double) * Phi.size());
+    d.Psi = (const double*)up(Psi.data(), sizeof(double) * Psi.size());
+    d.MFT = (const double*)up(MFT.data(), sizeof(double) * MFT.size());
+    d.src = I.a.source == "modes" ? 2 : (I.a.source == "sine" ? 1 : 0);
+    d.nmodes = (int)I.modes.size();
+    d.modes = (const double*)up(modes.data(), sizeof(double) * modes.size());
+    d.blk = (const int*)up(blk.data(), sizeof(int) * blk.size());
+    d.frow = (const int*)up(frow.data(), sizeof(int) * frow.size());
+    auto A = std::make_unique<Bcsr>();
+    A->nbr = A->nbc = nb;
+    A->bs_r = A->bs_c = nf;
+    A->nnzb = rowptr[nb];
+    A->prec = Prec::F64;
+    A->h_rowptr = rowptr;
+    A->h_colidx = colidx;
+    A->rowptr.alloc(sizeof(int) * rowptr.size());
+    A->colidx.alloc(sizeof(int) * std::max<size_t>(1, colidx.size()));
+    FH_HIP(hipMemcpyAsync(A->rowptr.p, rowptr.data(), sizeof(int) * rowptr.size(), hipMemcpyHostToDevice, c.stream));
+    FH_HIP(hipMemcpyAsync(A->colidx.p, colidx.data(), sizeof(int) * colidx.size(), hipMemcpyHostToDevice, c.stream));
+    const size_t nval = (size_t)A->nnzb * nf * nf;
+    A->vals.alloc(sizeof(double) * std::max<size_t>(1, nval));
+    FH_HIP(hipMemsetAsync(A->vals.p, 0, sizeof(double) * nval, c.stream));
+    FH_HIP(hipMemsetAsync(b_dev, 0, sizeof(double) * (size_t)nb * nf, c.stream));
+    d.vals = A->vals.as<double>();
+    d.b = b_dev;
+    launch_hho_local(c, d);  // synchronises
+    if (keep_host) {
+        A->h_vals.resize(nval);
+        FH_HIP(hipMemcpyAsync(A->h_vals.data(), A->vals.p, sizeof(double) * nval, hipMemcpyDeviceToHost, c.stream));
+        c.sync();
+    }
+    device_assemble_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return A;
 }
 
 }  // namespace fh

@@ -17,6 +17,8 @@
 #include <string>
 #include <vector>
 
+#include "fh_common.h"
+
 namespace fh {
 
 struct HhoArgs {
@@ -34,6 +36,7 @@ struct HhoArgs {
     double sigma = 0.0;            // lognormal standard deviation
     int subdomain = 8;             // cells per subdomain edge ("subdomain"), counted on the reference grid
     double aniso = 1e-2;           // z-scaling of the anisotropic subdomains ("subdomain")
+    bool host_assembly = true;     // false: only mesh, tables and pattern on the host (AssembleDevice)
     int ref_n = 0;                 // finest grid of the hierarchy (0: n).  Vertex perturbations and
                                    // subdomains are defined on it, so a mesh with n = ref_n / 2^l
                                    // is the geometric coarsening of the finest one.
@@ -68,6 +71,12 @@ public:
     void Prolongation(const HhoProblem& coarse, std::vector<int>& prp, std::vector<int>& pci,
                       std::vector<double>& pv) const;
     HhoTimes times;
+    bool host_assembled = false;       // vals and b hold the host assembly
+    mutable double device_assemble_s = 0;
+    // Local operators, static condensation and assembly on the GPU (one
+    // workgroup per cell): the block operator on the device (the host
+    // pattern rowptr / colidx) and the right-hand side into b_dev (nb*nf).
+    std::unique_ptr<Bcsr> AssembleDevice(Ctx& c, double* b_dev, bool keep_host) const;
     struct Impl;
     Impl* impl = nullptr;
 };

@@ -69,6 +69,7 @@ _SIGS = {
     "fh_nn_train_step": [_vp, _vp, _vp, _vp, _vp, C.c_float, C.POINTER(C.c_double)],
     "fh_solver_set_nn": [_vp, _vp, _vp, C.c_double],
     "fh_solver_set_stop": [_vp, C.c_double, C.c_int],
+    "fh_hho_assemble_gpu": [_vp, _vp, C.c_int, _pp, _vp],
     "fh_solver_apply_prec": [_vp, _vp, _vp],
     "fh_mat_export_market": [_vp, _vp, C.c_char_p],
     "fh_mat_import_market": [_vp, C.c_char_p, C.c_int, C.c_int, _pp],
@@ -116,7 +117,7 @@ class HhoArgs(C.Structure):
     _fields_ = [("dim", C.c_int), ("mesh", C.c_char_p), ("n", C.c_int), ("k", C.c_int), ("perturb", C.c_double),
                 ("seed", C.c_uint64), ("source", C.c_char_p), ("kappa", C.c_char_p), ("sigma", C.c_double),
                 ("subdomain", C.c_int), ("aniso", C.c_double), ("ref_n", C.c_int), ("source_seed", C.c_uint64),
-                ("source_modes", C.c_int)]
+                ("source_modes", C.c_int), ("defer_assembly", C.c_int)]
 
 
 class HhoInfo(C.Structure):
@@ -149,6 +150,34 @@ def _check(rc):
     if rc < 0:
         raise FhError("fhgpu error %d: %s" % (rc, lib().fh_last_error().decode()))
     return rc
+
+
+def _device_sync():
+    """Device-wide synchronisation.  The library runs on its own non-blocking
+    stream: inputs written by torch must be complete before its kernels read
+    them, and its outputs complete before torch (or the host) reads them or
+    frees a temporary input."""
+    try:
+        import torch
+    except ImportError:
+        return
+    if torch.cuda.is_available():
+        torch.cuda.synchronize()
+
+
+def _ordered(fn):
+    """Wraps a call that passes device tensors to the library with device
+    synchronisations before and after it."""
+    import functools
+
+    @functools.wraps(fn)
+    def wrapper(*a, **k):
+        _device_sync()
+        try:
+            return fn(*a, **k)
+        finally:
+            _device_sync()
+    return wrapper
 
 
 def _ptr(t):
@@ -280,20 +309,24 @@ def read_market_vector(path):
     return v
 
 
+@_ordered
 def spmv(ctx, A, x, y, alpha=1.0, beta=0.0):
     _check(lib().fh_spmv(ctx.h, A.h, alpha, _ptr(x), beta, _ptr(y), _vprec(x)))
 
 
+@_ordered
 def residual(ctx, A, b, x, r):
     _check(lib().fh_residual(ctx.h, A.h, _ptr(b), _ptr(x), _ptr(r), _vprec(x)))
 
 
+@_ordered
 def dot(ctx, x, y):
     out = C.c_double()
     _check(lib().fh_dot(ctx.h, x.numel(), _ptr(x), _ptr(y), _vprec(x), C.byref(out)))
     return out.value
 
 
+@_ordered
 def axpby(ctx, a, x, b, y):
     _check(lib().fh_axpby(ctx.h, x.numel(), a, _ptr(x), b, _ptr(y), _vprec(x)))
 
@@ -309,6 +342,7 @@ class BlockJacobi:
         _check(lib().fh_bjac_inverses(self.ctx.h, self.h, out))
         return out
 
+    @_ordered
     def sweep(self, b, x, work, nsweeps=1, x_is_zero=False, A=None):
         _check(lib().fh_bjac_sweep(self.ctx.h, self.h, (A or self.A).h, _ptr(b), _ptr(x), _ptr(work), nsweeps,
                                    int(x_is_zero), _vprec(x)))
@@ -339,6 +373,7 @@ class Colouring:
             pass
 
 
+@_ordered
 def mcgs_sweep(ctx, A, bj, col, direction, b, x):
     _check(lib().fh_mcgs_sweep(ctx.h, A.h, bj.h, col.h, direction, _ptr(b), _ptr(x), _vprec(x)))
 
@@ -390,6 +425,7 @@ class Solver:
         self.max_iter = max_iter
         _check(lib().fh_solver_set_stop(self.h, tol, max_iter))
 
+    @_ordered
     def apply_prec(self, r, z):
         """z = M r with the preconditioner alone (device FP64 vectors)."""
         _check(lib().fh_solver_apply_prec(self.h, _ptr(r), _ptr(z)))
@@ -407,6 +443,7 @@ class Solver:
         self.A = A
         _check(lib().fh_solver_setup(self.h, A.h))
 
+    @_ordered
     def solve(self, b, x):
         st = SolveStats()
         _check(lib().fh_solver_solve(self.h, _ptr(b), _ptr(x), C.byref(st)))
@@ -431,15 +468,16 @@ class HhoProblem:
     """Condensed HHO face system of -div(K grad u) = f on a structured mesh."""
 
     def __init__(self, dim=3, mesh="stetra", n=8, k=2, perturb=0.0, seed=1, source="pisine", kappa="one",
-                 sigma=0.0, subdomain=8, aniso=1e-2, ref_n=0, source_seed=1, source_modes=8):
+                 sigma=0.0, subdomain=8, aniso=1e-2, ref_n=0, source_seed=1, source_modes=8, defer_assembly=False):
         a = HhoArgs()
         self._strs = [mesh.encode(), source.encode(), kappa.encode()]
         a.dim, a.mesh, a.n, a.k, a.perturb, a.seed = dim, self._strs[0], n, k, perturb, seed
         a.source, a.kappa, a.sigma, a.subdomain, a.aniso = self._strs[1], self._strs[2], sigma, subdomain, aniso
         a.ref_n, a.source_seed, a.source_modes = ref_n, source_seed, source_modes
+        a.defer_assembly = int(defer_assembly)
         self.args = dict(dim=dim, mesh=mesh, n=n, k=k, perturb=perturb, seed=seed, source=source, kappa=kappa,
                          sigma=sigma, subdomain=subdomain, aniso=aniso, ref_n=ref_n, source_seed=source_seed,
-                         source_modes=source_modes)
+                         source_modes=source_modes, defer_assembly=defer_assembly)
         h = C.c_void_p()
         _check(lib().fh_hho_create(C.byref(a), C.byref(h)))
         self.h = h
@@ -448,6 +486,16 @@ class HhoProblem:
         self.info = {k: getattr(info, k) for k, _ in info._fields_}
         self.nb, self.nf, self.nnzb = info.nb, info.nf, info.nnzb
         self.n_unknowns = info.nb * info.nf
+
+    @_ordered
+    def assemble_gpu(self, ctx, keep_host=False):
+        """Local operators + static condensation + assembly on the GPU:
+        returns (device Matrix, device right-hand side tensor)."""
+        import torch
+        b = torch.zeros(self.n_unknowns, dtype=torch.float64, device="cuda")
+        h = C.c_void_p()
+        _check(lib().fh_hho_assemble_gpu(ctx.h, self.h, int(keep_host), C.byref(h), _ptr(b)))
+        return Matrix(ctx, h), b
 
     def matrix(self, ctx, keep_host=True):
         h = C.c_void_p()
@@ -529,9 +577,11 @@ class Mpnn:
         assert p.size == self.num_params()
         _check(lib().fh_nn_set_params(self.h, p))
 
+    @_ordered
     def forward(self, graph, r1, delta, bf16=True):
         _check(lib().fh_nn_forward(self.ctx.h, self.h, graph.h, _ptr(r1), _ptr(delta), int(bf16)))
 
+    @_ordered
     def loss_grad(self, graph, r1, e1, want_grad=True):
         loss = C.c_double()
         g = np.zeros(self.num_params(), dtype=np.float32) if want_grad else None
@@ -539,6 +589,7 @@ class Mpnn:
                                       g.ctypes.data if g is not None else None))
         return loss.value, g
 
+    @_ordered
     def train_step(self, graph, r1, e1, lr):
         loss = C.c_double()
         _check(lib().fh_nn_train_step(self.ctx.h, self.h, graph.h, _ptr(r1), _ptr(e1), lr, C.byref(loss)))

@@ -257,6 +257,7 @@ typedef struct fh_hho_args {
     int ref_n;              /* finest grid of a hierarchy (0: n); perturbation and subdomains live on it */
     uint64_t source_seed;   /* "modes" source: seed and number (1..64) of random sinusoid modes */
     int source_modes;
+    int defer_assembly;     /* 1: no host assembly at creation (use fh_hho_assemble_gpu) */
 } fh_hho_args;
 typedef struct fh_hho_info {
     int nb, nf, ncells, nfaces;
@@ -278,6 +279,12 @@ int fh_hho_l2_error(const fh_hho* h, const double* x, double* err);
  * mesh (coarse->n = fine->n / 2, same ref_n) to the fine one: nf x nf block
  * matrix (fine faces x coarse faces).  Replaces LevelForHHO::SetupProlongation_H
  * with CellInterp_Trace (src/Solver/Multigrid/MultigridForHHO/LevelForHHO.h:182-213). */
+/* Local HHO operators (Diff_HHOElement::InitHHO, Diff_HHOElement.h:104-150,
+ * 312-509), static condensation (Diffusion_HHO.h:624-652) and assembly on
+ * the GPU: one workgroup per cell, the cell matrices in LDS.  *out receives
+ * the device operator (host copy of the values with keep_host), b_dev (device,
+ * nb*nf FP64) the condensed right-hand side. */
+int fh_hho_assemble_gpu(fh_ctx* ctx, const fh_hho* h, int keep_host, fh_mat** out, double* b_dev);
 int fh_hho_prolongation(fh_ctx* ctx, const fh_hho* fine, const fh_hho* coarse, int keep_host, fh_mat** out);
 /* Cells of every interior face (2 ints per block row) and cell kappa. */
 int fh_hho_face_cells(const fh_hho* h, int* out);
