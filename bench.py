@@ -7,8 +7,10 @@ on the unit cube, 64^3 Cartesian cubes split into 6 tetrahedra (1.57 M
 cells), interior vertices perturbed by a seeded uniform +-0.2 h, HHO k = 2
 (6 unknowns per triangular face, 3.1 M faces, 18.7 M unknowns), kappa = 1,
 f = 3 pi^2 prod sin(pi x_i).  The plain method is FP64 CG preconditioned by
-one multigrid V-cycle with block smoothing; beside it: FP64 FCG with an FP32
-(and a BF16-stored) multigrid preconditioner.
+one V(1,1)-cycle of geometric multigrid with Galerkin coarse operators and
+degree-2 Chebyshev smoothing of D^-1 A (D = block diagonal); beside it: FP64
+FCG with an FP32 (and a BF16-stored) multigrid preconditioner, and FP64 FCG
+with the FP32 multigrid and the learned post-smoother.
 
 Prints one JSON line; progress goes to stderr.  --sweep runs a list of
 solver variants and prints one JSON line per variant instead.
@@ -129,12 +131,14 @@ def kernel_bench(args, G, np, torch):
     timed with HIP events on the library stream, plus a device copy of known
     size (bandwidth reference and calibration of the HBM byte counters)."""
     P = G.HhoProblem(dim=3, mesh="stetra", n=args.n, k=args.k, perturb=args.perturb, seed=args.seed,
-                     source="pisine")
+                     source="pisine", defer_assembly=True)
     nb, nf, nnzb = P.nb, P.nf, P.nnzb
     N = nb * nf
     ctx = G.Context(0)
-    A64 = P.matrix(ctx, keep_host=False)
-    del P
+    t = time.perf_counter()
+    A64, _ = P.assemble_gpu(ctx, keep_host=False)
+    ctx.sync()
+    log("GPU assembly of %d cells: %.3f s" % (P.info["ncells"], time.perf_counter() - t))
     out = {"kernel": "bcsr_spmv", "n": args.n, "unknowns": N, "blocks": nnzb, "reps": args.reps}
     x = torch.rand(N, dtype=torch.float64, device="cuda")
     for name, A, xv, sv, sx in (("f64", A64, x, 8, 8), ("f32", A64.convert("f32"), x.float(), 4, 4),
@@ -241,7 +245,7 @@ def main():
     ap.add_argument("--seed", type=int, default=2026)
     ap.add_argument("--prec", default="gmg", choices=["gmg", "amg"])
     ap.add_argument("--no-galerkin", action="store_true")
-    ap.add_argument("--smoother", default="sbgs")
+    ap.add_argument("--smoother", default="cheb2")
     ap.add_argument("--relax", type=float, default=1.0)
     ap.add_argument("--nu", type=int, default=1)
     ap.add_argument("--cycle", default="V")
@@ -273,19 +277,19 @@ def main():
     assert torch.cuda.is_available(), "no ROCm device"
     if args.kernel == "spmv":
         return kernel_bench(args, G, np, torch)
-    log("assembling HHO system n=%d k=%d (perturb %.2f)" % (args.n, args.k, args.perturb))
+    log("mesh and pattern of the HHO system n=%d k=%d (perturb %.2f)" % (args.n, args.k, args.perturb))
     P = G.HhoProblem(dim=3, mesh="stetra", n=args.n, k=args.k, perturb=args.perturb, seed=args.seed,
-                     source="pisine")
+                     source="pisine", defer_assembly=True)
     nb, nf, nnzb = P.nb, P.nf, P.nnzb
     N = nb * nf
-    log("system: %d faces x %d = %d unknowns, %d blocks, assembly %.1fs" % (nb, nf, N, nnzb, P.info["assemble_s"]))
+    log("system: %d faces x %d = %d unknowns, %d blocks" % (nb, nf, N, nnzb))
     ctx = G.Context(0)
     t = time.perf_counter()
-    hier = gmg.build_hierarchy(ctx, P)
+    hier = gmg.build_hierarchy(ctx, P)  # GPU local operators + condensation on every level
     hier_s = time.perf_counter() - t
     A = hier[0][1]
-    log("geometric hierarchy", [h[0].n_unknowns for h in hier], "%.1fs" % hier_s)
-    b = torch.from_numpy(P.rhs()).cuda()
+    log("geometric hierarchy", [h[0].n_unknowns for h in hier], "%.1fs (GPU assembly, host transfers)" % hier_s)
+    b = P.b_device
     x = torch.zeros_like(b)
     torch.cuda.synchronize()
 

@@ -411,6 +411,9 @@ int fh_solver_create(fh_ctx* ctx, const fh_solver_args* a, fh_solver** out) {
         s.galerkin = a->galerkin != 0;
         if (a->coarse_smoother) s.coarse_smoother = a->coarse_smoother;
         s.coarse_relax = a->coarse_relax;
+        s.fcg_truncation = a->fcg_truncation > 0 ? a->fcg_truncation : 1;
+        FH_CHECK(s.fcg_truncation <= 32, -2, "FCG truncation is limited to 32 directions");
+        s.fcg_restart = a->fcg_restart != 0;
         if (!s.coarse_smoother.empty())
             (void)fh::SmootherSpec::parse(s.coarse_smoother, 1, s.coarse_relax > 0 ? s.coarse_relax : s.relax);
         FH_CHECK(s.mg_prec != fh::Prec::BF16, -2, "multigrid vectors are FP64 or FP32");
@@ -526,11 +529,18 @@ int fh_nn_train_step(fh_ctx* ctx, fh_nn* nn, const fh_nngraph* g, const double* 
         if (loss) *loss = l;
     });
 }
-int fh_solver_set_nn(fh_solver* s, fh_nn* nn, const fh_nngraph* g, double omega) {
+int fh_solver_set_nn(fh_solver* s, fh_nn* nn, const fh_nngraph* g, double omega, int unit_step) {
     return guard([&] {
         NEED(s);
         FH_CHECK((nn == nullptr) == (g == nullptr), -2, "network and graph go together");
-        s->s->SetLearnedSmoother(nn ? nn->m.get() : nullptr, g ? g->g.get() : nullptr, omega > 0 ? omega : 1.0);
+        s->s->SetLearnedSmoother(nn ? nn->m.get() : nullptr, g ? g->g.get() : nullptr, omega > 0 ? omega : 1.0,
+                                 unit_step != 0);
+    });
+}
+int fh_solver_export_levels(fh_solver* s, const char* dir) {
+    return guard([&] {
+        NEED(s); NEED(dir);
+        s->s->ExportLevels(dir);
     });
 }
 int fh_solver_set_stop(fh_solver* s, double tol, int max_iter) {

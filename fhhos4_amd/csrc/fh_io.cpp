@@ -167,6 +167,19 @@ void market_read_csr(const std::string& path, int& nrows, int& ncols, std::vecto
     }
 }
 
+void market_write_csr(int nrows, int ncols, const int* rp, const int* ci, const double* v, const std::string& path) {
+    File out(path, "wb");
+    std::fprintf(out.f, "%%%%MatrixMarket matrix coordinate  real general\n%d %d %d\n", nrows, ncols, rp[nrows]);
+    char buf[160];
+    for (int i = 0; i < nrows; i++)
+        for (int p = rp[i]; p < rp[i + 1]; p++) {
+            int k = std::snprintf(buf, sizeof buf, "%d %d ", i + 1, ci[p] + 1);
+            k += put_value(buf + k, sizeof buf - k, v[p]);
+            buf[k++] = '\n';
+            FH_CHECK(std::fwrite(buf, 1, k, out.f) == (size_t)k, -11, "write error on " + path);
+        }
+}
+
 void market_write_vector(int64_t n, const double* v, const std::string& path) {
     File out(path, "wb");
     std::fprintf(out.f, "%%%%MatrixMarket matrix array real general\n%lld 1\n", (long long)n);

@@ -21,6 +21,9 @@ void market_write_matrix(Ctx& c, const Bcsr& A, const std::string& path);
 void market_read_csr(const std::string& path, int& nrows, int& ncols, std::vector<int>& rowptr,
                      std::vector<int>& colidx, std::vector<double>& vals);
 
+// Writes a scalar CSR matrix (columns ascending in every row).
+void market_write_csr(int nrows, int ncols, const int* rp, const int* ci, const double* v, const std::string& path);
+
 void market_write_vector(int64_t n, const double* v, const std::string& path);
 // Reads an "array real general" n x 1 file.
 std::vector<double> market_read_vector(const std::string& path);

@@ -276,17 +276,20 @@ __global__ void k_nn_gather_index(int64_t nnzb, const int* __restrict__ idx, con
 // s1 = delta.r1, s2 = delta.A delta.  If s1 > 0 and s2 > 0 the correction
 // alpha delta with alpha = s1 / s2 lowers the energy norm of the error by
 // s1^2 / s2; z += alpha delta.  Otherwise z += w D^-1 r1 (block Jacobi).
+// With unit_step the correction is delta itself, accepted when it lowers the
+// energy norm: delta.A delta - 2 delta.r1 < 0.
 __global__ void k_nn_select(int64_t count, int nf, const double* s, int slot_s1, int slot_s2,
                             const double* __restrict__ delta, const double* __restrict__ r1,
-                            const double* __restrict__ invD, double omega, double* __restrict__ z,
+                            const double* __restrict__ invD, double omega, int unit_step, double* __restrict__ z,
                             int* __restrict__ counters) {
     const double s1 = s[slot_s1], s2 = s[slot_s2];
-    const bool use = s1 > 0 && s2 > 0;
+    const bool use = unit_step ? (s2 - 2.0 * s1 < 0) : (s1 > 0 && s2 > 0);
+    const double alpha = unit_step ? 1.0 : s1 / s2;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t == 0) counters[use ? 0 : 1] += 1;
     if (t >= count) return;
     if (use) {
-        z[t] += (s1 / s2) * delta[t];
+        z[t] += alpha * delta[t];
     } else {
         const int64_t i = t / nf;
         const int a = (int)(t - i * nf);
@@ -394,9 +397,9 @@ void nn_gather_index(Ctx& c, int64_t nnzb, const int* idx, const float* src, flo
     FH_HIP(hipGetLastError());
 }
 void nn_select(Ctx& c, int64_t count, int nf, int slot_s1, int slot_s2, const double* delta, const double* r1,
-               const double* invD, double omega, double* z, int* counters) {
+               const double* invD, double omega, bool unit_step, double* z, int* counters) {
     hipLaunchKernelGGL(k_nn_select, dim3(g1(count)), dim3(256), 0, c.stream, count, nf, c.d_scalars, slot_s1, slot_s2,
-                       delta, r1, invD, omega, z, counters);
+                       delta, r1, invD, omega, (int)unit_step, z, counters);
     FH_HIP(hipGetLastError());
 }
 

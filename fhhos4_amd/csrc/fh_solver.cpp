@@ -4,8 +4,10 @@
 #include <chrono>
 #include <cmath>
 #include <cstdlib>
+#include <algorithm>
 #include <utility>
 
+#include "fh_io.h"
 #include "fh_kernels.h"
 #include "fh_nn.h"
 
@@ -15,6 +17,7 @@ namespace fh {
 enum : int {
     S_RZ = 0, S_RZOLD = 1, S_DAD = 2, S_RR = 3, S_RD = 4, S_ZADK = 5, S_DKADK = 6, S_BB = 7,
     S_KBASE = 32,  // K-cycle: 8 slots per level
+    S_FCG = 400,   // FCG(m): 2 slots per kept direction (m <= 32)
 };
 
 Ctx::Ctx(int dev) : device(dev) {
@@ -291,7 +294,9 @@ public:
             graph_exec_ = nullptr;
         }
         graph_mode_ = 0;
-        // eager reference time (this application's result stays in z)
+        // eager reference time, after one warm-up application (first launches
+        // load code objects); each application leaves the same result in z
+        apply_eager(r, z);
         c_.sync();
         auto t0 = std::chrono::steady_clock::now();
         apply_eager(r, z);
@@ -570,14 +575,58 @@ void Solver::SetLevel(int l, const Bcsr* A, const Bcsr* P) {
     given_levels_[l] = {A, P};
 }
 
-void Solver::SetLearnedSmoother(Mpnn* nn, const NnGraph* g, double omega) {
+void Solver::SetLearnedSmoother(Mpnn* nn, const NnGraph* g, double omega, bool unit_step) {
     FH_CHECK((nn == nullptr) == (g == nullptr), -2, "network and graph go together");
     nn_ = nn;
     nng_ = g;
     nn_omega_ = omega;
+    nn_unit_step_ = unit_step;
 }
 
 int Solver::NumLevels() const { return mg_ ? (int)mg_->L.size() : 1; }
+void Solver::ExportLevels(const std::string& dir) {
+    FH_CHECK(mg_ != nullptr, -2, "the solver has no multigrid hierarchy");
+    const auto& L = mg_->L;
+    for (size_t l = 0; l < L.size(); l++) {
+        const MgLevel& V = L[l];
+        const std::string pre = dir + "/level" + std::to_string(l) + "_";
+        market_write_matrix(c_, *V.A64, pre + "A.dat");
+        if (l + 1 == L.size()) continue;
+        if (V.P64) {
+            market_write_matrix(c_, *V.P64, pre + "P.dat");
+            market_write_matrix(c_, *V.R64_own, pre + "R.dat");
+            continue;
+        }
+        // transfers without a stored matrix: p-injection or aggregation
+        const int nf = V.A64->bs_r, nb = V.A64->nbr;
+        const int nfc = V.ptrunc ? V.nfc : nf;
+        std::vector<int> rows, cols;
+        if (V.ptrunc) {
+            for (int i = 0; i < nb; i++)
+                for (int a = 0; a < nfc; a++) { rows.push_back(i * nf + a); cols.push_back(i * nfc + a); }
+        } else {
+            std::vector<int> agg(nb);
+            FH_HIP(hipMemcpy(agg.data(), V.agg_map.p, sizeof(int) * nb, hipMemcpyDeviceToHost));
+            for (int i = 0; i < nb; i++)
+                for (int a = 0; a < nf; a++) { rows.push_back(i * nf + a); cols.push_back(agg[i] * nf + a); }
+        }
+        const int nr = nb * nf, nc = (int)L[l + 1].A64->rows();
+        std::vector<int> prp(nr + 1, 0), pci(rows.size()), rrp(nc + 1, 0), rci(rows.size());
+        std::vector<double> one(rows.size(), 1.0);
+        for (int r : rows) prp[r + 1]++;
+        for (int c : cols) rrp[c + 1]++;
+        for (int i = 0; i < nr; i++) prp[i + 1] += prp[i];
+        for (int i = 0; i < nc; i++) rrp[i + 1] += rrp[i];
+        std::vector<int> fp(prp.begin(), prp.end() - 1), fr(rrp.begin(), rrp.end() - 1);
+        for (size_t e = 0; e < rows.size(); e++) {  // rows ascending, one entry per row: already sorted
+            pci[fp[rows[e]]++] = cols[e];
+            rci[fr[cols[e]]++] = rows[e];
+        }
+        market_write_csr(nr, nc, prp.data(), pci.data(), one.data(), pre + "P.dat");
+        market_write_csr(nc, nr, rrp.data(), rci.data(), one.data(), pre + "R.dat");
+    }
+}
+
 int Solver::LevelUnknowns(int l) const {
     if (!mg_) return A_ ? (int)A_->rows() : 0;
     return (int)mg_->L.at(l).A64->rows();
@@ -667,7 +716,7 @@ void Solver::apply_prec(const DVec& r, DVec& z, DVec* Az) {
         int sl[2] = {kNnSlotS1, kNnSlotS2};
         launch_dots(c_, n, P, 2, xs, ys, sl);
         nn_select(c_, n, A_->bs_r, kNnSlotS1, kNnSlotS2, (const double*)delta_.data(), (const double*)r1_.data(),
-                  nng_->invD64.as<double>(), nn_omega_, (double*)z.data(), nn_counters_.as<int>());
+                  nng_->invD64.as<double>(), nn_omega_, nn_unit_step_, (double*)z.data(), nn_counters_.as<int>());
     }
     if (Az) launch_bcsr(c_, *A_, OP_SPMV, Prec::F64, z.data(), nullptr, Az->data(), 1.0, 0.0, 0.0, nullptr, Prec::F64, nullptr, 0);
 }
@@ -736,9 +785,26 @@ void Solver::solve_cg(const double* b, double* x) {
 }
 
 void Solver::solve_fcg(const double* b, double* x) {
-    // FlexibleConjugateGradient::Solve, truncation 1 (FlexibleConjugateGradient.h:47-171), x0 = 0
+    // FlexibleConjugateGradient::Solve (FlexibleConjugateGradient.h:47-171),
+    // x0 = 0.  z is A-orthogonalised against the kept previous directions,
+    // oldest first; when `Truncation` directions are kept the oldest is
+    // dropped (or all are, with Restart).  Directions live in a ring of
+    // Truncation + 1 buffers, so no vector is copied.
     const int64_t n = A_->rows();
     const Prec P = Prec::F64;
+    const int m = std::max(1, args_.fcg_truncation);
+    if ((int)fd_.size() != m + 1) {
+        fd_.clear();
+        fAd_.clear();
+        fd_.resize(m + 1);
+        fAd_.resize(m + 1);
+    }
+    for (int k = 0; k <= m; k++) {
+        fd_[k].resize(n, P);
+        fAd_[k].resize(n, P);
+    }
+    auto slot_dad = [](int k) { return S_FCG + 2 * k; };
+    auto slot_zad = [](int k) { return S_FCG + 2 * k + 1; };
     launch_fill(c_, n, x, P, 0.0);
     copy(c_, n, P, b, r_.data());
     dot1(c_, n, P, b, b, S_BB);
@@ -747,26 +813,45 @@ void Solver::solve_fcg(const double* b, double* x) {
     double res = 1.0;
     hist_.push_back(res);
     stats_.converged = 0;
-    bool have = false;
+    std::vector<int> kept;  // ring indices, oldest first
     auto stop = FH_STOP_TEST();
     while (!stop()) {
+        int cur = 0;
+        while (std::find(kept.begin(), kept.end(), cur) != kept.end()) cur++;
+        DVec& d = fd_[cur];
+        DVec& Ad = fAd_[cur];
         apply_prec(r_, z_, fused_ ? &Az_ : nullptr);
-        copy(c_, n, P, z_.data(), d_.data());
-        if (fused_) copy(c_, n, P, Az_.data(), Ad_.data());
-        if (have) {
-            dot1(c_, n, P, z_.data(), Adk_.data(), S_ZADK);
+        copy(c_, n, P, z_.data(), d.data());
+        if (fused_) copy(c_, n, P, Az_.data(), Ad.data());
+        // z . Adk for every kept direction (three per launch)
+        for (size_t q = 0; q < kept.size(); q += 3) {
+            const void* xs[3];
+            const void* ys[3];
+            int sl[3];
+            const int cnt = (int)std::min<size_t>(3, kept.size() - q);
+            for (int t = 0; t < cnt; t++) {
+                xs[t] = z_.data();
+                ys[t] = fAd_[kept[q + t]].data();
+                sl[t] = slot_zad(kept[q + t]);
+            }
+            launch_dots(c_, n, P, cnt, xs, ys, sl);
+        }
+        for (int k : kept) {
             // d -= (z.Adk / dk.Adk) dk
-            launch_axpby(c_, n, P, dk_.data(), d_.data(), -1.0, S_ZADK, S_DKADK, 1.0, -1, -1);
-            if (fused_) launch_axpby(c_, n, P, Adk_.data(), Ad_.data(), -1.0, S_ZADK, S_DKADK, 1.0, -1, -1);
+            launch_axpby(c_, n, P, fd_[k].data(), d.data(), -1.0, slot_zad(k), slot_dad(k), 1.0, -1, -1);
+            if (fused_)
+                launch_axpby(c_, n, P, fAd_[k].data(), Ad.data(), -1.0, slot_zad(k), slot_dad(k), 1.0, -1, -1);
         }
         if (!fused_)
-            launch_bcsr(c_, *A_, OP_SPMV, P, d_.data(), nullptr, Ad_.data(), 1.0, 0.0, 0.0, nullptr, P, nullptr, 0);
-        dot2(c_, n, P, d_.data(), Ad_.data(), S_DAD, r_.data(), d_.data(), S_RD);
-        launch_cg_update(c_, n, P, x, d_.data(), r_.data(), Ad_.data(), S_RD, S_DAD, 1.0);
-        std::swap(dk_.buf, d_.buf);
-        std::swap(Adk_.buf, Ad_.buf);
-        launch_scalar_op(c_, 4, S_DKADK, S_DAD, 0);
-        have = true;
+            launch_bcsr(c_, *A_, OP_SPMV, P, d.data(), nullptr, Ad.data(), 1.0, 0.0, 0.0, nullptr, P, nullptr, 0);
+        dot2(c_, n, P, d.data(), Ad.data(), S_DAD, r_.data(), d.data(), S_RD);
+        launch_cg_update(c_, n, P, x, d.data(), r_.data(), Ad.data(), S_RD, S_DAD, 1.0);
+        if ((int)kept.size() == m) {
+            if (args_.fcg_restart) kept.clear();
+            else kept.erase(kept.begin());
+        }
+        launch_scalar_op(c_, 4, slot_dad(cur), S_DAD, 0);
+        kept.push_back(cur);
         dot1(c_, n, P, r_.data(), r_.data(), S_RR);
         it++;
         double rr = read_scalar(S_RR);

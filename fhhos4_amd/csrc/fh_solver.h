@@ -94,6 +94,8 @@ struct SolverArgs {
     // launches; block Jacobi costs one.
     std::string coarse_smoother;
     double coarse_relax = 0.0;
+    int fcg_truncation = 1;          // FlexibleConjugateGradient::Truncation (FCG(1) by default)
+    bool fcg_restart = false;        // FlexibleConjugateGradient::Restart
 };
 
 class Solver {
@@ -105,9 +107,10 @@ public:
     void SetLevel(int l, const Bcsr* A, const Bcsr* P);
     // Learned post-smoother inside the preconditioner of FCG: after the
     // multigrid cycle z0 = M r, r1 = r - A z0, delta = net(r1); z = z0 +
-    // (delta.r1 / delta.A delta) delta when delta.r1 > 0, else the block-Jacobi
-    // step z0 + omega D^-1 r1.  The network and graph must outlive the solver.
-    void SetLearnedSmoother(Mpnn* nn, const NnGraph* g, double omega);
+    // (delta.r1 / delta.A delta) delta when delta.r1 > 0 (unit_step: z0 +
+    // delta when delta.A delta < 2 delta.r1), else the block-Jacobi step
+    // z0 + omega D^-1 r1.  The network and graph must outlive the solver.
+    void SetLearnedSmoother(Mpnn* nn, const NnGraph* g, double omega, bool unit_step = false);
     void Setup(const Bcsr& A);
     // Solves A x = b from x = 0 (FP64 device vectors of length A.rows()).
     void Solve(const double* b, double* x);
@@ -119,6 +122,10 @@ public:
     const SolveStats& Stats() const { return stats_; }
     const std::vector<double>& History() const { return hist_; }
     int NumLevels() const;
+    // Level::Setup with ExportComponents (Level.h:116-125): writes
+    // <dir>/level<l>_A.dat for every level and level<l>_P.dat, level<l>_R.dat
+    // for every level but the coarsest, in the exported-system format.
+    void ExportLevels(const std::string& dir);
     int LevelUnknowns(int l) const;
 
 private:
@@ -131,10 +138,12 @@ private:
     SolveStats stats_;
     std::vector<double> hist_;
     DVec r_, z_, d_, Ad_, Az_, dk_, Adk_, t_;
+    std::vector<DVec> fd_, fAd_;  // FCG(m) direction ring
     bool fused_ = false;
     Mpnn* nn_ = nullptr;
     const NnGraph* nng_ = nullptr;
     double nn_omega_ = 1.0;
+    bool nn_unit_step_ = false;
     DVec r1_, delta_, Adelta_;
     DevBuf nn_counters_;
     void apply_prec(const DVec& r, DVec& z, DVec* Az);

@@ -25,7 +25,8 @@ class SolverArgs(C.Structure):
                 ("nu1", C.c_int), ("nu2", C.c_int), ("pre_smoother", C.c_char_p), ("post_smoother", C.c_char_p),
                 ("relax", C.c_double), ("tol", C.c_double), ("max_iter", C.c_int), ("max_coarse", C.c_int),
                 ("theta", C.c_double), ("mg_prec", C.c_int), ("mg_store", C.c_int), ("galerkin", C.c_int),
-                ("coarse_smoother", C.c_char_p), ("coarse_relax", C.c_double)]
+                ("coarse_smoother", C.c_char_p), ("coarse_relax", C.c_double), ("fcg_truncation", C.c_int),
+                ("fcg_restart", C.c_int)]
 
 
 class SolveStats(C.Structure):
@@ -67,8 +68,9 @@ _SIGS = {
     "fh_nn_forward": [_vp, _vp, _vp, _vp, _vp, C.c_int],
     "fh_nn_loss_grad": [_vp, _vp, _vp, _vp, _vp, C.POINTER(C.c_double), _vp],
     "fh_nn_train_step": [_vp, _vp, _vp, _vp, _vp, C.c_float, C.POINTER(C.c_double)],
-    "fh_solver_set_nn": [_vp, _vp, _vp, C.c_double],
+    "fh_solver_set_nn": [_vp, _vp, _vp, C.c_double, C.c_int],
     "fh_solver_set_stop": [_vp, C.c_double, C.c_int],
+    "fh_solver_export_levels": [_vp, C.c_char_p],
     "fh_hho_assemble_gpu": [_vp, _vp, C.c_int, _pp, _vp],
     "fh_solver_apply_prec": [_vp, _vp, _vp],
     "fh_mat_export_market": [_vp, _vp, C.c_char_p],
@@ -403,11 +405,12 @@ class Solver:
 
     def __init__(self, ctx, solver="cg", prec="mg", cycle="V", nu=(1, 1), smoothers=("bj", "bj"), relax=1.0,
                  tol=1e-8, max_iter=200, max_coarse=1000, theta=0.25, mg_prec="f64", mg_store=None, wloops=0,
-                 galerkin=False, coarse_smoother=None, coarse_relax=0.0):
+                 galerkin=False, coarse_smoother=None, coarse_relax=0.0, fcg_truncation=1, fcg_restart=False):
         a = SolverArgs()
         self._strs = [s.encode() for s in (solver, prec or "", smoothers[0], smoothers[1], coarse_smoother or "")]
         a.solver, a.prec, a.pre_smoother, a.post_smoother, a.coarse_smoother = self._strs
         a.coarse_relax = coarse_relax
+        a.fcg_truncation, a.fcg_restart = fcg_truncation, int(fcg_restart)
         a.cycle = cycle.encode()
         a.wloops = wloops
         a.nu1, a.nu2 = nu
@@ -421,6 +424,10 @@ class Solver:
         self.ctx, self.h, self.max_iter = ctx, h, max_iter
         self._levels = []
 
+    def export_levels(self, directory):
+        """Writes level<l>_A/P/R.dat as the reference's -export mg does."""
+        _check(lib().fh_solver_export_levels(self.h, os.fsencode(directory)))
+
     def set_stop(self, tol, max_iter):
         self.max_iter = max_iter
         _check(lib().fh_solver_set_stop(self.h, tol, max_iter))
@@ -430,10 +437,10 @@ class Solver:
         """z = M r with the preconditioner alone (device FP64 vectors)."""
         _check(lib().fh_solver_apply_prec(self.h, _ptr(r), _ptr(z)))
 
-    def set_learned_smoother(self, nn, graph, omega=1.0):
+    def set_learned_smoother(self, nn, graph, omega=1.0, unit_step=False):
         """Learned post-smoother inside FCG (before setup)."""
         self._nn = (nn, graph)
-        _check(lib().fh_solver_set_nn(self.h, nn.h, graph.h, omega))
+        _check(lib().fh_solver_set_nn(self.h, nn.h, graph.h, omega, int(unit_step)))
 
     def set_level(self, level, A=None, P=None):
         self._levels.append((A, P))

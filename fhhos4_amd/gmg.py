@@ -8,15 +8,26 @@ def build_hierarchy(ctx, problem, max_coarse=1000, keep=None):
     """Returns [(problem_l, A_l, P_l)] from fine to coarse; P_l maps level
     l+1 to level l (None on the coarsest).  Coarsens while the level has more
     than max_coarse unknowns and n is even."""
-    levels = [(problem, problem.matrix(ctx, keep_host=True))]
+    levels = [(problem, _operator(ctx, problem))]
     while levels[-1][0].n_unknowns > max_coarse and levels[-1][0].args["n"] % 2 == 0 and levels[-1][0].args["n"] > 1:
         coarse = levels[-1][0].coarsened()
-        levels.append((coarse, coarse.matrix(ctx, keep_host=True)))
+        levels.append((coarse, _operator(ctx, coarse)))
     out = []
     for l, (p, A) in enumerate(levels):
         P = p.prolongation(ctx, levels[l + 1][0]) if l + 1 < len(levels) else None
         out.append((p, A, P))
     return out
+
+
+def _operator(ctx, problem):
+    """Level operator: assembled on the GPU for problems created with
+    defer_assembly (the right-hand side is kept as problem.b_device), else
+    uploaded from the host assembly."""
+    if problem.args.get("defer_assembly"):
+        A, b = problem.assemble_gpu(ctx, keep_host=True)
+        problem.b_device = b
+        return A
+    return problem.matrix(ctx, keep_host=True)
 
 
 def attach(solver, hierarchy):

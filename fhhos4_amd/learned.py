@@ -83,12 +83,12 @@ def mean_loss(nn, pairs):
     return float(np.mean([nn.loss_grad(sy.graph, r1, e1, want_grad=False)[0] for sy, r1, e1 in pairs]))
 
 
-def solve_with(ctx, system, b_host, nn=None, omega=1.0, solver_kw=None, tol=1e-8, max_iter=400):
+def solve_with(ctx, system, b_host, nn=None, omega=1.0, solver_kw=None, tol=1e-8, max_iter=400, unit_step=False):
     """FCG solve of one right-hand side with (nn) or without the learned
     post-smoother; returns (stats, x, seconds)."""
     s = G.Solver(ctx, solver="fcg", tol=tol, max_iter=max_iter, **(solver_kw or {}))
     if nn is not None:
-        s.set_learned_smoother(nn, system.graph, omega)
+        s.set_learned_smoother(nn, system.graph, omega, unit_step)
     s.setup(system.A)
     b = torch.from_numpy(np.ascontiguousarray(b_host)).cuda()
     x = torch.zeros_like(b)

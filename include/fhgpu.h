@@ -159,6 +159,8 @@ typedef struct fh_solver_args {
      * finest level's smoothers; coarse_relax <= 0 keeps relax. */
     const char* coarse_smoother;
     double coarse_relax;
+    int fcg_truncation;        /* FlexibleConjugateGradient::Truncation (<= 0: 1), <= 32 */
+    int fcg_restart;           /* FlexibleConjugateGradient::Restart (FlexibleConjugateGradient.h:18-25) */
 } fh_solver_args;
 
 typedef struct fh_solve_stats {
@@ -220,9 +222,16 @@ int fh_nn_loss_grad(fh_ctx* ctx, fh_nn* nn, const fh_nngraph* g, const double* r
 int fh_nn_train_step(fh_ctx* ctx, fh_nn* nn, const fh_nngraph* g, const double* r1, const double* e1, float lr,
                      double* loss);
 /* Enables the learned post-smoother in an FCG solver (before setup; the graph
- * must be built on the solver's operator); omega: fallback relaxation. */
-int fh_solver_set_nn(fh_solver* s, fh_nn* nn, const fh_nngraph* g, double omega);
+ * must be built on the solver's operator); omega: fallback relaxation.
+ * unit_step 0: z0 + (delta.r1 / delta.A delta) delta, accepted when
+ * delta.r1 > 0; 1: z0 + delta, accepted when delta.A delta < 2 delta.r1. */
+int fh_solver_set_nn(fh_solver* s, fh_nn* nn, const fh_nngraph* g, double omega, int unit_step);
 
+/* Level::Setup with ExportComponents (src/Solver/Multigrid/Level.h:116-125,
+ * ExportModule prefix "level<l>", Level.h:50-54): after setup, writes
+ * <dir>/level<l>_A.dat for every level and level<l>_P.dat / level<l>_R.dat
+ * for every level but the coarsest (exported-system format). */
+int fh_solver_export_levels(fh_solver* s, const char* dir);
 /* IterativeSolver::Tolerance and MaxIterations (IterativeSolver.h:11-23). */
 int fh_solver_set_stop(fh_solver* s, double tol, int max_iter);
 /* z = M r with the solver's preconditioner alone (Preconditioner::Apply,

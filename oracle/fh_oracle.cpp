@@ -805,25 +805,31 @@ int or_solve(const or_solver_cfg* cfg, const or_level_desc* levels, const double
             hist[nh++] = res;
         }
     } else if (cfg->krylov == 2) {
-        // FlexibleConjugateGradient::Solve, truncation 1 (FlexibleConjugateGradient.h:47-171)
-        std::vector<double> dk, Adk;
-        double dkAdk = 0;
-        bool have = false;
+        // FlexibleConjugateGradient::Solve (FlexibleConjugateGradient.h:47-171):
+        // A-orthogonalisation of z against the kept directions (oldest
+        // first), the oldest dropped beyond `truncation` (all with restart)
+        struct Dir { std::vector<double> d, Ad; double dAd; };
+        std::vector<Dir> kept;
+        const int m = cfg->fcg_truncation > 0 ? cfg->fcg_truncation : 1;
         while (!stop()) {
             apply(r.data(), z.data(), fused ? Az.data() : nullptr);
             d = z;
             if (fused) Ad = Az;
-            if (have) {
-                double zAdk = dot(n, z.data(), Adk.data());
-                for (int i = 0; i < n; i++) d[i] -= (zAdk / dkAdk) * dk[i];
-                if (fused) for (int i = 0; i < n; i++) Ad[i] -= (zAdk / dkAdk) * Adk[i];
+            for (const Dir& k : kept) {
+                double zAdk = dot(n, z.data(), k.Ad.data());
+                for (int i = 0; i < n; i++) d[i] -= (zAdk / k.dAd) * k.d[i];
+                if (fused) for (int i = 0; i < n; i++) Ad[i] -= (zAdk / k.dAd) * k.Ad[i];
             }
             if (!fused) spmv(A, d.data(), Ad.data());
             double dAd = dot(n, d.data(), Ad.data());
             double alpha = dot(n, r.data(), d.data()) / dAd;
             for (int i = 0; i < n; i++) x[i] += alpha * d[i];
             for (int i = 0; i < n; i++) r[i] -= alpha * Ad[i];
-            dk = d; Adk = Ad; dkAdk = dAd; have = true;
+            if ((int)kept.size() == m) {
+                if (cfg->fcg_restart) kept.clear();
+                else kept.erase(kept.begin());
+            }
+            kept.push_back(Dir{d, Ad, dAd});
             it++;
             res = nres(r.data());
             hist[nh++] = res;

@@ -88,6 +88,8 @@ typedef struct or_solver_cfg {
     int nlevels;       /* GMG: number of levels given in `levels` */
     int coarse_kind;   /* > 0: smoother kind of the levels below the finest (both sweeps) */
     double coarse_omega;
+    int fcg_truncation;  /* FCG: number of kept directions (<= 0: 1) */
+    int fcg_restart;     /* FCG: clear instead of dropping the oldest */
 } or_solver_cfg;
 
 typedef struct or_solver_out {

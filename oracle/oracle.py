@@ -26,7 +26,8 @@ class SolverCfg(C.Structure):
                 ("pre_kind", C.c_int), ("post_kind", C.c_int), ("pre_omega", C.c_double),
                 ("post_omega", C.c_double), ("nu1", C.c_int), ("nu2", C.c_int), ("cycle", C.c_char),
                 ("wloops", C.c_int), ("max_coarse", C.c_int), ("theta", C.c_double), ("tol", C.c_double),
-                ("max_iter", C.c_int), ("nlevels", C.c_int), ("coarse_kind", C.c_int), ("coarse_omega", C.c_double)]
+                ("max_iter", C.c_int), ("nlevels", C.c_int), ("coarse_kind", C.c_int), ("coarse_omega", C.c_double),
+                ("fcg_truncation", C.c_int), ("fcg_restart", C.c_int)]
 
 
 class SolverOut(C.Structure):
@@ -148,7 +149,8 @@ PREC_KIND = {"": 0, "none": 0, "bj": 1, "bgs": 2, "mg": 3, "aggregamg": 4}
 
 
 def solve(A, b, solver="cg", prec="mg", nf=1, cycle="V", nu=(1, 1), smoothers=("bj", "bj"), omega=1.0,
-          levels=None, tol=1e-8, max_iter=200, max_coarse=1000, theta=0.25, coarse_smoother=None, coarse_omega=0.0):
+          levels=None, tol=1e-8, max_iter=200, max_coarse=1000, theta=0.25, coarse_smoother=None, coarse_omega=0.0,
+          fcg_truncation=1, fcg_restart=False):
     """Runs the oracle solver; `levels` = [(A_l, P_l), ...] for GMG (P of the
     coarsest level is None).  Returns (x, hist, out)."""
     keep = []
@@ -175,6 +177,7 @@ def solve(A, b, solver="cg", prec="mg", nf=1, cycle="V", nu=(1, 1), smoothers=("
     cfg.pre_omega = 2.0 / 3.0 if smoothers[0] == "bj23" else omega
     cfg.post_omega = 2.0 / 3.0 if smoothers[1] == "bj23" else omega
     cfg.nu1, cfg.nu2 = nu
+    cfg.fcg_truncation, cfg.fcg_restart = fcg_truncation, int(fcg_restart)
     if coarse_smoother:
         cfg.coarse_kind = SMOOTHER_KIND[coarse_smoother]
         cfg.coarse_omega = 2.0 / 3.0 if coarse_smoother == "bj23" else (coarse_omega if coarse_omega > 0 else omega)

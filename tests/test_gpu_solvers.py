@@ -136,3 +136,49 @@ def test_mixed_precision_fcg_reaches_fp64_tolerance(ctx, mg_prec, mg_store):
           "rel. error vs FP64 oracle", err)
     assert err < 1e-7
     assert st["iterations"] <= oo.iterations + 3
+
+
+def test_exported_levels_match_reference(ctx, tmp_path):
+    """-export mg: level<l>_A/P/R.dat of the GPU solver's hierarchy equal the
+    reference's exported level matrices (geometric levels given from the
+    reference; aggregation levels built here, bit-exact indices)."""
+    import scipy.io as sio
+    # geometric hierarchy (operators and prolongations are the reference's own)
+    d = load("c1_n32_cg_mg_bj23")
+    A = csr(d, "A")
+    M = G.Matrix.from_csr(ctx, A, 2, keep_host=True)
+    s = G.Solver(ctx, solver="cg", prec="mg", smoothers=("bj23", "bj23"))
+    keep = []
+    for l, (Al, Pl) in enumerate(gmg_levels(d)):
+        Ml = G.Matrix.from_csr(ctx, Al, 2) if l > 0 else None
+        Pm = G.Matrix.from_csr(ctx, Pl, 2) if Pl is not None else None
+        keep += [Ml, Pm]
+        s.set_level(l, Ml, Pm)
+    s.setup(M)
+    s.export_levels(str(tmp_path))
+    for l in range(int(d["nlevels"])):
+        got = sio.mmread(str(tmp_path / ("level%d_A.dat" % l))).tocsr()
+        ref = A if l == 0 else csr(d, "level%d_A" % l)
+        assert abs(got - ref).max() == 0.0
+        if ("level%d_P_shape" % l) in d:
+            P = sio.mmread(str(tmp_path / ("level%d_P.dat" % l))).tocsr()
+            R = sio.mmread(str(tmp_path / ("level%d_R.dat" % l))).tocsr()
+            assert abs(P - csr(d, "level%d_P" % l)).max() == 0.0
+            assert abs(R - csr(d, "level%d_P" % l).T).max() == 0.0
+    # aggregation AMG (scalar): every coarse operator equals the reference's
+    d = load("tet8_k0_fcg_amg")
+    A = csr(d, "A")
+    M = G.Matrix.from_csr(ctx, A, 1, keep_host=True)
+    s = G.Solver(ctx, solver="fcg", prec="aggregamg", cycle="K", smoothers=("bgs", "rbgs"))
+    s.setup(M)
+    out = tmp_path / "amg"
+    out.mkdir()
+    s.export_levels(str(out))
+    for l in range(1, int(d["nlevels"])):
+        got = sio.mmread(str(out / ("level%d_A.dat" % l))).tocsr()
+        ref = csr(d, "level%d_A" % l)
+        assert got.shape == ref.shape
+        assert abs(got - ref).max() <= 1e-12 * abs(ref).max()
+        P = sio.mmread(str(out / ("level%d_P.dat" % (l - 1)))).tocsr()
+        Af = sio.mmread(str(out / ("level%d_A.dat" % (l - 1)))).tocsr()
+        assert abs(P.T @ Af @ P - got).max() <= 1e-12 * abs(got).max()

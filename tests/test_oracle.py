@@ -182,12 +182,16 @@ def test_multicolour_gs_reduces_to_lexicographic():
                                rtol=1e-12, atol=1e-12)
 
 
-def test_fcg_equals_cg_with_fixed_spd_preconditioner():
-    """With a fixed SPD preconditioner FCG(1) reproduces CG (Notay)."""
+@pytest.mark.parametrize("m,restart", [(1, False), (3, False), (4, True)])
+def test_fcg_equals_cg_with_fixed_spd_preconditioner(m, restart):
+    """With a fixed SPD preconditioner FCG(m) reproduces CG for any
+    truncation or restart (Notay): the extra A-orthogonalisations remove
+    components that are zero in exact arithmetic."""
     d = load("c1_n32_cg_mg_bj23")
     A = csr(d, "A")
     x1, h1, o1 = O.solve(A, d["b"], "cg", "mg", nf=2, smoothers=("bj23", "bj23"), levels=gmg_levels(d))
-    x2, h2, o2 = O.solve(A, d["b"], "fcg", "mg", nf=2, smoothers=("bj23", "bj23"), levels=gmg_levels(d))
+    x2, h2, o2 = O.solve(A, d["b"], "fcg", "mg", nf=2, smoothers=("bj23", "bj23"), levels=gmg_levels(d),
+                         fcg_truncation=m, fcg_restart=restart)
     assert o1.iterations == o2.iterations
     np.testing.assert_allclose(h1, h2, rtol=1e-6)
 
