@@ -383,7 +383,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic("stetra n=%d k=%d perturb %g seed %d" % (args.n, args.k, args.perturb,
                                                                                          args.seed)),
-                     "kernel": "k_bcsr<6,double,double,double,OP_SPMV> (fine level)",
+                     "kernel": "k_bcsr_grp<6,double,double,double,OP_SPMV> (fine level)",
                      "bytes_per_launch": bytes_fine, "mean_ms": spmv_ms, "launches": nl,
                      "smooth_mean_ms": smooth_ms, "smooth_launches": nls},
         "cpu_baseline": cpu,
