@@ -39,16 +39,19 @@ def spmv_bytes(nb, nf, nnzb, sv=8, sx=8):
     return nnzb * nf * nf * sv + 4 * nnzb + 4 * (nb + 1) + 2 * nb * nf * sx
 
 
-TRAFFIC_SUMMARY = os.path.join(REPO, "profiles", "bcsr_spmv_traffic_tet64_k2.md")
+# rocprofv3 FETCH_SIZE / WRITE_SIZE summaries of the fine-level SpMV, per layout
+TRAFFIC_SUMMARY = {"sliced": os.path.join(REPO, "profiles", "sliced_spmv_traffic_tet64_k2.md"),
+                   "bcsr": os.path.join(REPO, "profiles", "bcsr_spmv_traffic_tet64_k2.md")}
 
 
-def measured_traffic(workload_key):
+def measured_traffic(workload_key, layout):
     """HBM bytes per launch of the fine-level SpMV from the committed
-    rocprofv3 FETCH_SIZE / WRITE_SIZE summary of the same workload (scaled
-    by the counters' reading for a copy of known size), or None."""
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary of the same workload and
+    kernel (scaled by the counters' reading for a streaming kernel of known
+    size), or None."""
     import re
     try:
-        text = open(TRAFFIC_SUMMARY).read()
+        text = open(TRAFFIC_SUMMARY[layout]).read()
     except OSError:
         return None
     m = re.search(r"<!-- traffic: (\{.*?\}) -->", text)
@@ -329,6 +332,10 @@ def main():
     s64.solve(b, x)
     nl, spmv_ms = G.profile_end(ctx)
     bytes_fine = spmv_bytes(nb, nf, nnzb)
+    # the fine operator runs on its sliced copy unless FHGPU_SLICED=0
+    sliced = os.environ.get("FHGPU_SLICED", "1") != "0" and A.set_sliced(True)
+    kernel = ("k_sliced<6,double,double,double,OP_SPMV>" if sliced else
+              "k_bcsr_grp<6,double,double,double,OP_SPMV>") + " (fine level)"
     achieved = bytes_fine / (spmv_ms * 1e-3) / 1e9
     log("fine SpMV: %d launches, %.3f ms, %.0f GB/s" % (nl, spmv_ms, achieved))
     G.profile_begin(ctx, A, 2)
@@ -382,8 +389,9 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic("stetra n=%d k=%d perturb %g seed %d" % (args.n, args.k, args.perturb,
-                                                                                         args.seed)),
-                     "kernel": "k_bcsr_grp<6,double,double,double,OP_SPMV> (fine level)",
+                                                                                         args.seed),
+                                                 "sliced" if sliced else "bcsr"),
+                     "kernel": kernel,
                      "bytes_per_launch": bytes_fine, "mean_ms": spmv_ms, "launches": nl,
                      "smooth_mean_ms": smooth_ms, "smooth_launches": nls},
         "cpu_baseline": cpu,

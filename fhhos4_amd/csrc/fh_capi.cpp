@@ -142,6 +142,14 @@ int fh_mat_device_arrays(const fh_mat* A, const int** rowptr, const int** colidx
     });
 }
 
+int fh_mat_set_sliced(fh_ctx* ctx, const fh_mat* A, int on, int* active) {
+    return guard([&] {
+        NEED(ctx); NEED_DEV(ctx); NEED(A);
+        const bool r = fh::set_sliced(ctx->c, *A->A, on != 0);
+        if (active) *active = r ? 1 : 0;
+    });
+}
+
 int fh_mat_export_market(fh_ctx* ctx, const fh_mat* A, const char* path) {
     return guard([&] {
         NEED(ctx); NEED(A); NEED(path);

@@ -100,6 +100,19 @@ struct DVec {
     void* data() const { return buf.p; }
 };
 
+// Sliced execution layout of a square-block matrix whose rows fit one
+// wavefront.  A wavefront owns RW consecutive block rows; lane
+// (r, b, jp) = r * W * JP + b * JP + jp holds block slot b of row r and its
+// column group jp (cw = 2 columns for even block sizes, else 1; JP = nf/cw).
+// Values of slice s are stored [s][i][lane < ACT][cw] (ACT = RW * W * JP), so
+// load i of a wavefront reads ACT * cw contiguous values; cols holds the
+// block column of every (slice, row, slot), -1 for padding.
+struct Sliced {
+    int W = 0, RW = 0, cw = 1, nslices = 0;
+    DevBuf cols, vals;
+    int act(int nf) const { return RW * W * (nf / cw); }
+};
+
 // Block-CSR matrix with dense nf_r x nf_c blocks stored row-major, block
 // rows contiguous.  Value offsets are 64-bit (block index * nf_r * nf_c).
 struct Bcsr {
@@ -115,6 +128,12 @@ struct Bcsr {
     // entry exists in the scalar sparse matrix, even if its value is 0);
     // empty: an entry is stored iff its value is non-zero
     std::vector<uint8_t> h_mask;
+    // Sliced copy of the values used by full-matrix products (built from the
+    // block-CSR values; the values must not change afterwards).  State:
+    // 0 undecided (decided at the first full-matrix product), 1 built,
+    // -1 not used.
+    mutable std::unique_ptr<Sliced> sliced;
+    mutable int sliced_state = 0;
     int64_t rows() const { return (int64_t)nbr * bs_r; }
     int64_t cols() const { return (int64_t)nbc * bs_c; }
     size_t value_bytes() const { return (size_t)nnzb * bs_r * bs_c * prec_bytes(prec); }

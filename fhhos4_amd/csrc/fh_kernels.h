@@ -19,6 +19,12 @@ void launch_bcsr(Ctx& c, const Bcsr& A, BcsrMode mode, Prec vprec, const void* x
                  double alpha, double beta, double omega, const void* invD, Prec dprec,
                  const int* rows, int nrows);
 
+// Builds (on = true) or drops the sliced execution layout of A used by
+// full-matrix products; returns whether A now has one (false when a row of
+// A does not fit one wavefront).  Without a call the layout is chosen at the
+// first full-matrix product (rows fit, >= 4096 block rows, <= 25% padding).
+bool set_sliced(Ctx& c, const Bcsr& A, bool on);
+
 // y = omega * D^-1 b over all block rows (smoothing from a zero guess).
 void launch_bjac_zero(Ctx& c, int nbr, int nf, Prec vprec, const void* b, void* y, double omega,
                       const void* invD, Prec dprec, const int* rows, int nrows);

@@ -8,8 +8,16 @@
 // lane = (block slot, row in block) (k_bcsr).  Partial sums are reduced with
 // cross-lane shuffles.  Workgroup ids are remapped so that each XCD walks a
 // contiguous range of block rows (better reuse of x in that XCD's L2).
+//
+// Full-matrix products on operators whose rows fit one wavefront (the fine
+// levels: 7 blocks per row on tetrahedra) use a sliced copy of the values
+// instead (k_sliced): lane = (row, block slot, column pair), so every lane
+// gathers its two x entries once and every load instruction of the
+// wavefront reads one contiguous run of values.  Row sums are reduced
+// through LDS.
 #include "fh_kernels.h"
 
+#include <algorithm>
 #include <cstdlib>
 
 namespace fh {
@@ -195,6 +203,132 @@ __global__ __launch_bounds__(kWG) void k_bcsr_grp(int nbr, const int* __restrict
         for (int k = 0; k < NF; k++) z += (AT)ldv(D + k) * ((AT)bb[k] - acc[k]);
         y[row] = (TV)((AT)x[row] + (AT)omega * z);
     }
+}
+
+// Non-temporal loads of the streamed matrix values (read once per product;
+// keeping them out of the caches leaves L2 to the gathered x entries).
+typedef double nt_d2 __attribute__((ext_vector_type(2)));
+typedef float nt_f2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void ldnt(const double* p, double& a) { a = __builtin_nontemporal_load(p); }
+__device__ __forceinline__ void ldnt(const float* p, float& a) { a = __builtin_nontemporal_load(p); }
+__device__ __forceinline__ void ldnt(const bf16_t* p, float& a) {
+    a = __uint_as_float(((uint32_t)__builtin_nontemporal_load((const uint16_t*)p)) << 16);
+}
+__device__ __forceinline__ void ldnt2(const double* p, double& a, double& b) {
+    const nt_d2 v = __builtin_nontemporal_load((const nt_d2*)p);
+    a = v.x; b = v.y;
+}
+__device__ __forceinline__ void ldnt2(const float* p, float& a, float& b) {
+    const nt_f2 v = __builtin_nontemporal_load((const nt_f2*)p);
+    a = v.x; b = v.y;
+}
+__device__ __forceinline__ void ldnt2(const bf16_t* p, float& a, float& b) {
+    const uint32_t u = __builtin_nontemporal_load((const uint32_t*)p);
+    a = __uint_as_float(u << 16);
+    b = __uint_as_float(u & 0xffff0000u);
+}
+
+constexpr int kSlicedWaves = kWG / 64;  // slices per workgroup
+
+// Block-row operation on the sliced layout (all rows).  Lane
+// (r, b, jp) multiplies column group jp of block slot b of row r by its x
+// entries, for the NF rows i of the block; the NF partial sums per lane go
+// to LDS and lane (r, i) < RW * NF adds the W * JP sums of its row.
+template <int NF, class TA, class TV, class TD, int MODE>
+__global__ __launch_bounds__(kWG) void k_sliced(int nbr, int nslices, int W, int RW, const int* __restrict__ cols,
+                                                const TA* __restrict__ sv, const TV* x, const TV* __restrict__ b,
+                                                TV* y, double alpha, double beta, double omega,
+                                                const TD* __restrict__ invD) {
+    using AT = typename Acc<TV>::T;
+    constexpr int CW = NF % 2 == 0 ? 2 : 1;
+    constexpr int JP = NF / CW;
+    const int LPR = W * JP, ACT = RW * LPR;
+    __shared__ AT red[kSlicedWaves][64][NF + 1];
+    __shared__ AT res[kSlicedWaves][64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int s = xcd_remap(blockIdx.x, gridDim.x) * kSlicedWaves + wave;
+    AT acc[NF];
+#pragma unroll
+    for (int i = 0; i < NF; i++) acc[i] = 0;
+    if (s < nslices && lane < ACT) {
+        const int r = lane / LPR, rem = lane - r * LPR, bs = rem / JP, jp = rem - bs * JP;
+        const int c = cols[(int64_t)s * RW * W + r * W + bs];
+        if (c >= 0) {
+            const TA* base = sv + ((int64_t)s * NF * ACT + lane) * CW;
+            if constexpr (CW == 2) {
+                TV x0, x1;
+                ld2(x + (int64_t)c * NF + 2 * jp, x0, x1);
+#pragma unroll
+                for (int i = 0; i < NF; i++) {
+                    AT a0, a1;
+                    ldnt2(base + (int64_t)i * ACT * 2, a0, a1);
+                    acc[i] = a0 * (AT)x0 + a1 * (AT)x1;
+                }
+            } else {
+                const AT x0 = (AT)x[(int64_t)c * NF + jp];
+#pragma unroll
+                for (int i = 0; i < NF; i++) {
+                    AT a0;
+                    ldnt(base + (int64_t)i * ACT, a0);
+                    acc[i] = a0 * x0;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < NF; i++) red[wave][lane][i] = acc[i];
+    __syncthreads();
+    const bool fin = s < nslices && lane < RW * NF;
+    const int r = lane / NF, i = lane - r * NF;
+    const int row = s * RW + r;
+    AT sum = 0;
+    if (fin) {
+        for (int q = 0; q < LPR; q++) sum += red[wave][r * LPR + q][i];
+    }
+    const bool out = fin && row < nbr;
+    const int64_t e = (int64_t)row * NF + i;
+    if (MODE == OP_SPMV) {
+        if (out) {
+            AT v = (AT)alpha * sum;
+            if (beta != 0.0) v += (AT)beta * (AT)y[e];
+            y[e] = (TV)v;
+        }
+    } else if (MODE == OP_RESID) {
+        if (out) y[e] = (TV)((AT)b[e] - sum);
+    } else {
+        // y = x + omega D^-1 (b - A x): the row's NF residuals through LDS
+        if (out) res[wave][lane] = (AT)b[e] - sum;
+        __syncthreads();
+        if (out) {
+            const TD* D = invD + e * NF;
+            AT z = 0;
+#pragma unroll
+            for (int k = 0; k < NF; k++) z += (AT)ldv(D + k) * res[wave][r * NF + k];
+            y[e] = (TV)((AT)x[e] + (AT)omega * z);
+        }
+    }
+}
+
+// Block-CSR values -> sliced values (one thread per slice lane).
+template <int NF, class TA>
+__global__ void k_to_sliced(int nbr, int nslices, int W, int RW, const int* __restrict__ rp,
+                            const TA* __restrict__ val, TA* __restrict__ sv) {
+    constexpr int CW = NF % 2 == 0 ? 2 : 1;
+    constexpr int JP = NF / CW;
+    const int LPR = W * JP, ACT = RW * LPR;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)nslices * ACT) return;
+    const int s = (int)(t / ACT), lane = (int)(t - (int64_t)s * ACT);
+    const int r = lane / LPR, rem = lane - r * LPR, bs = rem / JP, jp = rem - bs * JP;
+    const int row = s * RW + r;
+    const bool ok = row < nbr && bs < rp[row + 1] - rp[row];
+    const int64_t blk = ok ? (int64_t)rp[row] + bs : 0;
+    for (int i = 0; i < NF; i++)
+        for (int u = 0; u < CW; u++) {
+            TA v{};
+            if (ok) v = val[(blk * NF + i) * NF + CW * jp + u];
+            sv[(((int64_t)s * NF + i) * ACT + lane) * CW + u] = v;
+        }
 }
 
 template <class TV, class TD>
@@ -455,6 +589,31 @@ void bcsr_dispatch_mode(Ctx& c, const Bcsr& A, BcsrMode mode, const void* x, con
     auto bb = (const TV*)b;
     auto yy = (TV*)y;
     auto D = (const TD*)invD;
+    if (!rows && A.sliced_state == 1) {
+        const Sliced& S = *A.sliced;
+        const bool aligned = S.cw == 1 || ((uintptr_t)x % (2 * sizeof(TV))) == 0;
+        if (aligned) {
+            const int nwg = (S.nslices + kSlicedWaves - 1) / kSlicedWaves;
+            auto sc = S.cols.as<int>();
+            auto sv = S.vals.as<TA>();
+            switch (mode) {
+                case OP_SPMV:
+                    hipLaunchKernelGGL((k_sliced<NF, TA, TV, TD, OP_SPMV>), dim3(nwg), dim3(kWG), 0, c.stream, A.nbr,
+                                       S.nslices, S.W, S.RW, sc, sv, xx, bb, yy, alpha, beta, omega, D);
+                    break;
+                case OP_RESID:
+                    hipLaunchKernelGGL((k_sliced<NF, TA, TV, TD, OP_RESID>), dim3(nwg), dim3(kWG), 0, c.stream, A.nbr,
+                                       S.nslices, S.W, S.RW, sc, sv, xx, bb, yy, alpha, beta, omega, D);
+                    break;
+                default:
+                    hipLaunchKernelGGL((k_sliced<NF, TA, TV, TD, OP_SMOOTH>), dim3(nwg), dim3(kWG), 0, c.stream, A.nbr,
+                                       S.nslices, S.W, S.RW, sc, sv, xx, bb, yy, alpha, beta, omega, D);
+                    break;
+            }
+            FH_HIP(hipGetLastError());
+            return;
+        }
+    }
     if constexpr (NF % 2 == 0) {
         const bool aligned = ((uintptr_t)x % (2 * sizeof(TV))) == 0;
         if (bcsr_variant() == 1 && aligned) {
@@ -515,7 +674,120 @@ void bcsr_dispatch_types(Ctx& c, const Bcsr& A, BcsrMode mode, Prec vp, const vo
         throw Error(-3, "vectors must be FP64 or FP32");
 }
 
+// Rows per wavefront and blocks per row slot of the sliced layout; false
+// when a row of A does not fit one wavefront (or the block size has no
+// kernel).
+bool sliced_geometry(const Bcsr& A, int& W, int& RW, int& cw) {
+    const int nf = A.bs_r;
+    if (A.bs_r != A.bs_c || A.nbr <= 0 || A.h_rowptr.size() != (size_t)A.nbr + 1 ||
+        A.h_colidx.size() != (size_t)A.nnzb || A.h_rowptr[A.nbr] != A.nnzb)
+        return false;
+    if (nf != 1 && nf != 2 && nf != 3 && nf != 4 && nf != 6 && nf != 10) return false;
+    cw = nf % 2 == 0 ? 2 : 1;
+    W = 0;
+    for (int i = 0; i < A.nbr; i++) W = std::max(W, A.h_rowptr[i + 1] - A.h_rowptr[i]);
+    if (W == 0 || W * (nf / cw) > 64) return false;
+    // every lane of a row holds a slot, and lane (r, i) < RW * nf writes
+    // output entry i of row r
+    RW = std::min(64 / (W * (nf / cw)), 64 / nf);
+    return true;
+}
+
+template <int NF, class TA>
+void sliced_convert(Ctx& c, const Bcsr& A, Sliced& S) {
+    const int64_t n = (int64_t)S.nslices * S.act(NF);
+    if (n == 0) return;
+    hipLaunchKernelGGL((k_to_sliced<NF, TA>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream, A.nbr,
+                       S.nslices, S.W, S.RW, A.rowptr.as<int>(), A.vals.as<TA>(), S.vals.as<TA>());
+    FH_HIP(hipGetLastError());
+}
+
+template <int NF>
+void sliced_convert_prec(Ctx& c, const Bcsr& A, Sliced& S) {
+    if (A.prec == Prec::F64) sliced_convert<NF, double>(c, A, S);
+    else if (A.prec == Prec::F32) sliced_convert<NF, float>(c, A, S);
+    else sliced_convert<NF, bf16_t>(c, A, S);
+}
+
+void sliced_build(Ctx& c, const Bcsr& A, int W, int RW, int cw) {
+    auto S = std::make_unique<Sliced>();
+    const int nf = A.bs_r;
+    S->W = W; S->RW = RW; S->cw = cw;
+    S->nslices = (A.nbr + RW - 1) / RW;
+    std::vector<int> hc((size_t)S->nslices * RW * W, -1);
+    for (int i = 0; i < A.nbr; i++)
+        for (int p = A.h_rowptr[i]; p < A.h_rowptr[i + 1]; p++)
+            hc[(size_t)(i / RW) * RW * W + (size_t)(i % RW) * W + (p - A.h_rowptr[i])] = A.h_colidx[p];
+    S->cols.alloc(sizeof(int) * hc.size());
+    FH_HIP(hipMemcpyAsync(S->cols.p, hc.data(), sizeof(int) * hc.size(), hipMemcpyHostToDevice, c.stream));
+    // layout [slice][i < nf][lane < ACT][cw]: k_to_sliced writes and
+    // k_sliced reads indices below nslices * nf * ACT * cw
+    const size_t nvals = (size_t)S->nslices * nf * S->act(nf) * cw;
+    S->vals.alloc(nvals * prec_bytes(A.prec));
+    FH_CHECK(S->vals.bytes == nvals * prec_bytes(A.prec) && hc.size() == (size_t)S->nslices * RW * W &&
+                 RW * nf <= 64 && S->act(nf) <= 64,
+             -10, "sliced layout sizes");
+    switch (nf) {
+        case 1: sliced_convert_prec<1>(c, A, *S); break;
+        case 2: sliced_convert_prec<2>(c, A, *S); break;
+        case 3: sliced_convert_prec<3>(c, A, *S); break;
+        case 4: sliced_convert_prec<4>(c, A, *S); break;
+        case 6: sliced_convert_prec<6>(c, A, *S); break;
+        default: sliced_convert_prec<10>(c, A, *S); break;
+    }
+    c.sync();  // host column array released on return
+    A.sliced = std::move(S);
+    A.sliced_state = 1;
+}
+
+// FHGPU_SLICED=0 keeps every product on the block-CSR kernels.
+bool sliced_enabled() {
+    static const bool on = [] {
+        const char* e = std::getenv("FHGPU_SLICED");
+        return e ? std::atoi(e) != 0 : true;
+    }();
+    return on;
+}
+
+// First full-matrix product on A: use the sliced layout when the rows fit a
+// wavefront, the matrix is large enough for the layout to pay and padding
+// adds at most 25% to the values.  Not decided while the stream is being
+// captured into a graph (no allocation there); the block-CSR kernels run
+// meanwhile.
+void sliced_decide(Ctx& c, const Bcsr& A) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    FH_HIP(hipStreamIsCapturing(c.stream, &cs));
+    if (cs != hipStreamCaptureStatusNone) return;
+    int W, RW, cw;
+    if (!sliced_enabled() || A.nbr < 4096 || !sliced_geometry(A, W, RW, cw)) {
+        A.sliced_state = -1;
+        return;
+    }
+    const int64_t nslices = (A.nbr + RW - 1) / RW;
+    if ((double)nslices * RW * W > 1.25 * (double)A.nnzb) {
+        A.sliced_state = -1;
+        return;
+    }
+    sliced_build(c, A, W, RW, cw);
+}
+
 }  // namespace
+
+bool set_sliced(Ctx& c, const Bcsr& A, bool on) {
+    if (!on) {
+        A.sliced.reset();
+        A.sliced_state = -1;
+        return false;
+    }
+    if (A.sliced_state == 1) return true;
+    int W, RW, cw;
+    if (!sliced_geometry(A, W, RW, cw)) {
+        A.sliced_state = -1;
+        return false;
+    }
+    sliced_build(c, A, W, RW, cw);
+    return true;
+}
 
 static void launch_bcsr_impl(Ctx& c, const Bcsr& A, BcsrMode mode, Prec vprec, const void* x, const void* b, void* y,
                              double alpha, double beta, double omega, const void* invD, Prec dprec, const int* rows,
@@ -524,6 +796,7 @@ static void launch_bcsr_impl(Ctx& c, const Bcsr& A, BcsrMode mode, Prec vprec, c
 void launch_bcsr(Ctx& c, const Bcsr& A, BcsrMode mode, Prec vprec, const void* x, const void* b, void* y,
                  double alpha, double beta, double omega, const void* invD, Prec dprec, const int* rows,
                  int nrows) {
+    if (!rows && A.sliced_state == 0) sliced_decide(c, A);
     const bool prof = c.prof_matrix == (const void*)&A && c.prof_mode == (int)mode && !rows;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (prof) {

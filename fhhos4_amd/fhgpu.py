@@ -58,6 +58,7 @@ _SIGS = {
                     C.POINTER(C.c_int)],
     "fh_mat_device_arrays": [_vp, _pp, _pp, _pp],
     "fh_mat_blocks": [_vp, _i32p, _i32p, _f64p],
+    "fh_mat_set_sliced": [_vp, _vp, C.c_int, C.POINTER(C.c_int)],
     "fh_nn_create": [_vp, C.c_int, C.c_int, C.c_uint64, _pp],
     "fh_nn_destroy": [_vp],
     "fh_nn_num_params": [_vp, C.POINTER(C.c_int64)],
@@ -273,6 +274,14 @@ class Matrix:
         _check(lib().fh_mat_blocks(self.h, rp, ci, v))
         bs = i["bs"]
         return sp.bsr_matrix((v.reshape(-1, bs, bs), ci, rp), shape=(i["nbr"] * bs, i["nbc"] * bs)).tocsr()
+
+    @_ordered
+    def set_sliced(self, on):
+        """Builds (on=True) or drops the sliced execution layout of full-matrix
+        products; returns whether the matrix now has one."""
+        a = C.c_int()
+        _check(lib().fh_mat_set_sliced(self.ctx.h, self.h, int(bool(on)), C.byref(a)))
+        return bool(a.value)
 
     def convert(self, prec):
         h = C.c_void_p()

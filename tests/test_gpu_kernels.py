@@ -129,3 +129,77 @@ def test_multicolour_gs_matches_oracle(ctx, name, nf):
         G.mcgs_sweep(ctx, M, bj, col, direction, dev(b), xd)
         got = host(ctx, xd)
         assert np.linalg.norm(got - want) <= 1e-12 * np.linalg.norm(want), direction
+
+
+def _modes(ctx, M, x, y0, b, vdt, bj=None):
+    """y = 1.5 A x - 0.5 y0, r = b - A x and two block-Jacobi sweeps on M."""
+    yd = dev(y0, vdt)
+    G.spmv(ctx, M, dev(x, vdt), yd, alpha=1.5, beta=-0.5)
+    rd = dev(np.zeros_like(x), vdt)
+    G.residual(ctx, M, dev(b, vdt), dev(x, vdt), rd)
+    out = [host(ctx, yd), host(ctx, rd)]
+    if bj is not None:
+        xd, wd = dev(x, vdt), dev(np.zeros_like(x), vdt)
+        bj.sweep(dev(b, vdt), xd, wd, nsweeps=2, A=M)
+        out.append(host(ctx, xd))
+    return out
+
+
+@pytest.mark.parametrize("name,nf", SYSTEMS)
+def test_sliced_layout_matches_oracle_and_block_csr(ctx, name, nf):
+    """The sliced execution layout gives the block-CSR results (FP64: to
+    summation-order roundoff; FP32 / BF16-stored: to FP32 roundoff) and the
+    oracle's, in the three modes of the block-row kernel."""
+    print("case", name, flush=True)
+    A = csr(load(name), "A")
+    rng = np.random.default_rng(6)
+    x, y0, b = (rng.standard_normal(A.shape[0]) for _ in range(3))
+    inv = O.block_inverse(A, nf)
+    want = [1.5 * O.spmv(A, x) - 0.5 * y0, b - O.spmv(A, x),
+            O.bj_sweep(A, nf, inv, 0.7, b, O.bj_sweep(A, nf, inv, 0.7, b, x))]
+    for prec, vdt, tol in (("f64", torch.float64, 1e-13), ("f32", torch.float32, 2e-6), ("bf16", torch.float32, 2e-6)):
+        res = {}
+        for sliced in (False, True):
+            M64 = G.Matrix.from_csr(ctx, A, nf)
+            M = M64 if prec == "f64" else M64.convert(prec)
+            assert M.set_sliced(sliced) == sliced
+            bj = G.BlockJacobi(ctx, M64, 0.7, dprec="f64" if prec == "f64" else "f32")
+            res[sliced] = _modes(ctx, M, x, y0, b, vdt, bj)
+        for k, (u, v) in enumerate(zip(res[False], res[True])):
+            assert np.linalg.norm(u - v) <= tol * np.linalg.norm(u), (prec, k)
+        if prec == "f64":
+            for k, (u, w) in enumerate(zip(res[True], want)):
+                assert np.linalg.norm(u - w) <= 1e-12 * np.linalg.norm(w), k
+
+
+def _random_blocks(rng, nbr, nbc, nf, wmax):
+    lens = rng.integers(0, wmax + 1, size=nbr)
+    lens[rng.integers(0, nbr)] = wmax
+    rp = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    ci = np.concatenate([np.sort(rng.choice(nbc, size=k, replace=False)) for k in lens]).astype(np.int32)
+    v = rng.standard_normal((int(rp[-1]), nf, nf))
+    return rp, ci, v
+
+
+@pytest.mark.parametrize("nf", [1, 2, 3, 4, 6, 10])
+def test_sliced_layout_irregular_rows(ctx, nf):
+    """Rows of 0..W blocks (W = the widest that fits one wavefront), a
+    non-square block pattern and a row count that is not a multiple of the
+    rows per wavefront: sliced products equal scipy's; a pattern whose rows do
+    not fit stays on the block-CSR kernels."""
+    import scipy.sparse as sp
+    rng = np.random.default_rng(nf)
+    cw = 2 if nf % 2 == 0 else 1
+    wfit = 64 // (nf // cw)
+    for nbr, wmax, fits in ((4099, min(wfit, 40), True), (517, 3, True), (300, wfit + 1, False)):
+        nbc = nbr + 37
+        rp, ci, v = _random_blocks(rng, nbr, nbc, nf, wmax)
+        S = sp.bsr_matrix((v, ci, rp), shape=(nbr * nf, nbc * nf)).tocsr()
+        M = G.Matrix.from_blocks(ctx, nbc, nf, rp, ci, v)
+        assert M.set_sliced(True) == fits
+        x = rng.standard_normal(nbc * nf)
+        y0 = rng.standard_normal(nbr * nf)
+        yd = dev(y0)
+        G.spmv(ctx, M, dev(x), yd, alpha=-2.0, beta=0.25)
+        want = -2.0 * (S @ x) + 0.25 * y0
+        assert np.linalg.norm(host(ctx, yd) - want) <= 1e-13 * np.linalg.norm(want), (nbr, wmax)
