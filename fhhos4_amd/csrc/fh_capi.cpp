@@ -9,6 +9,7 @@
 #include "fh_nn.h"
 #include "fh_solver.h"
 #include "fh_hho.h"
+#include "fh_uamg.h"
 
 struct fh_ctx {
     fh::Ctx c;
@@ -56,6 +57,19 @@ int guard(F&& f) {
 fh::Prec P(int p) {
     if (p < 0 || p > 2) throw fh::Error(-2, "invalid precision code " + std::to_string(p));
     return (fh::Prec)p;
+}
+fh::Csr csr_in(const fh_csr* m, const char* name) {
+    FH_CHECK(m && m->rowptr && m->nrows >= 0 && m->ncols >= 0, -2, std::string("missing matrix ") + name);
+    FH_CHECK(m->rowptr[m->nrows] == 0 || (m->colidx && m->vals), -2, std::string("null arrays in ") + name);
+    return fh::csr_copy(m->nrows, m->ncols, m->rowptr, m->colidx, m->vals);
+}
+void csr_out(const fh::Csr& M, int* nrows, int* ncols, int64_t* nnz, int* rowptr, int* colidx, double* vals) {
+    if (nrows) *nrows = M.nr;
+    if (ncols) *ncols = M.nc;
+    if (nnz) *nnz = M.nnz();
+    if (rowptr) std::memcpy(rowptr, M.rp.data(), sizeof(int) * M.rp.size());
+    if (colidx) std::memcpy(colidx, M.ci.data(), sizeof(int) * M.ci.size());
+    if (vals) std::memcpy(vals, M.v.data(), sizeof(double) * M.v.size());
 }
 #define NEED(ptr) FH_CHECK((ptr) != nullptr, -2, #ptr " is null")
 #define NEED_DEV(ctx) FH_CHECK(!(ctx)->c.host_only, -9, "this entry point needs a device context")
@@ -453,6 +467,16 @@ int fh_solver_setup(fh_solver* s, const fh_mat* A) {
         s->s->Setup(*A->A);
     });
 }
+int fh_solver_setup_uncondensed(fh_solver* s, const fh_mat* A, const fh_csr* A_TT, const fh_csr* A_TF,
+                                const fh_csr* A_FF, int cell_bs) {
+    return guard([&] {
+        NEED(s); NEED(A);
+        FH_CHECK(!s->ctx->c.host_only, -9, "solver set-up needs a device context");
+        FH_CHECK(A->A->prec == fh::Prec::F64, -2, "the system matrix must be FP64");
+        FH_CHECK(cell_bs > 0, -2, "cell block size must be positive");
+        s->s->SetupUncondensed(*A->A, csr_in(A_TT, "A_TT"), csr_in(A_TF, "A_TF"), csr_in(A_FF, "A_FF"), cell_bs);
+    });
+}
 int fh_solver_solve(fh_solver* s, const double* b, double* x, fh_solve_stats* st) {
     return guard([&] {
         NEED(s); NEED(b); NEED(x);
@@ -572,7 +596,85 @@ int fh_solver_history(const fh_solver* s, double* out, int cap) {
 }
 struct fh_hho {
     std::unique_ptr<fh::HhoProblem> p;
+    // uncondensed blocks, assembled at the first request
+    std::unique_ptr<fh::Csr> unc[3];
 };
+struct fh_uamg {
+    fh::UamgHierarchy h;
+};
+
+int fh_hho_uncondensed(fh_hho* h, int which, int* nrows, int* ncols, int64_t* nnz, int* rowptr, int* colidx,
+                       double* vals) {
+    return guard([&] {
+        NEED(h);
+        FH_CHECK(which >= 0 && which < 3, -2, "which: 0 = A_TT, 1 = A_TF, 2 = A_FF");
+        if (!h->unc[0]) {
+            fh::Csr tt, tf, ff;
+            h->p->Uncondensed(tt, tf, ff);
+            h->unc[0] = std::make_unique<fh::Csr>(std::move(tt));
+            h->unc[1] = std::make_unique<fh::Csr>(std::move(tf));
+            h->unc[2] = std::make_unique<fh::Csr>(std::move(ff));
+        }
+        csr_out(*h->unc[which], nrows, ncols, nnz, rowptr, colidx, vals);
+    });
+}
+int fh_hho_cell_unknowns(const fh_hho* h, int* nc) {
+    return guard([&] {
+        NEED(h); NEED(nc);
+        *nc = h->p->cell_unknowns();
+    });
+}
+
+int fh_uamg_build(const fh_csr* A, const fh_csr* A_TT, const fh_csr* A_TF, const fh_csr* A_FF, int cell_bs,
+                  int face_bs, double theta, int max_coarse, fh_uamg** out) {
+    return guard([&] {
+        NEED(out);
+        fh::UamgArgs a;
+        a.cell_bs = cell_bs;
+        a.face_bs = face_bs;
+        a.theta = theta;
+        a.max_coarse = max_coarse;
+        auto u = std::make_unique<fh_uamg>();
+        u->h = fh::uamg_build(csr_in(A, "A"), csr_in(A_TT, "A_TT"), csr_in(A_TF, "A_TF"), csr_in(A_FF, "A_FF"), a);
+        *out = u.release();
+    });
+}
+int fh_uamg_destroy(fh_uamg* u) {
+    return guard([&] { delete u; });
+}
+int fh_uamg_info(const fh_uamg* u, int* nlevels, int* npasses) {
+    return guard([&] {
+        NEED(u);
+        if (nlevels) *nlevels = (int)u->h.A.size();
+        if (npasses) *npasses = (int)u->h.passes.size();
+    });
+}
+int fh_uamg_level(const fh_uamg* u, int l, int which, int* nrows, int* ncols, int64_t* nnz, int* rowptr, int* colidx,
+                  double* vals) {
+    return guard([&] {
+        NEED(u);
+        FH_CHECK(which == 0 || which == 1, -2, "which: 0 = operator, 1 = prolongation");
+        const auto& v = which == 0 ? u->h.A : u->h.P;
+        FH_CHECK(l >= 0 && l < (int)v.size(), -2, "level out of range");
+        csr_out(v[l], nrows, ncols, nnz, rowptr, colidx, vals);
+    });
+}
+int fh_uamg_pass(const fh_uamg* u, int k, int* level, int* pass, int* counts, int* elem_agg, int* face_coarse) {
+    return guard([&] {
+        NEED(u);
+        FH_CHECK(k >= 0 && k < (int)u->h.passes.size(), -2, "pass out of range");
+        const fh::UamgPass& p = u->h.passes[k];
+        if (level) *level = p.level;
+        if (pass) *pass = p.pass;
+        if (counts) {
+            counts[0] = p.n_elems; counts[1] = p.n_faces;
+            counts[2] = p.n_coarse_elems; counts[3] = p.n_coarse_faces;
+        }
+        if (elem_agg && !p.elem_agg.empty()) std::memcpy(elem_agg, p.elem_agg.data(), sizeof(int) * p.elem_agg.size());
+        if (face_coarse && !p.face_coarse.empty())
+            std::memcpy(face_coarse, p.face_coarse.data(), sizeof(int) * p.face_coarse.size());
+    });
+}
 
 int fh_hho_create(const fh_hho_args* a, fh_hho** out) {
     return guard([&] {

@@ -300,7 +300,9 @@ struct HhoProblem::Impl {
     // or the reconstruction-from-faces operator RF (nr x m nfb) without source,
     // p = P [-A_TT^-1 A_TF; I] v_F (ReconstructionFromFacesMatrix,
     // Diff_HHOElement.h:212-232).
-    void local(int c, double* Acond, double* bcond, const double* xF, double* recon, double* RF = nullptr) const;
+    // Aloc: the uncondensed local matrix (nc + m nfb)^2, cell unknowns first.
+    void local(int c, double* Acond, double* bcond, const double* xF, double* recon, double* RF = nullptr,
+               double* Aloc = nullptr) const;
 };
 
 // ------------------------------------------------------------------- mesh
@@ -663,7 +665,8 @@ void HhoProblem::Impl::build_tables() {
 
 // Local HHO operators of cell c (Diff_HHOElement::InitHHO,
 // Diff_HHOElement.h:104-150, 312-509) and their static condensation.
-void HhoProblem::Impl::local(int c, double* Acond, double* bcond, const double* xF, double* recon, double* RF) const {
+void HhoProblem::Impl::local(int c, double* Acond, double* bcond, const double* xF, double* recon, double* RF,
+                             double* Aloc) const {
     const int m = nfc, nh = nc + m * nfb;
     V3 v0;
     double J[3][3], Ji[3][3], det;
@@ -795,6 +798,7 @@ void HhoProblem::Impl::local(int c, double* Acond, double* bcond, const double* 
                 for (int s = 0; s < nh; s++) A[r * nh + s] += d * DF[j * nh + s];
             }
     }
+    if (Aloc) std::copy(A.begin(), A.end(), Aloc);
     // right-hand side on the cell: B_T = int f psi_i
     std::vector<double> BT(nc, 0.0);
     if (bcond || recon) {
@@ -951,6 +955,77 @@ HhoProblem::HhoProblem(const HhoArgs& args) : impl(new Impl) {
     times.local_s = std::chrono::duration<double>(t3 - t2).count();
     times.assemble_s = std::chrono::duration<double>(t3 - t1).count();
 }
+
+void HhoProblem::Uncondensed(Csr& ATT, Csr& ATF, Csr& AFF) const {
+    const Impl& I = *impl;
+    const int nc = I.nc, nfb = I.nfb, m = I.nfc, nh = nc + m * nfb;
+    // Diffusion_HHO::Assemble (Diffusion_HHO.h:410-470): A_T_T blocks, A_T_F
+    // blocks on non-Dirichlet faces (A_T_ndF) and A_F_F face pairs of every
+    // cell (A_ndF_ndF), through coefficient lists that drop |v| <= 1e-15
+    // (NonZeroCoefficients.h:36-40) and sum duplicates.
+    struct Tri {
+        int i, j;
+        double v;
+    };
+    std::vector<std::vector<Tri>> tt(I.ncell), tf(I.ncell), ff(I.ncell);
+#pragma omp parallel
+    {
+        std::vector<double> Al((size_t)nh * nh);
+#pragma omp for schedule(dynamic, 256)
+        for (int c = 0; c < I.ncell; c++) {
+            I.local(c, nullptr, nullptr, nullptr, nullptr, nullptr, Al.data());
+            auto add = [](std::vector<Tri>& t, int i, int j, double v) {
+                if (std::fabs(v) > 1e-15) t.push_back({i, j, v});
+            };
+            for (int a = 0; a < nc; a++)
+                for (int b2 = 0; b2 < nc; b2++) add(tt[c], c * nc + a, c * nc + b2, Al[(size_t)a * nh + b2]);
+            for (int f = 0; f < m; f++) {
+                const int rf = I.funk[I.cf[(size_t)c * m + f]];
+                if (rf < 0) continue;
+                for (int a = 0; a < nc; a++)
+                    for (int b2 = 0; b2 < nfb; b2++)
+                        add(tf[c], c * nc + a, rf * nfb + b2, Al[(size_t)a * nh + nc + f * nfb + b2]);
+                for (int g = 0; g < m; g++) {
+                    const int rg = I.funk[I.cf[(size_t)c * m + g]];
+                    if (rg < 0) continue;
+                    for (int a = 0; a < nfb; a++)
+                        for (int b2 = 0; b2 < nfb; b2++)
+                            add(ff[c], rf * nfb + a, rg * nfb + b2,
+                                Al[(size_t)(nc + f * nfb + a) * nh + nc + g * nfb + b2]);
+                }
+            }
+        }
+    }
+    auto fill = [](const std::vector<std::vector<Tri>>& parts, int nr, int ncol, Csr& M) {
+        std::vector<std::vector<std::pair<int, double>>> rows(nr);
+        for (const auto& p : parts)
+            for (const Tri& t : p) rows[t.i].push_back({t.j, t.v});
+        M = Csr();
+        M.nr = nr;
+        M.nc = ncol;
+        M.rp.assign(nr + 1, 0);
+        for (int i = 0; i < nr; i++) {
+            auto& r = rows[i];
+            std::stable_sort(r.begin(), r.end(), [](const std::pair<int, double>& x, const std::pair<int, double>& y) {
+                return x.first < y.first;
+            });
+            for (size_t k = 0; k < r.size(); k++) {
+                if (k > 0 && r[k].first == M.ci.back())
+                    M.v.back() += r[k].second;
+                else {
+                    M.ci.push_back(r[k].first);
+                    M.v.push_back(r[k].second);
+                }
+            }
+            M.rp[i + 1] = (int)M.ci.size();
+        }
+    };
+    fill(tt, I.ncell * nc, I.ncell * nc, ATT);
+    fill(tf, I.ncell * nc, nb * nfb, ATF);
+    fill(ff, nb * nfb, nb * nfb, AFF);
+}
+
+int HhoProblem::cell_unknowns() const { return impl->nc; }
 
 HhoProblem::~HhoProblem() { delete impl; }
 

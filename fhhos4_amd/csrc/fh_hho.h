@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "fh_common.h"
+#include "fh_uamg.h"
 
 namespace fh {
 
@@ -70,6 +71,15 @@ public:
     // (CellInterp_Trace, LevelForHHO.h:192-213, 743-763, 899-954).
     void Prolongation(const HhoProblem& coarse, std::vector<int>& prp, std::vector<int>& pci,
                       std::vector<double>& pv) const;
+    // Uncondensed hybrid blocks, laid out as the reference keeps them for
+    // U-AMG (Diffusion_HHO::A_T_T, A_T_ndF, A_ndF_ndF; Diffusion_HHO.h:52-53,
+    // 410-470, 564-624): cells in mesh order with cell_unknowns() DOFs each,
+    // faces in block-row order.  Entries are in this module's local bases
+    // (orthonormal on the reference shape), so for k >= 1 they equal the
+    // reference's up to a change of the cell and face bases; their Schur
+    // complement is the condensed operator.
+    void Uncondensed(Csr& ATT, Csr& ATF, Csr& AFF) const;
+    int cell_unknowns() const;
     HhoTimes times;
     bool host_assembled = false;       // vals and b hold the host assembly
     mutable double device_assemble_s = 0;

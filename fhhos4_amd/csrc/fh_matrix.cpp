@@ -149,12 +149,6 @@ int greedy_colouring(const Bcsr& A, std::vector<int>& colour) {
 
 // --------------------------------------------------------------- aggregation
 namespace {
-struct Elem {
-    std::vector<std::pair<int, double>> nbr;
-    std::vector<int> strong;
-    int count = 0;
-    int agg = -1;
-};
 }  // namespace
 
 int pairwise_aggregate(const Bcsr& A, double theta, std::vector<int>& agg, std::vector<std::vector<int>>& members,
@@ -163,7 +157,7 @@ int pairwise_aggregate(const Bcsr& A, double theta, std::vector<int>& agg, std::
     FH_CHECK(A.bs_r == A.bs_c, -3, "aggregation needs square blocks");
     const int nb = A.nbr, bs = A.bs_r;
     const size_t bsz = (size_t)bs * bs;
-    std::vector<Elem> E(nb);
+    std::vector<AggElem> E(nb);
     std::vector<std::atomic<int>> counts(nb);
     for (auto& x : counts) x.store(0);
     // Neighbours in the order the scalar rows of the block row list them:
@@ -171,7 +165,7 @@ int pairwise_aggregate(const Bcsr& A, double theta, std::vector<int>& agg, std::
     // entry in A_ij.  Coupling = trace(A_ij).
 #pragma omp parallel for schedule(dynamic, 1024)
     for (int i = 0; i < nb; i++) {
-        Elem& e = E[i];
+        AggElem& e = E[i];
         const int s = A.h_rowptr[i], t = A.h_rowptr[i + 1];
         std::vector<char> seen(t - s, 0);
         for (int k = 0; k < bs; k++) {
@@ -204,7 +198,12 @@ int pairwise_aggregate(const Bcsr& A, double theta, std::vector<int>& agg, std::
         }
     }
     for (int i = 0; i < nb; i++) E[i].count = counts[i].load();
+    return pairwise_pairing(E, agg, members, coarsest);
+}
 
+int pairwise_pairing(std::vector<AggElem>& E, std::vector<int>& agg, std::vector<std::vector<int>>& members,
+                     bool& coarsest) {
+    const int nb = (int)E.size();
     // Priority ordering: breadth-first from the element with the fewest
     // "I am strong neighbour of" counts, neighbours numbered by ascending count.
     std::vector<int> order(nb, -1);
@@ -224,12 +223,13 @@ int pairwise_aggregate(const Bcsr& A, double theta, std::vector<int>& agg, std::
             order[on++] = scan;
             numbered[scan] = 1;
         }
-        const Elem& e = E[order[i]];
+        const AggElem& e = E[order[i]];
         un.clear();
         for (auto& q : e.nbr) if (!numbered[q.first]) un.push_back(q.first);
         if (un.empty()) continue;
         std::sort(un.begin(), un.end(), [&E](int a, int b) { return E[a].count < E[b].count; });
-        for (int u : un) { order[on++] = u; numbered[u] = 1; }
+        // an element listed twice (two shared faces) is numbered once
+        for (int u : un) if (!numbered[u]) { order[on++] = u; numbered[u] = 1; }
     }
     // Pairing
     int nagg = 0, remaining = nb;

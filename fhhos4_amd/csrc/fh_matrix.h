@@ -39,6 +39,24 @@ int greedy_colouring(const Bcsr& A, std::vector<int>& colour);
 int pairwise_aggregate(const Bcsr& A, double theta, std::vector<int>& agg, std::vector<std::vector<int>>& members,
                        bool& coarsest);
 
+// Element of an algebraic mesh for pairwise aggregation
+// (AlgebraicElement / HybridAlgebraicElement): neighbours with their
+// coupling, sorted by ascending coupling; strong neighbours in that order;
+// count = NElementsIAmStrongNeighbourOf.
+struct AggElem {
+    std::vector<std::pair<int, double>> nbr;
+    std::vector<int> strong;
+    int count = 0;
+    int agg = -1;
+};
+
+// PairwiseAggregation::Perform (PairwiseAggregation.h:9-130) on built
+// elements: priority ordering (breadth-first from the element with the
+// smallest count, unnumbered neighbours by ascending count) and pairing
+// with the available strong neighbour of smallest count.
+int pairwise_pairing(std::vector<AggElem>& E, std::vector<int>& agg, std::vector<std::vector<int>>& members,
+                     bool& coarsest);
+
 // Galerkin operator of a piecewise-constant (block identity) prolongation:
 // Ac(I,J) = sum_{k in I, l in J} A_kl, entries |a| <= 1e-15 dropped, host
 // result (structure + FP64 values), device copy uploaded.

@@ -639,7 +639,15 @@ void Solver::Setup(const Bcsr& A) {
     const std::string& p = args_.solver == "cg" || args_.solver == "fcg" ? args_.prec : args_.solver;
     mg_.reset();
     prec_sm_.reset();
-    if (p == "mg" || p == "aggregamg") {
+    if (p == "uamg") {
+        // levels from SetupUncondensed; coarse operators are the hierarchy's
+        FH_CHECK(uamg_ && !given_levels_.empty() && given_levels_[0].first == &A, -2,
+                 "the uamg multigrid needs the uncondensed blocks (SetupUncondensed)");
+        SolverArgs a = args_;
+        a.galerkin = false;
+        mg_ = std::make_unique<Multigrid>(c_, a);
+        mg_->setup(A, given_levels_, false);
+    } else if (p == "mg" || p == "aggregamg") {
         mg_ = std::make_unique<Multigrid>(c_, args_);
         std::vector<std::pair<const Bcsr*, const Bcsr*>> g = given_levels_;
         if (p == "mg") {
@@ -672,6 +680,67 @@ void Solver::Setup(const Bcsr& A) {
     stats_.levels = NumLevels();
     stats_.op_complexity = mg_ ? mg_->op_complexity : 1.0;
     stats_.setup_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// Scalar CSR of the stored entries of A (host values; the stored-entry mask
+// when A was made from a scalar CSR, else every non-zero entry).
+static Csr scalar_csr(const Bcsr& A) {
+    FH_CHECK(!A.h_vals.empty() && A.bs_r == A.bs_c, -3, "the operator needs host values (keep_host)");
+    const int bs = A.bs_r;
+    const size_t bsz = (size_t)bs * bs;
+    Csr S;
+    S.nr = (int)A.rows();
+    S.nc = (int)A.cols();
+    S.rp.assign(S.nr + 1, 0);
+    std::vector<int> order;
+    for (int I = 0; I < A.nbr; I++) {
+        order.assign(A.h_colidx.begin() + A.h_rowptr[I], A.h_colidx.begin() + A.h_rowptr[I + 1]);
+        std::vector<int> pos(order.size());
+        for (size_t t = 0; t < pos.size(); t++) pos[t] = A.h_rowptr[I] + (int)t;
+        std::sort(pos.begin(), pos.end(), [&A](int x, int y) { return A.h_colidx[x] < A.h_colidx[y]; });
+        for (int a = 0; a < bs; a++) {
+            for (int p : pos)
+                for (int b = 0; b < bs; b++) {
+                    const size_t e = (size_t)p * bsz + (size_t)a * bs + b;
+                    const bool stored = A.h_mask.empty() ? A.h_vals[e] != 0.0 : A.h_mask[e] != 0;
+                    if (!stored) continue;
+                    S.ci.push_back(A.h_colidx[p] * bs + b);
+                    S.v.push_back(A.h_vals[e]);
+                }
+            S.rp[I * bs + a + 1] = (int)S.ci.size();
+        }
+    }
+    return S;
+}
+
+void Solver::SetupUncondensed(const Bcsr& A, const Csr& ATT, const Csr& ATF, const Csr& AFF, int cell_bs) {
+    const std::string& p = args_.solver == "cg" || args_.solver == "fcg" ? args_.prec : args_.solver;
+    FH_CHECK(p == "uamg", -2, "uncondensed blocks are only used by the uamg multigrid");
+    UamgArgs ua;
+    ua.cell_bs = cell_bs;
+    ua.face_bs = A.bs_r;
+    ua.theta = args_.theta;
+    ua.max_coarse = args_.max_coarse;
+    uamg_ = std::make_unique<UamgHierarchy>(uamg_build(scalar_csr(A), ATT, ATF, AFF, ua));
+    uamg_mats_.clear();
+    given_levels_.clear();
+    const int nl = (int)uamg_->A.size();
+    std::vector<const Bcsr*> Al(nl, nullptr), Pl(nl, nullptr);
+    Al[0] = &A;
+    for (int l = 0; l < nl; l++) {
+        if (l > 0) {
+            const Csr& M = uamg_->A[l];
+            uamg_mats_.push_back(bcsr_from_csr(c_, M.nr, M.nc, A.bs_r, M.rp.data(), M.ci.data(), M.v.data(), true));
+            Al[l] = uamg_mats_.back().get();
+        }
+        if (l + 1 < nl) {
+            const Csr& M = uamg_->P[l];
+            uamg_mats_.push_back(bcsr_from_csr(c_, M.nr, M.nc, A.bs_r, M.rp.data(), M.ci.data(), M.v.data(), true));
+            Pl[l] = uamg_mats_.back().get();
+        }
+    }
+    for (int l = 0; l < nl; l++) SetLevel(l, Al[l], Pl[l]);
+    Setup(A);
 }
 
 void Solver::SetStop(double tol, int max_iter) {

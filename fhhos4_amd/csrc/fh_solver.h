@@ -17,6 +17,7 @@
 
 #include "fh_common.h"
 #include "fh_matrix.h"
+#include "fh_uamg.h"
 
 namespace fh {
 
@@ -112,6 +113,12 @@ public:
     // z0 + omega D^-1 r1.  The network and graph must outlive the solver.
     void SetLearnedSmoother(Mpnn* nn, const NnGraph* g, double omega, bool unit_step = false);
     void Setup(const Bcsr& A);
+    // IterativeSolver::Setup(A, A_T_T, A_T_F, A_F_F) (IterativeSolver.h:34-37)
+    // for the "uamg" multigrid (UncondensedAMG::Setup, UncondensedAMG.h:84-98):
+    // builds the hierarchy from the uncondensed blocks on the host, then sets
+    // up as with given levels.  A needs host values (keep_host).
+    void SetupUncondensed(const Bcsr& A, const Csr& ATT, const Csr& ATF, const Csr& AFF, int cell_bs);
+    const UamgHierarchy* Uamg() const { return uamg_.get(); }
     // Solves A x = b from x = 0 (FP64 device vectors of length A.rows()).
     void Solve(const double* b, double* x);
     // IterativeSolver::Tolerance / MaxIterations (IterativeSolver.h:11-23).
@@ -135,6 +142,9 @@ private:
     std::unique_ptr<Multigrid> mg_;
     std::unique_ptr<LevelSmoother> prec_sm_;   // bj / bgs as preconditioner
     std::vector<std::pair<const Bcsr*, const Bcsr*>> given_levels_;
+    // U-AMG hierarchy and the level matrices made from it
+    std::unique_ptr<UamgHierarchy> uamg_;
+    std::vector<std::unique_ptr<Bcsr>> uamg_mats_;
     SolveStats stats_;
     std::vector<double> hist_;
     DVec r_, z_, d_, Ad_, Az_, dk_, Adk_, t_;

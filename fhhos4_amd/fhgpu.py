@@ -113,6 +113,15 @@ _SIGS = {
     "fh_hho_prolongation": [_vp, _vp, _vp, C.c_int, _pp],
     "fh_hho_face_cells": [_vp, _i32p],
     "fh_hho_cell_kappa": [_vp, _f64p],
+    "fh_hho_uncondensed": [_vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int64), _vp, _vp, _vp],
+    "fh_hho_cell_unknowns": [_vp, C.POINTER(C.c_int)],
+    "fh_solver_setup_uncondensed": [_vp, _vp, _vp, _vp, _vp, C.c_int],
+    "fh_uamg_build": [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_double, C.c_int, _pp],
+    "fh_uamg_destroy": [_vp],
+    "fh_uamg_info": [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int)],
+    "fh_uamg_level": [_vp, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int64), _vp, _vp,
+                      _vp],
+    "fh_uamg_pass": [_vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), _vp, _vp, _vp],
 }
 
 
@@ -459,6 +468,13 @@ class Solver:
         self.A = A
         _check(lib().fh_solver_setup(self.h, A.h))
 
+    def setup_uncondensed(self, A, A_TT, A_TF, A_FF, cell_bs):
+        """Set-up of the "uamg" multigrid from the uncondensed blocks (scipy
+        matrices; A created with keep_host)."""
+        self.A = A
+        keep = [_CsrArg(M) for M in (A_TT, A_TF, A_FF)]
+        _check(lib().fh_solver_setup_uncondensed(self.h, A.h, *[C.byref(k.s) for k in keep], int(cell_bs)))
+
     @_ordered
     def solve(self, b, x):
         st = SolveStats()
@@ -476,6 +492,70 @@ class Solver:
     def __del__(self):
         try:
             lib().fh_solver_destroy(self.h)
+        except Exception:
+            pass
+
+
+class _Csr(C.Structure):
+    _fields_ = [("nrows", C.c_int), ("ncols", C.c_int), ("rowptr", C.POINTER(C.c_int)),
+                ("colidx", C.POINTER(C.c_int)), ("vals", C.POINTER(C.c_double))]
+
+
+class _CsrArg:
+    """fh_csr view of a scipy matrix (sorted CSR; arrays kept alive)."""
+
+    def __init__(self, M):
+        M = M.tocsr()
+        M.sort_indices()
+        self.rp = np.ascontiguousarray(M.indptr, dtype=np.int32)
+        self.ci = np.ascontiguousarray(M.indices, dtype=np.int32)
+        self.v = np.ascontiguousarray(M.data, dtype=np.float64)
+        self.s = _Csr(M.shape[0], M.shape[1], self.rp.ctypes.data_as(C.POINTER(C.c_int)),
+                      self.ci.ctypes.data_as(C.POINTER(C.c_int)), self.v.ctypes.data_as(C.POINTER(C.c_double)))
+
+
+def _csr_get(fn, *args):
+    """Two-call CSR read-out: sizes, then arrays -> scipy CSR."""
+    import scipy.sparse as sp
+    nr, nc, nnz = C.c_int(), C.c_int(), C.c_int64()
+    _check(fn(*args, C.byref(nr), C.byref(nc), C.byref(nnz), None, None, None))
+    rp = np.zeros(nr.value + 1, dtype=np.int32)
+    ci = np.zeros(nnz.value, dtype=np.int32)
+    v = np.zeros(nnz.value)
+    _check(fn(*args, None, None, None, rp.ctypes.data, ci.ctypes.data, v.ctypes.data))
+    return sp.csr_matrix((v, ci, rp), shape=(nr.value, nc.value))
+
+
+class UncondensedAMG:
+    """U-AMG hierarchy built on the host (UncondensedAMG set-up): level
+    operators A[l], prolongations P[l] (level l+1 -> l) and the aggregation of
+    every coarsening pass."""
+
+    def __init__(self, A, A_TT, A_TF, A_FF, cell_bs, face_bs, theta=0.25, max_coarse=1000):
+        args = [_CsrArg(M) for M in (A, A_TT, A_TF, A_FF)]
+        h = C.c_void_p()
+        _check(lib().fh_uamg_build(*[C.byref(a.s) for a in args], int(cell_bs), int(face_bs), float(theta),
+                                   int(max_coarse), C.byref(h)))
+        self.h = h
+        nl, npass = C.c_int(), C.c_int()
+        _check(lib().fh_uamg_info(self.h, C.byref(nl), C.byref(npass)))
+        self.A = [_csr_get(lib().fh_uamg_level, self.h, l, 0) for l in range(nl.value)]
+        self.P = [_csr_get(lib().fh_uamg_level, self.h, l, 1) for l in range(nl.value - 1)]
+        self.passes = []
+        for k in range(npass.value):
+            lv, ps = C.c_int(), C.c_int()
+            cnt = np.zeros(4, dtype=np.int32)
+            _check(lib().fh_uamg_pass(self.h, k, C.byref(lv), C.byref(ps), cnt.ctypes.data, None, None))
+            agg = np.full(max(cnt[0], 1), -1, dtype=np.int32)
+            fc = np.full(max(cnt[1], 1), -1, dtype=np.int32)
+            _check(lib().fh_uamg_pass(self.h, k, None, None, None, agg.ctypes.data, fc.ctypes.data))
+            self.passes.append({"level": lv.value, "pass": ps.value, "elems": int(cnt[0]), "faces": int(cnt[1]),
+                                "coarse_elems": int(cnt[2]), "coarse_faces": int(cnt[3]),
+                                "elem_agg": agg[:cnt[0]], "face_coarse": fc[:cnt[1]]})
+
+    def __del__(self):
+        try:
+            lib().fh_uamg_destroy(self.h)
         except Exception:
             pass
 
@@ -534,6 +614,13 @@ class HhoProblem:
         import scipy.sparse as sp
         rp, ci, v = self.blocks()
         return sp.bsr_matrix((v, ci, rp), shape=(self.n_unknowns, self.n_unknowns)).tocsr()
+
+    def uncondensed(self):
+        """(A_TT, A_TF, A_FF) as scipy CSR, and the cell unknowns per cell."""
+        out = tuple(_csr_get(lib().fh_hho_uncondensed, self.h, w) for w in range(3))
+        nc = C.c_int()
+        _check(lib().fh_hho_cell_unknowns(self.h, C.byref(nc)))
+        return out + (nc.value,)
 
     def l2_error(self, x):
         e = C.c_double()

@@ -309,6 +309,51 @@ int fh_hho_prolongation(fh_ctx* ctx, const fh_hho* fine, const fh_hho* coarse, i
 /* Cells of every interior face (2 ints per block row) and cell kappa. */
 int fh_hho_face_cells(const fh_hho* h, int* out);
 int fh_hho_cell_kappa(const fh_hho* h, double* out);
+/* Uncondensed hybrid blocks kept for U-AMG (Diffusion_HHO::A_T_T, A_T_ndF,
+ * A_ndF_ndF; Diffusion_HHO.h:52-53,410-470,564-624, kept when the solver is
+ * uamg, Program_Diffusion_HHO.h:180-186): which = 0 A_TT (cells x cells,
+ * cell_unknowns DOFs per cell), 1 A_TF (cells x non-Dirichlet face DOFs),
+ * 2 A_FF.  Scalar CSR, ascending columns; sizes always, arrays when non-NULL
+ * (rowptr nrows+1, colidx and vals nnz).  Assembled at the first call. */
+int fh_hho_uncondensed(fh_hho* h, int which, int* nrows, int* ncols, int64_t* nnz, int* rowptr, int* colidx,
+                       double* vals);
+/* Number of cell unknowns per cell (HHO->nCellUnknowns). */
+int fh_hho_cell_unknowns(const fh_hho* h, int* nc);
+
+/* --------------------------------------------- uncondensed AMG (U-AMG) */
+/* Host scalar CSR (RowMajor, int32 indices, strictly ascending columns in
+ * every row): the reference's SparseMatrix (src/Utils/Types.h:11-22). */
+typedef struct fh_csr {
+    int nrows, ncols;
+    const int* rowptr;
+    const int* colidx;
+    const double* vals;
+} fh_csr;
+/* IterativeSolver::Setup(A, A_T_T, A_T_F, A_F_F) (src/Solver/IterativeSolver.h:34-37,
+ * FlexibleConjugateGradient.h:39-44) for a solver whose multigrid is "uamg"
+ * (UncondensedAMG::Setup, src/Solver/Multigrid/UncondensedAMG/UncondensedAMG.h:84-98):
+ * the hierarchy is built on the host from the uncondensed blocks with the
+ * reference's default U-AMG options (main.cpp:1847-1868), then the levels run
+ * on the GPU.  A must be created with keep_host. */
+int fh_solver_setup_uncondensed(fh_solver* s, const fh_mat* A, const fh_csr* A_TT, const fh_csr* A_TF,
+                                const fh_csr* A_FF, int cell_bs);
+/* The same hierarchy without a solver (host only; no device needed):
+ * UncondensedLevel::CoarsenMesh for every level (UncondensedLevel.h:74-185). */
+typedef struct fh_uamg fh_uamg;
+int fh_uamg_build(const fh_csr* A, const fh_csr* A_TT, const fh_csr* A_TF, const fh_csr* A_FF, int cell_bs,
+                  int face_bs, double theta, int max_coarse, fh_uamg** out);
+int fh_uamg_destroy(fh_uamg* u);
+int fh_uamg_info(const fh_uamg* u, int* nlevels, int* npasses);
+/* which = 0: operator of level l (Level::OperatorMatrix); which = 1:
+ * prolongation from level l+1 to level l (Level::P, chained over the passes).
+ * Sizes always, arrays when non-NULL. */
+int fh_uamg_level(const fh_uamg* u, int l, int which, int* nrows, int* ncols, int64_t* nnz, int* rowptr, int* colidx,
+                  double* vals);
+/* Coarsening pass k (in order over all levels): its level and pass number,
+ * counts = {elements, faces, coarse elements, coarse faces}, the aggregate of
+ * every element (HybridElementAggregate::Number) and the coarse face of
+ * every face (-1 for a face removed inside an aggregate). */
+int fh_uamg_pass(const fh_uamg* u, int k, int* level, int* pass, int* counts, int* elem_agg, int* face_coarse);
 
 #ifdef __cplusplus
 }

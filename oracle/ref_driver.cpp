@@ -160,6 +160,17 @@ static int Run(const Opts& o)
 	}
 	else if (p == "aggregamg")
 		mg.H_CS = H_CoarsStgy::DoublePairwiseAggregation; // main.cpp:1874-1884
+	const bool uamg = p == "uamg";
+	if (uamg)
+	{
+		// Defaults resolved by main.cpp:1847-1868 for -s uamg / -p uamg
+		mg.UAMGMultigridProlong = UAMGProlongation::ChainedCoarseningProlongations;
+		mg.UAMGFaceProlong = UAMGFaceProlongation::BoundaryAggregatesInteriorAverage;
+		mg.UAMGCoarseningProlong = UAMGProlongation::ReconstructSmoothedTraceOrInject;
+		mg.H_CS = H_CoarsStgy::MultiplePairwiseAggregation;
+		mg.CoarseningFactor = 3.8;
+		mg.UseGalerkinOperator = true;
+	}
 	args.Actions.Export.MultigridComponents = o.exportMG;
 	args.OutputDirectory = o.out;
 	Utils::ProgramArgs = args;
@@ -180,7 +191,8 @@ static int Run(const Opts& o)
 	FunctionalBasis<Dim>* cellBasis = FunctionalBasisFactory<Dim>::Create(args.Discretization.ElemBasisCode, k, false);
 	FunctionalBasis<Dim - 1>* faceBasis = FunctionalBasisFactory<Dim - 1>::Create(args.Discretization.FaceBasisCode, k, false);
 	HHOParameters<Dim>* hho = new HHOParameters<Dim>(mesh, "hho", reconstructionBasis, cellBasis, faceBasis, args.Discretization.OrthogonalizeElemBasesCode, args.Discretization.OrthogonalizeFaceBasesCode);
-	Diffusion_HHO<Dim>* problem = new Diffusion_HHO<Dim>(mesh, testCase, hho, true, false);
+	// uncondensed blocks kept for U-AMG (Program_Diffusion_HHO.h:180-186)
+	Diffusion_HHO<Dim>* problem = new Diffusion_HHO<Dim>(mesh, testCase, hho, true, uamg);
 
 	auto t0 = chrono::steady_clock::now();
 	args.Actions.LogAssembly = false;
@@ -188,6 +200,12 @@ static int Run(const Opts& o)
 	auto t1 = chrono::steady_clock::now();
 	write_csr(o.out + "/A.csr", problem->A);
 	write_vec(o.out + "/b.vec", problem->b);
+	if (uamg)
+	{
+		write_csr(o.out + "/A_T_T.csr", problem->A_T_T);
+		write_csr(o.out + "/A_T_F.csr", problem->A_T_ndF);
+		write_csr(o.out + "/A_F_F.csr", problem->A_ndF_ndF);
+	}
 	if (o.exportDat)
 	{
 		// The reference's own Matrix Market export of the linear system
@@ -233,7 +251,10 @@ static int Run(const Opts& o)
 	it->ComputeExactSolution = false;
 	it->OnIterationEnd = [&hist](const IterationResult& r) { hist.push_back(r.NormalizedResidualNorm); };
 	auto t2 = chrono::steady_clock::now();
-	solver->Setup(problem->A);
+	if (uamg) // Program_Diffusion_HHO.h:268-269
+		it->Setup(problem->A, problem->A_T_T, problem->A_T_ndF, problem->A_ndF_ndF);
+	else
+		solver->Setup(problem->A);
 	auto t3 = chrono::steady_clock::now();
 	Vector x = it->Solve(problem->b, "0");
 	auto t4 = chrono::steady_clock::now();

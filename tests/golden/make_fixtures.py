@@ -12,6 +12,10 @@ repository compiled against the reference sources in place, see
                  exported by ``Level::Setup`` (Level.h:116-125),
 * ``agg1``     : first-pass pairwise aggregate of every block row
                  (AlgebraicMesh.h:54-125, PairwiseAggregation.h:9-54).
+* ``ATT_*``, ``ATF_*``, ``AFF_*`` : the uncondensed blocks A_T_T, A_T_ndF,
+                 A_ndF_ndF that U-AMG is set up from (Diffusion_HHO.h:52-53),
+* ``uamg_factors`` : coarsening factor after every U-AMG pass, as the
+                 reference prints it (UncondensedLevel.h:121-124).
 
 * ``dat_*``    : the reference's Matrix Market export (``A.dat``, ``b.dat``,
                  ``solutionFaces.dat``; ExportModule.h:68-87) as raw bytes.
@@ -66,7 +70,7 @@ def pack_csr(prefix, M, d):
     d[prefix + "_shape"] = np.array(M.shape, dtype=np.int64)
 
 
-def run(name, args, export_mg=False, keep_A=True, aggreg=None, export_dat=False):
+def run(name, args, export_mg=False, keep_A=True, aggreg=None, export_dat=False, uamg=False):
     if ONLY and name not in ONLY:
         return
     with tempfile.TemporaryDirectory() as tmp:
@@ -75,7 +79,7 @@ def run(name, args, export_mg=False, keep_A=True, aggreg=None, export_dat=False)
             cmd.append("-export-mg")
         if export_dat:
             cmd.append("-export-dat")
-        subprocess.run(cmd, check=True, stdout=subprocess.DEVNULL)
+        out = subprocess.run(cmd, check=True, stdout=subprocess.PIPE).stdout.decode(errors="replace")
         info = read_info(os.path.join(tmp, "info.txt"))
         d = {"cmd": np.array(" ".join(["refdrv"] + args))}
         if keep_A:
@@ -86,6 +90,10 @@ def run(name, args, export_mg=False, keep_A=True, aggreg=None, export_dat=False)
             d["hist"] = read_vec(os.path.join(tmp, "hist.vec"))
         for k, v in info.items():
             d["info_" + k] = np.array(v)
+        if uamg:
+            for blk, f in (("ATT", "A_T_T"), ("ATF", "A_T_F"), ("AFF", "A_F_F")):
+                pack_csr(blk, read_csr(os.path.join(tmp, f + ".csr")), d)
+            d["uamg_factors"] = np.array([float(l.split("=")[1]) for l in out.splitlines() if "Coarsening factor" in l])
         if export_dat:
             # the reference's Matrix Market files, byte for byte
             for f in ("A", "b", "solutionFaces"):
@@ -145,6 +153,19 @@ def main():
     # 2-D triangles (Square_TriangularMesh), k = 2: the learned-smoother family
     run("stri8_k2_cg_bj", ["-dim", "2", "-mesh", "stri", "-k", "2", "-n", "8", "-source", "pisine", "-s", "cg",
                            "-prec", "bj"])
+    # Uncondensed AMG (FCG + K-cycle U-AMG, the reference's default options):
+    # scalar and block hybrid systems, one with three levels
+    uk = ["-s", "fcg", "-prec", "uamg", "-cycle", "K,1,1", "-smoothers", "bj23,bj23"]
+    run("uamg_tet8_k0", ["-dim", "3", "-mesh", "stetra", "-k", "0", "-n", "8", "-source", "sine"] + uk,
+        export_mg=True, uamg=True)
+    run("uamg_tet12_k0", ["-dim", "3", "-mesh", "stetra", "-k", "0", "-n", "12", "-source", "sine"] + uk,
+        export_mg=True, uamg=True)
+    run("uamg_tet4_k1", ["-dim", "3", "-mesh", "stetra", "-k", "1", "-n", "4", "-source", "sine"] + uk,
+        export_mg=True, uamg=True)
+    run("uamg_tet4_k2", ["-dim", "3", "-mesh", "stetra", "-k", "2", "-n", "4", "-source", "sine"] + uk,
+        export_mg=True, uamg=True)
+    run("uamg_stri16_k2", ["-dim", "2", "-mesh", "stri", "-k", "2", "-n", "16", "-source", "pisine"] + uk,
+        export_mg=True, uamg=True)
     # Exported linear systems (Matrix Market text) of tiny 2-D and 3-D problems
     run("mm_c1_n4", c1s + ["-n", "4", "-s", "cg", "-prec", "mg", "-smoothers", "bj,bj"], export_dat=True)
     run("mm_tet2_k1", ["-dim", "3", "-mesh", "stetra", "-k", "1", "-n", "2", "-source", "sine", "-s", "cg",
