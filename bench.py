@@ -91,11 +91,12 @@ class Variant:
         return s
 
 
-def make_solver(ctx, G, hier, A, v, solver="cg", mg_prec="f64", mg_store=None, max_iter=1000):
+def make_solver(ctx, G, hier, A, v, solver="cg", mg_prec="f64", mg_store=None, max_iter=1000, coarse_store=None):
     from fhhos4_amd import gmg
     s = G.Solver(ctx, solver=solver, prec="mg" if v.prec == "gmg" else "aggregamg", cycle=v.cycle, nu=(v.nu, v.nu),
                  smoothers=(v.smoother, v.smoother), relax=v.relax, max_iter=max_iter, tol=1e-8, mg_prec=mg_prec,
-                 mg_store=mg_store, galerkin=v.galerkin, coarse_smoother=v.coarse, coarse_relax=v.coarse_relax)
+                 mg_store=mg_store, galerkin=v.galerkin, coarse_smoother=v.coarse, coarse_relax=v.coarse_relax,
+                 coarse_store=coarse_store)
     if v.prec == "gmg":
         gmg.attach(s, hier)
     t = time.perf_counter()
@@ -243,6 +244,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=64)
+    ap.add_argument("--sweep-mixed", action="store_true", help="with --sweep: FCG + FP32 / BF16-coarse multigrid")
     ap.add_argument("--k", type=int, default=2)
     ap.add_argument("--perturb", type=float, default=0.2)
     ap.add_argument("--seed", type=int, default=2026)
@@ -297,14 +299,16 @@ def main():
     torch.cuda.synchronize()
 
     if args.sweep:
-        variants = [Variant("gmg", True, "sbgs", 1.0, 1), Variant("gmg", True, "sbgs", 1.0, 1, "V", "cheb2"),
-                    Variant("gmg", True, "cheb3", 1.0, 1), Variant("gmg", True, "cheb2", 1.0, 1),
-                    Variant("gmg", True, "cheb4", 1.0, 1), Variant("gmg", False, "sbgs", 1.0, 1),
-                    Variant("gmg", False, "cheb3", 1.0, 1)]
+        variants = [Variant("gmg", True, "cheb2", 1.0, 1), Variant("gmg", True, "cheb3", 1.0, 1),
+                    Variant("gmg", False, "cheb2", 1.0, 1), Variant("gmg", False, "cheb3", 1.0, 1),
+                    Variant("gmg", False, "cheb4", 1.0, 1), Variant("gmg", False, "cheb2", 1.0, 2),
+                    Variant("gmg", False, "cheb2", 1.0, 1, "W")]
+        # FP64 CG, or (--sweep-mixed) FCG with the FP32 / BF16-coarse multigrid
+        mk = (dict(solver="fcg", mg_prec="f32", mg_store="f32", coarse_store="bf16") if args.sweep_mixed else {})
         for v in variants:
-            log("variant", v.label())
+            log("variant", v.label(), "mixed" if mk else "fp64")
             try:
-                s, su = make_solver(ctx, G, hier, A, v, max_iter=args.max_iter)
+                s, su = make_solver(ctx, G, hier, A, v, max_iter=args.max_iter, **mk)
                 log("  set-up %.1fs" % su)
                 ts, st = timed_solves(ctx, s, b, x, args.steps, args.warmup)
                 print(json.dumps({"variant": v.label(), "s": float(np.mean(ts)), "iterations": st["iterations"],
@@ -345,16 +349,18 @@ def main():
 
     mixed = {}
     if not args.no_mixed:
-        for store in ("f32", "bf16"):
-            log("set-up FCG + %s multigrid" % store)
+        # storage of the fine level / of the coarse levels
+        for store, cstore in (("f32", None), ("bf16", None), ("f32", "bf16")):
+            key = store if cstore is None else "%s_coarse_%s" % (store, cstore)
+            log("set-up FCG + %s multigrid" % key)
             s, su = make_solver(ctx, G, hier, A, v, solver="fcg", mg_prec="f32", mg_store=store,
-                                max_iter=args.max_iter)
+                                max_iter=args.max_iter, coarse_store=cstore)
             ts, st = timed_solves(ctx, s, b, x, args.steps, args.warmup)
             xm = x.cpu().numpy()
-            mixed[store] = {"s": float(np.mean(ts)), "iterations": st["iterations"], "setup_s": su,
-                            "rel_err_vs_fp64": float(np.linalg.norm(xm - x64) / np.linalg.norm(x64)),
-                            "l2": P.l2_error(xm), "speedup_vs_fp64": t64 / float(np.mean(ts))}
-            log("FCG + %s MG: %d iterations, %.3f s" % (store, st["iterations"], mixed[store]["s"]))
+            mixed[key] = {"s": float(np.mean(ts)), "iterations": st["iterations"], "setup_s": su,
+                          "rel_err_vs_fp64": float(np.linalg.norm(xm - x64) / np.linalg.norm(x64)),
+                          "l2": P.l2_error(xm), "speedup_vs_fp64": t64 / float(np.mean(ts))}
+            log("FCG + %s MG: %d iterations, %.3f s" % (key, st["iterations"], mixed[key]["s"]))
             del s
 
     learned = None

@@ -430,6 +430,9 @@ int fh_solver_create(fh_ctx* ctx, const fh_solver_args* a, fh_solver** out) {
         s.theta = a->theta > 0 ? a->theta : 0.25;
         s.mg_prec = P(a->mg_prec);
         s.mg_store = P(a->mg_store);
+        FH_CHECK(a->coarse_store == 0 || a->coarse_store == 1 || a->coarse_store == 2, -2,
+                 "coarse_store: 0 (as mg_store), FH_F32 or FH_BF16");
+        s.coarse_store = a->coarse_store == 0 ? s.mg_store : P(a->coarse_store);
         s.galerkin = a->galerkin != 0;
         if (a->coarse_smoother) s.coarse_smoother = a->coarse_smoother;
         s.coarse_relax = a->coarse_relax;
@@ -443,6 +446,8 @@ int fh_solver_create(fh_ctx* ctx, const fh_solver_args* a, fh_solver** out) {
                  "FP32/BF16 level matrices need FP32 multigrid vectors");
         FH_CHECK(s.mg_prec == fh::Prec::F64 || s.mg_store != fh::Prec::F64, -2,
                  "FP32 multigrid vectors need FP32 or BF16 level matrices");
+        FH_CHECK(s.coarse_store == s.mg_store || s.mg_prec == fh::Prec::F32, -2,
+                 "a separate coarse-level storage needs FP32 multigrid vectors");
         auto sv = new fh_solver;
         sv->ctx = ctx;
         sv->s = std::make_unique<fh::Solver>(ctx->c, s);

@@ -436,8 +436,9 @@ private:
             MgLevel& V = L[l];
             V.n = V.A64->rows();
             op_complexity += (double)V.A64->nnzb / fine_nnz;
-            if (a_.mg_store != Prec::F64) {
-                V.Aw_own = bcsr_convert(c_, *V.A64, a_.mg_store);
+            const Prec store = l > 0 ? a_.coarse_store : a_.mg_store;
+            if (store != Prec::F64) {
+                V.Aw_own = bcsr_convert(c_, *V.A64, store);
                 V.Aw = V.Aw_own.get();
             } else
                 V.Aw = V.A64;

@@ -88,6 +88,7 @@ struct SolverArgs {
     double theta = 0.25;             // strong coupling threshold
     Prec mg_prec = Prec::F64;        // working precision of the preconditioner vectors
     Prec mg_store = Prec::F64;       // storage of level matrices (F64 / F32 / BF16)
+    Prec coarse_store = Prec::F64;   // storage of the levels below the finest (set to mg_store unless given)
     bool galerkin = false;           // geometric multigrid: coarse operators P^T A P (UseGalerkinOperator)
     // Smoother of the levels below the finest ("" = the same as the finest)
     // and its relaxation (<= 0: relax).  Galerkin coarse levels are dense

@@ -172,6 +172,9 @@ typedef struct fh_solver_args {
     double coarse_relax;
     int fcg_truncation;        /* FlexibleConjugateGradient::Truncation (<= 0: 1), <= 32 */
     int fcg_restart;           /* FlexibleConjugateGradient::Restart (FlexibleConjugateGradient.h:18-25) */
+    /* Not in the reference: storage of the level matrices below the finest
+     * (0 = as mg_store, FH_F32, FH_BF16; needs FP32 multigrid vectors). */
+    int coarse_store;
 } fh_solver_args;
 
 typedef struct fh_solve_stats {

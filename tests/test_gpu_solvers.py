@@ -120,19 +120,20 @@ def test_amg_hierarchy_matches_reference_levels(ctx):
     assert sizes == ref
 
 
-@pytest.mark.parametrize("mg_prec,mg_store", [("f32", "f32"), ("f32", "bf16")])
-def test_mixed_precision_fcg_reaches_fp64_tolerance(ctx, mg_prec, mg_store):
-    """FP64 FCG with an FP32 (or BF16-stored) multigrid preconditioner
-    converges to the same 1e-8 relative residual; its solution is compared
-    with the FP64 oracle solve."""
+@pytest.mark.parametrize("mg_prec,mg_store,coarse_store", [("f32", "f32", None), ("f32", "bf16", None),
+                                                           ("f32", "f32", "bf16")])
+def test_mixed_precision_fcg_reaches_fp64_tolerance(ctx, mg_prec, mg_store, coarse_store):
+    """FP64 FCG with an FP32 (or BF16-stored, or BF16 on the coarse levels
+    only) multigrid preconditioner converges to the same 1e-8 relative
+    residual; its solution is compared with the FP64 oracle solve."""
     d = load("c1_n64_cg_mg_bj")
     A = csr(d, "A")
     x, hist, st = run_gpu(ctx, d, 2, levels=gmg_levels(d), solver="fcg", prec="mg", smoothers=("bj23", "bj23"),
-                          mg_prec=mg_prec, mg_store=mg_store)
+                          mg_prec=mg_prec, mg_store=mg_store, coarse_store=coarse_store)
     assert st["converged"] == 1 and hist[-1] < 1e-8
     xo, _, oo = O.solve(A, d["b"], "fcg", "mg", nf=2, smoothers=("bj23", "bj23"), levels=gmg_levels(d))
     err = np.linalg.norm(x - xo) / np.linalg.norm(xo)
-    print("mixed precision", mg_prec, mg_store, "iterations", st["iterations"], "vs FP64", oo.iterations,
+    print("mixed precision", mg_prec, mg_store, coarse_store, "iterations", st["iterations"], "vs FP64", oo.iterations,
           "rel. error vs FP64 oracle", err)
     assert err < 1e-7
     assert st["iterations"] <= oo.iterations + 3
