@@ -777,6 +777,12 @@ int fh_hho_l2_error(const fh_hho* h, const double* x, double* err) {
         *err = h->p->L2Error(x);
     });
 }
+int fh_hho_l2_error_gpu(fh_ctx* ctx, const fh_hho* h, const double* x_dev, double* err) {
+    return guard([&] {
+        NEED(ctx); NEED_DEV(ctx); NEED(h); NEED(x_dev); NEED(err);
+        *err = h->p->L2ErrorDevice(ctx->c, x_dev);
+    });
+}
 int fh_hho_prolongation(fh_ctx* ctx, const fh_hho* fine, const fh_hho* coarse, int keep_host, fh_mat** out) {
     return guard([&] {
         NEED(ctx); NEED(fine); NEED(coarse); NEED(out);

@@ -1244,11 +1244,10 @@ double HhoProblem::L2Error(const double* x) const {
     return nrm2 > 0 ? std::sqrt(err / nrm2) : std::sqrt(err);
 }
 
-std::unique_ptr<Bcsr> HhoProblem::AssembleDevice(Ctx& c, double* b_dev, bool keep_host) const {
-    FH_CHECK(!c.host_only, -9, "device assembly needs a device context");
-    const Impl& I = *impl;
+// Device description of the problem (geometry, tables and per-cell maps
+// uploaded; buffers owned by `keep`).
+static HhoDev make_device_problem(Ctx& c, const HhoProblem& H, const HhoProblem::Impl& I, std::vector<DevBuf>& keep) {
     const int m = I.nfc;
-    auto t0 = std::chrono::steady_clock::now();
     // per-cell maps: block of every (face, face) pair and block row of every face
     std::vector<int> blk((size_t)I.ncell * m * m, -1), frow((size_t)I.ncell * m, -1);
 #pragma omp parallel for schedule(static)
@@ -1257,10 +1256,11 @@ std::unique_ptr<Bcsr> HhoProblem::AssembleDevice(Ctx& c, double* b_dev, bool kee
         for (int f = 0; f < m; f++) rr[f] = frow[(size_t)cc * m + f] = I.funk[I.cf[(size_t)cc * m + f]];
         for (int f = 0; f < m; f++) {
             if (rr[f] < 0) continue;
-            const int* beg = &colidx[rowptr[rr[f]]];
-            const int* end = &colidx[rowptr[rr[f] + 1]];
+            const int* beg = &H.colidx[H.rowptr[rr[f]]];
+            const int* end = &H.colidx[H.rowptr[rr[f] + 1]];
             for (int g = 0; g < m; g++)
-                if (rr[g] >= 0) blk[((size_t)cc * m + f) * m + g] = (int)(std::lower_bound(beg, end, rr[g]) - &colidx[0]);
+                if (rr[g] >= 0)
+                    blk[((size_t)cc * m + f) * m + g] = (int)(std::lower_bound(beg, end, rr[g]) - &H.colidx[0]);
         }
     }
     std::vector<double> Vf(3 * I.V.size()), cqp(3 * I.cq.p.size()), modes;
@@ -1275,7 +1275,6 @@ std::unique_ptr<Bcsr> HhoProblem::AssembleDevice(Ctx& c, double* b_dev, bool kee
         Psi.insert(Psi.end(), T.Psi.begin(), T.Psi.end());
         MFT.insert(MFT.end(), T.MFT.begin(), T.MFT.end());
     }
-    std::vector<DevBuf> keep;
     auto up = [&](const void* h, size_t bytes) {
         keep.emplace_back(std::max<size_t>(bytes, 8));
         if (bytes) FH_HIP(hipMemcpyAsync(keep.back().p, h, bytes, hipMemcpyHostToDevice, c.stream));
@@ -1302,6 +1301,37 @@ std::unique_ptr<Bcsr> HhoProblem::AssembleDevice(Ctx& c, double* b_dev, bool kee
     d.modes = (const double*)up(modes.data(), sizeof(double) * modes.size());
     d.blk = (const int*)up(blk.data(), sizeof(int) * blk.size());
     d.frow = (const int*)up(frow.data(), sizeof(int) * frow.size());
+    c.sync();  // host staging vectors are released on return
+    return d;
+}
+
+double HhoProblem::L2ErrorDevice(Ctx& c, const double* x_dev) const {
+    FH_CHECK(!c.host_only, -9, "device post-processing needs a device context");
+    const Impl& I = *impl;
+    std::vector<DevBuf> keep;
+    HhoDev d = make_device_problem(c, *this, I, keep);
+    DevBuf out(sizeof(double) * 2 * std::max(1, I.ncell));
+    d.post = 1;
+    d.xsol = x_dev;
+    d.cell_err = out.as<double>();
+    launch_hho_local(c, d);  // synchronises
+    std::vector<double> h(2 * (size_t)I.ncell);
+    FH_HIP(hipMemcpyAsync(h.data(), out.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c.stream));
+    c.sync();
+    double err = 0, nrm2 = 0;  // summed in cell order
+    for (int cc = 0; cc < I.ncell; cc++) {
+        err += h[2 * (size_t)cc];
+        nrm2 += h[2 * (size_t)cc + 1];
+    }
+    return nrm2 > 0 ? std::sqrt(err / nrm2) : std::sqrt(err);
+}
+
+std::unique_ptr<Bcsr> HhoProblem::AssembleDevice(Ctx& c, double* b_dev, bool keep_host) const {
+    FH_CHECK(!c.host_only, -9, "device assembly needs a device context");
+    const Impl& I = *impl;
+    auto t0 = std::chrono::steady_clock::now();
+    std::vector<DevBuf> keep;
+    HhoDev d = make_device_problem(c, *this, I, keep);
     auto A = std::make_unique<Bcsr>();
     A->nbr = A->nbc = nb;
     A->bs_r = A->bs_c = nf;

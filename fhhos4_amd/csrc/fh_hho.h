@@ -60,6 +60,10 @@ public:
     // relative L2 error of the k+1 reconstruction of the solution x (face
     // unknowns, length nb*nf) against the exact solution of the source
     double L2Error(const double* x) const;
+    // The same on the GPU: x_dev is the face solution in device memory; the
+    // cell recovery, k+1 reconstruction and quadrature run one wavefront per
+    // cell (Diffusion_HHO.h:120-149, 855-913, 960-966).
+    double L2ErrorDevice(Ctx& c, const double* x_dev) const;
     // cell diffusion coefficient (isotropic part) of every cell
     std::vector<double> CellKappa() const;
     // face adjacency: cells of every interior face (2 per face)

@@ -28,12 +28,19 @@ struct HhoDev {
     const int* frow;               // nfc per cell: block row of the face, -1 for a Dirichlet face
     double* vals;                  // block values (nf x nf row-major), accumulated
     double* b;                     // right-hand side, accumulated
+    // post-processing mode (post = 1): instead of the scatter, recover the
+    // cell unknowns from the face solution xsol (nb x nfb), reconstruct the
+    // k+1 polynomial and write the cell's integrals of (u - p)^2 and u^2
+    // into cell_err[2c], cell_err[2c+1]
+    int post;
+    const double* xsol;
+    double* cell_err;
 };
 
 // LDS bytes per cell (one workgroup per cell).
 size_t hho_local_lds_bytes(const HhoDev& d);
 // Batched local operators + static condensation + scatter (vals and b must
-// be zeroed).  Returns after the launch (asynchronous on c.stream).
+// be zeroed), or the post-processing mode.  Synchronises the stream.
 void launch_hho_local(Ctx& c, const HhoDev& d);
 
 }  // namespace fh

@@ -45,6 +45,25 @@ __device__ double source_at(const HhoDev& d, double x, double y, double z) {
     return d.dim * w * w * s;
 }
 
+// Exact solution of the source (its Laplacian is -source_at).
+__device__ double exact_at(const HhoDev& d, double x, double y, double z) {
+    const double X[3] = {x, y, z};
+    if (d.src == 2) {
+        double u = 0;
+        for (int m = 0; m < d.nmodes; m++) {
+            const double* md = d.modes + 4 * m;
+            double s = 1;
+            for (int a = 0; a < d.dim; a++) s *= sin(md[1 + a] * M_PI * X[a]);
+            u += md[0] * s;
+        }
+        return u;
+    }
+    const double w = d.src == 1 ? 4 * M_PI : M_PI;
+    double s = 1;
+    for (int a = 0; a < d.dim; a++) s *= sin(w * X[a]);
+    return s;
+}
+
 // In-place Cholesky of the n x n matrix M (row-major, leading dimension n)
 // by columns; same summation order as the host factorisation.
 __device__ void lds_chol(double* M, int n, int t, int* err) {
@@ -301,9 +320,55 @@ __global__ __launch_bounds__(kT) void k_hho_local(HhoDev d, int* err) {
     lds_chol(T, nc, t, err);
     lds_chol_solve(T, nc, X, nF + 1, nF + 1, t);
     __syncthreads();
+    const int* frow = d.frow + (size_t)c * m;
+    if (d.post) {
+        // ---- recovery v_T = A_TT^-1 (B_T - A_TF v_F) = X[:, nF] - X[:, :nF] v_F,
+        // reconstruction p = P [v_T; v_F], L2 integrals at the cell rule
+        double* w = DF;       // nh: [v_T; v_F]
+        double* pc = DF + nh; // nr
+        for (int r = t; r < nF; r += kT) {
+            const int f = r / nfb, a = r - f * nfb, row = frow[f];
+            w[nc + r] = row >= 0 ? d.xsol[(size_t)row * nfb + a] : 0.0;  // homogeneous Dirichlet
+        }
+        __syncthreads();
+        for (int i = t; i < nc; i += kT) {
+            double s = X[i * (nF + 1) + nF];
+            for (int j = 0; j < nF; j++) s -= X[i * (nF + 1) + j] * w[nc + j];
+            w[i] = s;
+        }
+        __syncthreads();
+        for (int i = t; i < nr; i += kT) {
+            double s = 0;
+            for (int j = 0; j < nh; j++) s += P[i * nh + j] * w[j];
+            pc[i] = s;
+        }
+        __syncthreads();
+        const double isq = 1.0 / sqrt(adet);
+        double e2 = 0, u2 = 0;
+        for (int q = t; q < d.nq; q += kT) {
+            const double* xi = d.cqp + 3 * q;
+            double x[3];
+            for (int a = 0; a < 3; a++) x[a] = v0[a] + J[a][0] * xi[0] + J[a][1] * xi[1] + J[a][2] * xi[2];
+            double ph = 0;
+            for (int i = 0; i < nr; i++) ph += pc[i] * d.cellPsi[(size_t)q * nr + i];
+            ph *= isq;
+            const double u = exact_at(d, x[0], x[1], x[2]);
+            const double wq = d.cqw[q] * adet;
+            e2 += wq * (u - ph) * (u - ph);
+            u2 += wq * u * u;
+        }
+        for (int o = kT / 2; o > 0; o >>= 1) {  // one wavefront per cell
+            e2 += __shfl_xor(e2, o, kT);
+            u2 += __shfl_xor(u2, o, kT);
+        }
+        if (t == 0) {
+            d.cell_err[2 * (size_t)c] = e2;
+            d.cell_err[2 * (size_t)c + 1] = u2;
+        }
+        return;
+    }
     // ---- condensed blocks and right-hand side, added into the global system
     const int* blk = d.blk + (size_t)c * m * m;
-    const int* frow = d.frow + (size_t)c * m;
     const int bsz = nfb * nfb;
     for (int e = t; e < nF * nF; e += kT) {
         const int r = e / nF, s = e - r * nF;
@@ -335,6 +400,11 @@ size_t hho_local_lds_bytes(const HhoDev& d) {
 
 void launch_hho_local(Ctx& c, const HhoDev& d) {
     FH_CHECK(d.nfc <= 8, -3, "at most 8 faces per cell");
+    {
+        const size_t nh = (size_t)d.nc + (size_t)d.nfc * d.nfb;
+        FH_CHECK((size_t)d.nfc * d.nfb * nh >= nh + (size_t)d.nr, -3, "post-processing scratch does not fit");
+        FH_CHECK(!d.post || (d.xsol && d.cell_err), -2, "post-processing needs the face solution and cell output");
+    }
     const size_t lds = hho_local_lds_bytes(d);
     int dev = 0, maxlds = 0;
     FH_HIP(hipGetDevice(&dev));

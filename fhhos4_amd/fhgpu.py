@@ -115,6 +115,7 @@ _SIGS = {
     "fh_hho_cell_kappa": [_vp, _f64p],
     "fh_hho_uncondensed": [_vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int64), _vp, _vp, _vp],
     "fh_hho_cell_unknowns": [_vp, C.POINTER(C.c_int)],
+    "fh_hho_l2_error_gpu": [_vp, _vp, _vp, C.POINTER(C.c_double)],
     "fh_solver_setup_uncondensed": [_vp, _vp, _vp, _vp, _vp, C.c_int],
     "fh_uamg_build": [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_double, C.c_int, _pp],
     "fh_uamg_destroy": [_vp],
@@ -623,6 +624,14 @@ class HhoProblem:
         nc = C.c_int()
         _check(lib().fh_hho_cell_unknowns(self.h, C.byref(nc)))
         return out + (nc.value,)
+
+    @_ordered
+    def l2_error_gpu(self, ctx, x):
+        """Relative L2 error of the reconstruction on the GPU (x: device
+        FP64 tensor of the face unknowns)."""
+        e = C.c_double()
+        _check(lib().fh_hho_l2_error_gpu(ctx.h, self.h, _ptr(x), C.byref(e)))
+        return e.value
 
     def l2_error(self, x):
         e = C.c_double()

@@ -298,6 +298,11 @@ int fh_hho_blocks(const fh_hho* h, int* rowptr, int* colidx, double* vals);
 /* Relative L2 error of the k+1 reconstruction of the face solution x (host)
  * against the exact solution (Diffusion_HHO::L2Error, Diffusion_HHO.h:120-149). */
 int fh_hho_l2_error(const fh_hho* h, const double* x, double* err);
+/* The same on the GPU: x_dev is the face solution in device memory (nb*nf
+ * FP64).  Cell recovery, k+1 reconstruction and the quadrature of the error
+ * run one wavefront per cell (Diffusion_HHO.h:120-149, 855-913, 960-966);
+ * the per-cell integrals are summed on the host in cell order. */
+int fh_hho_l2_error_gpu(fh_ctx* ctx, const fh_hho* h, const double* x_dev, double* err);
 /* Geometric multigrid prolongation from the problem on the once-coarsened
  * mesh (coarse->n = fine->n / 2, same ref_n) to the fine one: nf x nf block
  * matrix (fine faces x coarse faces).  Replaces LevelForHHO::SetupProlongation_H
