@@ -178,6 +178,50 @@ def kernel_bench(args, G, np, torch):
     print(json.dumps(out), flush=True)
 
 
+def hetero_bench(args, G, np, torch):
+    """Heterogeneous diffusion on Cartesian hexahedra (n^3 cells, HHO k): kappa
+    piecewise constant per 8^3-cell subdomain, lognormal with sigma = 3
+    (seeded, contrast ~1e6), one subdomain in eight anisotropic
+    diag(1, 1, 1e-2).  FP64 CG with the re-discretised geometric multigrid
+    against FP64 FCG with the FP32 / BF16-stored / BF16-coarse multigrid; the
+    error of each against the FP64 solution.  (Galerkin operators need a
+    full-rank prolongation, which CellInterp_Trace is not on hexahedra.)"""
+    from fhhos4_amd import gmg
+    P = G.HhoProblem(dim=3, mesh="cart", n=args.n, k=args.k, kappa="subdomain", sigma=3.0, subdomain=8, aniso=1e-2,
+                     seed=args.seed, source="pisine", defer_assembly=True)
+    N = P.n_unknowns
+    ctx = G.Context(0)
+    t = time.perf_counter()
+    hier = gmg.build_hierarchy(ctx, P)
+    log("hexahedral hierarchy", [h[0].n_unknowns for h in hier], "%.1fs" % (time.perf_counter() - t))
+    A, b = hier[0][1], P.b_device
+    x = torch.zeros_like(b)
+    v = Variant("gmg", False, args.smoother, args.relax, args.nu)
+    s64, su = make_solver(ctx, G, hier, A, v, max_iter=args.max_iter)
+    ts, st = timed_solves(ctx, s64, b, x, args.steps, args.warmup)
+    t64 = float(np.mean(ts))
+    x64 = x.cpu().numpy()
+    out = {"workload": "3-D heterogeneous diffusion, %d^3 hexahedra, HHO k=%d (%d unknowns), kappa lognormal per 8^3 "
+                       "subdomain (sigma 3), 1/8 anisotropic diag(1,1,1e-2)" % (args.n, args.k, N),
+           "solver": "gmg %s" % v.label(), "fp64_cg": {"s": t64, "iterations": st["iterations"], "setup_s": su,
+                                                      "converged": st["converged"]}}
+    log("FP64 CG + %s: %d iterations, %.3f s" % (v.label(), st["iterations"], t64))
+    del s64
+    for store, cstore in (("f32", None), ("bf16", None), ("f32", "bf16")):
+        key = store if cstore is None else "%s_coarse_%s" % (store, cstore)
+        s, su = make_solver(ctx, G, hier, A, v, solver="fcg", mg_prec="f32", mg_store=store, coarse_store=cstore,
+                            max_iter=args.max_iter)
+        ts, st = timed_solves(ctx, s, b, x, args.steps, args.warmup)
+        xm = x.cpu().numpy()
+        out["fcg_" + key] = {"s": float(np.mean(ts)), "iterations": st["iterations"], "converged": st["converged"],
+                             "rel_err_vs_fp64": float(np.linalg.norm(xm - x64) / np.linalg.norm(x64)),
+                             "speedup_vs_fp64": t64 / float(np.mean(ts))}
+        log("FCG + %s MG: %d iterations, %.3f s, rel. error vs FP64 %.2e" % (key, st["iterations"], out["fcg_" + key]["s"],
+                                                                            out["fcg_" + key]["rel_err_vs_fp64"]))
+        del s
+    print(json.dumps(out), flush=True)
+
+
 def learned_leg(args, G, ctx, hier, A, P, b, x, x64, t64, v):
     """FCG with the FP32 multigrid and the learned post-smoother.  The
     network is trained on the GPU (library kernels) on pairs from plain
@@ -245,6 +289,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=64)
     ap.add_argument("--sweep-mixed", action="store_true", help="with --sweep: FCG + FP32 / BF16-coarse multigrid")
+    ap.add_argument("--config", default="", choices=["", "hetero"],
+                    help="hetero: the heterogeneous hexahedral case (64^3 hexahedra, k=2, subdomain lognormal kappa)")
     ap.add_argument("--k", type=int, default=2)
     ap.add_argument("--perturb", type=float, default=0.2)
     ap.add_argument("--seed", type=int, default=2026)
@@ -280,6 +326,9 @@ def main():
 
     G = F.fhgpu
     assert torch.cuda.is_available(), "no ROCm device"
+    if args.config == "hetero":
+        hetero_bench(args, G, np, torch)
+        return
     if args.kernel == "spmv":
         return kernel_bench(args, G, np, torch)
     log("mesh and pattern of the HHO system n=%d k=%d (perturb %.2f)" % (args.n, args.k, args.perturb))

@@ -529,7 +529,10 @@ void HhoProblem::Impl::build_mesh() {
     std::vector<double> sub_kappa;
     int nsub = 0;
     if (a.kappa == "subdomain") {
-        FH_CHECK(a.subdomain % ratio == 0, -2, "subdomains must be unions of coarse cells");
+        // a cell lies in one subdomain, or (on coarse levels of a hierarchy)
+        // covers whole subdomains, whose tensors it then averages
+        FH_CHECK(a.subdomain % ratio == 0 || ratio % std::max(1, a.subdomain) == 0, -2,
+                 "subdomains must be unions of cells or of whole subdomains");
         int ns = std::max(1, rn / std::max(1, a.subdomain));
         nsub = ns;
         sub_kappa.resize((size_t)ns * ns * (dim == 3 ? ns : 1));
@@ -543,10 +546,30 @@ void HhoProblem::Impl::build_mesh() {
             int bc = kind == 0 ? c / (dim == 3 ? 6 : 2) : c;
             int ix = (bc % n) * ratio, iy = ((bc / n) % n) * ratio, iz = (dim == 3 ? bc / (n * n) : 0) * ratio;
             int sd = std::max(1, a.subdomain);
-            int sx = std::min(ix / sd, nsub - 1), sy = std::min(iy / sd, nsub - 1), sz = std::min(iz / sd, nsub - 1);
-            int s = (dim == 3 ? sz * nsub * nsub : 0) + sy * nsub + sx;
-            kap = sub_kappa[s];
-            if (s % 8 == 7) kz = a.aniso;  // one subdomain in eight is anisotropic
+            // subdomains covered by the cell (one unless the cell is coarser)
+            const int span = std::max(1, ratio / sd);
+            double kxy = 0, kzz = 0;
+            int cnt = 0;
+            for (int dz = 0; dz < (dim == 3 ? span : 1); dz++)
+                for (int dy = 0; dy < span; dy++)
+                    for (int dx = 0; dx < span; dx++) {
+                        int sx = std::min(ix / sd + dx, nsub - 1), sy = std::min(iy / sd + dy, nsub - 1),
+                            sz = std::min(iz / sd + dz, nsub - 1);
+                        int s = (dim == 3 ? sz * nsub * nsub : 0) + sy * nsub + sx;
+                        const double ks = sub_kappa[s];
+                        kxy += ks;
+                        kzz += s % 8 == 7 ? ks * a.aniso : ks;  // one subdomain in eight is anisotropic
+                        cnt++;
+                    }
+            if (cnt == 1) {
+                const int s0 = (dim == 3 ? std::min(iz / sd, nsub - 1) * nsub * nsub : 0) +
+                               std::min(iy / sd, nsub - 1) * nsub + std::min(ix / sd, nsub - 1);
+                kap = sub_kappa[s0];
+                kz = s0 % 8 == 7 ? a.aniso : 1.0;
+            } else {
+                kap = kxy / cnt;
+                kz = kzz / cnt / kap;
+            }
         } else if (a.kappa != "one")
             throw Error(-2, "unknown kappa field '" + a.kappa + "'");
         K[(size_t)c * 9 + 0] = kap;

@@ -137,3 +137,16 @@ def test_heterogeneous_subdomain_field():
     assert len(np.unique(kap)) == 8  # one value per 4^3-cell subdomain
     fc = P.face_cells()
     assert fc.shape == (P.nb, 2) and (fc >= 0).all()
+
+
+def test_coarse_cells_average_subdomain_kappa():
+    """On a coarse level whose cells cover several kappa subdomains, each cell
+    carries the mean of the subdomain tensors it covers (the finer level with
+    one subdomain per cell gives them)."""
+    kw = dict(dim=3, mesh="cart", k=0, kappa="subdomain", sigma=3.0, subdomain=8, aniso=1e-2, ref_n=64, seed=7)
+    fine = G.HhoProblem(n=8, **kw).cell_kappa().reshape(8, 8, 8)      # one subdomain per cell
+    coarse = G.HhoProblem(n=4, **kw).cell_kappa().reshape(4, 4, 4)    # 2x2x2 subdomains per cell
+    want = fine.reshape(4, 2, 4, 2, 4, 2).mean(axis=(1, 3, 5))
+    np.testing.assert_allclose(coarse, want, rtol=1e-14)
+    with pytest.raises(G.FhError):
+        G.HhoProblem(n=5, **dict(kw, ref_n=15))
