@@ -222,6 +222,73 @@ def hetero_bench(args, G, np, torch):
     print(json.dumps(out), flush=True)
 
 
+def learned2d_bench(args, G, np, torch):
+    """Learned post-smoother on 2-D perturbed triangles (HHO k = 2, per-cell
+    lognormal kappa, sigma = 2): trained on the --train-n2d mesh with random
+    sinusoid right-hand sides (pairs from plain FCG solves), evaluated on the
+    --n mesh with unseen kappa and right-hand-side seeds.  FCG with the
+    aggregation AMG (symmetric multicolour block Gauss-Seidel) without a
+    post-step, with the block-Jacobi post-step (the untrained network) and
+    with the trained network; energy-norm gain on held-out pairs of the
+    evaluation mesh.  Perturbed quadrilaterals are not affine, so the
+    affine triangles of the same vertex perturbation stand in for them."""
+    from fhhos4_amd import gmg
+    from fhhos4_amd import learned as Lr
+    ctx = G.Context(0)
+    if args.prec2d == "aggregamg":
+        # K-cycle, as the reference's aggregation AMG defaults to (main.cpp:1874-1884)
+        solver = dict(prec="aggregamg", cycle="K", nu=(1, 1), smoothers=("sbgs", "sbgs"), fcg_truncation=8)
+        label = "FCG(8) + aggregation AMG K(1,1) sbgs"
+    else:
+        solver = dict(prec="mg", cycle="V", nu=(1, 1), smoothers=("cheb2", "cheb2"), galerkin=True,
+                      fcg_truncation=8)
+        label = "FCG(8) + geometric MG V(1,1) cheb2, Galerkin coarse operators"
+
+    def system(problem, tol=1e-12):
+        hier = gmg.build_hierarchy(ctx, problem) if args.prec2d == "gmg" else None
+        return Lr.System(ctx, problem, solver, tol=tol, max_iter=args.max_iter, hierarchy=hier)
+
+    base = dict(dim=2, mesh="stri", k=2, perturb=0.2, kappa="lognormal", sigma=2.0)
+    t0 = time.perf_counter()
+    train_sys = system(G.HhoProblem(n=args.train_n2d, seed=11, source="pisine", **base))
+    rhs = [G.HhoProblem(n=args.train_n2d, seed=11, source="modes", source_seed=1000 + j, **base).rhs()
+           for j in range(args.train_rhs)]
+    pairs = Lr.make_pairs(ctx, train_sys, rhs, iters=(1, 3, 6))
+    t_pairs = time.perf_counter() - t0
+    nf = train_sys.problem.nf
+    nn = G.Mpnn(ctx, nf, layers=3, seed=1)
+    before = Lr.mean_loss(nn, pairs)
+    t1 = time.perf_counter()
+    hist = Lr.train(nn, pairs, epochs=args.train_epochs, lr=2e-3, log=log)
+    t_train = time.perf_counter() - t1
+    log("trained on %d^2 x 2 triangles: %d pairs (%.1fs), %.1fs, energy error %.3f -> %.3f"
+        % (args.train_n2d, len(pairs), t_pairs, t_train, before, hist[-1]))
+    P = G.HhoProblem(n=args.n, seed=12, source="modes", source_seed=5000, **base)
+    ev = system(P)
+    held = Lr.make_pairs(ctx, ev, [G.HhoProblem(n=args.n, seed=12, source="modes", source_seed=6000 + j,
+                                                **base).rhs() for j in range(2)], iters=(3,))
+    untrained = G.Mpnn(ctx, nf, layers=3, seed=1)
+    gain = {"untrained": Lr.mean_loss(untrained, held), "trained": Lr.mean_loss(nn, held), "pairs": len(held)}
+    b = P.rhs()
+    out = {"workload": "2-D perturbed triangles, %d^2 x 2 cells, HHO k=2 (%d unknowns), lognormal kappa sigma 2; "
+                       "trained on %d^2 x 2 (%d pairs, %d epochs)" % (args.n, P.n_unknowns, args.train_n2d,
+                                                                      len(pairs), args.train_epochs),
+           "solver": label,
+           "train": {"pairs_s": t_pairs, "train_s": t_train, "energy_error_before": before, "after": hist[-1]},
+           "held_out_energy_error": gain}
+    xs = {}
+    for key, net, unit in (("plain", None, False), ("block_jacobi_post", untrained, False),
+                           ("learned", nn, False), ("learned_unit_step", nn, True)):
+        st, x, sec = Lr.solve_with(ctx, ev, b, net, solver_kw=solver, max_iter=args.max_iter, unit_step=unit)
+        xs[key] = x.cpu().numpy()
+        out[key] = {"s": sec, "iterations": st["iterations"], "converged": st["converged"],
+                    "accepted": st["nn_used"], "fallback": st["nn_fallback"]}
+        log("%s: %d iterations, %.3f s" % (key, st["iterations"], sec))
+    for key in xs:
+        out[key]["rel_diff_vs_plain"] = float(np.linalg.norm(xs[key] - xs["plain"]) / np.linalg.norm(xs["plain"]))
+    print(json.dumps(out), flush=True)
+
+
 def learned_leg(args, G, ctx, hier, A, P, b, x, x64, t64, v):
     """FCG with the FP32 multigrid and the learned post-smoother.  The
     network is trained on the GPU (library kernels) on pairs from plain
@@ -289,8 +356,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=64)
     ap.add_argument("--sweep-mixed", action="store_true", help="with --sweep: FCG + FP32 / BF16-coarse multigrid")
-    ap.add_argument("--config", default="", choices=["", "hetero"],
-                    help="hetero: the heterogeneous hexahedral case (64^3 hexahedra, k=2, subdomain lognormal kappa)")
+    ap.add_argument("--config", default="", choices=["", "hetero", "learned2d"],
+                    help="hetero: the heterogeneous hexahedral case (64^3 hexahedra, k=2, subdomain lognormal kappa); "
+                         "learned2d: the learned post-smoother on 2-D triangles (use --n 512)")
+    ap.add_argument("--train-n2d", type=int, default=128)
+    ap.add_argument("--prec2d", default="gmg", choices=["gmg", "aggregamg"])
     ap.add_argument("--k", type=int, default=2)
     ap.add_argument("--perturb", type=float, default=0.2)
     ap.add_argument("--seed", type=int, default=2026)
@@ -328,6 +398,9 @@ def main():
     assert torch.cuda.is_available(), "no ROCm device"
     if args.config == "hetero":
         hetero_bench(args, G, np, torch)
+        return
+    if args.config == "learned2d":
+        learned2d_bench(args, G, np, torch)
         return
     if args.kernel == "spmv":
         return kernel_bench(args, G, np, torch)

@@ -538,9 +538,27 @@ void HhoProblem::Impl::build_mesh() {
         sub_kappa.resize((size_t)ns * ns * (dim == 3 ? ns : 1));
         for (auto& s : sub_kappa) s = std::exp(a.sigma * N01(krng));
     }
+    // Per-cell lognormal field.  The draws belong to the cells of the finest
+    // mesh (ref_n): a coarse level of a hierarchy gives each cell the mean
+    // coefficient of the fine cells in its background box.
+    std::vector<double> box_kappa;
+    const int cpb = kind == 0 ? (dim == 3 ? 6 : 2) : 1;  // cells per background box
+    if (a.kappa == "lognormal" && ratio > 1) {
+        const size_t nbox = (size_t)n * n * (dim == 3 ? n : 1);
+        box_kappa.assign(nbox, 0.0);
+        const size_t nfine = (size_t)rn * rn * (dim == 3 ? rn : 1) * cpb;
+        for (size_t f = 0; f < nfine; f++) {
+            const double kf = std::exp(a.sigma * N01(krng));
+            const size_t fb = f / cpb;
+            const size_t fx = fb % rn, fy = (fb / rn) % rn, fz = dim == 3 ? fb / ((size_t)rn * rn) : 0;
+            box_kappa[((fz / ratio) * n + fy / ratio) * n + fx / ratio] += kf;
+        }
+        const double per_box = (double)nfine / (double)nbox;
+        for (auto& v : box_kappa) v /= per_box;
+    }
     for (int c = 0; c < ncell; c++) {
         double kap = 1.0, kz = 1.0;
-        if (a.kappa == "lognormal") kap = std::exp(a.sigma * N01(krng));
+        if (a.kappa == "lognormal") kap = box_kappa.empty() ? std::exp(a.sigma * N01(krng)) : box_kappa[c / cpb];
         else if (a.kappa == "subdomain") {
             // background cell of the cell (tets: 6 per cube)
             int bc = kind == 0 ? c / (dim == 3 ? 6 : 2) : c;

@@ -4,11 +4,11 @@
 //
 // Layout: node features are row-major n x 32 FP32 arrays (one 128-byte row
 // per face), so the 32 lanes of a node read or write one row with one
-// coalesced access.  A workgroup of 256 threads = 8 nodes x 32 channels; the
-// layer weights are staged in LDS once per workgroup, the node's own row and
-// its neighbour aggregate are staged per node, and every lane computes one
-// output channel (a 64-long dot product from LDS).  Weights are FP32 for
-// training and BF16 (FP32 accumulate) for inference.
+// coalesced access.  A workgroup of 256 threads = 8 nodes x 32 channels; every
+// lane computes one output channel.  The message-passing layer keeps its
+// lane's weight rows in registers and strides over node batches staged in
+// LDS; the smaller kernels read the weights from the cache.  Weights are FP32
+// for training and BF16 (FP32 accumulate) for inference.
 //
 // Weight gradients are sums over nodes of outer products; each workgroup of
 // a fixed-size grid accumulates its own share in FP64 and a second kernel
@@ -60,65 +60,330 @@ __global__ void k_nn_input(int n, int nf, const float* __restrict__ v, const flo
     x0[t] = val;
 }
 
-// Encoder: h = relu(We x + be), x of width DIN (<= 12).
+// Encoder: h = relu(We x + be), x of width din <= kMaxDin.  Lane c keeps row c
+// of We in registers; the grid strides over nodes.
+constexpr int kMaxDin = 12;
+constexpr int kEncodeMaxWGs = 8192;
+
 template <class TW>
 __global__ __launch_bounds__(256) void k_nn_encode(int n, int din, const float* __restrict__ x,
                                                     const TW* __restrict__ We, const TW* __restrict__ be,
                                                     float* __restrict__ h) {
+    __shared__ float sWe[H * kMaxDin + H];
+    for (int e = threadIdx.x; e < H * din; e += blockDim.x) sWe[e] = wload(We + e);
+    if (threadIdx.x < H) sWe[H * kMaxDin + threadIdx.x] = wload(be + threadIdx.x);
+    __syncthreads();
     const int c = threadIdx.x & (H - 1);
-    const int64_t i = (int64_t)blockIdx.x * kNodesPerWG + (threadIdx.x >> 5);
-    if (i >= n) return;
-    float acc = wload(be + c);
-    const float* xi = x + i * din;
-    for (int k = 0; k < din; k++) acc += wload(We + c * din + k) * xi[k];
-    h[i * H + c] = acc > 0.f ? acc : 0.f;
+    float wr[kMaxDin];
+#pragma unroll
+    for (int k = 0; k < kMaxDin; k++) wr[k] = k < din ? sWe[c * din + k] : 0.f;
+    const float bias = sWe[H * kMaxDin + c];
+    for (int64_t i = (int64_t)blockIdx.x * kNodesPerWG + (threadIdx.x >> 5); i < n;
+         i += (int64_t)gridDim.x * kNodesPerWG) {
+        float acc = bias;
+        const float* xi = x + i * din;
+#pragma unroll
+        for (int k = 0; k < kMaxDin; k++)
+            if (k < din) acc += wr[k] * xi[k];
+        h[i * H + c] = acc > 0.f ? acc : 0.f;
+    }
 }
 
 // Message-passing layer: m_i = sum_j w_ij h_j ; h'_i = relu(S h_i + N m_i + c).
 // m is stored for the backward pass when m_out != nullptr.
+//
+// Lane c of a 32-lane half-wave owns output channel c and keeps row c of S
+// and of N in registers for the whole launch (staged once per workgroup, not
+// once per node).  The grid strides over batches of kLayerBatch nodes: every
+// thread gathers kLayerBatch / 8 nodes' (h_i, m_i) rows into LDS, then
+// computes their channel c from LDS broadcasts (16-byte reads; the row pitch
+// of 2H + 4 floats puts the two half-waves' rows in different banks).  The
+// batch loop bound depends on blockIdx only, so both barriers are uniform.
+constexpr int kLayerBatch = 32;            // nodes per workgroup iteration
+constexpr int kLayerPer = kLayerBatch / 8; // nodes per thread per iteration
+constexpr int kLayerPitch = 2 * H + 4;
+constexpr int kLayerMaxWGs = 768;  // 3 workgroups of 4 waves per CU (164 VGPRs: 3 waves per SIMD)
+
 template <class TW>
 __global__ __launch_bounds__(256) void k_nn_layer(int n, const int* __restrict__ rp, const int* __restrict__ ci,
                                                    const float* __restrict__ w, const float* __restrict__ hin,
                                                    const TW* __restrict__ S, const TW* __restrict__ N,
                                                    const TW* __restrict__ cb, float* __restrict__ hout,
                                                    float* __restrict__ m_out) {
-    __shared__ float sS[H * H], sN[H * H];
-    __shared__ float sx[kNodesPerWG][2 * H];
+    __shared__ __align__(16) float sx[kLayerBatch][kLayerPitch];
+    __shared__ float sW[2 * H * H];  // S^T then N^T: sW[k H + c] = S[c][k]
+    // Weights reach the registers through LDS: one coalesced read per
+    // workgroup (every lane of the grid reading the same few lines from L2
+    // would serialise on their channels).
     for (int e = threadIdx.x; e < H * H; e += blockDim.x) {
-        sS[e] = wload(S + e);
-        sN[e] = wload(N + e);
+        const int row = e / H, k = e - row * H;
+        sW[k * H + row] = wload(S + e);
+        sW[H * H + k * H + row] = wload(N + e);
     }
-    const int c = threadIdx.x & (H - 1), q = threadIdx.x >> 5;
-    const int64_t i = (int64_t)blockIdx.x * kNodesPerWG + q;
-    const bool ok = i < n;
-    float m = 0.f, hs = 0.f;
-    if (ok) {
-        hs = hin[i * H + c];
-        for (int p = rp[i]; p < rp[i + 1]; p++) m += w[p] * hin[(int64_t)ci[p] * H + c];
-        if (m_out) m_out[i * H + c] = m;
-    }
-    sx[q][c] = hs;
-    sx[q][H + c] = m;
     __syncthreads();
-    if (!ok) return;
-    float acc = wload(cb + c);
-#pragma unroll 8
-    for (int k = 0; k < H; k++) acc += sS[c * H + k] * sx[q][k] + sN[c * H + k] * sx[q][H + k];
-    hout[i * H + c] = acc > 0.f ? acc : 0.f;
+    const int c = threadIdx.x & (H - 1), q = threadIdx.x >> 5;
+    float ws[H], wn[H];
+#pragma unroll
+    for (int k = 0; k < H; k++) {
+        ws[k] = sW[k * H + c];
+        wn[k] = sW[H * H + k * H + c];
+    }
+    const float bias = wload(cb + c);
+    for (int64_t base = (int64_t)blockIdx.x * kLayerBatch; base < n; base += (int64_t)gridDim.x * kLayerBatch) {
+        // gather.  Lane c of the half-wave fetches edge c of each of its
+        // kLayerPer rows (one coalesced load for the row's columns and
+        // weights); edge e is then broadcast from lane e, and the neighbour
+        // rows of four edge positions x kLayerPer nodes are loaded before
+        // they are summed (edge order kept per node).  Rows longer than 32
+        // edges read the rest directly.
+        int beg[kLayerPer], deg[kLayerPer], dmax = 0;
+        float m[kLayerPer], hs[kLayerPer];
+#pragma unroll
+        for (int r = 0; r < kLayerPer; r++) {
+            const int64_t i = base + r * 8 + q;
+            beg[r] = 0;
+            deg[r] = 0;
+            hs[r] = 0.f;
+            m[r] = 0.f;
+            if (i < n) {
+                beg[r] = rp[i];
+                deg[r] = rp[i + 1] - beg[r];
+                hs[r] = hin[i * H + c];
+            }
+            dmax = deg[r] > dmax ? deg[r] : dmax;
+        }
+        dmax = max(dmax, __shfl_xor(dmax, 32));  // wave-uniform trip count
+        int colv[kLayerPer];
+        float wv[kLayerPer];
+#pragma unroll
+        for (int r = 0; r < kLayerPer; r++) {
+            colv[r] = c < deg[r] ? ci[beg[r] + c] : 0;
+            wv[r] = c < deg[r] ? w[beg[r] + c] : 0.f;
+        }
+        for (int e0 = 0; e0 < dmax; e0 += 4) {
+            float v[4][kLayerPer], wt[4][kLayerPer];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int e = e0 + u;
+#pragma unroll
+                for (int r = 0; r < kLayerPer; r++) {
+                    int col;
+                    float we;
+                    if (e < 32) {
+                        col = __shfl(colv[r], e, 32);
+                        we = __shfl(wv[r], e, 32);
+                    } else {
+                        col = e < deg[r] ? ci[beg[r] + e] : 0;
+                        we = e < deg[r] ? w[beg[r] + e] : 0.f;
+                    }
+                    wt[u][r] = we;
+                    v[u][r] = e < deg[r] ? hin[(int64_t)col * H + c] : 0.f;
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+#pragma unroll
+                for (int r = 0; r < kLayerPer; r++)
+                    if (e0 + u < deg[r]) m[r] += wt[u][r] * v[u][r];
+        }
+#pragma unroll
+        for (int r = 0; r < kLayerPer; r++) {
+            const int slot = r * 8 + q;
+            const int64_t i = base + slot;
+            if (m_out && i < n) m_out[i * H + c] = m[r];
+            sx[slot][c] = hs[r];
+            sx[slot][H + c] = m[r];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < kLayerPer; r++) {
+            const int slot = r * 8 + q;
+            const int64_t i = base + slot;
+            const float4* x4 = reinterpret_cast<const float4*>(&sx[slot][0]);
+            float acc = bias;
+#pragma unroll
+            for (int k4 = 0; k4 < H / 4; k4++) {
+                const float4 hv = x4[k4], mv = x4[H / 4 + k4];
+                acc += ws[4 * k4 + 0] * hv.x + wn[4 * k4 + 0] * mv.x;
+                acc += ws[4 * k4 + 1] * hv.y + wn[4 * k4 + 1] * mv.y;
+                acc += ws[4 * k4 + 2] * hv.z + wn[4 * k4 + 2] * mv.z;
+                acc += ws[4 * k4 + 3] * hv.w + wn[4 * k4 + 3] * mv.w;
+            }
+            if (i < n) hout[i * H + c] = acc > 0.f ? acc : 0.f;
+        }
+        __syncthreads();
+    }
 }
 
-// Output: delta_i = sigma (u_i + Wo h_i + bo), thread per unknown.
+// Inference layer (BF16 weights): the same layer with the dense part on the
+// matrix cores.  A wave owns tiles of 32 consecutive nodes.
+//  1. The tile's row pointers come in one load, its edge list (contiguous in
+//     the CSR arrays) in a few coalesced loads, into the wave's LDS.
+//  2. Each half-wave walks the edges of its 16 nodes in order, kMChunk rows
+//     in flight (lane c = channel c), keeps a running sum per node and writes
+//     (h_i, m_i) to the tile in LDS as BF16.  Tiles with more than kMEdges
+//     edges walk their nodes one by one instead.
+//  3. Four v_mfma_f32_32x32x16_bf16 multiply the 32 x 64 tile by the 64 x 32
+//     matrix [S N]^T (FP32 accumulate).  A lane supplies row lane%32 of the
+//     tile and column lane%32 of the weights with the same k elements, and
+//     receives column lane%32 of the product: rows 8 (r / 4) + 4 (lane / 32)
+//     + r % 4 of its 16 accumulators.
+// Activations are rounded to BF16 (round to nearest even) for the product;
+// the neighbour sum m_i is accumulated in FP32 in edge order.
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+constexpr int kMTile = 32;
+constexpr int kMWaves = 4;
+constexpr int kMBatch = kMTile * kMWaves;  // nodes per workgroup iteration
+constexpr int kMPitch = 2 * H + 8;         // BF16 elements per tile row (144 bytes)
+constexpr int kMEdges = 384;               // staged edges per tile
+constexpr int kMChunk = 16;                // neighbour rows in flight per half-wave
+constexpr int kMMaxWGs = 1024;             // 4 workgroups per CU
+
+__device__ __forceinline__ __bf16 to_bf16(float f) {
+    uint32_t u = __float_as_uint(f);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return __builtin_bit_cast(__bf16, (uint16_t)(u >> 16));
+}
+
+__global__ __launch_bounds__(256) void k_nn_layer_mfma(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                                                        const float* __restrict__ w, const float* __restrict__ hin,
+                                                        const bf16w* __restrict__ S, const bf16w* __restrict__ N,
+                                                        const bf16w* __restrict__ cb, float* __restrict__ hout) {
+    __shared__ __align__(16) __bf16 tile[kMWaves][kMTile][kMPitch];
+    __shared__ uint16_t sB[2 * H * H];  // sB[k H + j] = [S N]^T[k][j]
+    __shared__ int s_col[kMWaves][kMEdges];
+    __shared__ float s_w[kMWaves][kMEdges];
+    __shared__ uint8_t s_node[kMWaves][kMEdges];
+    __shared__ int s_row[kMWaves][kMTile + 1];
+    for (int e = threadIdx.x; e < H * H; e += blockDim.x) {
+        const int row = e / H, k = e - row * H;
+        sB[k * H + row] = S[e].u;
+        sB[(H + k) * H + row] = N[e].u;
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, half = lane >> 5, c = lane & 31;
+    bf16x8_t bop[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++)
+#pragma unroll
+        for (int u = 0; u < 8; u++) bop[t][u] = __builtin_bit_cast(__bf16, sB[(16 * t + 8 * half + u) * H + c]);
+    const float bias = wload(cb + c);
+    // XCD-aware batch order: workgroups are dealt to the 8 XCDs round robin,
+    // so XCD x (blockIdx % 8) walks the x-th contiguous slab of batches and
+    // the neighbour rows a tile gathers were mostly fetched into the same
+    // XCD's L2 by the tiles next to it.
+    const int64_t nbat = ((int64_t)n + kMBatch - 1) / kMBatch;
+    int64_t first = blockIdx.x, end = nbat, step = gridDim.x;
+    if (gridDim.x % 8 == 0) {
+        const int64_t slab = (nbat + 7) / 8, xcd = blockIdx.x % 8;
+        first = xcd * slab + blockIdx.x / 8;
+        end = min(nbat, (xcd + 1) * slab);
+        step = gridDim.x / 8;
+    }
+    for (int64_t b = first; b < end; b += step) {
+        const int64_t tb = b * kMBatch + (int64_t)wv * kMTile;
+        const int nt = (int)max<int64_t>(0, min<int64_t>(kMTile, (int64_t)n - tb));  // nodes of this tile
+        // 1. row pointers and edge list
+        const int rpv = lane <= nt && nt > 0 ? rp[tb + lane] : 0;
+        const int e_base = __shfl(rpv, 0);
+        const int ne = __shfl(rpv, nt) - e_base;
+        const bool staged = ne <= kMEdges;
+        if (lane <= nt) s_row[wv][lane] = rpv - e_base;
+        if (staged) {
+            for (int e = lane; e < ne; e += 64) {
+                s_col[wv][e] = ci[e_base + e];
+                s_w[wv][e] = w[e_base + e];
+            }
+            const int rnext = __shfl(rpv, lane + 1 < 64 ? lane + 1 : 63);
+            if (lane < nt)
+                for (int e = rpv - e_base; e < rnext - e_base; e++) s_node[wv][e] = (uint8_t)lane;
+        }
+        __syncthreads();
+        // 2. (h_i, m_i) rows of the tile
+        const int r0 = min(16 * half, nt), r1 = min(16 * half + 16, nt);
+        {
+            float hs[16];
+#pragma unroll
+            for (int r = 0; r < 16; r++) hs[r] = r0 + r < r1 ? hin[(tb + r0 + r) * H + c] : 0.f;
+#pragma unroll
+            for (int r = 0; r < 16; r++)
+                if (r0 + r < r1) {
+                    tile[wv][r0 + r][c] = to_bf16(hs[r]);
+                    tile[wv][r0 + r][H + c] = to_bf16(0.f);
+                }
+        }
+        if (staged) {
+            const int es = r0 < r1 ? s_row[wv][r0] : 0, ee = r0 < r1 ? s_row[wv][r1] : 0;
+            int cur = r0;
+            float run = 0.f;
+            for (int e0 = es; e0 < ee; e0 += kMChunk) {
+                float v[kMChunk];
+#pragma unroll
+                for (int u = 0; u < kMChunk; u++) {
+                    const int e = e0 + u;
+                    v[u] = e < ee ? hin[(int64_t)s_col[wv][e] * H + c] : 0.f;
+                }
+#pragma unroll
+                for (int u = 0; u < kMChunk; u++) {
+                    const int e = e0 + u;
+                    if (e < ee) {
+                        const int nd = s_node[wv][e];
+                        if (nd != cur) {
+                            tile[wv][cur][H + c] = to_bf16(run);
+                            run = 0.f;
+                            cur = nd;
+                        }
+                        run += s_w[wv][e] * v[u];
+                    }
+                }
+            }
+            if (ee > es) tile[wv][cur][H + c] = to_bf16(run);
+        } else {
+            for (int r = r0; r < r1; r++) {
+                const int64_t i = tb + r;
+                float m = 0.f;
+                for (int p = rp[i]; p < rp[i + 1]; p++) m += w[p] * hin[(int64_t)ci[p] * H + c];
+                tile[wv][r][H + c] = to_bf16(m);
+            }
+        }
+        __syncthreads();
+        // 3. dense part on the matrix cores (rows >= nt hold stale values and are not stored)
+        floatx16 acc;
+#pragma unroll
+        for (int r = 0; r < 16; r++) acc[r] = 0.f;
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            const bf16x8_t a = *reinterpret_cast<const bf16x8_t*>(&tile[wv][c][16 * t + 8 * half]);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bop[t], acc, 0, 0, 0);
+        }
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            const int row = 8 * (r / 4) + 4 * half + (r % 4);
+            const float y = acc[r] + bias;
+            if (row < nt) hout[(tb + row) * H + c] = y > 0.f ? y : 0.f;
+        }
+        __syncthreads();
+    }
+}
+
+// Output: delta_i = sigma (u_i + Wo h_i + bo), thread per unknown (nf <= kMaxNf).
+constexpr int kMaxNf = 10;
 template <class TW>
 __global__ void k_nn_out(int n, int nf, const float* __restrict__ h, const float* __restrict__ u,
                          const TW* __restrict__ Wo, const TW* __restrict__ bo, const double* s, int slot_sigma,
                          double* __restrict__ delta) {
+    __shared__ float sWo[kMaxNf * H + kMaxNf];
+    for (int e = threadIdx.x; e < nf * H; e += blockDim.x) sWo[e] = wload(Wo + e);
+    if (threadIdx.x < nf) sWo[kMaxNf * H + threadIdx.x] = wload(bo + threadIdx.x);
+    __syncthreads();
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (int64_t)n * nf) return;
     const int i = (int)(t / nf), a = (int)(t - (int64_t)i * nf);
     const float* hi = h + (int64_t)i * H;
-    float y = wload(bo + a);
+    float y = sWo[kMaxNf * H + a];
 #pragma unroll 8
-    for (int k = 0; k < H; k++) y += wload(Wo + a * H + k) * hi[k];
+    for (int k = 0; k < H; k++) y += sWo[a * H + k] * hi[k];
     delta[t] = s[slot_sigma] * (double)(u[t] + y);
 }
 
@@ -317,7 +582,9 @@ void nn_input(Ctx& c, int n, int nf, const float* v, const float* feat, int slot
     FH_HIP(hipGetLastError());
 }
 void nn_encode(Ctx& c, int n, int din, const float* x, const void* We, const void* be, bool bf16, float* h) {
-    const int g = (n + kNodesPerWG - 1) / kNodesPerWG;
+    FH_CHECK(din >= 1 && din <= kMaxDin, -3, "encoder input width out of range");
+    if (n <= 0) return;
+    const int g = (int)std::min<int64_t>(((int64_t)n + kNodesPerWG - 1) / kNodesPerWG, kEncodeMaxWGs);
     if (bf16)
         hipLaunchKernelGGL(k_nn_encode<bf16w>, dim3(g), dim3(256), 0, c.stream, n, din, x, (const bf16w*)We,
                            (const bf16w*)be, h);
@@ -328,17 +595,22 @@ void nn_encode(Ctx& c, int n, int din, const float* x, const void* We, const voi
 }
 void nn_layer(Ctx& c, int n, const int* rp, const int* ci, const float* w, const float* hin, const void* S,
               const void* N, const void* cb, bool bf16, float* hout, float* m_out) {
-    const int g = (n + kNodesPerWG - 1) / kNodesPerWG;
-    if (bf16)
-        hipLaunchKernelGGL(k_nn_layer<bf16w>, dim3(g), dim3(256), 0, c.stream, n, rp, ci, w, hin, (const bf16w*)S,
-                           (const bf16w*)N, (const bf16w*)cb, hout, m_out);
-    else
+    const int g = (int)std::min<int64_t>(((int64_t)n + kLayerBatch - 1) / kLayerBatch, kLayerMaxWGs);
+    if (n <= 0) return;
+    if (bf16) {
+        FH_CHECK(m_out == nullptr, -3, "the BF16 layer is inference only (no stored aggregate)");
+        int gm = (int)std::min<int64_t>(((int64_t)n + kMBatch - 1) / kMBatch, kMMaxWGs);
+        if (gm >= 64) gm -= gm % 8;  // a multiple of the XCD count selects the slab order
+        hipLaunchKernelGGL(k_nn_layer_mfma, dim3(gm), dim3(256), 0, c.stream, n, rp, ci, w, hin, (const bf16w*)S,
+                           (const bf16w*)N, (const bf16w*)cb, hout);
+    } else
         hipLaunchKernelGGL(k_nn_layer<float>, dim3(g), dim3(256), 0, c.stream, n, rp, ci, w, hin, (const float*)S,
                            (const float*)N, (const float*)cb, hout, m_out);
     FH_HIP(hipGetLastError());
 }
 void nn_out(Ctx& c, int n, int nf, const float* h, const float* u, const void* Wo, const void* bo, bool bf16,
             int slot_sigma, double* delta) {
+    FH_CHECK(nf >= 1 && nf <= kMaxNf, -3, "face block size out of range for the output layer");
     const int g = g1((int64_t)n * nf);
     if (bf16)
         hipLaunchKernelGGL(k_nn_out<bf16w>, dim3(g), dim3(256), 0, c.stream, n, nf, h, u, (const bf16w*)Wo,

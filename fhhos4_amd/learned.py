@@ -14,18 +14,23 @@ import numpy as np
 import torch
 
 from . import fhgpu as G
+from . import gmg
 
 
 class System:
     """One operator on the device with its solver, network graph and
-    features."""
+    features.  With `hierarchy` (gmg.build_hierarchy of the problem) the
+    solver is a geometric multigrid one (solver_kw with prec="mg")."""
 
-    def __init__(self, ctx, problem, solver_kw, tol=1e-12, max_iter=400):
+    def __init__(self, ctx, problem, solver_kw, tol=1e-12, max_iter=400, hierarchy=None):
         self.problem = problem
-        self.A = problem.matrix(ctx, keep_host=True)
+        self.hierarchy = hierarchy
+        self.A = hierarchy[0][1] if hierarchy else problem.matrix(ctx, keep_host=True)
         self.features = G.face_features(problem)
         self.graph = G.NnGraph(ctx, self.A, self.features)
         self.solver = G.Solver(ctx, solver="fcg", tol=tol, max_iter=max_iter, **solver_kw)
+        if hierarchy:
+            gmg.attach(self.solver, hierarchy)
         self.solver.setup(self.A)
         self.n = problem.n_unknowns
         self.tol, self.max_iter = tol, max_iter
@@ -89,6 +94,8 @@ def solve_with(ctx, system, b_host, nn=None, omega=1.0, solver_kw=None, tol=1e-8
     s = G.Solver(ctx, solver="fcg", tol=tol, max_iter=max_iter, **(solver_kw or {}))
     if nn is not None:
         s.set_learned_smoother(nn, system.graph, omega, unit_step)
+    if getattr(system, "hierarchy", None):
+        gmg.attach(s, system.hierarchy)
     s.setup(system.A)
     b = torch.from_numpy(np.ascontiguousarray(b_host)).cuda()
     x = torch.zeros_like(b)

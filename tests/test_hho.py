@@ -150,3 +150,25 @@ def test_coarse_cells_average_subdomain_kappa():
     np.testing.assert_allclose(coarse, want, rtol=1e-14)
     with pytest.raises(G.FhError):
         G.HhoProblem(n=5, **dict(kw, ref_n=15))
+
+
+@pytest.mark.parametrize("dim,mesh,cpb", [(2, "stri", 2), (3, "stetra", 6), (2, "cart", 1)])
+def test_coarse_cells_average_lognormal_kappa(dim, mesh, cpb):
+    """Per-cell lognormal kappa on a coarse level (the problem's coarsened())
+    is the mean of the fine cells' coefficients in each background box,
+    equal for the simplices sharing a box; the fine level is unchanged."""
+    n = 8
+    P = G.HhoProblem(dim=dim, mesh=mesh, n=n, k=0, kappa="lognormal", sigma=2.0, seed=5)
+    fine = P.cell_kappa()
+    assert np.array_equal(fine, G.HhoProblem(**dict(P.args, ref_n=n)).cell_kappa())
+    C = P.coarsened()
+    coarse = C.cell_kappa()
+    box = fine.reshape((n,) * dim + (cpb,)).mean(axis=-1)  # boxes are numbered x fastest
+
+    def coarse_mean(r):
+        m = n // r
+        return box.reshape(*[x for _ in range(dim) for x in (m, r)]).mean(axis=tuple(range(1, 2 * dim, 2)))
+
+    for level, r in ((C, 2), (C.coarsened(), 4)):
+        got = level.cell_kappa().reshape(-1, cpb)
+        np.testing.assert_allclose(got, np.repeat(coarse_mean(r).reshape(-1, 1), cpb, axis=1), rtol=1e-13)
