@@ -329,7 +329,11 @@ def learned_leg(args, G, ctx, hier, A, P, b, x, x64, t64, v):
         % (len(pairs), t_pairs, t_train, before, hist[-1]))
     feats = G.face_features(P)
     graph = G.NnGraph(ctx, A, feats)
-    s = G.Solver(ctx, solver="fcg", tol=1e-8, max_iter=args.max_iter, mg_prec="f32", mg_store="f32", **kw)
+    # FCG(8): the network makes the preconditioner vary from one application
+    # to the next, and eight kept directions hold the A-orthogonality that
+    # FCG(1) loses under it
+    s = G.Solver(ctx, solver="fcg", tol=1e-8, max_iter=args.max_iter, mg_prec="f32", mg_store="f32",
+                 fcg_truncation=args.learned_fcg_m, **kw)
     gmg.attach(s, hier)
     s.set_learned_smoother(nn, graph, 1.0)
     t2 = time.perf_counter()
@@ -360,6 +364,7 @@ def main():
                     help="hetero: the heterogeneous hexahedral case (64^3 hexahedra, k=2, subdomain lognormal kappa); "
                          "learned2d: the learned post-smoother on 2-D triangles (use --n 512)")
     ap.add_argument("--train-n2d", type=int, default=128)
+    ap.add_argument("--learned-fcg-m", type=int, default=8, help="FCG truncation of the learned leg")
     ap.add_argument("--prec2d", default="gmg", choices=["gmg", "aggregamg"])
     ap.add_argument("--k", type=int, default=2)
     ap.add_argument("--perturb", type=float, default=0.2)
@@ -378,8 +383,8 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--train-n", type=int, default=16, help="mesh of the learned smoother's training problems")
-    ap.add_argument("--train-rhs", type=int, default=8)
-    ap.add_argument("--train-epochs", type=int, default=6)
+    ap.add_argument("--train-rhs", type=int, default=100)
+    ap.add_argument("--train-epochs", type=int, default=8)
     ap.add_argument("--sweep", action="store_true", help="time a list of solver variants")
     ap.add_argument("--kernel", choices=["spmv"], help="time one kernel family alone")
     ap.add_argument("--reps", type=int, default=20)
