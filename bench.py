@@ -375,7 +375,7 @@ def main():
     ap.add_argument("--relax", type=float, default=1.0)
     ap.add_argument("--nu", type=int, default=1)
     ap.add_argument("--cycle", default="V")
-    ap.add_argument("--coarse-smoother", default=None, help="smoother of the levels below the finest")
+    ap.add_argument("--coarse-smoother", default="cheb1", help="smoother of the levels below the finest")
     ap.add_argument("--coarse-relax", type=float, default=0.0)
     ap.add_argument("--max-iter", type=int, default=1000)
     ap.add_argument("--cpu-n", type=int, default=16)

@@ -82,7 +82,7 @@ SmootherSpec SmootherSpec::parse(const std::string& code, int iterations, double
     else if (code == "bgs" || code == "gs" || code == "bsor" || code == "sor") { s.kind = SmootherKind::BlockGS; s.sweep = Sweep::Forward; }
     else if (code == "rbgs" || code == "rgs" || code == "rbsor" || code == "rsor") { s.kind = SmootherKind::BlockGS; s.sweep = Sweep::Backward; }
     else if (code == "sbgs" || code == "sgs") { s.kind = SmootherKind::BlockGS; s.sweep = Sweep::Symmetric; }
-    else if (code == "cheb2" || code == "cheb3" || code == "cheb4") {
+    else if (code == "cheb1" || code == "cheb2" || code == "cheb3" || code == "cheb4") {
         s.kind = SmootherKind::Chebyshev;
         s.degree = code[4] - '0';
         return s;

@@ -322,7 +322,7 @@ void galerkin(const Csr& A, int nf, const std::vector<int>& agg, int nagg, Csr& 
 
 // ----------------------------------------------------------------- multigrid
 enum { SM_BJ = 1, SM_BGS_F = 2, SM_BGS_B = 3, SM_BGS_S = 4, SM_MC_F = 5, SM_MC_B = 6, SM_MC_S = 7,
-       SM_CHEB2 = 8, SM_CHEB3 = 9, SM_CHEB4 = 10 };
+       SM_CHEB2 = 8, SM_CHEB3 = 9, SM_CHEB4 = 10, SM_CHEB1 = 11 };
 
 // out = D^-1 (b - A x), block by block
 void dinv_resid(const Csr& A, int nf, const double* invD, const double* b, const double* x, double* out) {
@@ -388,7 +388,7 @@ struct MG {
             if (kind >= SM_CHEB2) {
                 // Chebyshev polynomial of D^-1 A on [0.25, 1.1] x lambda_max estimate
                 if (L.lmax == 0) L.lmax = cheb_lmax(L.A, nf, L.invD.data());
-                const int deg = 2 + (kind - SM_CHEB2);
+                const int deg = kind == SM_CHEB1 ? 1 : 2 + (kind - SM_CHEB2);
                 const double hi = 1.1 * L.lmax, lo = 0.25 * L.lmax;
                 const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sigma = theta / delta;
                 double rho = 1.0 / sigma;

@@ -75,8 +75,8 @@ typedef struct or_solver_cfg {
     double omega;      /* relaxation of block Jacobi */
     int pre_kind, post_kind;   /* 1 bj, 2 bgs forward, 3 bgs backward, 4 bgs symmetric,
                                   5/6/7 multicolour bgs forward/backward/symmetric,
-                                  8/9/10 Chebyshev of degree 2/3/4 of D^-1 A on
-                                  [0.25, 1.1] x its power-iteration lambda_max */
+                                  8/9/10/11 Chebyshev of degree 2/3/4/1 of D^-1 A
+                                  on [0.25, 1.1] x its power-iteration lambda_max */
     double pre_omega, post_omega;
     int nu1, nu2;
     char cycle;        /* 'V', 'W' or 'K' */

@@ -144,7 +144,7 @@ def amg_hierarchy(A, nf, theta=0.25, max_coarse=1000):
 
 SMOOTHER_KIND = {"bj": 1, "bj23": 1, "bgs": 2, "rbgs": 3, "sbgs": 4, "gs": 2, "rgs": 3, "sgs": 4,
                  # multicolour block Gauss-Seidel (greedy colouring), the ordering of the HIP path
-                 "mcbgs": 5, "mcrbgs": 6, "mcsbgs": 7, "cheb2": 8, "cheb3": 9, "cheb4": 10}
+                 "mcbgs": 5, "mcrbgs": 6, "mcsbgs": 7, "cheb2": 8, "cheb3": 9, "cheb4": 10, "cheb1": 11}
 PREC_KIND = {"": 0, "none": 0, "bj": 1, "bgs": 2, "mg": 3, "aggregamg": 4}
 
 

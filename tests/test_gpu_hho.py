@@ -39,6 +39,7 @@ CASES = [
     dict(galerkin=True, smoothers=("sbgs", "sbgs"), relax=1.0, coarse_smoother="bj", coarse_relax=0.5),
     dict(galerkin=True, smoothers=("sbgs", "sbgs"), relax=1.0, coarse_smoother="cheb2"),
     dict(galerkin=True, smoothers=("cheb3", "cheb3"), relax=1.0),
+    dict(galerkin=True, smoothers=("cheb2", "cheb2"), relax=1.0, coarse_smoother="cheb1"),
     dict(galerkin=False, smoothers=("bj", "bj"), relax=0.5),
     dict(galerkin=False, smoothers=("sbgs", "sbgs"), relax=1.0, cycle="W"),
     dict(galerkin=True, smoothers=("sbgs", "sbgs"), relax=1.0, solver="fcg", fcg_truncation=4),
@@ -46,7 +47,7 @@ CASES = [
 
 
 @pytest.mark.parametrize("case", CASES, ids=["galerkin-sbgs", "galerkin-sbgs-coarse-bj", "galerkin-sbgs-coarse-cheb2",
-                                             "galerkin-cheb3", "rediscr-bj", "rediscr-sbgs-W", "galerkin-sbgs-fcg4"])
+                                             "galerkin-cheb3", "galerkin-cheb2-coarse-cheb1", "rediscr-bj", "rediscr-sbgs-W", "galerkin-sbgs-fcg4"])
 def test_gmg_matches_oracle(ctx, system, case):
     P, H, Hh, gal = system
     cycle = case.get("cycle", "V")

@@ -260,3 +260,45 @@ def test_mpnn_oracle_structure():
     loss, _ = O.mpnn_loss_grad(A, nf, feat, params, r1, e1, 3, False)
     diff = e1 - d1
     assert abs(loss - diff @ (A @ diff) / (e1 @ r1)) <= 1e-12 * loss
+
+
+@pytest.mark.parametrize("smoother,deg", [("cheb1", 1), ("cheb2", 2), ("cheb3", 3), ("cheb4", 4)])
+def test_chebyshev_two_grid_matches_closed_form(smoother, deg):
+    """One two-grid cycle from x = 0 (V(1,1), Chebyshev pre- and
+    post-smoothing, exact Galerkin coarse solve) against its closed form
+    x = x* + E (I - P Ac^-1 P^T A) E (0 - x*), where the Chebyshev error
+    propagator is E = p(D^-1 A), p(t) = T_d((theta - t)/delta) / T_d(theta/delta)
+    on [0.25, 1.1] lambda_max, evaluated through the eigenvectors of
+    D^-1/2 A D^-1/2 and numpy's Chebyshev series (not a recurrence).
+    lambda_max(D^-1 A) is well separated, so the oracle's power-iteration
+    estimate equals it to rounding."""
+    import scipy.sparse as sp
+    from numpy.polynomial import chebyshev as C
+    n = 24
+    rng = np.random.default_rng(5)
+    L = sp.diags([-np.ones(n - 1), 2.5 * np.ones(n), -np.ones(n - 1)], [-1, 0, 1]).toarray()
+    v = 1.0 + rng.random(n)
+    A = L + 40.0 * np.outer(v, v) / n
+    P = np.zeros((n, n // 4))
+    for j in range(n // 4):
+        P[4 * j:4 * j + 4, j] = 1.0
+    Ac = P.T @ A @ P
+    b = rng.standard_normal(n)
+    levels = [(sp.csr_matrix(A), sp.csr_matrix(P)), (sp.csr_matrix(Ac), None)]
+    x, _, out = O.solve(sp.csr_matrix(A), b, "none", "mg", nf=1, smoothers=(smoother, smoother), nu=(1, 1),
+                        levels=levels, tol=1e-30, max_iter=1)
+    assert out.iterations == 1
+    d = np.diag(A)
+    lam, U = np.linalg.eigh(A / np.sqrt(np.outer(d, d)))
+    assert lam[-2] < 0.2 * lam[-1]
+    hi, lo = 1.1 * lam[-1], 0.25 * lam[-1]
+    theta, delta = 0.5 * (hi + lo), 0.5 * (hi - lo)
+    coef = np.zeros(deg + 1)
+    coef[deg] = 1.0
+    p = C.chebval((theta - lam) / delta, coef) / C.chebval(theta / delta, coef)
+    E = (U * p) @ U.T
+    E = E / np.sqrt(d)[:, None] * np.sqrt(d)[None, :]  # D^-1/2 (U p U^T) D^1/2
+    xs = np.linalg.solve(A, b)
+    K = np.eye(n) - P @ np.linalg.solve(Ac, P.T @ A)
+    want = xs + E @ K @ E @ (-xs)
+    np.testing.assert_allclose(x, want, rtol=1e-9, atol=1e-12 * np.abs(xs).max())
