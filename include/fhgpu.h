@@ -155,7 +155,9 @@ typedef struct fh_solver_args {
     char cycle;                /* 'V' | 'W' | 'K' (Multigrid.h:18-19) */
     int wloops;
     int nu1, nu2;              /* pre/post smoothing iterations */
-    const char* pre_smoother;  /* SmootherFactory codes: bj bj23 bgs rbgs sbgs gs rgs sgs j j23 */
+    const char* pre_smoother;  /* SmootherFactory codes: bj bj23 bgs rbgs sbgs gs rgs sgs j j23;
+                                  not in the reference: cheb1..cheb4, a Chebyshev polynomial
+                                  of degree 1-4 of D^-1 A on [0.25, 1.1] x lambda_max */
     const char* post_smoother;
     double relax;              /* RelaxationParameter */
     double tol;                /* relative residual ||r||/||b|| */
