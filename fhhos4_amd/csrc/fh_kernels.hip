@@ -251,28 +251,26 @@ __global__ __launch_bounds__(kWG) void k_sliced(int nbr, int nslices, int W, int
 #pragma unroll
     for (int i = 0; i < NF; i++) acc[i] = 0;
     if (s < nslices && lane < ACT) {
+        // The value loads do not wait for the column index: padding slots
+        // hold zero values, and their x is taken as zero without a load.
         const int r = lane / LPR, rem = lane - r * LPR, bs = rem / JP, jp = rem - bs * JP;
         const int c = cols[(int64_t)s * RW * W + r * W + bs];
-        if (c >= 0) {
-            const TA* base = sv + ((int64_t)s * NF * ACT + lane) * CW;
-            if constexpr (CW == 2) {
-                TV x0, x1;
-                ld2(x + (int64_t)c * NF + 2 * jp, x0, x1);
+        const TA* base = sv + ((int64_t)s * NF * ACT + lane) * CW;
+        if constexpr (CW == 2) {
+            AT a[NF][2];
 #pragma unroll
-                for (int i = 0; i < NF; i++) {
-                    AT a0, a1;
-                    ldnt2(base + (int64_t)i * ACT * 2, a0, a1);
-                    acc[i] = a0 * (AT)x0 + a1 * (AT)x1;
-                }
-            } else {
-                const AT x0 = (AT)x[(int64_t)c * NF + jp];
+            for (int i = 0; i < NF; i++) ldnt2(base + (int64_t)i * ACT * 2, a[i][0], a[i][1]);
+            TV x0 = 0, x1 = 0;
+            if (c >= 0) ld2(x + (int64_t)c * NF + 2 * jp, x0, x1);
 #pragma unroll
-                for (int i = 0; i < NF; i++) {
-                    AT a0;
-                    ldnt(base + (int64_t)i * ACT, a0);
-                    acc[i] = a0 * x0;
-                }
-            }
+            for (int i = 0; i < NF; i++) acc[i] = a[i][0] * (AT)x0 + a[i][1] * (AT)x1;
+        } else {
+            AT a[NF];
+#pragma unroll
+            for (int i = 0; i < NF; i++) ldnt(base + (int64_t)i * ACT, a[i]);
+            const AT x0 = c >= 0 ? (AT)x[(int64_t)c * NF + jp] : (AT)0;
+#pragma unroll
+            for (int i = 0; i < NF; i++) acc[i] = a[i] * x0;
         }
     }
 #pragma unroll
