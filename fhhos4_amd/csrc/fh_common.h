@@ -107,9 +107,15 @@ struct DVec {
 // Values of slice s are stored [s][i][lane < ACT][cw] (ACT = RW * W * JP), so
 // load i of a wavefront reads ACT * cw contiguous values; cols holds the
 // block column of every (slice, row, slot), -1 for padding.
+// Execution copies of a block-CSR operator's values for full-matrix
+// products.  kind 0 (rows that fit a wavefront): slices of RW rows with W
+// block slots each.  kind 1 (wide rows): every row split into chunks of W
+// block slots (RW = 1), chunk_ptr[row] .. chunk_ptr[row + 1] being its
+// chunks, one wavefront per row.
 struct Sliced {
-    int W = 0, RW = 0, cw = 1, nslices = 0;
-    DevBuf cols, vals;
+    int kind = 0;
+    int W = 0, RW = 0, cw = 1, nslices = 0;  // kind 1: nslices = number of chunks
+    DevBuf cols, vals, chunk_ptr;
     int act(int nf) const { return RW * W * (nf / cw); }
 };
 

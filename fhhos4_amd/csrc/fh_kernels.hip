@@ -307,6 +307,113 @@ __global__ __launch_bounds__(kWG) void k_sliced(int nbr, int nslices, int W, int
     }
 }
 
+// Block-row operation on the chunked layout of wide rows (Galerkin coarse
+// operators, restrictions: 40-100 blocks per row).  One wavefront per row;
+// lane (b, jp) takes column group jp of block slot b of each chunk of W
+// slots, so each value load of the wavefront reads one contiguous run; two
+// chunks are in flight at a time.  The NF row sums are reduced across the
+// wavefront with xor-shuffles, after which every lane holds all of them.
+template <int NF, class TA, class TV, class TD, int MODE>
+__global__ __launch_bounds__(kWG) void k_wide(int nbr, int W, const int* __restrict__ cp,
+                                              const int* __restrict__ cols, const TA* __restrict__ sv, const TV* x,
+                                              const TV* __restrict__ b, TV* y, double alpha, double beta,
+                                              double omega, const TD* __restrict__ invD) {
+    using AT = typename Acc<TV>::T;
+    constexpr int CW = NF % 2 == 0 ? 2 : 1;
+    constexpr int JP = NF / CW;
+    const int ACT = W * JP;
+    const int lane = threadIdx.x & 63;
+    const int row = xcd_remap(blockIdx.x, gridDim.x) * kSlicedWaves + (int)(threadIdx.x >> 6);
+    if (row >= nbr) return;  // wave-uniform; no barrier below
+    const int bs = lane / JP, jp = lane - bs * JP;
+    const bool act = lane < ACT;
+    AT acc[NF];
+#pragma unroll
+    for (int i = 0; i < NF; i++) acc[i] = 0;
+    const int q0 = cp[row], q1 = cp[row + 1];
+    for (int q = q0; q < q1; q += 2) {
+        AT a[2][NF][CW];
+        int c[2];
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+            const int qq = q + u;
+            const bool ok = act && qq < q1;
+            c[u] = ok ? cols[(int64_t)qq * W + bs] : -1;
+            const TA* base = sv + ((int64_t)qq * NF * ACT + lane) * CW;
+#pragma unroll
+            for (int i = 0; i < NF; i++) {
+#pragma unroll
+                for (int k = 0; k < CW; k++) a[u][i][k] = 0;
+                if (ok) {
+                    if constexpr (CW == 2) ldnt2(base + (int64_t)i * ACT * 2, a[u][i][0], a[u][i][1]);
+                    else ldnt(base + (int64_t)i * ACT, a[u][i][0]);
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+            if constexpr (CW == 2) {
+                TV x0 = 0, x1 = 0;
+                if (c[u] >= 0) ld2(x + (int64_t)c[u] * NF + 2 * jp, x0, x1);
+#pragma unroll
+                for (int i = 0; i < NF; i++) acc[i] += a[u][i][0] * (AT)x0 + a[u][i][1] * (AT)x1;
+            } else {
+                const AT x0 = c[u] >= 0 ? (AT)x[(int64_t)c[u] * NF + jp] : (AT)0;
+#pragma unroll
+                for (int i = 0; i < NF; i++) acc[i] += a[u][i][0] * x0;
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < NF; i++)
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) acc[i] += __shfl_xor(acc[i], off, 64);
+    if (lane >= NF) return;
+    const int64_t e = (int64_t)row * NF + lane;
+    AT sum = 0;
+#pragma unroll
+    for (int i = 0; i < NF; i++)
+        if (lane == i) sum = acc[i];
+    if (MODE == OP_SPMV) {
+        AT v = (AT)alpha * sum;
+        if (beta != 0.0) v += (AT)beta * (AT)y[e];
+        y[e] = (TV)v;
+    } else if (MODE == OP_RESID) {
+        y[e] = (TV)((AT)b[e] - sum);
+    } else {
+        // y = x + omega D^-1 (b - A x); every lane holds the row's NF sums
+        const TD* D = invD + e * NF;
+        const TV* bb = b + (int64_t)row * NF;
+        AT z = 0;
+#pragma unroll
+        for (int k = 0; k < NF; k++) z += (AT)ldv(D + k) * ((AT)bb[k] - acc[k]);
+        y[e] = (TV)((AT)x[e] + (AT)omega * z);
+    }
+}
+
+// Block-CSR values -> chunked values of wide rows (one thread per chunk lane).
+template <int NF, class TA>
+__global__ void k_to_wide(int nchunks, int W, const int* __restrict__ chunk_row, const int* __restrict__ cp,
+                          const int* __restrict__ rp, const TA* __restrict__ val, TA* __restrict__ sv) {
+    constexpr int CW = NF % 2 == 0 ? 2 : 1;
+    constexpr int JP = NF / CW;
+    const int ACT = W * JP;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)nchunks * ACT) return;
+    const int q = (int)(t / ACT), lane = (int)(t - (int64_t)q * ACT);
+    const int bs = lane / JP, jp = lane - bs * JP;
+    const int row = chunk_row[q];
+    const int slot = (q - cp[row]) * W + bs;
+    const bool ok = slot < rp[row + 1] - rp[row];
+    const int64_t blk = ok ? (int64_t)rp[row] + slot : 0;
+    for (int i = 0; i < NF; i++)
+        for (int u = 0; u < CW; u++) {
+            TA v{};
+            if (ok) v = val[(blk * NF + i) * NF + CW * jp + u];
+            sv[(((int64_t)q * NF + i) * ACT + lane) * CW + u] = v;
+        }
+}
+
 // Block-CSR values -> sliced values (one thread per slice lane).
 template <int NF, class TA>
 __global__ void k_to_sliced(int nbr, int nslices, int W, int RW, const int* __restrict__ rp,
@@ -587,7 +694,33 @@ void bcsr_dispatch_mode(Ctx& c, const Bcsr& A, BcsrMode mode, const void* x, con
     auto bb = (const TV*)b;
     auto yy = (TV*)y;
     auto D = (const TD*)invD;
-    if (!rows && A.sliced_state == 1) {
+    if (!rows && A.sliced_state == 1 && A.sliced->kind == 1) {
+        const Sliced& S = *A.sliced;
+        const bool aligned = S.cw == 1 || ((uintptr_t)x % (2 * sizeof(TV))) == 0;
+        if (aligned) {
+            const int nwg = (A.nbr + kSlicedWaves - 1) / kSlicedWaves;
+            auto cp = S.chunk_ptr.as<int>();
+            auto sc = S.cols.as<int>();
+            auto sv = S.vals.as<TA>();
+            switch (mode) {
+                case OP_SPMV:
+                    hipLaunchKernelGGL((k_wide<NF, TA, TV, TD, OP_SPMV>), dim3(nwg), dim3(kWG), 0, c.stream, A.nbr,
+                                       S.W, cp, sc, sv, xx, bb, yy, alpha, beta, omega, D);
+                    break;
+                case OP_RESID:
+                    hipLaunchKernelGGL((k_wide<NF, TA, TV, TD, OP_RESID>), dim3(nwg), dim3(kWG), 0, c.stream, A.nbr,
+                                       S.W, cp, sc, sv, xx, bb, yy, alpha, beta, omega, D);
+                    break;
+                default:
+                    hipLaunchKernelGGL((k_wide<NF, TA, TV, TD, OP_SMOOTH>), dim3(nwg), dim3(kWG), 0, c.stream, A.nbr,
+                                       S.W, cp, sc, sv, xx, bb, yy, alpha, beta, omega, D);
+                    break;
+            }
+            FH_HIP(hipGetLastError());
+            return;
+        }
+    }
+    if (!rows && A.sliced_state == 1 && A.sliced->kind == 0) {
         const Sliced& S = *A.sliced;
         const bool aligned = S.cw == 1 || ((uintptr_t)x % (2 * sizeof(TV))) == 0;
         if (aligned) {
@@ -691,6 +824,76 @@ bool sliced_geometry(const Bcsr& A, int& W, int& RW, int& cw) {
     return true;
 }
 
+// Chunk width of the wide-row layout (block slots per chunk = 64 / column
+// groups per block) and the number of chunks; false without host arrays or
+// a kernel for the block size.
+bool wide_geometry(const Bcsr& A, int& W, int& cw, int64_t& nchunks) {
+    const int nf = A.bs_r;
+    if (A.bs_r != A.bs_c || A.nbr <= 0 || A.h_rowptr.size() != (size_t)A.nbr + 1 ||
+        A.h_colidx.size() != (size_t)A.nnzb || A.h_rowptr[A.nbr] != A.nnzb)
+        return false;
+    if (nf != 1 && nf != 2 && nf != 3 && nf != 4 && nf != 6 && nf != 10) return false;
+    cw = nf % 2 == 0 ? 2 : 1;
+    W = 64 / (nf / cw);
+    nchunks = 0;
+    for (int i = 0; i < A.nbr; i++) nchunks += (A.h_rowptr[i + 1] - A.h_rowptr[i] + W - 1) / W;
+    return nchunks > 0 && nchunks < INT32_MAX / 64;
+}
+
+template <int NF, class TA>
+void wide_convert(Ctx& c, const Bcsr& A, Sliced& S, const int* chunk_row) {
+    const int64_t n = (int64_t)S.nslices * S.act(NF);
+    if (n == 0) return;
+    hipLaunchKernelGGL((k_to_wide<NF, TA>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream, S.nslices, S.W,
+                       chunk_row, S.chunk_ptr.as<int>(), A.rowptr.as<int>(), A.vals.as<TA>(), S.vals.as<TA>());
+    FH_HIP(hipGetLastError());
+}
+
+template <int NF>
+void wide_convert_prec(Ctx& c, const Bcsr& A, Sliced& S, const int* chunk_row) {
+    if (A.prec == Prec::F64) wide_convert<NF, double>(c, A, S, chunk_row);
+    else if (A.prec == Prec::F32) wide_convert<NF, float>(c, A, S, chunk_row);
+    else wide_convert<NF, bf16_t>(c, A, S, chunk_row);
+}
+
+void wide_build(Ctx& c, const Bcsr& A, int W, int cw, int64_t nchunks) {
+    auto S = std::make_unique<Sliced>();
+    const int nf = A.bs_r;
+    S->kind = 1;
+    S->W = W; S->RW = 1; S->cw = cw;
+    S->nslices = (int)nchunks;
+    std::vector<int> cp(A.nbr + 1, 0), crow((size_t)nchunks), hc((size_t)nchunks * W, -1);
+    for (int i = 0; i < A.nbr; i++) {
+        const int len = A.h_rowptr[i + 1] - A.h_rowptr[i];
+        cp[i + 1] = cp[i] + (len + W - 1) / W;
+        for (int q = cp[i]; q < cp[i + 1]; q++) crow[q] = i;
+        for (int k = 0; k < len; k++) hc[(size_t)cp[i] * W + k] = A.h_colidx[A.h_rowptr[i] + k];
+    }
+    FH_CHECK(cp[A.nbr] == nchunks && S->act(nf) <= 64 && nf <= 64, -10, "wide layout sizes");
+    S->chunk_ptr.alloc(sizeof(int) * cp.size());
+    S->cols.alloc(sizeof(int) * hc.size());
+    DevBuf d_crow(sizeof(int) * crow.size());
+    FH_HIP(hipMemcpyAsync(S->chunk_ptr.p, cp.data(), sizeof(int) * cp.size(), hipMemcpyHostToDevice, c.stream));
+    FH_HIP(hipMemcpyAsync(S->cols.p, hc.data(), sizeof(int) * hc.size(), hipMemcpyHostToDevice, c.stream));
+    FH_HIP(hipMemcpyAsync(d_crow.p, crow.data(), sizeof(int) * crow.size(), hipMemcpyHostToDevice, c.stream));
+    // layout [chunk][i < nf][lane < W * JP][cw]
+    const size_t nvals = (size_t)nchunks * nf * S->act(nf) * cw;
+    S->vals.alloc(nvals * prec_bytes(A.prec));
+    FH_CHECK(S->vals.bytes == nvals * prec_bytes(A.prec), -10, "wide layout sizes");
+    const int* cr = d_crow.as<int>();
+    switch (nf) {
+        case 1: wide_convert_prec<1>(c, A, *S, cr); break;
+        case 2: wide_convert_prec<2>(c, A, *S, cr); break;
+        case 3: wide_convert_prec<3>(c, A, *S, cr); break;
+        case 4: wide_convert_prec<4>(c, A, *S, cr); break;
+        case 6: wide_convert_prec<6>(c, A, *S, cr); break;
+        default: wide_convert_prec<10>(c, A, *S, cr); break;
+    }
+    c.sync();  // host arrays and the chunk-row map released on return
+    A.sliced = std::move(S);
+    A.sliced_state = 1;
+}
+
 template <int NF, class TA>
 void sliced_convert(Ctx& c, const Bcsr& A, Sliced& S) {
     const int64_t n = (int64_t)S.nslices * S.act(NF);
@@ -757,16 +960,24 @@ void sliced_decide(Ctx& c, const Bcsr& A) {
     FH_HIP(hipStreamIsCapturing(c.stream, &cs));
     if (cs != hipStreamCaptureStatusNone) return;
     int W, RW, cw;
-    if (!sliced_enabled() || A.nbr < 4096 || !sliced_geometry(A, W, RW, cw)) {
+    if (!sliced_enabled() || A.nbr < 4096) {
         A.sliced_state = -1;
         return;
     }
-    const int64_t nslices = (A.nbr + RW - 1) / RW;
-    if ((double)nslices * RW * W > 1.25 * (double)A.nnzb) {
-        A.sliced_state = -1;
+    if (sliced_geometry(A, W, RW, cw)) {
+        const int64_t nslices = (A.nbr + RW - 1) / RW;
+        if ((double)nslices * RW * W > 1.25 * (double)A.nnzb) A.sliced_state = -1;
+        else sliced_build(c, A, W, RW, cw);
         return;
     }
-    sliced_build(c, A, W, RW, cw);
+    // rows wider than a wavefront: the chunked layout, under the same bound
+    // on padding.  BF16 storage stays on the block-CSR kernel, which is as
+    // fast there (4-byte value loads either way).
+    int64_t nchunks;
+    if (A.prec != Prec::BF16 && wide_geometry(A, W, cw, nchunks) && (double)nchunks * W <= 1.25 * (double)A.nnzb)
+        wide_build(c, A, W, cw, nchunks);
+    else
+        A.sliced_state = -1;
 }
 
 }  // namespace
@@ -779,12 +990,17 @@ bool set_sliced(Ctx& c, const Bcsr& A, bool on) {
     }
     if (A.sliced_state == 1) return true;
     int W, RW, cw;
-    if (!sliced_geometry(A, W, RW, cw)) {
-        A.sliced_state = -1;
-        return false;
+    if (sliced_geometry(A, W, RW, cw)) {
+        sliced_build(c, A, W, RW, cw);
+        return true;
     }
-    sliced_build(c, A, W, RW, cw);
-    return true;
+    int64_t nchunks;
+    if (wide_geometry(A, W, cw, nchunks)) {
+        wide_build(c, A, W, cw, nchunks);
+        return true;
+    }
+    A.sliced_state = -1;
+    return false;
 }
 
 static void launch_bcsr_impl(Ctx& c, const Bcsr& A, BcsrMode mode, Prec vprec, const void* x, const void* b, void* y,

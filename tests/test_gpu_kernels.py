@@ -185,21 +185,69 @@ def _random_blocks(rng, nbr, nbc, nf, wmax):
 def test_sliced_layout_irregular_rows(ctx, nf):
     """Rows of 0..W blocks (W = the widest that fits one wavefront), a
     non-square block pattern and a row count that is not a multiple of the
-    rows per wavefront: sliced products equal scipy's; a pattern whose rows do
-    not fit stays on the block-CSR kernels."""
+    rows per wavefront: sliced products equal scipy's.  Patterns whose rows
+    do not fit a wavefront take the chunked wide-row layout (several chunks
+    per row, empty rows, a row of exactly one chunk) with the same result."""
     import scipy.sparse as sp
     rng = np.random.default_rng(nf)
     cw = 2 if nf % 2 == 0 else 1
     wfit = 64 // (nf // cw)
-    for nbr, wmax, fits in ((4099, min(wfit, 40), True), (517, 3, True), (300, wfit + 1, False)):
+    for nbr, wmax in ((4099, min(wfit, 40)), (517, 3), (300, wfit + 1), (4101, 3 * wfit + 5), (64, wfit)):
         nbc = nbr + 37
         rp, ci, v = _random_blocks(rng, nbr, nbc, nf, wmax)
         S = sp.bsr_matrix((v, ci, rp), shape=(nbr * nf, nbc * nf)).tocsr()
         M = G.Matrix.from_blocks(ctx, nbc, nf, rp, ci, v)
-        assert M.set_sliced(True) == fits
+        assert M.set_sliced(True)
         x = rng.standard_normal(nbc * nf)
         y0 = rng.standard_normal(nbr * nf)
         yd = dev(y0)
         G.spmv(ctx, M, dev(x), yd, alpha=-2.0, beta=0.25)
         want = -2.0 * (S @ x) + 0.25 * y0
         assert np.linalg.norm(host(ctx, yd) - want) <= 1e-13 * np.linalg.norm(want), (nbr, wmax)
+
+
+@pytest.mark.parametrize("nf", [1, 2, 3, 4, 6, 10])
+def test_wide_row_layout_modes(ctx, nf):
+    """The chunked layout of wide rows (the Galerkin coarse operators) in the
+    three modes of the block-row kernel, FP64 against the block-CSR kernel
+    and the oracle, FP32 and BF16-stored against the block-CSR kernel."""
+    import scipy.sparse as sp
+    rng = np.random.default_rng(40 + nf)
+    cw = 2 if nf % 2 == 0 else 1
+    wfit = 64 // (nf // cw)
+    nbr = 4500
+    lens = rng.integers(wfit + 1, 3 * wfit, size=nbr)
+    lens[7] = 0
+    rp = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+
+    def row_cols(i, k):  # the diagonal and k - 1 other columns, sorted
+        if k == 0:
+            return np.zeros(0, dtype=np.int64)
+        c = rng.choice(nbr - 1, size=k - 1, replace=False)
+        return np.sort(np.concatenate([[i], c + (c >= i)]))
+
+    ci = np.concatenate([row_cols(i, k) for i, k in enumerate(lens)]).astype(np.int32)
+    v = rng.standard_normal((int(rp[-1]), nf, nf))
+    for i in range(nbr):  # dominant diagonal blocks: block Jacobi is well defined
+        for p in range(rp[i], rp[i + 1]):
+            if ci[p] == i:
+                v[p] += 4.0 * nf * lens[i] * np.eye(nf)
+    A = sp.bsr_matrix((v, ci, rp), shape=(nbr * nf, nbr * nf)).tocsr()
+    A = (A + sp.kron(sp.diags((lens == 0).astype(float)), np.eye(nf))).tocsr()  # the empty row: identity block
+    x, y0, b = (rng.standard_normal(nbr * nf) for _ in range(3))
+    inv = O.block_inverse(A, nf)
+    want = [1.5 * O.spmv(A, x) - 0.5 * y0, b - O.spmv(A, x),
+            O.bj_sweep(A, nf, inv, 0.7, b, O.bj_sweep(A, nf, inv, 0.7, b, x))]
+    for prec, vdt, tol in (("f64", torch.float64, 1e-13), ("f32", torch.float32, 2e-6), ("bf16", torch.float32, 2e-6)):
+        res = {}
+        for layout in (False, True):
+            M64 = G.Matrix.from_csr(ctx, A, nf)
+            M = M64 if prec == "f64" else M64.convert(prec)
+            assert M.set_sliced(layout) == layout
+            bj = G.BlockJacobi(ctx, M64, 0.7, dprec="f64" if prec == "f64" else "f32")
+            res[layout] = _modes(ctx, M, x, y0, b, vdt, bj)
+        for k, (u, w) in enumerate(zip(res[False], res[True])):
+            assert np.linalg.norm(u - w) <= tol * np.linalg.norm(u), (prec, k)
+        if prec == "f64":
+            for k, (u, w) in enumerate(zip(res[True], want)):
+                assert np.linalg.norm(u - w) <= 1e-12 * np.linalg.norm(w), k
