@@ -56,16 +56,19 @@ int fh_mat_info(const fh_mat* A, int* nbr, int* nbc, int* bs, int64_t* nnzb, int
 int fh_mat_blocks(const fh_mat* A, int* rowptr, int* colidx, double* vals);
 /* Device pointers of the block structure and values (for profiling tools). */
 int fh_mat_device_arrays(const fh_mat* A, const int** rowptr, const int** colidx, const void** vals);
-/* Execution layout of full-matrix products.  on = 1 builds a sliced copy of
- * the values (a wavefront owns consecutive block rows, lane = (row, block
- * slot, column pair)); *active (may be NULL) tells whether A has one, which
- * needs every row to fit one wavefront (blocks per row * n_f / 2 <= 64 for
- * even n_f, blocks per row * n_f <= 64 for odd n_f).  on = 0 drops it and
- * keeps A on the block-CSR kernels.  Without a call the library builds it at
- * the first full-matrix product when the rows fit, the matrix has at least
- * 4096 block rows and padding adds at most 25% to the values.  No reference
- * counterpart: the reference has one CSR storage (src/Utils/Types.h:11-22);
- * the sliced copy is a GPU execution detail of the same operator. */
+/* Execution layout of full-matrix products.  on = 1 builds a second copy of
+ * the values: a sliced copy when every row fits one wavefront (blocks per
+ * row * n_f / 2 <= 64 for even n_f, blocks per row * n_f <= 64 for odd n_f;
+ * a wavefront owns consecutive block rows, lane = (row, block slot, column
+ * pair)), otherwise a chunked copy (a wavefront per row, rows cut into
+ * chunks of 64 / (n_f / 2) or 64 / n_f block slots); *active (may be NULL)
+ * tells whether A has one.  on = 0 drops it and keeps A on the block-CSR
+ * kernels.  Without a call the library builds one at the first full-matrix
+ * product when the matrix has at least 4096 block rows and padding adds at
+ * most 25% to the values (the chunked copy: FP64 or FP32 storage only).  No
+ * reference counterpart: the reference has one CSR storage
+ * (src/Utils/Types.h:11-22); the copy is a GPU execution detail of the same
+ * operator. */
 int fh_mat_set_sliced(fh_ctx* ctx, const fh_mat* A, int on, int* active);
 
 /* -------------------------------------------- exported linear system */
