@@ -18,6 +18,7 @@
 #include "fh_kernels.h"
 
 #include <algorithm>
+#include <type_traits>
 #include <cstdlib>
 
 namespace fh {
@@ -228,6 +229,17 @@ __device__ __forceinline__ void ldnt2(const bf16_t* p, float& a, float& b) {
     b = __uint_as_float(u & 0xffff0000u);
 }
 
+// Four BF16 values (8 bytes): block rows 2h and 2h + 1, two columns each.
+__device__ __forceinline__ void ldnt4(const bf16_t* p, float& a00, float& a01, float& a10, float& a11) {
+    const uint64_t u = __builtin_nontemporal_load((const uint64_t*)p);
+    const uint32_t lo = (uint32_t)u, hi = (uint32_t)(u >> 32);
+    a00 = __uint_as_float(lo << 16);
+    a01 = __uint_as_float(lo & 0xffff0000u);
+    a10 = __uint_as_float(hi << 16);
+    a11 = __uint_as_float(hi & 0xffff0000u);
+}
+template <class T> __device__ __forceinline__ void ldnt4(const T*, float&, float&, float&, float&) {}
+
 constexpr int kSlicedWaves = kWG / 64;  // slices per workgroup
 
 // Block-row operation on the sliced layout (all rows).  Lane
@@ -331,6 +343,9 @@ __global__ __launch_bounds__(kWG) void k_wide(int nbr, int W, const int* __restr
 #pragma unroll
     for (int i = 0; i < NF; i++) acc[i] = 0;
     const int q0 = cp[row], q1 = cp[row + 1];
+    // BF16 storage with even NF packs block rows 2h, 2h + 1 of the lane's
+    // column pair into one 8-byte load: [chunk][h < NF/2][lane][4]
+    constexpr bool PK = std::is_same<TA, bf16_t>::value && CW == 2;
     for (int q = q0; q < q1; q += 2) {
         AT a[2][NF][CW];
         int c[2];
@@ -341,12 +356,22 @@ __global__ __launch_bounds__(kWG) void k_wide(int nbr, int W, const int* __restr
             c[u] = ok ? cols[(int64_t)qq * W + bs] : -1;
             const TA* base = sv + ((int64_t)qq * NF * ACT + lane) * CW;
 #pragma unroll
-            for (int i = 0; i < NF; i++) {
+            for (int i = 0; i < NF; i++)
 #pragma unroll
                 for (int k = 0; k < CW; k++) a[u][i][k] = 0;
-                if (ok) {
-                    if constexpr (CW == 2) ldnt2(base + (int64_t)i * ACT * 2, a[u][i][0], a[u][i][1]);
-                    else ldnt(base + (int64_t)i * ACT, a[u][i][0]);
+            if (ok) {
+                if constexpr (PK) {
+                    const TA* b4 = sv + ((int64_t)qq * NF * ACT + (int64_t)lane * 2) * CW;
+#pragma unroll
+                    for (int h = 0; h < NF / 2; h++)
+                        ldnt4(b4 + (int64_t)h * ACT * 4, a[u][2 * h][0], a[u][2 * h][1], a[u][2 * h + 1][0],
+                              a[u][2 * h + 1][1]);
+                } else {
+#pragma unroll
+                    for (int i = 0; i < NF; i++) {
+                        if constexpr (CW == 2) ldnt2(base + (int64_t)i * ACT * 2, a[u][i][0], a[u][i][1]);
+                        else ldnt(base + (int64_t)i * ACT, a[u][i][0]);
+                    }
                 }
             }
         }
@@ -406,11 +431,15 @@ __global__ void k_to_wide(int nchunks, int W, const int* __restrict__ chunk_row,
     const int slot = (q - cp[row]) * W + bs;
     const bool ok = slot < rp[row + 1] - rp[row];
     const int64_t blk = ok ? (int64_t)rp[row] + slot : 0;
+    constexpr bool PK = std::is_same<TA, bf16_t>::value && CW == 2;
     for (int i = 0; i < NF; i++)
         for (int u = 0; u < CW; u++) {
             TA v{};
             if (ok) v = val[(blk * NF + i) * NF + CW * jp + u];
-            sv[(((int64_t)q * NF + i) * ACT + lane) * CW + u] = v;
+            if constexpr (PK)  // [chunk][i / 2][lane][(i % 2) * 2 + u]
+                sv[(((int64_t)q * (NF / 2) + i / 2) * ACT + lane) * 4 + (i % 2) * 2 + u] = v;
+            else
+                sv[(((int64_t)q * NF + i) * ACT + lane) * CW + u] = v;
         }
 }
 
@@ -971,10 +1000,11 @@ void sliced_decide(Ctx& c, const Bcsr& A) {
         return;
     }
     // rows wider than a wavefront: the chunked layout, under the same bound
-    // on padding.  BF16 storage stays on the block-CSR kernel, which is as
-    // fast there (4-byte value loads either way).
+    // on padding.  BF16 storage with odd n_f stays on the block-CSR kernel
+    // (2-byte value loads either way).
     int64_t nchunks;
-    if (A.prec != Prec::BF16 && wide_geometry(A, W, cw, nchunks) && (double)nchunks * W <= 1.25 * (double)A.nnzb)
+    if ((A.prec != Prec::BF16 || A.bs_r % 2 == 0) && wide_geometry(A, W, cw, nchunks) &&
+        (double)nchunks * W <= 1.25 * (double)A.nnzb)
         wide_build(c, A, W, cw, nchunks);
     else
         A.sliced_state = -1;
