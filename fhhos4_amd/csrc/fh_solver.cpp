@@ -813,6 +813,7 @@ void Solver::Solve(const double* b, double* x) {
 // IterativeSolver::StoppingCriteriaReached (IterativeSolver.h:224-247)
 #define FH_STOP_TEST()                                                                        \
     [&]() -> bool {                                                                           \
+        if (args_.max_iter == 0) return true;                                                 \
         if (it == 0) return false;                                                            \
         FH_CHECK(!std::isinf(res), -7, "divergence of the solver");                           \
         FH_CHECK(!std::isnan(res), -8, "the residual is NaN.");                               \

@@ -779,6 +779,8 @@ int or_solve(const or_solver_cfg* cfg, const or_level_desc* levels, const double
     int status = 1;
     std::vector<double> r(b, b + n), z(n), d(n), Ad(n), Az(n);
     auto stop = [&]() {
+        // IterativeSolver::StoppingCriteriaReached (IterativeSolver.h:224-247)
+        if (cfg->max_iter == 0) return true;
         if (it == 0) return false;
         if (!std::isfinite(res)) { status = -4; return true; }
         if (res < cfg->tol) { status = 0; return true; }

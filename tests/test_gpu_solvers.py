@@ -183,3 +183,27 @@ def test_exported_levels_match_reference(ctx, tmp_path):
         P = sio.mmread(str(out / ("level%d_P.dat" % (l - 1)))).tocsr()
         Af = sio.mmread(str(out / ("level%d_A.dat" % (l - 1)))).tocsr()
         assert abs(P.T @ Af @ P - got).max() <= 1e-12 * abs(got).max()
+
+
+def test_stopping_rules_match_the_reference(ctx):
+    """IterativeSolver::StoppingCriteriaReached (IterativeSolver.h:224-247):
+    max_iter = 0 stops before the first iteration, leaving the zero initial
+    guess; a zero right-hand side makes CG's first step 0/0 and the NaN
+    residual is reported as an error, as the reference's FatalError ("the
+    residual is NaN.").  The oracle behaves the same (test_oracle.py)."""
+    d = load("c1_n32_cg_mg_bj23")
+    M = G.Matrix.from_csr(ctx, csr(d, "A"), 2)
+    s = G.Solver(ctx, solver="cg", prec="bj", tol=1e-8, max_iter=200)
+    s.setup(M)
+    b = dev(d["b"])
+    x = torch.full_like(b, 3.0)
+    s.set_stop(1e-8, 0)
+    st = s.solve(b, x)
+    ctx.sync()
+    assert st["iterations"] == 0 and st["converged"] == 0
+    assert not x.cpu().numpy().any()
+    s.set_stop(1e-8, 200)
+    with pytest.raises(G.FhError, match="NaN"):
+        s.solve(torch.zeros_like(b), x)
+    st = s.solve(b, x)  # the solver stays usable after the error
+    assert st["converged"] == 1

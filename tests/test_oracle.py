@@ -302,3 +302,16 @@ def test_chebyshev_two_grid_matches_closed_form(smoother, deg):
     K = np.eye(n) - P @ np.linalg.solve(Ac, P.T @ A)
     want = xs + E @ K @ E @ (-xs)
     np.testing.assert_allclose(x, want, rtol=1e-9, atol=1e-12 * np.abs(xs).max())
+
+
+def test_oracle_stopping_rules():
+    """IterativeSolver::StoppingCriteriaReached (IterativeSolver.h:224-247):
+    max_iter = 0 stops before the first iteration (x stays the zero initial
+    guess); a zero right-hand side gives a NaN residual in CG's first step,
+    which the reference reports as a fatal error."""
+    d = load("c1_n32_cg_mg_bj23")
+    A = csr(d, "A")
+    x, hist, out = O.solve(A, d["b"], "cg", "none", nf=2, max_iter=0)
+    assert out.iterations == 0 and not x.any() and len(hist) == 1
+    with pytest.raises(RuntimeError):
+        O.solve(A, np.zeros(A.shape[0]), "cg", "none", nf=2)
