@@ -9,6 +9,7 @@ enum BcsrMode : int {
     OP_SPMV = 0,    // y = alpha * A x + beta * y
     OP_RESID = 1,   // y = b - A x
     OP_SMOOTH = 2,  // y = x + omega * D^-1 (b - A x)   (block Jacobi / GS colour)
+    OP_CHEB = 3,    // d = alpha * d + beta * D^-1 (b - A x), y = x + d   (Chebyshev step)
 };
 
 // Block-row operation over all block rows, or over `rows[0..nrows)` when
@@ -18,6 +19,11 @@ enum BcsrMode : int {
 void launch_bcsr(Ctx& c, const Bcsr& A, BcsrMode mode, Prec vprec, const void* x, const void* b, void* y,
                  double alpha, double beta, double omega, const void* invD, Prec dprec,
                  const int* rows, int nrows);
+
+// One Chebyshev step over all block rows, fused: d = a d + c D^-1 (b - A x)
+// (d not read when a = 0) and xnew = x + d.  xnew must not alias x.
+void launch_bcsr_cheb(Ctx& c, const Bcsr& A, Prec vprec, const void* x, const void* b, void* xnew, void* d,
+                      double a, double cc, const void* invD, Prec dprec);
 
 // Builds (on = true) or drops the sliced execution layout of A used by
 // full-matrix products; returns whether A now has one (false when a row of

@@ -57,12 +57,27 @@ __device__ __forceinline__ int xcd_remap(int bid, int nwg) {
     return xcd * per + (xcd < rem ? xcd : rem) + idx;
 }
 
+// Smoothing output of row entry e with z = D^-1 (b - A x): y = x + omega z
+// (OP_SMOOTH), or the Chebyshev step d = alpha d + beta z, y = x + d with d
+// in aux (OP_CHEB; d is not read when alpha = 0).
+template <int MODE, class TV, class AT>
+__device__ __forceinline__ void smooth_out(int64_t e, AT z, const TV* x, TV* y, TV* aux, double alpha, double beta,
+                                           double omega) {
+    if constexpr (MODE == OP_CHEB) {
+        const AT dv = (alpha != 0.0 ? (AT)alpha * (AT)aux[e] : (AT)0) + (AT)beta * z;
+        aux[e] = (TV)dv;
+        y[e] = (TV)((AT)x[e] + dv);
+    } else {
+        y[e] = (TV)((AT)x[e] + (AT)omega * z);
+    }
+}
+
 template <int NF, class TA, class TV, class TD, int MODE>
 __global__ __launch_bounds__(kWG) void k_bcsr(int nbr, const int* __restrict__ rp, const int* __restrict__ ci,
                                                const TA* __restrict__ val, const TV* x, const TV* __restrict__ b,
                                                TV* y, double alpha, double beta, double omega,
                                                const TD* __restrict__ invD, const int* __restrict__ rows,
-                                               int nrows) {
+                                               int nrows, TV* __restrict__ aux) {
     using AT = typename Acc<TV>::T;
     constexpr int SLOTS = 64 / NF;
     constexpr int ACT = SLOTS * NF;
@@ -108,7 +123,7 @@ __global__ __launch_bounds__(kWG) void k_bcsr(int nbr, const int* __restrict__ r
             AT rj = __shfl(r, j, 64);
             if (lane < NF) z += (AT)ldv(D + j) * rj;
         }
-        if (lane < NF) y[row] = (TV)((AT)x[row] + (AT)omega * z);
+        if (lane < NF) smooth_out<MODE>(row, z, x, y, aux, alpha, beta, omega);
     }
 }
 
@@ -139,7 +154,7 @@ __global__ __launch_bounds__(kWG) void k_bcsr_grp(int nbr, const int* __restrict
                                                    const TA* __restrict__ val, const TV* x,
                                                    const TV* __restrict__ b, TV* y, double alpha, double beta,
                                                    double omega, const TD* __restrict__ invD,
-                                                   const int* __restrict__ rows, int nrows) {
+                                                   const int* __restrict__ rows, int nrows, TV* __restrict__ aux) {
     static_assert(NF % 2 == 0, "even block size");
     using AT = typename Acc<TV>::T;
     constexpr int G = 16, U = 4;
@@ -202,7 +217,7 @@ __global__ __launch_bounds__(kWG) void k_bcsr_grp(int nbr, const int* __restrict
         AT z = 0;
 #pragma unroll
         for (int k = 0; k < NF; k++) z += (AT)ldv(D + k) * ((AT)bb[k] - acc[k]);
-        y[row] = (TV)((AT)x[row] + (AT)omega * z);
+        smooth_out<MODE>(row, z, x, y, aux, alpha, beta, omega);
     }
 }
 
@@ -250,7 +265,7 @@ template <int NF, class TA, class TV, class TD, int MODE>
 __global__ __launch_bounds__(kWG) void k_sliced(int nbr, int nslices, int W, int RW, const int* __restrict__ cols,
                                                 const TA* __restrict__ sv, const TV* x, const TV* __restrict__ b,
                                                 TV* y, double alpha, double beta, double omega,
-                                                const TD* __restrict__ invD) {
+                                                const TD* __restrict__ invD, TV* __restrict__ aux) {
     using AT = typename Acc<TV>::T;
     constexpr int CW = NF % 2 == 0 ? 2 : 1;
     constexpr int JP = NF / CW;
@@ -261,32 +276,32 @@ __global__ __launch_bounds__(kWG) void k_sliced(int nbr, int nslices, int W, int
     const int s = xcd_remap(blockIdx.x, gridDim.x) * kSlicedWaves + wave;
     AT acc[NF];
 #pragma unroll
-    for (int i = 0; i < NF; i++) acc[i] = 0;
+    for (int k = 0; k < NF; k++) acc[k] = 0;
     if (s < nslices && lane < ACT) {
         // The value loads do not wait for the column index: padding slots
         // hold zero values, and their x is taken as zero without a load.
-        const int r = lane / LPR, rem = lane - r * LPR, bs = rem / JP, jp = rem - bs * JP;
-        const int c = cols[(int64_t)s * RW * W + r * W + bs];
+        const int rv = lane / LPR, rem = lane - rv * LPR, bs = rem / JP, jp = rem - bs * JP;
+        const int c = cols[(int64_t)s * RW * W + rv * W + bs];
         const TA* base = sv + ((int64_t)s * NF * ACT + lane) * CW;
         if constexpr (CW == 2) {
             AT a[NF][2];
 #pragma unroll
-            for (int i = 0; i < NF; i++) ldnt2(base + (int64_t)i * ACT * 2, a[i][0], a[i][1]);
+            for (int k = 0; k < NF; k++) ldnt2(base + (int64_t)k * ACT * 2, a[k][0], a[k][1]);
             TV x0 = 0, x1 = 0;
             if (c >= 0) ld2(x + (int64_t)c * NF + 2 * jp, x0, x1);
 #pragma unroll
-            for (int i = 0; i < NF; i++) acc[i] = a[i][0] * (AT)x0 + a[i][1] * (AT)x1;
+            for (int k = 0; k < NF; k++) acc[k] = a[k][0] * (AT)x0 + a[k][1] * (AT)x1;
         } else {
             AT a[NF];
 #pragma unroll
-            for (int i = 0; i < NF; i++) ldnt(base + (int64_t)i * ACT, a[i]);
+            for (int k = 0; k < NF; k++) ldnt(base + (int64_t)k * ACT, a[k]);
             const AT x0 = c >= 0 ? (AT)x[(int64_t)c * NF + jp] : (AT)0;
 #pragma unroll
-            for (int i = 0; i < NF; i++) acc[i] = a[i] * x0;
+            for (int k = 0; k < NF; k++) acc[k] = a[k] * x0;
         }
     }
 #pragma unroll
-    for (int i = 0; i < NF; i++) red[wave][lane][i] = acc[i];
+    for (int k = 0; k < NF; k++) red[wave][lane][k] = acc[k];
     __syncthreads();
     const bool fin = s < nslices && lane < RW * NF;
     const int r = lane / NF, i = lane - r * NF;
@@ -314,7 +329,7 @@ __global__ __launch_bounds__(kWG) void k_sliced(int nbr, int nslices, int W, int
             AT z = 0;
 #pragma unroll
             for (int k = 0; k < NF; k++) z += (AT)ldv(D + k) * res[wave][r * NF + k];
-            y[e] = (TV)((AT)x[e] + (AT)omega * z);
+            smooth_out<MODE>(e, z, x, y, aux, alpha, beta, omega);
         }
     }
 }
@@ -329,7 +344,8 @@ template <int NF, class TA, class TV, class TD, int MODE>
 __global__ __launch_bounds__(kWG) void k_wide(int nbr, int W, const int* __restrict__ cp,
                                               const int* __restrict__ cols, const TA* __restrict__ sv, const TV* x,
                                               const TV* __restrict__ b, TV* y, double alpha, double beta,
-                                              double omega, const TD* __restrict__ invD) {
+                                              double omega, const TD* __restrict__ invD,
+                                              TV* __restrict__ aux) {
     using AT = typename Acc<TV>::T;
     constexpr int CW = NF % 2 == 0 ? 2 : 1;
     constexpr int JP = NF / CW;
@@ -412,7 +428,7 @@ __global__ __launch_bounds__(kWG) void k_wide(int nbr, int W, const int* __restr
         AT z = 0;
 #pragma unroll
         for (int k = 0; k < NF; k++) z += (AT)ldv(D + k) * ((AT)bb[k] - acc[k]);
-        y[e] = (TV)((AT)x[e] + (AT)omega * z);
+        smooth_out<MODE>(e, z, x, y, aux, alpha, beta, omega);
     }
 }
 
@@ -713,7 +729,7 @@ int bcsr_variant() {
 
 template <int NF, class TA, class TV, class TD>
 void bcsr_dispatch_mode(Ctx& c, const Bcsr& A, BcsrMode mode, const void* x, const void* b, void* y, double alpha,
-                        double beta, double omega, const void* invD, const int* rows, int nrows) {
+                        double beta, double omega, const void* invD, const int* rows, int nrows, void* aux) {
     int count = rows ? nrows : A.nbr;
     if (count <= 0) return;
     auto rp = A.rowptr.as<int>();
@@ -723,6 +739,7 @@ void bcsr_dispatch_mode(Ctx& c, const Bcsr& A, BcsrMode mode, const void* x, con
     auto bb = (const TV*)b;
     auto yy = (TV*)y;
     auto D = (const TD*)invD;
+    auto ax = (TV*)aux;
     if (!rows && A.sliced_state == 1 && A.sliced->kind == 1) {
         const Sliced& S = *A.sliced;
         const bool aligned = S.cw == 1 || ((uintptr_t)x % (2 * sizeof(TV))) == 0;
@@ -734,15 +751,19 @@ void bcsr_dispatch_mode(Ctx& c, const Bcsr& A, BcsrMode mode, const void* x, con
             switch (mode) {
                 case OP_SPMV:
                     hipLaunchKernelGGL((k_wide<NF, TA, TV, TD, OP_SPMV>), dim3(nwg), dim3(kWG), 0, c.stream, A.nbr,
-                                       S.W, cp, sc, sv, xx, bb, yy, alpha, beta, omega, D);
+                                       S.W, cp, sc, sv, xx, bb, yy, alpha, beta, omega, D, ax);
                     break;
                 case OP_RESID:
                     hipLaunchKernelGGL((k_wide<NF, TA, TV, TD, OP_RESID>), dim3(nwg), dim3(kWG), 0, c.stream, A.nbr,
-                                       S.W, cp, sc, sv, xx, bb, yy, alpha, beta, omega, D);
+                                       S.W, cp, sc, sv, xx, bb, yy, alpha, beta, omega, D, ax);
+                    break;
+                case OP_CHEB:
+                    hipLaunchKernelGGL((k_wide<NF, TA, TV, TD, OP_CHEB>), dim3(nwg), dim3(kWG), 0, c.stream, A.nbr,
+                                       S.W, cp, sc, sv, xx, bb, yy, alpha, beta, omega, D, ax);
                     break;
                 default:
                     hipLaunchKernelGGL((k_wide<NF, TA, TV, TD, OP_SMOOTH>), dim3(nwg), dim3(kWG), 0, c.stream, A.nbr,
-                                       S.W, cp, sc, sv, xx, bb, yy, alpha, beta, omega, D);
+                                       S.W, cp, sc, sv, xx, bb, yy, alpha, beta, omega, D, ax);
                     break;
             }
             FH_HIP(hipGetLastError());
@@ -759,15 +780,19 @@ void bcsr_dispatch_mode(Ctx& c, const Bcsr& A, BcsrMode mode, const void* x, con
             switch (mode) {
                 case OP_SPMV:
                     hipLaunchKernelGGL((k_sliced<NF, TA, TV, TD, OP_SPMV>), dim3(nwg), dim3(kWG), 0, c.stream, A.nbr,
-                                       S.nslices, S.W, S.RW, sc, sv, xx, bb, yy, alpha, beta, omega, D);
+                                       S.nslices, S.W, S.RW, sc, sv, xx, bb, yy, alpha, beta, omega, D, ax);
                     break;
                 case OP_RESID:
                     hipLaunchKernelGGL((k_sliced<NF, TA, TV, TD, OP_RESID>), dim3(nwg), dim3(kWG), 0, c.stream, A.nbr,
-                                       S.nslices, S.W, S.RW, sc, sv, xx, bb, yy, alpha, beta, omega, D);
+                                       S.nslices, S.W, S.RW, sc, sv, xx, bb, yy, alpha, beta, omega, D, ax);
+                    break;
+                case OP_CHEB:
+                    hipLaunchKernelGGL((k_sliced<NF, TA, TV, TD, OP_CHEB>), dim3(nwg), dim3(kWG), 0, c.stream, A.nbr,
+                                       S.nslices, S.W, S.RW, sc, sv, xx, bb, yy, alpha, beta, omega, D, ax);
                     break;
                 default:
                     hipLaunchKernelGGL((k_sliced<NF, TA, TV, TD, OP_SMOOTH>), dim3(nwg), dim3(kWG), 0, c.stream, A.nbr,
-                                       S.nslices, S.W, S.RW, sc, sv, xx, bb, yy, alpha, beta, omega, D);
+                                       S.nslices, S.W, S.RW, sc, sv, xx, bb, yy, alpha, beta, omega, D, ax);
                     break;
             }
             FH_HIP(hipGetLastError());
@@ -781,15 +806,19 @@ void bcsr_dispatch_mode(Ctx& c, const Bcsr& A, BcsrMode mode, const void* x, con
             switch (mode) {
                 case OP_SPMV:
                     hipLaunchKernelGGL((k_bcsr_grp<NF, TA, TV, TD, OP_SPMV>), dim3(nwg), dim3(kWG), 0, c.stream, A.nbr,
-                                       rp, ci, val, xx, bb, yy, alpha, beta, omega, D, rows, nrows);
+                                       rp, ci, val, xx, bb, yy, alpha, beta, omega, D, rows, nrows, ax);
                     break;
                 case OP_RESID:
                     hipLaunchKernelGGL((k_bcsr_grp<NF, TA, TV, TD, OP_RESID>), dim3(nwg), dim3(kWG), 0, c.stream,
-                                       A.nbr, rp, ci, val, xx, bb, yy, alpha, beta, omega, D, rows, nrows);
+                                       A.nbr, rp, ci, val, xx, bb, yy, alpha, beta, omega, D, rows, nrows, ax);
+                    break;
+                case OP_CHEB:
+                    hipLaunchKernelGGL((k_bcsr_grp<NF, TA, TV, TD, OP_CHEB>), dim3(nwg), dim3(kWG), 0, c.stream,
+                                       A.nbr, rp, ci, val, xx, bb, yy, alpha, beta, omega, D, rows, nrows, ax);
                     break;
                 default:
                     hipLaunchKernelGGL((k_bcsr_grp<NF, TA, TV, TD, OP_SMOOTH>), dim3(nwg), dim3(kWG), 0, c.stream,
-                                       A.nbr, rp, ci, val, xx, bb, yy, alpha, beta, omega, D, rows, nrows);
+                                       A.nbr, rp, ci, val, xx, bb, yy, alpha, beta, omega, D, rows, nrows, ax);
                     break;
             }
             FH_HIP(hipGetLastError());
@@ -800,15 +829,19 @@ void bcsr_dispatch_mode(Ctx& c, const Bcsr& A, BcsrMode mode, const void* x, con
     switch (mode) {
         case OP_SPMV:
             hipLaunchKernelGGL((k_bcsr<NF, TA, TV, TD, OP_SPMV>), dim3(nwg), dim3(kWG), 0, c.stream, A.nbr, rp, ci, val,
-                               xx, bb, yy, alpha, beta, omega, D, rows, nrows);
+                               xx, bb, yy, alpha, beta, omega, D, rows, nrows, ax);
             break;
         case OP_RESID:
             hipLaunchKernelGGL((k_bcsr<NF, TA, TV, TD, OP_RESID>), dim3(nwg), dim3(kWG), 0, c.stream, A.nbr, rp, ci,
-                               val, xx, bb, yy, alpha, beta, omega, D, rows, nrows);
+                               val, xx, bb, yy, alpha, beta, omega, D, rows, nrows, ax);
+            break;
+        case OP_CHEB:
+            hipLaunchKernelGGL((k_bcsr<NF, TA, TV, TD, OP_CHEB>), dim3(nwg), dim3(kWG), 0, c.stream, A.nbr, rp, ci,
+                               val, xx, bb, yy, alpha, beta, omega, D, rows, nrows, ax);
             break;
         default:
             hipLaunchKernelGGL((k_bcsr<NF, TA, TV, TD, OP_SMOOTH>), dim3(nwg), dim3(kWG), 0, c.stream, A.nbr, rp, ci,
-                               val, xx, bb, yy, alpha, beta, omega, D, rows, nrows);
+                               val, xx, bb, yy, alpha, beta, omega, D, rows, nrows, ax);
             break;
     }
     FH_HIP(hipGetLastError());
@@ -817,17 +850,17 @@ void bcsr_dispatch_mode(Ctx& c, const Bcsr& A, BcsrMode mode, const void* x, con
 template <int NF>
 void bcsr_dispatch_types(Ctx& c, const Bcsr& A, BcsrMode mode, Prec vp, const void* x, const void* b, void* y,
                          double alpha, double beta, double omega, const void* invD, Prec dp, const int* rows,
-                         int nrows) {
+                         int nrows, void* aux) {
     if (vp == Prec::F64) {
-        FH_CHECK(A.prec == Prec::F64 && (mode != OP_SMOOTH || dp == Prec::F64), -3,
+        FH_CHECK(A.prec == Prec::F64 && ((mode != OP_SMOOTH && mode != OP_CHEB) || dp == Prec::F64), -3,
                  "FP64 vectors need FP64 matrix and FP64 block inverses");
-        bcsr_dispatch_mode<NF, double, double, double>(c, A, mode, x, b, y, alpha, beta, omega, invD, rows, nrows);
+        bcsr_dispatch_mode<NF, double, double, double>(c, A, mode, x, b, y, alpha, beta, omega, invD, rows, nrows, aux);
     } else if (vp == Prec::F32) {
-        FH_CHECK(mode != OP_SMOOTH || dp == Prec::F32, -3, "FP32 smoothing needs FP32 block inverses");
+        FH_CHECK((mode != OP_SMOOTH && mode != OP_CHEB) || dp == Prec::F32, -3, "FP32 smoothing needs FP32 block inverses");
         if (A.prec == Prec::F32)
-            bcsr_dispatch_mode<NF, float, float, float>(c, A, mode, x, b, y, alpha, beta, omega, invD, rows, nrows);
+            bcsr_dispatch_mode<NF, float, float, float>(c, A, mode, x, b, y, alpha, beta, omega, invD, rows, nrows, aux);
         else if (A.prec == Prec::BF16)
-            bcsr_dispatch_mode<NF, bf16_t, float, float>(c, A, mode, x, b, y, alpha, beta, omega, invD, rows, nrows);
+            bcsr_dispatch_mode<NF, bf16_t, float, float>(c, A, mode, x, b, y, alpha, beta, omega, invD, rows, nrows, aux);
         else
             throw Error(-3, "FP32 vectors need an FP32 or BF16 matrix");
     } else
@@ -1035,7 +1068,7 @@ bool set_sliced(Ctx& c, const Bcsr& A, bool on) {
 
 static void launch_bcsr_impl(Ctx& c, const Bcsr& A, BcsrMode mode, Prec vprec, const void* x, const void* b, void* y,
                              double alpha, double beta, double omega, const void* invD, Prec dprec, const int* rows,
-                             int nrows);
+                             int nrows, void* aux);
 
 void launch_bcsr(Ctx& c, const Bcsr& A, BcsrMode mode, Prec vprec, const void* x, const void* b, void* y,
                  double alpha, double beta, double omega, const void* invD, Prec dprec, const int* rows,
@@ -1048,24 +1081,31 @@ void launch_bcsr(Ctx& c, const Bcsr& A, BcsrMode mode, Prec vprec, const void* x
         FH_HIP(hipEventCreate(&e1));
         FH_HIP(hipEventRecord(e0, c.stream));
     }
-    launch_bcsr_impl(c, A, mode, vprec, x, b, y, alpha, beta, omega, invD, dprec, rows, nrows);
+    launch_bcsr_impl(c, A, mode, vprec, x, b, y, alpha, beta, omega, invD, dprec, rows, nrows, nullptr);
     if (prof) {
         FH_HIP(hipEventRecord(e1, c.stream));
         c.prof_events.push_back({e0, e1});
     }
 }
 
+void launch_bcsr_cheb(Ctx& c, const Bcsr& A, Prec vprec, const void* x, const void* b, void* xnew, void* d,
+                      double a, double cc, const void* invD, Prec dprec) {
+    FH_CHECK(xnew != x && d != nullptr && d != x && d != xnew, -3, "Chebyshev step: x, xnew and d must differ");
+    if (A.sliced_state == 0) sliced_decide(c, A);
+    launch_bcsr_impl(c, A, OP_CHEB, vprec, x, b, xnew, a, cc, 1.0, invD, dprec, nullptr, 0, d);
+}
+
 static void launch_bcsr_impl(Ctx& c, const Bcsr& A, BcsrMode mode, Prec vprec, const void* x, const void* b, void* y,
                              double alpha, double beta, double omega, const void* invD, Prec dprec, const int* rows,
-                             int nrows) {
+                             int nrows, void* aux) {
     FH_CHECK(A.bs_r == A.bs_c, -3, "block-row kernel needs square blocks");
     switch (A.bs_r) {
-        case 1: bcsr_dispatch_types<1>(c, A, mode, vprec, x, b, y, alpha, beta, omega, invD, dprec, rows, nrows); break;
-        case 2: bcsr_dispatch_types<2>(c, A, mode, vprec, x, b, y, alpha, beta, omega, invD, dprec, rows, nrows); break;
-        case 3: bcsr_dispatch_types<3>(c, A, mode, vprec, x, b, y, alpha, beta, omega, invD, dprec, rows, nrows); break;
-        case 4: bcsr_dispatch_types<4>(c, A, mode, vprec, x, b, y, alpha, beta, omega, invD, dprec, rows, nrows); break;
-        case 6: bcsr_dispatch_types<6>(c, A, mode, vprec, x, b, y, alpha, beta, omega, invD, dprec, rows, nrows); break;
-        case 10: bcsr_dispatch_types<10>(c, A, mode, vprec, x, b, y, alpha, beta, omega, invD, dprec, rows, nrows); break;
+        case 1: bcsr_dispatch_types<1>(c, A, mode, vprec, x, b, y, alpha, beta, omega, invD, dprec, rows, nrows, aux); break;
+        case 2: bcsr_dispatch_types<2>(c, A, mode, vprec, x, b, y, alpha, beta, omega, invD, dprec, rows, nrows, aux); break;
+        case 3: bcsr_dispatch_types<3>(c, A, mode, vprec, x, b, y, alpha, beta, omega, invD, dprec, rows, nrows, aux); break;
+        case 4: bcsr_dispatch_types<4>(c, A, mode, vprec, x, b, y, alpha, beta, omega, invD, dprec, rows, nrows, aux); break;
+        case 6: bcsr_dispatch_types<6>(c, A, mode, vprec, x, b, y, alpha, beta, omega, invD, dprec, rows, nrows, aux); break;
+        case 10: bcsr_dispatch_types<10>(c, A, mode, vprec, x, b, y, alpha, beta, omega, invD, dprec, rows, nrows, aux); break;
         default: throw Error(-3, "unsupported block size " + std::to_string(A.bs_r) + " (1,2,3,4,6,10)");
     }
 }

@@ -189,15 +189,19 @@ void smoother_apply(Ctx& c, const Bcsr& A, LevelSmoother& S, Prec vp, const void
                 if (k == 0 && x_is_zero) {
                     launch_bjac_zero(c, A.nbr, nf, vp, b, work.data(), 1.0, S.invD.p, S.dprec, nullptr, 0);
                     launch_fill(c, x.n, x.data(), vp, 0.0);
-                } else
-                    launch_bcsr(c, A, OP_SMOOTH, vp, x.data(), b, work.data(), 1.0, 0.0, 1.0, S.invD.p, S.dprec,
-                                nullptr, 0);
-                if (k == 0)
                     launch_cheb_update(c, x.n, vp, work.data(), x.data(), S.d.data(), 0.0, 1.0 / theta);
-                else {
-                    const double rn = 1.0 / (2.0 * sigma - rho);
-                    launch_cheb_update(c, x.n, vp, work.data(), x.data(), S.d.data(), rn * rho, 2.0 * rn / delta);
-                    rho = rn;
+                } else {
+                    // d = a d + c D^-1 (b - A x) and x + d in one block-row pass
+                    // into work, which then becomes x
+                    double a = 0.0, cc = 1.0 / theta;
+                    if (k > 0) {
+                        const double rn = 1.0 / (2.0 * sigma - rho);
+                        a = rn * rho;
+                        cc = 2.0 * rn / delta;
+                        rho = rn;
+                    }
+                    launch_bcsr_cheb(c, A, vp, x.data(), b, work.data(), S.d.data(), a, cc, S.invD.p, S.dprec);
+                    std::swap(x.buf, work.buf);
                 }
                 x_is_zero = false;
             }
