@@ -247,7 +247,7 @@ int fh_convert(fh_ctx* ctx, int64_t n, const void* src, int sprec, void* dst, in
 int fh_profile_begin(fh_ctx* ctx, const fh_mat* A, int mode) {
     return guard([&] {
         NEED(ctx); NEED_DEV(ctx); NEED(A);
-        FH_CHECK(mode >= 0 && mode <= 2, -2, "mode must be 0, 1 or 2");
+        FH_CHECK(mode >= 0 && mode <= 3, -2, "mode must be 0, 1, 2 or 3");
         ctx->c.prof_matrix = A->A.get();
         ctx->c.prof_mode = mode;
         ctx->c.prof_events.clear();

@@ -1092,7 +1092,18 @@ void launch_bcsr_cheb(Ctx& c, const Bcsr& A, Prec vprec, const void* x, const vo
                       double a, double cc, const void* invD, Prec dprec) {
     FH_CHECK(xnew != x && d != nullptr && d != x && d != xnew, -3, "Chebyshev step: x, xnew and d must differ");
     if (A.sliced_state == 0) sliced_decide(c, A);
+    const bool prof = c.prof_matrix == (const void*)&A && c.prof_mode == (int)OP_CHEB;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (prof) {
+        FH_HIP(hipEventCreate(&e0));
+        FH_HIP(hipEventCreate(&e1));
+        FH_HIP(hipEventRecord(e0, c.stream));
+    }
     launch_bcsr_impl(c, A, OP_CHEB, vprec, x, b, xnew, a, cc, 1.0, invD, dprec, nullptr, 0, d);
+    if (prof) {
+        FH_HIP(hipEventRecord(e1, c.stream));
+        c.prof_events.push_back({e0, e1});
+    }
 }
 
 static void launch_bcsr_impl(Ctx& c, const Bcsr& A, BcsrMode mode, Prec vprec, const void* x, const void* b, void* y,

@@ -746,7 +746,8 @@ def face_features(problem):
 
 
 def profile_begin(ctx, A, mode=0):
-    """Time every launch of `mode` (0 spmv, 1 residual, 2 smoothing) on A."""
+    """Time every launch of `mode` (0 spmv, 1 residual, 2 smoothing, 3 fused
+    Chebyshev step) on A."""
     _check(lib().fh_profile_begin(ctx.h, A.h, mode))
 
 

@@ -106,9 +106,9 @@ int fh_axpby(fh_ctx* ctx, int64_t n, double a, const void* x, double b, void* y,
 int fh_convert(fh_ctx* ctx, int64_t n, const void* src, int sprec, void* dst, int dprec);
 
 /* In-situ timing of the block-row kernel: every launch of operation `mode`
- * (0 spmv, 1 residual, 2 smoothing) on matrix A is bracketed by HIP events on
- * the context's stream until fh_profile_end, which returns the number of
- * launches and their mean duration in ms.  Replaces the reference's Timer
+ * (0 spmv, 1 residual, 2 smoothing, 3 fused Chebyshev step) on matrix A is
+ * bracketed by HIP events on the context's stream until fh_profile_end,
+ * which returns the number of launches and their mean duration in ms.  Replaces the reference's Timer
  * accounting of matrix-vector products (src/Utils/Timer.h:60-123). */
 int fh_profile_begin(fh_ctx* ctx, const fh_mat* A, int mode);
 int fh_profile_end(fh_ctx* ctx, int* launches, double* mean_ms);
