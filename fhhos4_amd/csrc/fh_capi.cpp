@@ -164,6 +164,16 @@ int fh_mat_set_sliced(fh_ctx* ctx, const fh_mat* A, int on, int* active) {
     });
 }
 
+int fh_mat_layout(const fh_mat* A, int* layout) {
+    return guard([&] {
+        NEED(A); NEED(layout);
+        const fh::Bcsr& M = *A->A;
+        int v = 0;
+        if (M.sliced_state == 1 && M.sliced) v = M.sliced->kind == 1 ? 3 : (M.sliced->next ? 2 : 1);
+        *layout = v;
+    });
+}
+
 int fh_mat_export_market(fh_ctx* ctx, const fh_mat* A, const char* path) {
     return guard([&] {
         NEED(ctx); NEED(A); NEED(path);

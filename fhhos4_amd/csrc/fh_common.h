@@ -109,13 +109,18 @@ struct DVec {
 // block column of every (slice, row, slot), -1 for padding.
 // Execution copies of a block-CSR operator's values for full-matrix
 // products.  kind 0 (rows that fit a wavefront): slices of RW rows with W
-// block slots each.  kind 1 (wide rows): every row split into chunks of W
-// block slots (RW = 1), chunk_ptr[row] .. chunk_ptr[row + 1] being its
-// chunks, one wavefront per row.
+// block slots each, over the nrows rows of this part; rowmap (empty: the
+// identity) gives the operator row of part row r, and `next` holds a
+// further part (rows split by length into classes of their own width).
+// kind 1 (wide rows): every row split into chunks of W block slots
+// (RW = 1), chunk_ptr[row] .. chunk_ptr[row + 1] being its chunks, one
+// wavefront per row.
 struct Sliced {
     int kind = 0;
     int W = 0, RW = 0, cw = 1, nslices = 0;  // kind 1: nslices = number of chunks
-    DevBuf cols, vals, chunk_ptr;
+    int nrows = 0;
+    DevBuf cols, vals, chunk_ptr, rowmap;
+    std::unique_ptr<Sliced> next;
     int act(int nf) const { return RW * W * (nf / cw); }
 };
 

@@ -265,7 +265,8 @@ template <int NF, class TA, class TV, class TD, int MODE>
 __global__ __launch_bounds__(kWG) void k_sliced(int nbr, int nslices, int W, int RW, const int* __restrict__ cols,
                                                 const TA* __restrict__ sv, const TV* x, const TV* __restrict__ b,
                                                 TV* y, double alpha, double beta, double omega,
-                                                const TD* __restrict__ invD, TV* __restrict__ aux) {
+                                                const TD* __restrict__ invD, TV* __restrict__ aux,
+                                                const int* __restrict__ rowmap) {
     using AT = typename Acc<TV>::T;
     constexpr int CW = NF % 2 == 0 ? 2 : 1;
     constexpr int JP = NF / CW;
@@ -310,8 +311,8 @@ __global__ __launch_bounds__(kWG) void k_sliced(int nbr, int nslices, int W, int
     if (fin) {
         for (int q = 0; q < LPR; q++) sum += red[wave][r * LPR + q][i];
     }
-    const bool out = fin && row < nbr;
-    const int64_t e = (int64_t)row * NF + i;
+    const bool out = fin && row < nbr;  // nbr: rows of this part
+    const int64_t e = (int64_t)(out && rowmap ? rowmap[row] : row) * NF + i;
     if (MODE == OP_SPMV) {
         if (out) {
             AT v = (AT)alpha * sum;
@@ -462,7 +463,7 @@ __global__ void k_to_wide(int nchunks, int W, const int* __restrict__ chunk_row,
 // Block-CSR values -> sliced values (one thread per slice lane).
 template <int NF, class TA>
 __global__ void k_to_sliced(int nbr, int nslices, int W, int RW, const int* __restrict__ rp,
-                            const TA* __restrict__ val, TA* __restrict__ sv) {
+                            const TA* __restrict__ val, TA* __restrict__ sv, const int* __restrict__ rowmap) {
     constexpr int CW = NF % 2 == 0 ? 2 : 1;
     constexpr int JP = NF / CW;
     const int LPR = W * JP, ACT = RW * LPR;
@@ -470,8 +471,9 @@ __global__ void k_to_sliced(int nbr, int nslices, int W, int RW, const int* __re
     if (t >= (int64_t)nslices * ACT) return;
     const int s = (int)(t / ACT), lane = (int)(t - (int64_t)s * ACT);
     const int r = lane / LPR, rem = lane - r * LPR, bs = rem / JP, jp = rem - bs * JP;
-    const int row = s * RW + r;
-    const bool ok = row < nbr && bs < rp[row + 1] - rp[row];
+    const int prow = s * RW + r;
+    const int row = prow < nbr ? (rowmap ? rowmap[prow] : prow) : 0;
+    const bool ok = prow < nbr && bs < rp[row + 1] - rp[row];
     const int64_t blk = ok ? (int64_t)rp[row] + bs : 0;
     for (int i = 0; i < NF; i++)
         for (int u = 0; u < CW; u++) {
@@ -770,34 +772,36 @@ void bcsr_dispatch_mode(Ctx& c, const Bcsr& A, BcsrMode mode, const void* x, con
             return;
         }
     }
-    if (!rows && A.sliced_state == 1 && A.sliced->kind == 0) {
-        const Sliced& S = *A.sliced;
-        const bool aligned = S.cw == 1 || ((uintptr_t)x % (2 * sizeof(TV))) == 0;
-        if (aligned) {
+    if (!rows && A.sliced_state == 1 && A.sliced->kind == 0 &&
+        (A.sliced->cw == 1 || ((uintptr_t)x % (2 * sizeof(TV))) == 0)) {
+        for (const Sliced* P = A.sliced.get(); P; P = P->next.get()) {
+            const Sliced& S = *P;
             const int nwg = (S.nslices + kSlicedWaves - 1) / kSlicedWaves;
             auto sc = S.cols.as<int>();
             auto sv = S.vals.as<TA>();
+            auto rm = S.rowmap.as<int>();
+            const int nrp = S.nrows;
             switch (mode) {
                 case OP_SPMV:
-                    hipLaunchKernelGGL((k_sliced<NF, TA, TV, TD, OP_SPMV>), dim3(nwg), dim3(kWG), 0, c.stream, A.nbr,
-                                       S.nslices, S.W, S.RW, sc, sv, xx, bb, yy, alpha, beta, omega, D, ax);
+                    hipLaunchKernelGGL((k_sliced<NF, TA, TV, TD, OP_SPMV>), dim3(nwg), dim3(kWG), 0, c.stream, nrp,
+                                       S.nslices, S.W, S.RW, sc, sv, xx, bb, yy, alpha, beta, omega, D, ax, rm);
                     break;
                 case OP_RESID:
-                    hipLaunchKernelGGL((k_sliced<NF, TA, TV, TD, OP_RESID>), dim3(nwg), dim3(kWG), 0, c.stream, A.nbr,
-                                       S.nslices, S.W, S.RW, sc, sv, xx, bb, yy, alpha, beta, omega, D, ax);
+                    hipLaunchKernelGGL((k_sliced<NF, TA, TV, TD, OP_RESID>), dim3(nwg), dim3(kWG), 0, c.stream, nrp,
+                                       S.nslices, S.W, S.RW, sc, sv, xx, bb, yy, alpha, beta, omega, D, ax, rm);
                     break;
                 case OP_CHEB:
-                    hipLaunchKernelGGL((k_sliced<NF, TA, TV, TD, OP_CHEB>), dim3(nwg), dim3(kWG), 0, c.stream, A.nbr,
-                                       S.nslices, S.W, S.RW, sc, sv, xx, bb, yy, alpha, beta, omega, D, ax);
+                    hipLaunchKernelGGL((k_sliced<NF, TA, TV, TD, OP_CHEB>), dim3(nwg), dim3(kWG), 0, c.stream, nrp,
+                                       S.nslices, S.W, S.RW, sc, sv, xx, bb, yy, alpha, beta, omega, D, ax, rm);
                     break;
                 default:
-                    hipLaunchKernelGGL((k_sliced<NF, TA, TV, TD, OP_SMOOTH>), dim3(nwg), dim3(kWG), 0, c.stream, A.nbr,
-                                       S.nslices, S.W, S.RW, sc, sv, xx, bb, yy, alpha, beta, omega, D, ax);
+                    hipLaunchKernelGGL((k_sliced<NF, TA, TV, TD, OP_SMOOTH>), dim3(nwg), dim3(kWG), 0, c.stream, nrp,
+                                       S.nslices, S.W, S.RW, sc, sv, xx, bb, yy, alpha, beta, omega, D, ax, rm);
                     break;
             }
             FH_HIP(hipGetLastError());
-            return;
         }
+        return;
     }
     if constexpr (NF % 2 == 0) {
         const bool aligned = ((uintptr_t)x % (2 * sizeof(TV))) == 0;
@@ -960,8 +964,9 @@ template <int NF, class TA>
 void sliced_convert(Ctx& c, const Bcsr& A, Sliced& S) {
     const int64_t n = (int64_t)S.nslices * S.act(NF);
     if (n == 0) return;
-    hipLaunchKernelGGL((k_to_sliced<NF, TA>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream, A.nbr,
-                       S.nslices, S.W, S.RW, A.rowptr.as<int>(), A.vals.as<TA>(), S.vals.as<TA>());
+    hipLaunchKernelGGL((k_to_sliced<NF, TA>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream, S.nrows,
+                       S.nslices, S.W, S.RW, A.rowptr.as<int>(), A.vals.as<TA>(), S.vals.as<TA>(),
+                       S.rowmap.as<int>());
     FH_HIP(hipGetLastError());
 }
 
@@ -972,17 +977,27 @@ void sliced_convert_prec(Ctx& c, const Bcsr& A, Sliced& S) {
     else sliced_convert<NF, bf16_t>(c, A, S);
 }
 
-void sliced_build(Ctx& c, const Bcsr& A, int W, int RW, int cw) {
+// One part of the sliced layout over `rows` (ascending operator rows; null:
+// all rows, no row map), W = its widest row.
+std::unique_ptr<Sliced> sliced_part(Ctx& c, const Bcsr& A, int W, int RW, int cw, const std::vector<int>* rows) {
     auto S = std::make_unique<Sliced>();
     const int nf = A.bs_r;
     S->W = W; S->RW = RW; S->cw = cw;
-    S->nslices = (A.nbr + RW - 1) / RW;
+    S->nrows = rows ? (int)rows->size() : A.nbr;
+    S->nslices = (S->nrows + RW - 1) / RW;
     std::vector<int> hc((size_t)S->nslices * RW * W, -1);
-    for (int i = 0; i < A.nbr; i++)
+    for (int k = 0; k < S->nrows; k++) {
+        const int i = rows ? (*rows)[k] : k;
+        FH_CHECK(A.h_rowptr[i + 1] - A.h_rowptr[i] <= W, -10, "sliced part narrower than a row");
         for (int p = A.h_rowptr[i]; p < A.h_rowptr[i + 1]; p++)
-            hc[(size_t)(i / RW) * RW * W + (size_t)(i % RW) * W + (p - A.h_rowptr[i])] = A.h_colidx[p];
+            hc[(size_t)(k / RW) * RW * W + (size_t)(k % RW) * W + (p - A.h_rowptr[i])] = A.h_colidx[p];
+    }
     S->cols.alloc(sizeof(int) * hc.size());
     FH_HIP(hipMemcpyAsync(S->cols.p, hc.data(), sizeof(int) * hc.size(), hipMemcpyHostToDevice, c.stream));
+    if (rows) {
+        S->rowmap.alloc(sizeof(int) * std::max<size_t>(1, rows->size()));
+        FH_HIP(hipMemcpyAsync(S->rowmap.p, rows->data(), sizeof(int) * rows->size(), hipMemcpyHostToDevice, c.stream));
+    }
     // layout [slice][i < nf][lane < ACT][cw]: k_to_sliced writes and
     // k_sliced reads indices below nslices * nf * ACT * cw
     const size_t nvals = (size_t)S->nslices * nf * S->act(nf) * cw;
@@ -998,9 +1013,40 @@ void sliced_build(Ctx& c, const Bcsr& A, int W, int RW, int cw) {
         case 6: sliced_convert_prec<6>(c, A, *S); break;
         default: sliced_convert_prec<10>(c, A, *S); break;
     }
-    c.sync();  // host column array released on return
-    A.sliced = std::move(S);
+    c.sync();  // host arrays released on return
+    return S;
+}
+
+// Rows per wavefront for rows of at most W blocks.
+int sliced_rw(int nf, int cw, int W) { return std::min(64 / (W * (nf / cw)), 64 / nf); }
+
+void sliced_build(Ctx& c, const Bcsr& A, int W, int RW, int cw) {
+    A.sliced = sliced_part(c, A, W, RW, cw, nullptr);
     A.sliced_state = 1;
+}
+
+// Rows split by length into two classes (length <= t and > t), each part as
+// wide as its widest row.  Returns the padded slot count of the best split
+// (t in *t_best), or -1 when no split has two nonempty classes.
+int64_t sliced_split(const Bcsr& A, int cw, int& t_best) {
+    const int nf = A.bs_r;
+    int W = 0;
+    for (int i = 0; i < A.nbr; i++) W = std::max(W, A.h_rowptr[i + 1] - A.h_rowptr[i]);
+    std::vector<int64_t> cnt(W + 1, 0);
+    for (int i = 0; i < A.nbr; i++) cnt[A.h_rowptr[i + 1] - A.h_rowptr[i]]++;
+    int64_t best = -1;
+    for (int t = 1; t < W; t++) {
+        int64_t n1 = 0, n2 = 0;
+        int w1 = 0;
+        for (int l = 0; l <= t; l++)
+            if (cnt[l]) { n1 += cnt[l]; w1 = std::max(w1, l); }
+        for (int l = t + 1; l <= W; l++) n2 += cnt[l];
+        if (n1 == 0 || n2 == 0 || w1 == 0) continue;
+        const int rw1 = sliced_rw(nf, cw, w1), rw2 = sliced_rw(nf, cw, W);
+        const int64_t slots = (n1 + rw1 - 1) / rw1 * rw1 * w1 + (n2 + rw2 - 1) / rw2 * rw2 * W;
+        if (best < 0 || slots < best) { best = slots; t_best = t; }
+    }
+    return best;
 }
 
 // FHGPU_SLICED=0 keeps every product on the block-CSR kernels.
@@ -1028,8 +1074,31 @@ void sliced_decide(Ctx& c, const Bcsr& A) {
     }
     if (sliced_geometry(A, W, RW, cw)) {
         const int64_t nslices = (A.nbr + RW - 1) / RW;
-        if ((double)nslices * RW * W > 1.25 * (double)A.nnzb) A.sliced_state = -1;
-        else sliced_build(c, A, W, RW, cw);
+        if ((double)nslices * RW * W <= 1.25 * (double)A.nnzb) {
+            sliced_build(c, A, W, RW, cw);
+            return;
+        }
+        // uneven rows (the prolongations: 1-7 blocks): two parts, short and
+        // long rows, each as wide as its widest row, on operators large
+        // enough for the second launch to pay
+        constexpr int kSplitMinRows = 32768;
+        int t = 0;
+        const int64_t slots = A.nbr >= kSplitMinRows ? sliced_split(A, cw, t) : -1;
+        if (slots < 0 || (double)slots > 1.25 * (double)A.nnzb) {
+            A.sliced_state = -1;
+            return;
+        }
+        std::vector<int> shortr, longr;
+        int w1 = 0;
+        for (int i = 0; i < A.nbr; i++) {
+            const int len = A.h_rowptr[i + 1] - A.h_rowptr[i];
+            if (len <= t) { shortr.push_back(i); w1 = std::max(w1, len); }
+            else longr.push_back(i);
+        }
+        auto first = sliced_part(c, A, w1, sliced_rw(A.bs_r, cw, w1), cw, &shortr);
+        first->next = sliced_part(c, A, W, RW, cw, &longr);
+        A.sliced = std::move(first);
+        A.sliced_state = 1;
         return;
     }
     // rows wider than a wavefront: the chunked layout, under the same bound

@@ -59,6 +59,7 @@ _SIGS = {
     "fh_mat_device_arrays": [_vp, _pp, _pp, _pp],
     "fh_mat_blocks": [_vp, _i32p, _i32p, _f64p],
     "fh_mat_set_sliced": [_vp, _vp, C.c_int, C.POINTER(C.c_int)],
+    "fh_mat_layout": [_vp, C.POINTER(C.c_int)],
     "fh_nn_create": [_vp, C.c_int, C.c_int, C.c_uint64, _pp],
     "fh_nn_destroy": [_vp],
     "fh_nn_num_params": [_vp, C.POINTER(C.c_int64)],
@@ -292,6 +293,12 @@ class Matrix:
         a = C.c_int()
         _check(lib().fh_mat_set_sliced(self.ctx.h, self.h, int(bool(on)), C.byref(a)))
         return bool(a.value)
+
+    def layout(self):
+        """0 block CSR (or undecided), 1 sliced, 2 sliced in two parts, 3 chunked."""
+        a = C.c_int()
+        _check(lib().fh_mat_layout(self.h, C.byref(a)))
+        return a.value
 
     def convert(self, prec):
         h = C.c_void_p()

@@ -70,6 +70,11 @@ int fh_mat_device_arrays(const fh_mat* A, const int** rowptr, const int** colidx
  * (src/Utils/Types.h:11-22); the copy is a GPU execution detail of the same
  * operator. */
 int fh_mat_set_sliced(fh_ctx* ctx, const fh_mat* A, int on, int* active);
+/* Execution layout of A's full-matrix products: 0 block CSR (or not decided
+ * yet), 1 sliced copy, 2 sliced copy in two parts (short and long rows, each
+ * with its own width: uneven rows such as a prolongation's), 3 chunked copy.
+ * No reference counterpart (an execution detail). */
+int fh_mat_layout(const fh_mat* A, int* layout);
 
 /* -------------------------------------------- exported linear system */
 /* Writes the scalar operator as Matrix Market text, byte-compatible with the

@@ -251,3 +251,54 @@ def test_wide_row_layout_modes(ctx, nf):
         if prec == "f64":
             for k, (u, w) in enumerate(zip(res[True], want)):
                 assert np.linalg.norm(u - w) <= 1e-12 * np.linalg.norm(w), k
+
+
+@pytest.mark.parametrize("nf", [1, 2, 3, 6, 10])
+def test_sliced_two_part_layout_modes(ctx, nf):
+    """Rows of two length classes (1-3 blocks and nearly the widest that fits
+    a wavefront, like a prolongation's uneven rows) pad a one-width sliced
+    copy by far more than 25%: the first product splits the rows into two
+    parts, each with its own width and a row map.  The three modes equal the
+    block-CSR kernel's results (FP64, FP32, BF16-stored) and the oracle's."""
+    import scipy.sparse as sp
+    rng = np.random.default_rng(70 + nf)
+    cw = 2 if nf % 2 == 0 else 1
+    wfit = 64 // (nf // cw)
+    nbr = 33001  # the split layout is used from 32768 block rows on
+    lens = np.where(rng.random(nbr) < 0.5, rng.integers(1, 4, size=nbr), rng.integers(max(wfit - 1, 4), wfit + 1, size=nbr))
+    rp = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    offsets = np.arange(1, 200)
+
+    def row_cols(i, k):  # the diagonal and k - 1 distinct nearby columns, sorted
+        c = (i + rng.choice(offsets, size=k - 1, replace=False) * rng.choice([-1, 1], size=k - 1)) % nbr
+        c = np.unique(c[c != i])
+        while len(c) < k - 1:  # rare collisions of +o and -o' modulo nbr
+            c = np.unique(np.concatenate([c, [(i + 200 + len(c)) % nbr]]))
+        return np.sort(np.concatenate([[i], c[:k - 1]]))
+
+    ci = np.concatenate([row_cols(i, k) for i, k in enumerate(lens)]).astype(np.int32)
+    v = rng.standard_normal((int(rp[-1]), nf, nf))
+    for i in range(nbr):
+        for p in range(rp[i], rp[i + 1]):
+            if ci[p] == i:
+                v[p] += 4.0 * nf * lens[i] * np.eye(nf)
+    A = sp.bsr_matrix((v, ci, rp), shape=(nbr * nf, nbr * nf)).tocsr()
+    x, y0, b = (rng.standard_normal(nbr * nf) for _ in range(3))
+    inv = O.block_inverse(A, nf)
+    want = [1.5 * O.spmv(A, x) - 0.5 * y0, b - O.spmv(A, x),
+            O.bj_sweep(A, nf, inv, 0.7, b, O.bj_sweep(A, nf, inv, 0.7, b, x))]
+    for prec, vdt, tol in (("f64", torch.float64, 1e-13), ("f32", torch.float32, 2e-6), ("bf16", torch.float32, 2e-6)):
+        res = {}
+        for auto in (False, True):
+            M64 = G.Matrix.from_csr(ctx, A, nf)
+            M = M64 if prec == "f64" else M64.convert(prec)
+            if not auto:
+                assert not M.set_sliced(False)
+            bj = G.BlockJacobi(ctx, M64, 0.7, dprec="f64" if prec == "f64" else "f32")
+            res[auto] = _modes(ctx, M, x, y0, b, vdt, bj)
+            assert M.layout() == (2 if auto else 0), (prec, M.layout())
+        for k, (u, w) in enumerate(zip(res[False], res[True])):
+            assert np.linalg.norm(u - w) <= tol * np.linalg.norm(u), (prec, k)
+        if prec == "f64":
+            for k, (u, w) in enumerate(zip(res[True], want)):
+                assert np.linalg.norm(u - w) <= 1e-12 * np.linalg.norm(w), k
