@@ -65,7 +65,9 @@ int fh_mat_device_arrays(const fh_mat* A, const int** rowptr, const int** colidx
  * tells whether A has one.  on = 0 drops it and keeps A on the block-CSR
  * kernels.  Without a call the library builds one at the first full-matrix
  * product when the matrix has at least 4096 block rows and padding adds at
- * most 25% to the values (the chunked copy: FP64 or FP32 storage only).  No
+ * most 25% to the values (the chunked copy: FP64, FP32, or BF16 with even
+ * n_f; uneven rows of 32768 or more block rows: a sliced copy in two parts,
+ * short and long rows).  No
  * reference counterpart: the reference has one CSR storage
  * (src/Utils/Types.h:11-22); the copy is a GPU execution detail of the same
  * operator. */
