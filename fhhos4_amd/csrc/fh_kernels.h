@@ -31,9 +31,10 @@ void launch_bcsr_cheb(Ctx& c, const Bcsr& A, Prec vprec, const void* x, const vo
 // first full-matrix product (rows fit, >= 4096 block rows, <= 25% padding).
 bool set_sliced(Ctx& c, const Bcsr& A, bool on);
 
-// y = omega * D^-1 b over all block rows (smoothing from a zero guess).
+// y = omega * D^-1 b over all block rows (smoothing from a zero guess); the
+// same values also into y2 when it is not null (first Chebyshev step).
 void launch_bjac_zero(Ctx& c, int nbr, int nf, Prec vprec, const void* b, void* y, double omega,
-                      const void* invD, Prec dprec, const int* rows, int nrows);
+                      const void* invD, Prec dprec, const int* rows, int nrows, void* y2 = nullptr);
 
 // Batched inverse of the diagonal blocks of A (FP64 values, Gauss-Jordan
 // with partial pivoting per block); diag[i] = block index of A_ii.  Writes

@@ -485,7 +485,7 @@ __global__ void k_to_sliced(int nbr, int nslices, int W, int RW, const int* __re
 
 template <class TV, class TD>
 __global__ void k_bjac_zero(int count, int nf, const TV* __restrict__ b, TV* __restrict__ y, double omega,
-                            const TD* __restrict__ invD, const int* __restrict__ rows) {
+                            const TD* __restrict__ invD, const int* __restrict__ rows, TV* __restrict__ y2) {
     using AT = typename Acc<TV>::T;
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (int64_t)count * nf) return;
@@ -495,7 +495,9 @@ __global__ void k_bjac_zero(int count, int nf, const TV* __restrict__ b, TV* __r
     const TV* bb = b + (int64_t)br * nf;
     AT z = 0;
     for (int j = 0; j < nf; j++) z += (AT)ldv(D + j) * (AT)bb[j];
-    y[(int64_t)br * nf + i] = (TV)((AT)omega * z);
+    const TV v = (TV)((AT)omega * z);
+    y[(int64_t)br * nf + i] = v;
+    if (y2) y2[(int64_t)br * nf + i] = v;  // first Chebyshev step: iterate and direction
 }
 
 template <class TD>
@@ -1191,17 +1193,17 @@ static void launch_bcsr_impl(Ctx& c, const Bcsr& A, BcsrMode mode, Prec vprec, c
 }
 
 void launch_bjac_zero(Ctx& c, int nbr, int nf, Prec vp, const void* b, void* y, double omega, const void* invD,
-                      Prec dp, const int* rows, int nrows) {
+                      Prec dp, const int* rows, int nrows, void* y2) {
     int count = rows ? nrows : nbr;
     if (count <= 0) return;
     int64_t nt = (int64_t)count * nf;
     int g = (int)((nt + 255) / 256);
     if (vp == Prec::F64 && dp == Prec::F64)
         hipLaunchKernelGGL((k_bjac_zero<double, double>), dim3(g), dim3(256), 0, c.stream, count, nf, (const double*)b,
-                           (double*)y, omega, (const double*)invD, rows);
+                           (double*)y, omega, (const double*)invD, rows, (double*)y2);
     else if (vp == Prec::F32 && dp == Prec::F32)
         hipLaunchKernelGGL((k_bjac_zero<float, float>), dim3(g), dim3(256), 0, c.stream, count, nf, (const float*)b,
-                           (float*)y, omega, (const float*)invD, rows);
+                           (float*)y, omega, (const float*)invD, rows, (float*)y2);
     else
         throw Error(-3, "unsupported precision pair for block Jacobi");
     FH_HIP(hipGetLastError());

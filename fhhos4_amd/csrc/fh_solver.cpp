@@ -187,9 +187,9 @@ void smoother_apply(Ctx& c, const Bcsr& A, LevelSmoother& S, Prec vp, const void
             double rho = 1.0 / sigma;
             for (int k = 0; k < S.spec.degree; k++) {
                 if (k == 0 && x_is_zero) {
-                    launch_bjac_zero(c, A.nbr, nf, vp, b, work.data(), 1.0, S.invD.p, S.dprec, nullptr, 0);
-                    launch_fill(c, x.n, x.data(), vp, 0.0);
-                    launch_cheb_update(c, x.n, vp, work.data(), x.data(), S.d.data(), 0.0, 1.0 / theta);
+                    // from x = 0: d = D^-1 b / theta and x = d in one pass
+                    launch_bjac_zero(c, A.nbr, nf, vp, b, x.data(), 1.0 / theta, S.invD.p, S.dprec, nullptr, 0,
+                                     S.d.data());
                 } else {
                     // d = a d + c D^-1 (b - A x) and x + d in one block-row pass
                     // into work, which then becomes x
