@@ -343,8 +343,7 @@ public:
     void apply_eager(const double* r, double* z) {
         MgLevel& F = L[0];
         if (vp_ == Prec::F64) {
-            copy(c_, F.n, Prec::F64, r, F.b.data());
-            cycle(0, F.b.data(), F.x, true, nullptr);
+            cycle(0, r, F.x, true, nullptr);  // the cycle only reads its right-hand side
             copy(c_, F.n, Prec::F64, F.x.data(), z);
         } else {
             launch_convert(c_, F.n, r, Prec::F64, F.b.data(), vp_);
