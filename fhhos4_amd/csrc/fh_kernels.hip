@@ -14,7 +14,11 @@
 // instead (k_sliced): lane = (row, block slot, column pair), so every lane
 // gathers its two x entries once and every load instruction of the
 // wavefront reads one contiguous run of values.  Row sums are reduced
-// through LDS.
+// through LDS.  Operators with uneven rows (the prolongations) get the
+// sliced copy in two parts, short and long rows, each with a row map;
+// operators with rows wider than a wavefront (the Galerkin coarse levels)
+// get a chunked copy with one wavefront per row (k_wide).  The smoothing
+// modes include a fused Chebyshev step (OP_CHEB).
 #include "fh_kernels.h"
 
 #include <algorithm>
