@@ -50,8 +50,6 @@ void launch_fill(Ctx& c, int64_t n, void* x, Prec p, double v);
 // (hb, ic, id).
 void launch_axpby(Ctx& c, int64_t n, Prec p, const void* x, void* y, double ha, int ia, int ib, double hb,
                   int ic, int id);
-// Chebyshev smoothing step: d = a d + c (t - x); x += d.
-void launch_cheb_update(Ctx& c, int64_t n, Prec p, const void* t, void* x, void* d, double a, double cc);
 // x += a d ; r -= a q with a = s[inum] / s[iden]  (CG / FCG update).
 void launch_cg_update(Ctx& c, int64_t n, Prec p, void* x, const void* d, void* r, const void* q, int inum, int iden,
                       double sign);

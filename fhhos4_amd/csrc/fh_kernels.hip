@@ -575,18 +575,6 @@ __global__ void k_axpby(int64_t n, const T* __restrict__ x, T* __restrict__ y, c
 // Chebyshev smoothing step: d = a d + c (t - x); x = x + d, where
 // t = x + D^-1 (b - A x) was produced by the block-row kernel.
 template <class T>
-__global__ void k_cheb_update(int64_t n, const T* __restrict__ t, T* __restrict__ x, T* __restrict__ d, double a,
-                              double c) {
-    using AT = typename Acc<T>::T;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        const AT xv = (AT)x[i];
-        const AT dv = (a != 0.0 ? (AT)a * (AT)d[i] : (AT)0) + (AT)c * ((AT)t[i] - xv);
-        d[i] = (T)dv;
-        x[i] = (T)(xv + dv);
-    }
-}
-
-template <class T>
 __global__ void k_cg_update(int64_t n, T* __restrict__ x, const T* __restrict__ d, T* __restrict__ r,
                             const T* __restrict__ q, const double* s, int inum, int iden, double sign) {
     using AT = typename Acc<T>::T;
@@ -1264,20 +1252,6 @@ void launch_fill(Ctx& c, int64_t n, void* x, Prec p, double v) {
     if (p == Prec::F64) hipLaunchKernelGGL((k_fill<double>), dim3(g), dim3(256), 0, c.stream, n, (double*)x, v);
     else if (p == Prec::F32) hipLaunchKernelGGL((k_fill<float>), dim3(g), dim3(256), 0, c.stream, n, (float*)x, v);
     else hipLaunchKernelGGL((k_fill<bf16_t>), dim3(g), dim3(256), 0, c.stream, n, (bf16_t*)x, v);
-    FH_HIP(hipGetLastError());
-}
-
-void launch_cheb_update(Ctx& c, int64_t n, Prec p, const void* t, void* x, void* d, double a, double cc) {
-    if (n <= 0) return;
-    int g = grid_for(n);
-    if (p == Prec::F64)
-        hipLaunchKernelGGL((k_cheb_update<double>), dim3(g), dim3(256), 0, c.stream, n, (const double*)t, (double*)x,
-                           (double*)d, a, cc);
-    else if (p == Prec::F32)
-        hipLaunchKernelGGL((k_cheb_update<float>), dim3(g), dim3(256), 0, c.stream, n, (const float*)t, (float*)x,
-                           (float*)d, a, cc);
-    else
-        throw Error(-3, "Chebyshev smoothing runs on FP64 or FP32 vectors");
     FH_HIP(hipGetLastError());
 }
 
