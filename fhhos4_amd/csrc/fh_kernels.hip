@@ -489,7 +489,8 @@ __global__ void k_to_sliced(int nbr, int nslices, int W, int RW, const int* __re
 
 template <class TV, class TD>
 __global__ void k_bjac_zero(int count, int nf, const TV* __restrict__ b, TV* __restrict__ y, double omega,
-                            const TD* __restrict__ invD, const int* __restrict__ rows, TV* __restrict__ y2) {
+                            const TD* __restrict__ invD, const int* __restrict__ rows, TV* __restrict__ y2,
+                            int vec2) {
     using AT = typename Acc<TV>::T;
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (int64_t)count * nf) return;
@@ -498,7 +499,19 @@ __global__ void k_bjac_zero(int count, int nf, const TV* __restrict__ b, TV* __r
     const TD* D = invD + ((int64_t)br * nf + i) * nf;
     const TV* bb = b + (int64_t)br * nf;
     AT z = 0;
-    for (int j = 0; j < nf; j++) z += (AT)ldv(D + j) * (AT)bb[j];
+    if (vec2) {
+        // even block size and 2-value-aligned b and invD: 2-value loads
+        for (int j = 0; j < nf; j += 2) {
+            TD d0, d1;
+            TV b0, b1;
+            ld2(D + j, d0, d1);
+            ld2(bb + j, b0, b1);
+            z += (AT)d0 * (AT)b0;
+            z += (AT)d1 * (AT)b1;
+        }
+    } else {
+        for (int j = 0; j < nf; j++) z += (AT)ldv(D + j) * (AT)bb[j];
+    }
     const TV v = (TV)((AT)omega * z);
     y[(int64_t)br * nf + i] = v;
     if (y2) y2[(int64_t)br * nf + i] = v;  // first Chebyshev step: iterate and direction
@@ -1190,12 +1203,14 @@ void launch_bjac_zero(Ctx& c, int nbr, int nf, Prec vp, const void* b, void* y, 
     if (count <= 0) return;
     int64_t nt = (int64_t)count * nf;
     int g = (int)((nt + 255) / 256);
+    const size_t vb = 2 * prec_bytes(vp), db = 2 * prec_bytes(dp);
+    const int vec2 = nf % 2 == 0 && (uintptr_t)b % vb == 0 && (uintptr_t)invD % db == 0;
     if (vp == Prec::F64 && dp == Prec::F64)
         hipLaunchKernelGGL((k_bjac_zero<double, double>), dim3(g), dim3(256), 0, c.stream, count, nf, (const double*)b,
-                           (double*)y, omega, (const double*)invD, rows, (double*)y2);
+                           (double*)y, omega, (const double*)invD, rows, (double*)y2, vec2);
     else if (vp == Prec::F32 && dp == Prec::F32)
         hipLaunchKernelGGL((k_bjac_zero<float, float>), dim3(g), dim3(256), 0, c.stream, count, nf, (const float*)b,
-                           (float*)y, omega, (const float*)invD, rows, (float*)y2);
+                           (float*)y, omega, (const float*)invD, rows, (float*)y2, vec2);
     else
         throw Error(-3, "unsupported precision pair for block Jacobi");
     FH_HIP(hipGetLastError());
