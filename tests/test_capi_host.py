@@ -99,3 +99,15 @@ def test_invalid_arguments_are_reported(hctx):
         G.Matrix.from_csr(hctx, A, 2)
     with pytest.raises(G.FhError, match="smoother|cycle"):
         s = G.Solver(hctx, solver="cg", prec="mg", cycle="X")
+
+
+def test_execution_layout_query(hctx):
+    """A host matrix has no execution copy (layout 0, block CSR); asking for
+    one needs a device context; a null output pointer is an argument error."""
+    import ctypes as C
+    d = load("tet4_k2")
+    A = G.Matrix.from_csr(hctx, csr(d, "A"), 6)
+    assert A.layout() == 0
+    with pytest.raises(G.FhError, match="device"):
+        A.set_sliced(True)
+    assert G.lib().fh_mat_layout(A.h, None) != 0
