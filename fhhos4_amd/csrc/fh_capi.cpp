@@ -435,7 +435,7 @@ int fh_solver_create(fh_ctx* ctx, const fh_solver_args* a, fh_solver** out) {
         if (a->post_smoother) s.post_smoother = a->post_smoother;
         s.relax = a->relax > 0 ? a->relax : 1.0;
         s.tol = a->tol > 0 ? a->tol : 1e-8;
-        s.max_iter = a->max_iter > 0 ? a->max_iter : 200;
+        s.max_iter = a->max_iter >= 0 ? a->max_iter : 200;  // -1: the reference's default (ProgramArguments.h:95)
         s.max_coarse = a->max_coarse > 0 ? a->max_coarse : 1000;
         s.theta = a->theta > 0 ? a->theta : 0.25;
         s.mg_prec = P(a->mg_prec);

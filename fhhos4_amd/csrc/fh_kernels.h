@@ -30,6 +30,9 @@ void launch_bcsr_cheb(Ctx& c, const Bcsr& A, Prec vprec, const void* x, const vo
 // A does not fit one wavefront).  Without a call the layout is chosen at the
 // first full-matrix product (rows fit, >= 4096 block rows, <= 25% padding).
 bool set_sliced(Ctx& c, const Bcsr& A, bool on);
+// Counter of execution-layout changes (built, dropped or decided) of any
+// matrix; a graph captured before a change must not be replayed after it.
+uint64_t layout_epoch();
 
 // y = omega * D^-1 b over all block rows (smoothing from a zero guess); the
 // same values also into y2 when it is not null (first Chebyshev step).

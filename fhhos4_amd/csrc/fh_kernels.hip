@@ -21,6 +21,8 @@
 // modes include a fused Chebyshev step (OP_CHEB).
 #include "fh_kernels.h"
 
+#include <atomic>
+
 #include <algorithm>
 #include <type_traits>
 #include <cstdlib>
@@ -1057,6 +1059,10 @@ int64_t sliced_split(const Bcsr& A, int cw, int& t_best) {
 }
 
 // FHGPU_SLICED=0 keeps every product on the block-CSR kernels.
+// Counts changes of any matrix's execution layout: a captured graph whose
+// kernels read a layout copy is replayed only while the count is unchanged.
+std::atomic<uint64_t> g_layout_epoch{0};
+
 bool sliced_enabled() {
     static const bool on = [] {
         const char* e = std::getenv("FHGPU_SLICED");
@@ -1074,6 +1080,7 @@ void sliced_decide(Ctx& c, const Bcsr& A) {
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     FH_HIP(hipStreamIsCapturing(c.stream, &cs));
     if (cs != hipStreamCaptureStatusNone) return;
+    g_layout_epoch.fetch_add(1);
     int W, RW, cw;
     if (!sliced_enabled() || A.nbr < 4096) {
         A.sliced_state = -1;
@@ -1123,12 +1130,14 @@ void sliced_decide(Ctx& c, const Bcsr& A) {
 
 bool set_sliced(Ctx& c, const Bcsr& A, bool on) {
     if (!on) {
+        if (A.sliced) g_layout_epoch.fetch_add(1);  // captured graphs may hold the dropped copy
         A.sliced.reset();
         A.sliced_state = -1;
         return false;
     }
     if (A.sliced_state == 1) return true;
     int W, RW, cw;
+    g_layout_epoch.fetch_add(1);
     if (sliced_geometry(A, W, RW, cw)) {
         sliced_build(c, A, W, RW, cw);
         return true;
@@ -1141,6 +1150,8 @@ bool set_sliced(Ctx& c, const Bcsr& A, bool on) {
     A.sliced_state = -1;
     return false;
 }
+
+uint64_t layout_epoch() { return g_layout_epoch.load(); }
 
 static void launch_bcsr_impl(Ctx& c, const Bcsr& A, BcsrMode mode, Prec vprec, const void* x, const void* b, void* y,
                              double alpha, double beta, double omega, const void* invD, Prec dprec, const int* rows,

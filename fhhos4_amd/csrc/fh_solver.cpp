@@ -288,7 +288,7 @@ public:
             apply_eager(r, z);
             return;
         }
-        if (graph_r_ == r && graph_z_ == z && graph_mode_ != 0) {
+        if (graph_r_ == r && graph_z_ == z && graph_mode_ != 0 && graph_epoch_ == layout_epoch()) {
             if (graph_mode_ == 2) FH_HIP(hipGraphLaunch(graph_exec_, c_.stream));
             else apply_eager(r, z);
             return;
@@ -322,6 +322,7 @@ public:
         FH_HIP(e);
         graph_r_ = r;
         graph_z_ = z;
+        graph_epoch_ = layout_epoch();
         FH_HIP(hipGraphLaunch(graph_exec_, c_.stream));  // first replay (uploads the graph)
         c_.sync();
         t0 = std::chrono::steady_clock::now();
@@ -367,6 +368,9 @@ private:
     hipGraphExec_t graph_exec_ = nullptr;
     const double* graph_r_ = nullptr;
     double* graph_z_ = nullptr;
+    // layout epoch at capture: a matrix whose sliced / chunked copy was
+    // dropped or rebuilt since then makes the graph stale
+    uint64_t graph_epoch_ = 0;
 
     // FHGPU_GRAPH=0 runs every cycle eagerly (for timing comparisons).
     static bool graphs_enabled() {
@@ -818,10 +822,13 @@ void Solver::Solve(const double* b, double* x) {
     [&]() -> bool {                                                                           \
         if (args_.max_iter == 0) return true;                                                 \
         if (it == 0) return false;                                                            \
+        if (it >= args_.max_iter) {                                                           \
+            stats_.converged = res < args_.tol ? 1 : 0;                                       \
+            return true;                                                                      \
+        }                                                                                     \
         FH_CHECK(!std::isinf(res), -7, "divergence of the solver");                           \
         FH_CHECK(!std::isnan(res), -8, "the residual is NaN.");                               \
         if (res < args_.tol) { stats_.converged = 1; return true; }                           \
-        if (it >= args_.max_iter) return true;                                                \
         return false;                                                                         \
     }
 

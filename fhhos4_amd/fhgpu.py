@@ -202,6 +202,35 @@ def _ptr(t):
     return t.data_ptr()
 
 
+def _dev(t, n=None, like=None, dtype=None, name="vector"):
+    """Device pointer of tensor t after checking what the C entry points
+    assume but cannot see (they take no lengths): a contiguous tensor on the
+    ROCm device, of dtype `dtype` (or that of `like`), with exactly n
+    entries.  Raw integer pointers are passed through unchecked."""
+    if isinstance(t, int):
+        return t
+    import torch
+    if not isinstance(t, torch.Tensor):
+        raise FhError("%s must be a torch tensor on the device (or a raw pointer)" % name)
+    if not t.is_cuda:
+        raise FhError("%s must be in device memory" % name)
+    if not t.is_contiguous():
+        raise FhError("%s must be contiguous" % name)
+    want = dtype if dtype is not None else (like.dtype if like is not None and not isinstance(like, int) else None)
+    if want is not None and t.dtype != want:
+        raise FhError("%s has dtype %s, expected %s" % (name, t.dtype, want))
+    if t.dtype not in (torch.float64, torch.float32):
+        raise FhError("%s must be float64 or float32" % name)
+    if n is not None and t.numel() != n:
+        raise FhError("%s has %d entries, expected %d" % (name, t.numel(), n))
+    return t.data_ptr()
+
+
+def _f64():
+    import torch
+    return torch.float64
+
+
 def _vprec(t):
     try:
         import torch
@@ -305,6 +334,16 @@ class Matrix:
         _check(lib().fh_mat_convert(self.ctx.h, self.h, _PREC[prec], C.byref(h)))
         return Matrix(self.ctx, h)
 
+    @property
+    def rows(self):
+        i = self.info()
+        return i["nbr"] * i["bs"]
+
+    @property
+    def cols(self):
+        i = self.info()
+        return i["nbc"] * i["bs"]
+
     def info(self):
         nbr, nbc, bs, pr = C.c_int(), C.c_int(), C.c_int(), C.c_int()
         nnzb = C.c_int64()
@@ -339,24 +378,34 @@ def read_market_vector(path):
 
 @_ordered
 def spmv(ctx, A, x, y, alpha=1.0, beta=0.0):
-    _check(lib().fh_spmv(ctx.h, A.h, alpha, _ptr(x), beta, _ptr(y), _vprec(x)))
+    px = _dev(x, A.cols, name="x")
+    py = _dev(y, A.rows, like=x, name="y")
+    _check(lib().fh_spmv(ctx.h, A.h, alpha, px, beta, py, _vprec(x)))
 
 
 @_ordered
 def residual(ctx, A, b, x, r):
-    _check(lib().fh_residual(ctx.h, A.h, _ptr(b), _ptr(x), _ptr(r), _vprec(x)))
+    n = A.rows
+    px = _dev(x, n, name="x")
+    pb = _dev(b, n, like=x, name="b")
+    pr = _dev(r, n, like=x, name="r")
+    _check(lib().fh_residual(ctx.h, A.h, pb, px, pr, _vprec(x)))
 
 
 @_ordered
 def dot(ctx, x, y):
     out = C.c_double()
-    _check(lib().fh_dot(ctx.h, x.numel(), _ptr(x), _ptr(y), _vprec(x), C.byref(out)))
+    px = _dev(x, name="x")
+    py = _dev(y, x.numel(), like=x, name="y")
+    _check(lib().fh_dot(ctx.h, x.numel(), px, py, _vprec(x), C.byref(out)))
     return out.value
 
 
 @_ordered
 def axpby(ctx, a, x, b, y):
-    _check(lib().fh_axpby(ctx.h, x.numel(), a, _ptr(x), b, _ptr(y), _vprec(x)))
+    px = _dev(x, name="x")
+    py = _dev(y, x.numel(), like=x, name="y")
+    _check(lib().fh_axpby(ctx.h, x.numel(), a, px, b, py, _vprec(x)))
 
 
 class BlockJacobi:
@@ -372,8 +421,12 @@ class BlockJacobi:
 
     @_ordered
     def sweep(self, b, x, work, nsweeps=1, x_is_zero=False, A=None):
-        _check(lib().fh_bjac_sweep(self.ctx.h, self.h, (A or self.A).h, _ptr(b), _ptr(x), _ptr(work), nsweeps,
-                                   int(x_is_zero), _vprec(x)))
+        M = A or self.A
+        n = M.rows
+        px = _dev(x, n, name="x")
+        pb = _dev(b, n, like=x, name="b")
+        pw = _dev(work, n, like=x, name="work")
+        _check(lib().fh_bjac_sweep(self.ctx.h, self.h, M.h, pb, px, pw, nsweeps, int(x_is_zero), _vprec(x)))
 
     def __del__(self):
         try:
@@ -403,7 +456,10 @@ class Colouring:
 
 @_ordered
 def mcgs_sweep(ctx, A, bj, col, direction, b, x):
-    _check(lib().fh_mcgs_sweep(ctx.h, A.h, bj.h, col.h, direction, _ptr(b), _ptr(x), _vprec(x)))
+    n = A.rows
+    px = _dev(x, n, name="x")
+    pb = _dev(b, n, like=x, name="b")
+    _check(lib().fh_mcgs_sweep(ctx.h, A.h, bj.h, col.h, direction, pb, px, _vprec(x)))
 
 
 def aggregate_pairwise(ctx, A, theta=0.25):
@@ -460,10 +516,17 @@ class Solver:
         self.max_iter = max_iter
         _check(lib().fh_solver_set_stop(self.h, tol, max_iter))
 
+    def _n(self):
+        if getattr(self, "A", None) is None:
+            raise FhError("setup() must be called first")
+        return self.A.rows
+
     @_ordered
     def apply_prec(self, r, z):
         """z = M r with the preconditioner alone (device FP64 vectors)."""
-        _check(lib().fh_solver_apply_prec(self.h, _ptr(r), _ptr(z)))
+        n = self._n()
+        _check(lib().fh_solver_apply_prec(self.h, _dev(r, n, dtype=_f64(), name="r"),
+                                          _dev(z, n, dtype=_f64(), name="z")))
 
     def set_learned_smoother(self, nn, graph, omega=1.0, unit_step=False):
         """Learned post-smoother inside FCG (before setup)."""
@@ -487,8 +550,11 @@ class Solver:
 
     @_ordered
     def solve(self, b, x):
+        n = self._n()
+        pb = _dev(b, n, dtype=_f64(), name="b")
+        px = _dev(x, n, dtype=_f64(), name="x")
         st = SolveStats()
-        _check(lib().fh_solver_solve(self.h, _ptr(b), _ptr(x), C.byref(st)))
+        _check(lib().fh_solver_solve(self.h, pb, px, C.byref(st)))
         return st.as_dict()
 
     def history(self):
@@ -637,7 +703,8 @@ class HhoProblem:
         """Relative L2 error of the reconstruction on the GPU (x: device
         FP64 tensor of the face unknowns)."""
         e = C.c_double()
-        _check(lib().fh_hho_l2_error_gpu(ctx.h, self.h, _ptr(x), C.byref(e)))
+        _check(lib().fh_hho_l2_error_gpu(ctx.h, self.h, _dev(x, self.n_unknowns, dtype=_f64(), name="x"),
+                                         C.byref(e)))
         return e.value
 
     def l2_error(self, x):
@@ -700,20 +767,26 @@ class Mpnn:
 
     @_ordered
     def forward(self, graph, r1, delta, bf16=True):
-        _check(lib().fh_nn_forward(self.ctx.h, self.h, graph.h, _ptr(r1), _ptr(delta), int(bf16)))
+        n = graph.A.rows
+        _check(lib().fh_nn_forward(self.ctx.h, self.h, graph.h, _dev(r1, n, dtype=_f64(), name="r1"),
+                                   _dev(delta, n, dtype=_f64(), name="delta"), int(bf16)))
 
     @_ordered
     def loss_grad(self, graph, r1, e1, want_grad=True):
         loss = C.c_double()
         g = np.zeros(self.num_params(), dtype=np.float32) if want_grad else None
-        _check(lib().fh_nn_loss_grad(self.ctx.h, self.h, graph.h, _ptr(r1), _ptr(e1), C.byref(loss),
+        n = graph.A.rows
+        _check(lib().fh_nn_loss_grad(self.ctx.h, self.h, graph.h, _dev(r1, n, dtype=_f64(), name="r1"),
+                                      _dev(e1, n, dtype=_f64(), name="e1"), C.byref(loss),
                                       g.ctypes.data if g is not None else None))
         return loss.value, g
 
     @_ordered
     def train_step(self, graph, r1, e1, lr):
         loss = C.c_double()
-        _check(lib().fh_nn_train_step(self.ctx.h, self.h, graph.h, _ptr(r1), _ptr(e1), lr, C.byref(loss)))
+        n = graph.A.rows
+        _check(lib().fh_nn_train_step(self.ctx.h, self.h, graph.h, _dev(r1, n, dtype=_f64(), name="r1"),
+                                      _dev(e1, n, dtype=_f64(), name="e1"), lr, C.byref(loss)))
         return loss.value
 
     def __del__(self):

@@ -171,7 +171,8 @@ typedef struct fh_solver_args {
     const char* post_smoother;
     double relax;              /* RelaxationParameter */
     double tol;                /* relative residual ||r||/||b|| */
-    int max_iter;
+    int max_iter;              /* MaxIterations; 0 stops before the first iteration
+                                  (IterativeSolver.h:229-230); -1: the default 200 */
     int max_coarse;            /* MatrixMaxSizeForCoarsestLevel (ProgramArguments.h:61) */
     double theta;              /* strong coupling threshold (SolverFactory.h:95) */
     int mg_prec;               /* working precision of the multigrid vectors: FH_F64 | FH_F32 */
