@@ -736,6 +736,9 @@ int fh_hho_get_info(const fh_hho* h, fh_hho_info* info) {
         info->mesh_s = p.times.mesh_s;
         info->local_s = p.times.local_s;
         info->assemble_s = p.times.assemble_s;
+        info->redraws = p.redraws();
+        info->kept_vertices = p.kept_vertices();
+        info->min_quality = p.min_quality();
     });
 }
 int fh_hho_matrix(fh_ctx* ctx, const fh_hho* h, int keep_host, fh_mat** out) {

@@ -201,12 +201,150 @@ inline V3 cross(const V3& a, const V3& b) {
     return {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
 }
 
+// ------------------------------------------------------ perturbed grids
+inline uint64_t splitmix64(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+inline double det3(const V3& a, const V3& b, const V3& c) { return dot3(a, cross(b, c)); }
+
+// Cube split of Cube_CartesianTetrahedralMesh (corner code x + 2y + 4z).
+const int kTetSplit[6][4] = {{2, 3, 7, 1}, {7, 2, 1, 0}, {0, 1, 7, 5}, {7, 2, 0, 6}, {5, 7, 0, 6}, {5, 0, 4, 6}};
+
+// Smallest cell quality in the background box (bx, by, bz) of a grid of
+// spacing h: the Jacobian determinant over h^dim, oriented so that the
+// unperturbed cell has quality 1.  Simplices (stetra, stri): the affine
+// Jacobian of every cell of the box split.  Bilinear / trilinear cells
+// (hex, quad): the Jacobian at each corner, from the corner's edges.
+double box_quality(const std::vector<V3>& X, int dim, int nv1, const std::string& mesh, int bx, int by, int bz,
+                   double h) {
+    auto vid = [&](int ix, int iy, int iz) { return ((size_t)iz * nv1 + iy) * nv1 + ix; };
+    V3 P[8];
+    const int ncorner = dim == 3 ? 8 : 4;
+    for (int cc = 0; cc < ncorner; cc++) P[cc] = X[vid(bx + (cc & 1), by + ((cc >> 1) & 1), dim == 3 ? bz + ((cc >> 2) & 1) : 0)];
+    double q = 1e300;
+    if (mesh == "stetra") {
+        static const double* sgn = [] {
+            static double s[6];
+            for (int t = 0; t < 6; t++) {
+                V3 v[4];
+                for (int i = 0; i < 4; i++) v[i] = {double(kTetSplit[t][i] & 1), double((kTetSplit[t][i] >> 1) & 1), double((kTetSplit[t][i] >> 2) & 1)};
+                s[t] = det3(sub(v[1], v[0]), sub(v[2], v[0]), sub(v[3], v[0])) > 0 ? 1.0 : -1.0;
+            }
+            return s;
+        }();
+        const double h3 = h * h * h;
+        for (int t = 0; t < 6; t++) {
+            const V3& v0 = P[kTetSplit[t][0]];
+            q = std::min(q, sgn[t] * det3(sub(P[kTetSplit[t][1]], v0), sub(P[kTetSplit[t][2]], v0), sub(P[kTetSplit[t][3]], v0)) / h3);
+        }
+    } else if (mesh == "stri") {
+        // (v00, v10, v01) and (v01, v10, v11), both counter-clockwise
+        auto cr = [](const V3& a, const V3& b, const V3& c) { return (b[0] - a[0]) * (c[1] - a[1]) - (b[1] - a[1]) * (c[0] - a[0]); };
+        q = std::min(cr(P[0], P[1], P[2]), cr(P[2], P[1], P[3])) / (h * h);
+    } else if (dim == 3) {
+        const double h3 = h * h * h;
+        for (int cc = 0; cc < 8; cc++) {
+            V3 e[3];
+            for (int d = 0; d < 3; d++) {
+                e[d] = sub(P[cc ^ (1 << d)], P[cc]);
+                if ((cc >> d) & 1) e[d] = {-e[d][0], -e[d][1], -e[d][2]};
+            }
+            q = std::min(q, det3(e[0], e[1], e[2]) / h3);
+        }
+    } else {
+        for (int cc = 0; cc < 4; cc++) {
+            V3 e0 = sub(P[cc ^ 1], P[cc]), e1 = sub(P[cc ^ 2], P[cc]);
+            if (cc & 1) e0 = {-e0[0], -e0[1], 0};
+            if (cc & 2) e1 = {-e1[0], -e1[1], 0};
+            q = std::min(q, (e0[0] * e1[1] - e0[1] * e1[0]) / (h * h));
+        }
+    }
+    return q;
+}
+
+// Vertex positions of the (rn + 1)^dim grid of the unit square / cube with
+// every interior vertex moved by perturb * h * u, u uniform in [-1, 1]^dim
+// drawn from a counter-based hash of (seed, vertex, coordinate, attempt).
+// Vertices are placed one by one in index order; a draw is rejected, and
+// the next attempt drawn, when a cell of the vertex's boxes (vertices placed
+// so far at their new positions, the others at their grid positions) has a
+// quality below kQualityFloor.  After kMaxDraws rejected draws the vertex
+// stays at its grid position, which passed that test when its neighbours
+// were placed.  Every cell is tested with all its vertices final when its
+// last vertex is placed, so no cell of the mesh has a quality below the
+// floor (in particular none is inverted).
+constexpr double kQualityFloor = 0.1;
+constexpr int kMaxDraws = 64;
+struct PerturbedGrid {
+    std::vector<V3> X;
+    int64_t redraws = 0, kept = 0;  // rejected draws; vertices left at their grid position
+    double min_quality = 1.0;
+};
+PerturbedGrid perturbed_grid(int dim, int rn, const std::string& mesh, double perturb, uint64_t seed) {
+    PerturbedGrid G;
+    const int nv1 = rn + 1;
+    const double h = 1.0 / rn;
+    const int nz = dim == 3 ? nv1 : 1;
+    G.X.resize((size_t)nv1 * nv1 * nz);
+    for (int iz = 0; iz < nz; iz++)
+        for (int iy = 0; iy < nv1; iy++)
+            for (int ix = 0; ix < nv1; ix++) G.X[((size_t)iz * nv1 + iy) * nv1 + ix] = {ix * h, iy * h, dim == 3 ? iz * h : 0.0};
+    if (!(perturb > 0)) return G;
+    auto quality_around = [&](int ix, int iy, int iz) {
+        double q = 1e300;
+        for (int bz = (dim == 3 ? iz - 1 : 0); bz <= (dim == 3 ? iz : 0); bz++)
+            for (int by = iy - 1; by <= iy; by++)
+                for (int bx = ix - 1; bx <= ix; bx++) {
+                    if (bx < 0 || by < 0 || bz < 0 || bx >= rn || by >= rn || (dim == 3 && bz >= rn)) continue;
+                    q = std::min(q, box_quality(G.X, dim, nv1, mesh, bx, by, bz, h));
+                }
+        return q;
+    };
+    for (int iz = 1; iz < (dim == 3 ? rn : 2); iz++) {
+        const int z = dim == 3 ? iz : 0;
+        for (int iy = 1; iy < rn; iy++)
+            for (int ix = 1; ix < rn; ix++) {
+                const size_t v = ((size_t)z * nv1 + iy) * nv1 + ix;
+                const V3 base = G.X[v];
+                const uint64_t gid = ((uint64_t)z * nv1 + (uint64_t)iy) * nv1 + (uint64_t)ix;
+                bool ok = false;
+                for (int a = 0; a < kMaxDraws && !ok; a++) {
+                    V3 p = base;
+                    for (int d = 0; d < dim; d++) {
+                        uint64_t key = seed * 0x100000001B3ull ^ (gid * 4 + d);
+                        if (a > 0) key ^= (uint64_t)a * 0xD1B54A32D192ED03ull;
+                        const uint64_t r = splitmix64(key);
+                        const double u = (double)(r >> 11) * (1.0 / 9007199254740992.0) * 2.0 - 1.0;
+                        p[d] += perturb * h * u;
+                    }
+                    G.X[v] = p;
+                    ok = quality_around(ix, iy, z) >= kQualityFloor;
+                    if (!ok) G.redraws++;
+                }
+                if (!ok) {
+                    G.X[v] = base;
+                    G.kept++;
+                }
+            }
+    }
+    for (int bz = 0; bz < (dim == 3 ? rn : 1); bz++)
+        for (int by = 0; by < rn; by++)
+            for (int bx = 0; bx < rn; bx++) G.min_quality = std::min(G.min_quality, box_quality(G.X, dim, nv1, mesh, bx, by, bz, h));
+    return G;
+}
+
 }  // namespace
 
 // ------------------------------------------------------------------- Impl
 struct HhoProblem::Impl {
     HhoArgs a;
-    int dim = 3, kind = 0;   // cell kind: 0 simplex, 1 box
+    int dim = 3, kind = 0;   // cell kind: 0 simplex, 1 box (affine), 2 bilinear / trilinear quadrilateral or hexahedron
+    int64_t redraws = 0, kept_vertices = 0;  // perturbation: rejected draws, vertices left on the grid
+    double min_quality = 1.0;                // smallest cell quality (Jacobian determinant / h^dim)
     int nvc = 0, nfc = 0, nvf = 0;
     std::vector<V3> V;
     std::vector<int> cv;     // cell vertices (nvc per cell), local order of the mesher
@@ -294,6 +432,29 @@ struct HhoProblem::Impl {
 
     void build_mesh();
     void build_tables();
+
+    // ---- general cells (kind 2): bilinear quadrilaterals / trilinear
+    // hexahedra.  Cell and reconstruction polynomials are tensor Legendre
+    // products in the physical coordinates of the cell, scaled to its
+    // bounding box; face polynomials are tensor Legendre products in the
+    // face's own parameters (s, t) in [-1, 1]^2 (bilinear faces), the basis
+    // and order of the reference's Cartesian faces (LegendreBasis2D,
+    // FunctionalBasis/Legendre/LegendreBasis.h:80-120; face map
+    // Geometry/CartesianShape.h:345-421).  Integrals use tensor Gauss rules
+    // of k + 2 points per direction through the cell / face maps.
+    std::vector<double> g1x, g1w;        // (k+2)-point Gauss rule on [-1, 1]
+    std::vector<double> l1x, l1w;        // (k+4)-point rule (L2 error)
+    std::vector<I3> rexp;                // reconstruction exponents, graded (first nc: degree <= k)
+    std::vector<std::array<int, 2>> fexp;  // face exponents
+    void cell_frame(int c, V3& ctr, V3& hw) const;
+    void cell_map(int c, const V3& xi, V3& x, double J[3][3], double& det) const;
+    void cell_basis(const V3& ctr, const V3& hw, const V3& x, double* psi, double (*g)[3]) const;
+    void face_map(int fid, double s, double t, V3& x, V3& xs, V3& xt) const;
+    void face_basis(double s, double t, double* phi) const;
+    double face_diameter(int fid) const;
+    void local_general(int c, double* Acond, double* bcond, const double* xF, double* recon, double* RF,
+                       double* Aloc) const;
+    double l2_general(int c, const double* recon, double& u2) const;
     void geometry(int c, V3& v0, double J[3][3], double Ji[3][3], double& det) const;
     // Local condensation; outputs (optional): condensed matrix (m nfb)^2 and rhs;
     // or, for post-processing, the reconstruction coefficients from face values;
@@ -312,38 +473,27 @@ void HhoProblem::Impl::build_mesh() {
     const double h = 1.0 / n;
     FH_CHECK(n >= 1 && n <= 1000, -2, "mesh size n out of range");
     const int nv1 = n + 1;
-    // vertices (x fastest, then y, then z), perturbed inside the domain
+    // vertices (x fastest, then y, then z), perturbed inside the domain.
+    // The perturbation lives on the finest grid of a hierarchy (ref_n):
+    // coarsened meshes reuse the positions of its vertices.
     int nverts = dim == 3 ? nv1 * nv1 * nv1 : nv1 * nv1;
     V.resize(nverts);
-    // Perturbation of a vertex = counter-based uniform(-1,1) draws (splitmix64
-    // of the seed and the vertex's index on the reference grid) times
-    // perturb * h_ref: coarsened meshes reuse the positions of the finest.
     const int rn = a.ref_n > 0 ? a.ref_n : n;
     FH_CHECK(rn % n == 0, -2, "ref_n must be a multiple of n");
     const int ratio = rn / n;
-    const double href = 1.0 / rn;
-    auto splitmix = [](uint64_t x) {
-        x += 0x9E3779B97F4A7C15ull;
-        x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
-        x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
-        return x ^ (x >> 31);
-    };
+    const std::string grid_kind = (a.mesh == "hex" || a.mesh == "quad" || a.mesh == "cart") ? "box" : a.mesh;
+    FH_CHECK(a.perturb >= 0 && a.perturb < 0.5, -2, "perturb must be in [0, 0.5)");
+    PerturbedGrid PG = perturbed_grid(dim, rn, grid_kind, a.perturb, a.seed);
+    redraws = PG.redraws;
+    kept_vertices = PG.kept;
+    min_quality = PG.min_quality;
     for (int iz = 0; iz < (dim == 3 ? nv1 : 1); iz++)
         for (int iy = 0; iy < nv1; iy++)
-            for (int ix = 0; ix < nv1; ix++) {
-                int id = (iz * nv1 + iy) * nv1 + ix;
-                V3 p{ix * h, iy * h, dim == 3 ? iz * h : 0.0};
-                bool interior = ix > 0 && ix < n && iy > 0 && iy < n && (dim == 2 || (iz > 0 && iz < n));
-                if (a.perturb > 0 && interior) {
-                    uint64_t gid = ((uint64_t)(iz * ratio) * (rn + 1) + (uint64_t)(iy * ratio)) * (rn + 1) + (uint64_t)(ix * ratio);
-                    for (int d = 0; d < dim; d++) {
-                        uint64_t r = splitmix(a.seed * 0x100000001B3ull ^ (gid * 4 + d));
-                        double u = (double)(r >> 11) * (1.0 / 9007199254740992.0) * 2.0 - 1.0;
-                        p[d] += a.perturb * href * u;
-                    }
-                }
-                V[id] = p;
-            }
+            for (int ix = 0; ix < nv1; ix++)
+                V[(iz * nv1 + iy) * nv1 + ix] =
+                    PG.X[((size_t)(iz * ratio) * (rn + 1) + (size_t)(iy * ratio)) * (rn + 1) + (size_t)(ix * ratio)];
+    PG.X.clear();
+    PG.X.shrink_to_fit();
     auto vid = [&](int ix, int iy, int iz) { return (iz * nv1 + iy) * nv1 + ix; };
 
     std::unordered_map<uint64_t, int> fmap;
@@ -466,9 +616,14 @@ void HhoProblem::Impl::build_mesh() {
                 if (ix != n - 1) add_face(up, lower(ix + 1, iy), vid(ix + 1, iy + 1, 0), vid(ix + 1, iy, 0));
                 if (iy != n - 1) add_face(up, lower(ix, iy + 1), vid(ix, iy + 1, 0), vid(ix + 1, iy + 1, 0));
             }
-    } else if (a.mesh == "cart") {
-        FH_CHECK(a.perturb == 0, -2, "perturbed Cartesian cells are not affine; use stetra");
-        kind = 1;
+    } else if (a.mesh == "cart" || a.mesh == "hex" || a.mesh == "quad") {
+        // "cart": axis-aligned boxes (affine, tabulated on the reference box);
+        // "hex" / "quad": the same cells and faces with the vertices free to
+        // move (trilinear / bilinear maps, general-cell local operators)
+        FH_CHECK(a.mesh != "cart" || a.perturb == 0, -2, "perturbed Cartesian cells are not affine; use hex / quad");
+        FH_CHECK(a.mesh != "hex" || dim == 3, -2, "hex meshes are 3-D");
+        FH_CHECK(a.mesh != "quad" || dim == 2, -2, "quad meshes are 2-D");
+        kind = a.mesh == "cart" ? 1 : 2;
         nvc = dim == 3 ? 8 : 4;
         nfc = 2 * dim;
         nvf = dim == 3 ? 4 : 2;
@@ -517,7 +672,7 @@ void HhoProblem::Impl::build_mesh() {
                 if (fcells[2 * id + 1] >= 0) interior_list.push_back(id);
             }
     } else
-        throw Error(-2, "unknown mesh '" + a.mesh + "' (stetra | stri | cart)");
+        throw Error(-2, "unknown mesh '" + a.mesh + "' (stetra | stri | cart | hex | quad)");
     nface = (int)(fcells.size() / 2);
     funk.assign(nface, -1);
     for (size_t i = 0; i < interior_list.size(); i++) funk[interior_list[i]] = (int)i;
@@ -627,6 +782,34 @@ void HhoProblem::Impl::geometry(int c, V3& v0, double J[3][3], double Ji[3][3], 
 void HhoProblem::Impl::build_tables() {
     const int k = a.k;
     FH_CHECK(k >= 0 && k <= 3, -2, "face degree k must be 0..3");
+    if (kind == 2) {
+        std::vector<double> x, w;
+        gauss_legendre01(k + 2, x, w);
+        g1x.resize(x.size()); g1w.resize(x.size());
+        for (size_t i = 0; i < x.size(); i++) { g1x[i] = 2 * x[i] - 1; g1w[i] = 2 * w[i]; }
+        gauss_legendre01(k + 4, x, w);
+        l1x.resize(x.size()); l1w.resize(x.size());
+        for (size_t i = 0; i < x.size(); i++) { l1x[i] = 2 * x[i] - 1; l1w[i] = 2 * w[i]; }
+        rexp.clear();
+        for (int d = 0; d <= k + 1; d++) {
+            if (dim == 2)
+                for (int aa = d; aa >= 0; aa--) rexp.push_back({aa, d - aa, 0});
+            else
+                for (int aa = d; aa >= 0; aa--)
+                    for (int bb = d - aa; bb >= 0; bb--) rexp.push_back({aa, bb, d - aa - bb});
+        }
+        nr = (int)rexp.size();
+        nc = 0;
+        for (auto& e : rexp) if (e[0] + e[1] + e[2] <= k) nc++;
+        fexp.clear();
+        for (int d = 0; d <= k; d++) {
+            if (dim == 2) fexp.push_back({d, 0});
+            else
+                for (int j = 0; j <= d; j++) fexp.push_back({d - j, j});
+        }
+        nfb = (int)fexp.size();
+        return;
+    }
     cq = ref_quad(dim, kind, k + 4);
     fq = ref_quad(dim - 1, kind, k + 3);
     rb = make_basis(dim, k + 1, cq);
@@ -704,10 +887,393 @@ void HhoProblem::Impl::build_tables() {
     }
 }
 
+// ------------------------------------------- general (bi/trilinear) cells
+namespace {
+// Legendre polynomials L_0..L_p at x and their derivatives: Bonnet's
+// recurrence and L'_n = L'_{n-2} + (2n - 1) L_{n-1}.
+inline void legendre_all(int p, double x, double* L, double* dL) {
+    L[0] = 1;
+    dL[0] = 0;
+    if (p >= 1) { L[1] = x; dL[1] = 1; }
+    for (int q = 2; q <= p; q++) {
+        L[q] = ((2 * q - 1) * x * L[q - 1] - (q - 1) * L[q - 2]) / q;
+        dL[q] = dL[q - 2] + (2 * q - 1) * L[q - 1];
+    }
+}
+}  // namespace
+
+// Centre (vertex mean) and half-widths of the bounding box of cell c.
+void HhoProblem::Impl::cell_frame(int c, V3& ctr, V3& hw) const {
+    V3 lo{1e300, 1e300, 1e300}, hi{-1e300, -1e300, -1e300};
+    ctr = {0, 0, 0};
+    for (int l = 0; l < nvc; l++) {
+        const V3& p = V[cv[(size_t)c * nvc + l]];
+        for (int d = 0; d < 3; d++) {
+            ctr[d] += p[d] / nvc;
+            lo[d] = std::min(lo[d], p[d]);
+            hi[d] = std::max(hi[d], p[d]);
+        }
+    }
+    for (int d = 0; d < 3; d++) hw[d] = d < dim ? 0.5 * (hi[d] - lo[d]) : 1.0;
+}
+
+// Bilinear / trilinear map of cell c at xi in [-1, 1]^dim (corner code
+// x + 2y + 4z), its Jacobian J[r][q] = dx_r / dxi_q and determinant.
+void HhoProblem::Impl::cell_map(int c, const V3& xi, V3& x, double J[3][3], double& det) const {
+    x = {0, 0, 0};
+    for (int r = 0; r < 3; r++)
+        for (int q = 0; q < 3; q++) J[r][q] = 0;
+    const int ncorner = dim == 3 ? 8 : 4;
+    for (int cc = 0; cc < ncorner; cc++) {
+        double f[3], df[3];
+        for (int d = 0; d < 3; d++) {
+            const double sg = ((cc >> d) & 1) ? 1.0 : -1.0;
+            f[d] = d < dim ? 0.5 * (1 + sg * xi[d]) : 1.0;
+            df[d] = d < dim ? 0.5 * sg : 0.0;
+        }
+        const V3& P = V[cv[(size_t)c * nvc + cc]];
+        const double N = f[0] * f[1] * f[2];
+        const double dN[3] = {df[0] * f[1] * f[2], f[0] * df[1] * f[2], f[0] * f[1] * df[2]};
+        for (int r = 0; r < 3; r++) {
+            x[r] += N * P[r];
+            for (int q = 0; q < dim; q++) J[r][q] += dN[q] * P[r];
+        }
+    }
+    if (dim == 2) J[2][2] = 1;
+    det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) - J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+          J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+}
+
+// Reconstruction basis (the first nc functions: the cell basis) at the
+// physical point x: products of Legendre polynomials of (x - ctr) / hw.
+void HhoProblem::Impl::cell_basis(const V3& ctr, const V3& hw, const V3& x, double* psi, double (*g)[3]) const {
+    const int p = a.k + 1;
+    double L[3][8], dL[3][8];
+    for (int d = 0; d < 3; d++) {
+        if (d < dim) legendre_all(p, (x[d] - ctr[d]) / hw[d], L[d], dL[d]);
+        else { L[d][0] = 1; dL[d][0] = 0; }
+    }
+    for (int i = 0; i < nr; i++) {
+        const I3& e = rexp[i];
+        psi[i] = L[0][e[0]] * L[1][e[1]] * L[2][e[2]];
+        if (g) {
+            g[i][0] = dL[0][e[0]] / hw[0] * L[1][e[1]] * L[2][e[2]];
+            g[i][1] = dim >= 2 ? L[0][e[0]] * dL[1][e[1]] / hw[1] * L[2][e[2]] : 0.0;
+            g[i][2] = dim == 3 ? L[0][e[0]] * L[1][e[1]] * dL[2][e[2]] / hw[2] : 0.0;
+        }
+    }
+}
+
+// Face map: segment a + (s + 1)/2 (b - a) (2-D), or the bilinear patch of the
+// corners a (-,-), b (+,-), c (+,+), d (-,+) (3-D); xs, xt its tangents.
+void HhoProblem::Impl::face_map(int fid, double s, double t, V3& x, V3& xs, V3& xt) const {
+    const int* f = &fv[(size_t)fid * nvf];
+    if (dim == 2) {
+        const V3 &A = V[f[0]], &B = V[f[1]];
+        for (int d = 0; d < 3; d++) {
+            x[d] = 0.5 * (1 - s) * A[d] + 0.5 * (1 + s) * B[d];
+            xs[d] = 0.5 * (B[d] - A[d]);
+            xt[d] = 0;
+        }
+        return;
+    }
+    const V3 &A = V[f[0]], &B = V[f[1]], &Cc = V[f[2]], &D = V[f[3]];
+    for (int d = 0; d < 3; d++) {
+        x[d] = 0.25 * ((1 - s) * (1 - t) * A[d] + (1 + s) * (1 - t) * B[d] + (1 + s) * (1 + t) * Cc[d] + (1 - s) * (1 + t) * D[d]);
+        xs[d] = 0.25 * ((1 - t) * (B[d] - A[d]) + (1 + t) * (Cc[d] - D[d]));
+        xt[d] = 0.25 * ((1 - s) * (D[d] - A[d]) + (1 + s) * (Cc[d] - B[d]));
+    }
+}
+
+void HhoProblem::Impl::face_basis(double s, double t, double* phi) const {
+    double Ls[8], dLs[8], Lt[8], dLt[8];
+    legendre_all(a.k, s, Ls, dLs);
+    legendre_all(a.k, t, Lt, dLt);
+    for (int j = 0; j < nfb; j++) phi[j] = Ls[fexp[j][0]] * (dim == 3 ? Lt[fexp[j][1]] : 1.0);
+}
+
+// Longest edge of the face (the side of a square face: CartesianShape's
+// diameter is its largest width, Geometry/CartesianShape.h:140).
+double HhoProblem::Impl::face_diameter(int fid) const {
+    const int* f = &fv[(size_t)fid * nvf];
+    if (dim == 2) return nrm(sub(V[f[1]], V[f[0]]));
+    double h = 0;
+    for (int e = 0; e < 4; e++) h = std::max(h, nrm(sub(V[f[(e + 1) % 4]], V[f[e]])));
+    return h;
+}
+
+// Local HHO operators of a bilinear / trilinear cell (Diff_HHOElement::InitHHO,
+// Diff_HHOElement.h:104-150): the saddle-point reconstruction with the
+// mean-value condition (:313-402), consistency P^T S P (:323), the "hho"
+// stabilisation with the general projections (:411-451: Dt = M_T^-1 N_T P - I,
+// face trace projections Mf^-1 int phi psi), and the static condensation.
+// Face integrals use the vector area element x_s x x_t (outward), so the
+// integration by parts of the reconstruction is exact on warped faces; the
+// stabilisation weight (K n.n)/h_F takes the unit normal at the face centre.
+// Outputs as local().
+void HhoProblem::Impl::local_general(int c, double* Acond, double* bcond, const double* xF, double* recon, double* RF,
+                                     double* Aloc) const {
+    const int m = nfc, nh = nc + m * nfb, nF = m * nfb;
+    const int nq1 = (int)g1x.size();
+    const double* Kc = &K[(size_t)c * 9];
+    V3 ctr, hw;
+    cell_frame(c, ctr, hw);
+    thread_local std::vector<double> S, Lm, MT, BT, B, P, A, Dt, Mf, TF, PF, DF, tmp, W, X;
+    S.assign((size_t)nr * nr, 0.0);
+    Lm.assign(nr, 0.0);
+    MT.assign((size_t)nc * nr, 0.0);
+    BT.assign(nc, 0.0);
+    B.assign((size_t)nr * nh, 0.0);
+    std::vector<double> psi(nr), g(3 * nr), phi(nfb);
+    // ---- cell integrals
+    const int nqc = dim == 3 ? nq1 * nq1 * nq1 : nq1 * nq1;
+    for (int q = 0; q < nqc; q++) {
+        const int i0 = q % nq1, i1 = (q / nq1) % nq1, i2 = dim == 3 ? q / (nq1 * nq1) : 0;
+        V3 xi{g1x[i0], g1x[i1], dim == 3 ? g1x[i2] : 0.0};
+        const double w = g1w[i0] * g1w[i1] * (dim == 3 ? g1w[i2] : 1.0);
+        V3 x;
+        double J[3][3], det;
+        cell_map(c, xi, x, J, det);
+        FH_CHECK(det > 0, -6, "cell " + std::to_string(c) + " is inverted at a quadrature point");
+        const double wd = w * det;
+        cell_basis(ctr, hw, x, psi.data(), (double(*)[3])g.data());
+        for (int i = 0; i < nr; i++) {
+            const double* gi = &g[3 * i];
+            double Kg[3];
+            for (int r = 0; r < 3; r++) Kg[r] = Kc[r * 3] * gi[0] + Kc[r * 3 + 1] * gi[1] + Kc[r * 3 + 2] * gi[2];
+            for (int j = 0; j < nr; j++) {
+                const double* gj = &g[3 * j];
+                S[(size_t)i * nr + j] += wd * (Kg[0] * gj[0] + Kg[1] * gj[1] + Kg[2] * gj[2]);
+            }
+            Lm[i] += wd * psi[i];
+        }
+        for (int i = 0; i < nc; i++)
+            for (int j = 0; j < nr; j++) MT[(size_t)i * nr + j] += wd * psi[i] * psi[j];
+        if (bcond || recon) {
+            const double fw = wd * source(x);
+            for (int i = 0; i < nc; i++) BT[i] += fw * psi[i];
+        }
+    }
+    for (int i = 0; i < nr; i++)
+        for (int j = 0; j < nc; j++) B[(size_t)i * nh + j] = S[(size_t)i * nr + j];
+    // ---- face integrals
+    Mf.assign((size_t)m * nfb * nfb, 0.0);
+    TF.assign((size_t)m * nfb * nr, 0.0);
+    std::vector<double> coef(m);
+    const int nqf = dim == 3 ? nq1 * nq1 : nq1;
+    for (int f = 0; f < m; f++) {
+        const int fid = cf[(size_t)c * nfc + f];
+        V3 x0, xs, xt;
+        face_map(fid, 0, 0, x0, xs, xt);
+        V3 nc0 = dim == 3 ? cross(xs, xt) : V3{xs[1], -xs[0], 0};
+        const double sg = dot3(nc0, sub(x0, ctr)) < 0 ? -1.0 : 1.0;  // outward
+        const double l0 = nrm(nc0);
+        const V3 nu{sg * nc0[0] / l0, sg * nc0[1] / l0, sg * nc0[2] / l0};
+        double Kn = 0;
+        for (int r = 0; r < 3; r++)
+            for (int q = 0; q < 3; q++) Kn += nu[r] * Kc[r * 3 + q] * nu[q];
+        coef[f] = Kn / face_diameter(fid);
+        double* Mff = &Mf[(size_t)f * nfb * nfb];
+        double* TFf = &TF[(size_t)f * nfb * nr];
+        for (int q = 0; q < nqf; q++) {
+            const int i0 = q % nq1, i1 = dim == 3 ? q / nq1 : 0;
+            const double st = g1x[i0], tt = dim == 3 ? g1x[i1] : 0.0;
+            const double w = g1w[i0] * (dim == 3 ? g1w[i1] : 1.0);
+            V3 x, s1, t1;
+            face_map(fid, st, tt, x, s1, t1);
+            V3 nv = dim == 3 ? cross(s1, t1) : V3{s1[1], -s1[0], 0};
+            for (int d = 0; d < 3; d++) nv[d] *= sg;
+            const double da = nrm(nv);
+            cell_basis(ctr, hw, x, psi.data(), (double(*)[3])g.data());
+            face_basis(st, tt, phi.data());
+            for (int i = 0; i < nr; i++) {
+                const double* gi = &g[3 * i];
+                double Kgn = 0;
+                for (int r = 0; r < 3; r++) Kgn += (Kc[r * 3] * gi[0] + Kc[r * 3 + 1] * gi[1] + Kc[r * 3 + 2] * gi[2]) * nv[r];
+                const double wk = w * Kgn;
+                for (int j = 0; j < nfb; j++) B[(size_t)i * nh + nc + f * nfb + j] += wk * phi[j];
+                for (int j = 0; j < nc; j++) B[(size_t)i * nh + j] -= wk * psi[j];
+            }
+            for (int aa = 0; aa < nfb; aa++) {
+                const double wa = w * da * phi[aa];
+                for (int bb = 0; bb < nfb; bb++) Mff[aa * nfb + bb] += wa * phi[bb];
+                for (int i = 0; i < nr; i++) TFf[(size_t)aa * nr + i] += wa * psi[i];
+            }
+        }
+    }
+    for (int j = 0; j < nh; j++) B[j] = 0.0;  // degree-0 row: no equation (Diff_HHOElement.h:366-367)
+    // ---- reconstruction: rows 1.. solve S' P' = B'; row 0 from the mean
+    // value  sum_i (int psi_i) P_i = [int psi_j (cell), 0 (faces)]
+    P.assign((size_t)nr * nh, 0.0);
+    const int n1 = nr - 1;
+    tmp.assign((size_t)n1 * n1, 0.0);
+    for (int i = 0; i < n1; i++)
+        for (int j = 0; j < n1; j++) tmp[(size_t)i * n1 + j] = S[(size_t)(i + 1) * nr + j + 1];
+    FH_CHECK(n1 == 0 || chol(n1, tmp.data()), -6, "reconstruction stiffness not SPD");
+    for (int i = 1; i < nr; i++)
+        for (int j = 0; j < nh; j++) P[(size_t)i * nh + j] = B[(size_t)i * nh + j];
+    if (n1) chol_solve(n1, tmp.data(), &P[nh], nh);
+    for (int j = 0; j < nh; j++) {
+        double v = j < nc ? Lm[j] : 0.0;
+        for (int i = 1; i < nr; i++) v -= Lm[i] * P[(size_t)i * nh + j];
+        P[j] = v / Lm[0];
+    }
+    // ---- consistency P^T S P
+    W.assign((size_t)nr * nh, 0.0);
+    for (int i = 0; i < nr; i++)
+        for (int l = 0; l < nr; l++) {
+            const double sv = S[(size_t)i * nr + l];
+            if (sv == 0.0) continue;
+            for (int j = 0; j < nh; j++) W[(size_t)i * nh + j] += sv * P[(size_t)l * nh + j];
+        }
+    A.assign((size_t)nh * nh, 0.0);
+    for (int i = 0; i < nr; i++)
+        for (int r = 0; r < nh; r++) {
+            const double pv = P[(size_t)i * nh + r];
+            if (pv == 0.0) continue;
+            for (int j = 0; j < nh; j++) A[(size_t)r * nh + j] += pv * W[(size_t)i * nh + j];
+        }
+    // ---- Dt = M_T^-1 N_T P - [I 0]   (N_T = MT, M_T = MT[:, :nc])
+    std::vector<double> MTc((size_t)nc * nc);
+    for (int i = 0; i < nc; i++)
+        for (int j = 0; j < nc; j++) MTc[(size_t)i * nc + j] = MT[(size_t)i * nr + j];
+    FH_CHECK(chol(nc, MTc.data()), -6, "cell mass matrix not SPD");
+    Dt.assign((size_t)nc * nh, 0.0);
+    for (int i = 0; i < nc; i++)
+        for (int l = 0; l < nr; l++) {
+            const double mv = MT[(size_t)i * nr + l];
+            for (int j = 0; j < nh; j++) Dt[(size_t)i * nh + j] += mv * P[(size_t)l * nh + j];
+        }
+    chol_solve(nc, MTc.data(), Dt.data(), nh);
+    for (int i = 0; i < nc; i++) Dt[(size_t)i * nh + i] -= 1.0;
+    // ---- stabilisation, face by face
+    PF.assign((size_t)nfb * nr, 0.0);
+    DF.assign((size_t)nfb * nh, 0.0);
+    std::vector<double> Lf((size_t)nfb * nfb), MD((size_t)nfb * nh);
+    for (int f = 0; f < m; f++) {
+        const double* Mff = &Mf[(size_t)f * nfb * nfb];
+        std::copy(Mff, Mff + (size_t)nfb * nfb, Lf.begin());
+        FH_CHECK(chol(nfb, Lf.data()), -6, "face mass matrix not SPD");
+        std::copy(&TF[(size_t)f * nfb * nr], &TF[(size_t)(f + 1) * nfb * nr], PF.begin());
+        chol_solve(nfb, Lf.data(), PF.data(), nr);  // ProjF = Mf^-1 int phi psi
+        std::fill(DF.begin(), DF.end(), 0.0);
+        for (int aa = 0; aa < nfb; aa++) {
+            for (int i = 0; i < nr; i++) {
+                const double pv = PF[(size_t)aa * nr + i];
+                for (int j = 0; j < nh; j++) DF[(size_t)aa * nh + j] += pv * P[(size_t)i * nh + j];
+            }
+            DF[(size_t)aa * nh + nc + f * nfb + aa] -= 1.0;
+            for (int i = 0; i < nc; i++) {
+                const double pv = PF[(size_t)aa * nr + i];
+                for (int j = 0; j < nh; j++) DF[(size_t)aa * nh + j] -= pv * Dt[(size_t)i * nh + j];
+            }
+        }
+        // A += coef DF^T Mf DF
+        std::fill(MD.begin(), MD.end(), 0.0);
+        for (int aa = 0; aa < nfb; aa++)
+            for (int bb = 0; bb < nfb; bb++) {
+                const double mv = Mff[aa * nfb + bb];
+                for (int j = 0; j < nh; j++) MD[(size_t)aa * nh + j] += mv * DF[(size_t)bb * nh + j];
+            }
+        for (int aa = 0; aa < nfb; aa++)
+            for (int r = 0; r < nh; r++) {
+                const double d = coef[f] * DF[(size_t)aa * nh + r];
+                if (d == 0.0) continue;
+                for (int j = 0; j < nh; j++) A[(size_t)r * nh + j] += d * MD[(size_t)aa * nh + j];
+            }
+    }
+    if (Aloc) std::copy(A.begin(), A.end(), Aloc);
+    // ---- static condensation (Diffusion_HHO.h:624-652, HHOCellSpace.h:170-203)
+    if (RF && !bcond && !recon) std::fill(BT.begin(), BT.end(), 0.0);
+    tmp.assign((size_t)nc * nc, 0.0);
+    for (int i = 0; i < nc; i++)
+        for (int j = 0; j < nc; j++) tmp[(size_t)i * nc + j] = A[(size_t)i * nh + j];
+    FH_CHECK(chol(nc, tmp.data()), -6, "A_TT not SPD");
+    if (Acond || bcond) {
+        X.assign((size_t)nc * (nF + 1), 0.0);
+        for (int i = 0; i < nc; i++) {
+            for (int j = 0; j < nF; j++) X[(size_t)i * (nF + 1) + j] = A[(size_t)i * nh + nc + j];
+            X[(size_t)i * (nF + 1) + nF] = BT[i];
+        }
+        chol_solve(nc, tmp.data(), X.data(), nF + 1);
+        if (Acond)
+            for (int r = 0; r < nF; r++)
+                for (int s2 = 0; s2 < nF; s2++) {
+                    double v = A[(size_t)(nc + r) * nh + nc + s2];
+                    for (int i = 0; i < nc; i++) v -= A[(size_t)i * nh + nc + r] * X[(size_t)i * (nF + 1) + s2];
+                    Acond[(size_t)r * nF + s2] = v;
+                }
+        if (bcond)
+            for (int r = 0; r < nF; r++) {
+                double v = 0;
+                for (int i = 0; i < nc; i++) v -= A[(size_t)i * nh + nc + r] * X[(size_t)i * (nF + 1) + nF];
+                bcond[r] = v;
+            }
+    }
+    if (RF) {
+        X.assign((size_t)nc * nF, 0.0);
+        for (int i = 0; i < nc; i++)
+            for (int j = 0; j < nF; j++) X[(size_t)i * nF + j] = A[(size_t)i * nh + nc + j];
+        chol_solve(nc, tmp.data(), X.data(), nF);
+        for (int i = 0; i < nr; i++)
+            for (int j = 0; j < nF; j++) {
+                double v = P[(size_t)i * nh + nc + j];
+                for (int l = 0; l < nc; l++) v -= P[(size_t)i * nh + l] * X[(size_t)l * nF + j];
+                RF[(size_t)i * nF + j] = v;
+            }
+    }
+    if (recon) {
+        std::vector<double> vT(nc);
+        for (int i = 0; i < nc; i++) {
+            double v = BT[i];
+            for (int j = 0; j < nF; j++) v -= A[(size_t)i * nh + nc + j] * xF[j];
+            vT[i] = v;
+        }
+        chol_solve(nc, tmp.data(), vT.data(), 1);
+        for (int i = 0; i < nr; i++) {
+            double v = 0;
+            for (int j = 0; j < nc; j++) v += P[(size_t)i * nh + j] * vT[j];
+            for (int j = 0; j < nF; j++) v += P[(size_t)i * nh + nc + j] * xF[j];
+            recon[i] = v;
+        }
+    }
+}
+
+// Integrals of (u - p)^2 (returned) and u^2 (u2) over cell c, p the
+// reconstruction with coefficients recon, on the (k+4)-point rule.
+double HhoProblem::Impl::l2_general(int c, const double* recon, double& u2) const {
+    V3 ctr, hw;
+    cell_frame(c, ctr, hw);
+    std::vector<double> psi(nr);
+    const int n1 = (int)l1x.size();
+    const int nq = dim == 3 ? n1 * n1 * n1 : n1 * n1;
+    double e2 = 0;
+    u2 = 0;
+    for (int q = 0; q < nq; q++) {
+        const int i0 = q % n1, i1 = (q / n1) % n1, i2 = dim == 3 ? q / (n1 * n1) : 0;
+        V3 xi{l1x[i0], l1x[i1], dim == 3 ? l1x[i2] : 0.0};
+        const double w = l1w[i0] * l1w[i1] * (dim == 3 ? l1w[i2] : 1.0);
+        V3 x;
+        double J[3][3], det;
+        cell_map(c, xi, x, J, det);
+        cell_basis(ctr, hw, x, psi.data(), nullptr);
+        double ph = 0;
+        for (int i = 0; i < nr; i++) ph += recon[i] * psi[i];
+        const double u = exact(x);
+        e2 += w * det * (u - ph) * (u - ph);
+        u2 += w * det * u * u;
+    }
+    return e2;
+}
+
 // Local HHO operators of cell c (Diff_HHOElement::InitHHO,
 // Diff_HHOElement.h:104-150, 312-509) and their static condensation.
 void HhoProblem::Impl::local(int c, double* Acond, double* bcond, const double* xF, double* recon, double* RF,
                              double* Aloc) const {
+    if (kind == 2) {
+        local_general(c, Acond, bcond, xF, recon, RF, Aloc);
+        return;
+    }
     const int m = nfc, nh = nc + m * nfb;
     V3 v0;
     double J[3][3], Ji[3][3], det;
@@ -1151,9 +1717,14 @@ void HhoProblem::Prolongation(const HhoProblem& coarse, std::vector<int>& prp, s
             const int* fvv = &Fm.fv[(size_t)f * Fm.nvf];
             V3 a0 = Fm.V[fvv[0]];
             V3 E1 = sub(Fm.V[fvv[1]], a0);
-            V3 E2 = dim == 3 ? sub(Fm.V[Fm.kind == 1 ? fvv[3] : fvv[2]], a0) : V3{0, 0, 0};
+            V3 E2 = dim == 3 ? sub(Fm.V[Fm.kind >= 1 ? fvv[3] : fvv[2]], a0) : V3{0, 0, 0};
             double sF = dim == 3 ? nrm(cross(E1, E2)) : nrm(E1);
             V3 nF_ = dim == 3 ? cross(E1, E2) : V3{E1[1], -E1[0], 0};
+            if (Fm.kind == 2) {  // normal at the face centre
+                V3 x0, s1, t1;
+                Fm.face_map(f, 0, 0, x0, s1, t1);
+                nF_ = dim == 3 ? cross(s1, t1) : V3{s1[1], -s1[0], 0};
+            }
             double nn = nrm(nF_);
             for (int d = 0; d < 3; d++) nF_[d] /= nn;
             // weights: normal diffusivity (Weight, LevelForHHO.h:743-763)
@@ -1174,6 +1745,32 @@ void HhoProblem::Prolongation(const HhoProblem& coarse, std::vector<int>& prp, s
                 int C = cs[s];
                 if (C < 0) continue;
                 // trace projection of the coarse reconstruction basis on the fine face
+                if (Fm.kind == 2) {
+                    // general cells: Tr = Mf^-1 int_F phi psi_C (fine face basis,
+                    // coarse cell's physical basis)
+                    V3 ctr, hw;
+                    Cm.cell_frame(C, ctr, hw);
+                    const int nq1 = (int)Fm.g1x.size(), nqf = dim == 3 ? nq1 * nq1 : nq1;
+                    std::vector<double> Mf((size_t)nfb * nfb, 0.0);
+                    std::fill(Tr.begin(), Tr.end(), 0.0);
+                    for (int q = 0; q < nqf; q++) {
+                        const int i0 = q % nq1, i1 = dim == 3 ? q / nq1 : 0;
+                        const double st = Fm.g1x[i0], tt = dim == 3 ? Fm.g1x[i1] : 0.0;
+                        const double wq = Fm.g1w[i0] * (dim == 3 ? Fm.g1w[i1] : 1.0);
+                        V3 x, s1, t1;
+                        Fm.face_map(f, st, tt, x, s1, t1);
+                        const double da = dim == 3 ? nrm(cross(s1, t1)) : nrm(s1);
+                        Cm.cell_basis(ctr, hw, x, psi.data(), nullptr);
+                        Fm.face_basis(st, tt, chi.data());
+                        for (int j = 0; j < nfb; j++) {
+                            const double wj = wq * da * chi[j];
+                            for (int i = 0; i < nr; i++) Tr[(size_t)j * nr + i] += wj * psi[i];
+                            for (int i = 0; i < nfb; i++) Mf[(size_t)j * nfb + i] += wj * chi[i];
+                        }
+                    }
+                    FH_CHECK(chol(nfb, Mf.data()), -6, "face mass matrix not SPD");
+                    chol_solve(nfb, Mf.data(), Tr.data(), nr);
+                } else {
                 V3 v0;
                 double J[3][3], Ji[3][3], det;
                 Cm.geometry(C, v0, J, Ji, det);
@@ -1192,6 +1789,7 @@ void HhoProblem::Prolongation(const HhoProblem& coarse, std::vector<int>& prp, s
                     double wq = Fm.fq.w[q] * ssq * isq;
                     for (int j = 0; j < nfb; j++)
                         for (int i = 0; i < nr; i++) Tr[(size_t)j * nr + i] += wq * chi[j] * psi[i];
+                }
                 }
                 const double* RF = &RFall[(size_t)C * nr * nF];
                 for (int lf = 0; lf < nfc; lf++) {
@@ -1231,6 +1829,9 @@ void HhoProblem::Prolongation(const HhoProblem& coarse, std::vector<int>& prp, s
 }
 
 int HhoProblem::ncells() const { return impl->ncell; }
+int64_t HhoProblem::redraws() const { return impl->redraws; }
+int64_t HhoProblem::kept_vertices() const { return impl->kept_vertices; }
+double HhoProblem::min_quality() const { return impl->min_quality; }
 int HhoProblem::nfaces() const { return impl->nface; }
 
 std::vector<double> HhoProblem::CellKappa() const {
@@ -1263,6 +1864,12 @@ double HhoProblem::L2Error(const double* x) const {
                 for (int a = 0; a < nf; a++) xF[f * nf + a] = r >= 0 ? x[(size_t)r * nf + a] : 0.0;
             }
             I.local(c, nullptr, nullptr, xF.data(), rec.data());
+            if (I.kind == 2) {
+                double u2 = 0;
+                err += I.l2_general(c, rec.data(), u2);
+                nrm2 += u2;
+                continue;
+            }
             V3 v0;
             double J[3][3], Ji[3][3], det;
             I.geometry(c, v0, J, Ji, det);
@@ -1349,6 +1956,7 @@ static HhoDev make_device_problem(Ctx& c, const HhoProblem& H, const HhoProblem:
 double HhoProblem::L2ErrorDevice(Ctx& c, const double* x_dev) const {
     FH_CHECK(!c.host_only, -9, "device post-processing needs a device context");
     const Impl& I = *impl;
+    FH_CHECK(I.kind != 2, -2, "general (bilinear / trilinear) cells: device kernel not built in this library");
     std::vector<DevBuf> keep;
     HhoDev d = make_device_problem(c, *this, I, keep);
     DevBuf out(sizeof(double) * 2 * std::max(1, I.ncell));
@@ -1370,6 +1978,7 @@ double HhoProblem::L2ErrorDevice(Ctx& c, const double* x_dev) const {
 std::unique_ptr<Bcsr> HhoProblem::AssembleDevice(Ctx& c, double* b_dev, bool keep_host) const {
     FH_CHECK(!c.host_only, -9, "device assembly needs a device context");
     const Impl& I = *impl;
+    FH_CHECK(I.kind != 2, -2, "general (bilinear / trilinear) cells: device kernel not built in this library");
     auto t0 = std::chrono::steady_clock::now();
     std::vector<DevBuf> keep;
     HhoDev d = make_device_problem(c, *this, I, keep);

@@ -25,6 +25,7 @@ namespace fh {
 struct HhoArgs {
     int dim = 3;
     std::string mesh = "stetra";   // "stetra" (cube split into 6 tets) | "cart" (squares / cubes) | "stri" (2-D triangles)
+                                   // | "hex" / "quad" (trilinear / bilinear cells of the grid, perturbable)
     int n = 8;                     // cells per direction of the background grid
     int k = 2;                     // face degree
     double perturb = 0.0;          // interior vertex perturbation, fraction of h (uniform per coordinate)
@@ -85,6 +86,11 @@ public:
     void Uncondensed(Csr& ATT, Csr& ATF, Csr& AFF) const;
     int cell_unknowns() const;
     HhoTimes times;
+    // mesh perturbation: rejected draws, vertices left on their grid
+    // position, smallest cell quality (Jacobian determinant / h^dim)
+    int64_t redraws() const;
+    int64_t kept_vertices() const;
+    double min_quality() const;
     bool host_assembled = false;       // vals and b hold the host assembly
     mutable double device_assemble_s = 0;
     // Local operators, static condensation and assembly on the GPU (one

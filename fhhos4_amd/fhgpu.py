@@ -136,7 +136,8 @@ class HhoArgs(C.Structure):
 
 class HhoInfo(C.Structure):
     _fields_ = [("nb", C.c_int), ("nf", C.c_int), ("ncells", C.c_int), ("nfaces", C.c_int), ("nnzb", C.c_int64),
-                ("mesh_s", C.c_double), ("local_s", C.c_double), ("assemble_s", C.c_double)]
+                ("mesh_s", C.c_double), ("local_s", C.c_double), ("assemble_s", C.c_double),
+                ("redraws", C.c_int64), ("kept_vertices", C.c_int64), ("min_quality", C.c_double)]
 
 
 def lib():

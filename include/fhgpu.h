@@ -280,7 +280,9 @@ typedef struct fh_hho fh_hho;
 typedef struct fh_hho_args {
     int dim;                /* 2 or 3 */
     const char* mesh;       /* "stetra" (cube -> 6 tets, Cube_CartesianTetrahedralMesh.h) | "stri" (square ->
-                               2 triangles, Square_TriangularMesh.h) | "cart" */
+                               2 triangles, Square_TriangularMesh.h) | "cart" (Cube_CartesianMesh.h,
+                               Square_CartesianMesh.h) | "hex" / "quad": the cells and faces of "cart" as
+                               trilinear / bilinear maps of their (perturbable) vertices */
     int n;                  /* background cells per direction */
     int k;                  /* face polynomial degree 0..3 */
     double perturb;         /* interior vertex perturbation (fraction of h) */
@@ -299,6 +301,12 @@ typedef struct fh_hho_info {
     int nb, nf, ncells, nfaces;
     int64_t nnzb;
     double mesh_s, local_s, assemble_s;
+    /* vertex perturbation: draws rejected because a cell of the vertex would
+     * fall below the quality floor 0.1 (Jacobian determinant / h^dim, 1 on
+     * the unperturbed grid), vertices left on the grid after 64 rejected
+     * draws, and the smallest cell quality of the mesh */
+    int64_t redraws, kept_vertices;
+    double min_quality;
 } fh_hho_info;
 int fh_hho_create(const fh_hho_args* args, fh_hho** out);
 int fh_hho_destroy(fh_hho* h);

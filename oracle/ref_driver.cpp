@@ -100,7 +100,30 @@ struct Opts
 	bool aggregOnly = false;
 	int aggregBlock = -1;
 	bool solve = true;
+	string kappaFile; // per-element isotropic kappa (int64 count + doubles, mesh element order)
 };
+
+// Gives every element a physical part of its own with the isotropic tensor
+// kappa[i] (the reference's way of a piecewise-constant field: one named
+// PhysicalGroup per part with a constant Tensor, Mesh/PhysicalGroup.h,
+// TestCases/Diffusion/Tensor.h:20-23).
+template <int Dim>
+static void SetCellKappa(Mesh<Dim>* mesh, const string& path)
+{
+	FILE* f = fopen(path.c_str(), "rb");
+	if (!f) { fprintf(stderr, "cannot open %s\n", path.c_str()); exit(2); }
+	int64_t n = 0;
+	if (fread(&n, sizeof(int64_t), 1, f) != 1 || n != (int64_t)mesh->Elements.size()) { fprintf(stderr, "kappa file size mismatch\n"); exit(2); }
+	vector<double> k(n);
+	if (fread(k.data(), sizeof(double), n, f) != (size_t)n) { fprintf(stderr, "short kappa file\n"); exit(2); }
+	fclose(f);
+	for (int64_t i = 0; i < n; i++)
+	{
+		PhysicalGroup<Dim>* pg = new PhysicalGroup<Dim>(1000 + (int)i, "cell" + to_string(i));
+		pg->ConstantDiffTensor = new Tensor<Dim>(k[i]);
+		mesh->Elements[i]->PhysicalPart = pg;
+	}
+}
 
 // In-house structured meshes, as MeshFactory<Dim>::BuildMesh builds them for
 // these codes (src/Mesher/MeshFactory.h:142-143,381-399).
@@ -185,6 +208,8 @@ static int Run(const Opts& o)
 	Mesh<Dim>* mesh = BuildMesh<Dim>(o);
 	mesh->SetDiffusionField(&testCase->DiffField);
 	mesh->SetBoundaryConditions(&testCase->BC);
+	if (!o.kappaFile.empty())
+		SetCellKappa<Dim>(mesh, o.kappaFile);
 
 	int k = args.Discretization.PolyDegree - 1;
 	FunctionalBasis<Dim>* reconstructionBasis = FunctionalBasisFactory<Dim>::Create(args.Discretization.ElemBasisCode, k + 1, false);
@@ -305,6 +330,7 @@ int main(int argc, char** argv)
 		else if (a == "-aggreg") o.aggregOnly = true;
 		else if (a == "-aggreg-block") o.aggregBlock = stoi(next());
 		else if (a == "-nosolve") o.solve = false;
+		else if (a == "-kappa-file") o.kappaFile = next();
 		else { fprintf(stderr, "unknown option %s\n", a.c_str()); return 2; }
 	}
 	if (o.dim == 2) return Run<2>(o);

@@ -70,11 +70,27 @@ def pack_csr(prefix, M, d):
     d[prefix + "_shape"] = np.array(M.shape, dtype=np.int64)
 
 
-def run(name, args, export_mg=False, keep_A=True, aggreg=None, export_dat=False, uamg=False):
+def cell_kappa(dim, n, sigma, seed):
+    """Per-cell lognormal kappa as this package's HHO module draws it for a
+    "cart" / "hex" / "quad" mesh (the same cells in the same order as the
+    reference's Cube_CartesianMesh / Square_CartesianMesh)."""
+    sys.path.insert(0, REPO)
+    from fhhos4_amd import fhgpu as G
+    P = G.HhoProblem(dim=dim, mesh="cart", n=n, k=0, kappa="lognormal", sigma=sigma, seed=seed, defer_assembly=True)
+    return P.cell_kappa()
+
+
+def run(name, args, export_mg=False, keep_A=True, aggreg=None, export_dat=False, uamg=False, kappa=None):
     if ONLY and name not in ONLY:
         return
     with tempfile.TemporaryDirectory() as tmp:
         cmd = [REFDRV] + args + ["-out", tmp]
+        if kappa is not None:
+            kf = os.path.join(tmp, "kappa.bin")
+            with open(kf, "wb") as fh:
+                fh.write(np.array([kappa.size], dtype=np.int64).tobytes())
+                fh.write(np.ascontiguousarray(kappa, dtype=np.float64).tobytes())
+            cmd += ["-kappa-file", kf]
         if export_mg:
             cmd.append("-export-mg")
         if export_dat:
@@ -82,6 +98,8 @@ def run(name, args, export_mg=False, keep_A=True, aggreg=None, export_dat=False,
         out = subprocess.run(cmd, check=True, stdout=subprocess.PIPE).stdout.decode(errors="replace")
         info = read_info(os.path.join(tmp, "info.txt"))
         d = {"cmd": np.array(" ".join(["refdrv"] + args))}
+        if kappa is not None:
+            d["kappa"] = np.asarray(kappa, dtype=np.float64)
         if keep_A:
             pack_csr("A", read_csr(os.path.join(tmp, "A.csr")), d)
             d["b"] = read_vec(os.path.join(tmp, "b.vec"))
@@ -166,6 +184,13 @@ def main():
         export_mg=True, uamg=True)
     run("uamg_stri16_k2", ["-dim", "2", "-mesh", "stri", "-k", "2", "-n", "16", "-source", "pisine"] + uk,
         export_mg=True, uamg=True)
+    # Cartesian cells with a per-cell lognormal kappa (one physical part per
+    # cell): the unperturbed members of the hexahedral (k = 3, sigma 3) and
+    # quadrilateral (k = 2, sigma 2) benchmark families
+    run("hex4_k3_lognormal", ["-dim", "3", "-mesh", "cart", "-k", "3", "-n", "4", "-source", "pisine", "-s", "cg",
+                              "-prec", "bj"], kappa=cell_kappa(3, 4, 3.0, 7))
+    run("quad8_k2_lognormal", ["-dim", "2", "-mesh", "cart", "-k", "2", "-n", "8", "-source", "pisine", "-s", "cg",
+                               "-prec", "bj"], kappa=cell_kappa(2, 8, 2.0, 5))
     # Exported linear systems (Matrix Market text) of tiny 2-D and 3-D problems
     run("mm_c1_n4", c1s + ["-n", "4", "-s", "cg", "-prec", "mg", "-smoothers", "bj,bj"], export_dat=True)
     run("mm_tet2_k1", ["-dim", "3", "-mesh", "stetra", "-k", "1", "-n", "2", "-source", "sine", "-s", "cg",

@@ -207,3 +207,29 @@ def test_stopping_rules_match_the_reference(ctx):
         s.solve(torch.zeros_like(b), x)
     st = s.solve(b, x)  # the solver stays usable after the error
     assert st["converged"] == 1
+
+
+def test_graph_recaptured_after_layout_change(ctx):
+    """A multigrid cycle replayed from a HIP graph reads the sliced copy of
+    the fine operator; dropping that copy (fh_mat_set_sliced(A, 0)) must make
+    the next solve capture again on block CSR, with the same result."""
+    from fhhos4_amd import gmg
+    P = G.HhoProblem(dim=3, mesh="stetra", n=12, k=1, perturb=0.1, seed=3, source="pisine")
+    hier = gmg.build_hierarchy(ctx, P)
+    A = hier[0][1]
+    s = G.Solver(ctx, solver="cg", prec="mg", nu=(1, 1), smoothers=("cheb2", "cheb2"), galerkin=True,
+                 coarse_smoother="cheb1", max_iter=200)
+    gmg.attach(s, hier)
+    s.setup(A)
+    b = dev(P.rhs())
+    x = torch.zeros_like(b)
+    st0 = s.solve(b, x)   # decides the layout, captures and replays the cycle
+    x0 = x.cpu().numpy()
+    assert A.layout() in (1, 2), A.layout()
+    s.solve(b, x)
+    assert not A.set_sliced(False)
+    assert A.layout() == 0
+    st1 = s.solve(b, x)   # same (r, z) pair: must not replay the stale graph
+    x1 = x.cpu().numpy()
+    assert st1["iterations"] == st0["iterations"]
+    assert np.linalg.norm(x1 - x0) / np.linalg.norm(x0) < 1e-12
