@@ -1,19 +1,22 @@
 #!/usr/bin/env python3
 """Time to 1e-8 relative residual of the condensed HHO face system on one GPU.
 
-Workload (one "step" = one solve from x = 0 to ||r||/||b|| < 1e-8 with the
-operator, the multigrid hierarchy and b already resident in HBM): 3-D Poisson
-on the unit cube, 64^3 Cartesian cubes split into 6 tetrahedra (1.57 M
-cells), interior vertices perturbed by a seeded uniform +-0.2 h, HHO k = 2
-(6 unknowns per triangular face, 3.1 M faces, 18.7 M unknowns), kappa = 1,
-f = 3 pi^2 prod sin(pi x_i).  The plain method is FP64 CG preconditioned by
-one V(1,1)-cycle of geometric multigrid with Galerkin coarse operators and
-degree-2 Chebyshev smoothing of D^-1 A (D = block diagonal); beside it: FP64
-FCG with an FP32 (and a BF16-stored) multigrid preconditioner, and FP64 FCG
-with the FP32 multigrid and the learned post-smoother.
+Default workload (one "step" = one solve from x = 0 to ||r||/||b|| < 1e-8
+with the operator, the multigrid hierarchy and b already resident in HBM):
+3-D diffusion -div(kappa grad u) = f on the unit cube, 96^3 hexahedra with
+interior vertices perturbed by a seeded uniform +-0.15 h (trilinear cells,
+warped faces), HHO k = 3 (10 unknowns per face, 2.63 M interior faces,
+26.3 M unknowns), kappa lognormal per cell with sigma = 3, f = 3 pi^2 prod
+sin(pi x_i).  The plain method is FP64 CG preconditioned by one V(1,1)-cycle
+of a p-then-h multigrid (p-injection to the face means, then Galerkin
+h-levels of the k = 0 problem with CellInterp_Trace prolongations) with
+degree-2 Chebyshev smoothing of D^-1 A on the finest level (D = block
+diagonal) and degree 1 below; beside it: FP64 FCG with FP32 and BF16-stored
+multigrid preconditioners and with the learned post-smoother.
 
-Prints one JSON line; progress goes to stderr.  --sweep runs a list of
-solver variants and prints one JSON line per variant instead.
+--config c2 runs the tetrahedral workload (64^3 x 6 perturbed tetrahedra,
+k = 2, kappa = 1), hetero and learned2d the other families.  Prints one JSON
+line; progress goes to stderr.
 """
 import argparse
 import json
@@ -353,6 +356,115 @@ def learned_leg(args, G, ctx, hier, A, P, b, x, x64, t64, v):
     return out
 
 
+def c5_bench(args, G, np, torch):
+    """The hexahedral k = 3 workload (see the module docstring)."""
+    from fhhos4_amd import gmg
+    pm = tuple(int(t) for t in args.p_modes.split(","))
+    log("mesh and pattern: %d^3 hexahedra, k=%d, perturb %.2f, lognormal kappa sigma %.1f"
+        % (args.hex_n, args.hex_k, args.hex_perturb, args.hex_sigma))
+    P = G.HhoProblem(dim=3, mesh="hex", n=args.hex_n, k=args.hex_k, perturb=args.hex_perturb, seed=args.seed,
+                     kappa="lognormal", sigma=args.hex_sigma, source="pisine", defer_assembly=True)
+    nb, nf, nnzb = P.nb, P.nf, P.nnzb
+    N = nb * nf
+    log("system: %d faces x %d = %d unknowns, %d blocks; cell quality >= %.3f (%d redrawn draws)"
+        % (nb, nf, N, nnzb, P.info["min_quality"], P.info["redraws"]))
+    ctx = G.Context(0)
+    t = time.perf_counter()
+    H = gmg.build_hp_hierarchy(ctx, P, p_modes=pm)
+    hier_s = time.perf_counter() - t
+    A, b = H["A"], P.b_device
+    log("hierarchy (GPU assembly, host k=0 prolongations) %.1f s; p-levels keep %s modes, h-levels %s"
+        % (hier_s, pm, [h[0].n_unknowns for h in H["h"]]))
+    x = torch.zeros_like(b)
+    torch.cuda.synchronize()
+    base = dict(prec="mg", nu=(args.nu, args.nu), smoothers=(args.smoother, args.smoother),
+                coarse_smoother=args.coarse_smoother, galerkin=True, tol=1e-8, max_iter=args.max_iter)
+    label = "hp-MG V(%d,%d) p%s+Galerkin h(k=0) %s, coarse levels %s" % (args.nu, args.nu, args.p_modes,
+                                                                         args.smoother, args.coarse_smoother)
+
+    def solver(**kw):
+        a = dict(base)
+        a.update(kw)
+        s = G.Solver(ctx, **a)
+        gmg.attach_hp(s, H)
+        t0 = time.perf_counter()
+        s.setup(A)
+        return s, time.perf_counter() - t0
+
+    log("set-up FP64 MG-PCG:", label)
+    s64, setup64 = solver(solver="cg")
+    levels = [s64.level_size(l) for l in range(s64_levels(s64))]
+    log("levels", levels, "setup %.1fs" % setup64)
+    ts64, st64 = timed_solves(ctx, s64, b, x, args.steps, args.warmup)
+    t64 = float(np.mean(ts64))
+    x64 = x.cpu().numpy()
+    t = time.perf_counter()
+    l2_64 = P.l2_error_gpu(ctx, x)
+    post_s = time.perf_counter() - t
+    log("FP64 MG-PCG: %d iterations, %.3f s (%.1f ms/it), res %.2e, L2 %.4e"
+        % (st64["iterations"], t64, 1e3 * t64 / max(1, st64["iterations"]), st64["final_res"], l2_64))
+    # in-situ durations of the fine-level kernels: SpMV (CG product and
+    # residual kernel family) and the fused Chebyshev step
+    prof = {}
+    for mode, name in ((0, "spmv"), (1, "residual"), (3, "cheb_step")):
+        G.profile_begin(ctx, A, mode)
+        s64.solve(b, x)
+        nl, ms = G.profile_end(ctx)
+        prof[name] = (nl, ms)
+    layout = {0: "bcsr", 1: "sliced", 2: "sliced2", 3: "chunked"}[A.layout()]
+    del s64
+    bytes_spmv = spmv_bytes(nb, nf, nnzb)
+    # fused Chebyshev step: the product's bytes + D^-1 (nb nf^2 doubles) +
+    # b read + direction read and written + new iterate written
+    bytes_cheb = bytes_spmv + nb * nf * nf * 8 + 4 * N * 8
+    kern = {}
+    for name, byt in (("spmv", bytes_spmv), ("residual", bytes_spmv + N * 8), ("cheb_step", bytes_cheb)):
+        nl, ms = prof[name]
+        if nl:
+            kern[name] = {"launches_per_solve": nl, "mean_ms": ms, "bytes": byt, "GBs": byt / (ms * 1e-3) / 1e9,
+                          "ms_per_solve": nl * ms}
+            log("fine %s (%s): %d launches, %.3f ms, %.0f GB/s" % (name, layout, nl, ms, kern[name]["GBs"]))
+    dom = max(kern, key=lambda k: kern[k]["ms_per_solve"])
+    mixed = {}
+    if not args.no_mixed:
+        for store, cstore in (("f32", None), ("bf16", None), ("f32", "bf16")):
+            key = store if cstore is None else "%s_coarse_%s" % (store, cstore)
+            log("set-up FCG + %s multigrid" % key)
+            s, su = solver(solver="fcg", mg_prec="f32", mg_store=store, coarse_store=cstore)
+            ts, st = timed_solves(ctx, s, b, x, args.leg_steps, 1)
+            xm = x.cpu().numpy()
+            mixed[key] = {"s": float(np.mean(ts)), "iterations": st["iterations"], "setup_s": su,
+                          "converged": st["converged"],
+                          "rel_err_vs_fp64": float(np.linalg.norm(xm - x64) / np.linalg.norm(x64)),
+                          "l2": P.l2_error_gpu(ctx, x), "speedup_vs_fp64": t64 / float(np.mean(ts))}
+            log("FCG + %s MG: %d iterations, %.3f s (x%.2f), rel. err vs FP64 %.1e"
+                % (key, st["iterations"], mixed[key]["s"], t64 / mixed[key]["s"], mixed[key]["rel_err_vs_fp64"]))
+            del s
+    best = min(mixed.values(), key=lambda v: v["s"]) if mixed else None
+    gb_iter = None
+    line = {
+        "metric": METRIC, "value": t64, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": t64 * 1e3, "higher_is_better": False, "scaling": "strong", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "3-D diffusion, %d^3 trilinear hexahedra perturbed +-%.2fh, HHO k=%d (%d unknowns), "
+                               "per-cell lognormal kappa sigma %.1f; CG + %s, tol 1e-8"
+                               % (args.hex_n, args.hex_perturb, args.hex_k, N, args.hex_sigma, label),
+                   "unknowns": N, "faces": nb, "blocks": nnzb, "levels": levels},
+        "iterations": st64["iterations"], "setup_s": setup64, "hierarchy_s": hier_s, "l2_error": l2_64,
+        "postprocess_s": post_s, "dof_it_per_s": N * st64["iterations"] / t64,
+        "fine_kernels": kern, "fine_layout": layout,
+        "mixed_fcg": mixed,
+        "best_mixed_vs_fp64": (best["s"] / t64) if best else None,
+        "roofline": {"bound": "hbm", "achieved": kern[dom]["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": kern[dom]["GBs"] / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "fine-level %s (%s layout, n_f = %d)" % (dom, layout, nf),
+                     "bytes_per_launch": kern[dom]["bytes"], "mean_ms": kern[dom]["mean_ms"],
+                     "launches": kern[dom]["launches_per_solve"]},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -360,7 +472,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=64)
     ap.add_argument("--sweep-mixed", action="store_true", help="with --sweep: FCG + FP32 / BF16-coarse multigrid")
-    ap.add_argument("--config", default="", choices=["", "hetero", "learned2d"],
+    ap.add_argument("--config", default="c5", choices=["c5", "c2", "hetero", "learned2d"],
                     help="hetero: the heterogeneous hexahedral case (64^3 hexahedra, k=2, subdomain lognormal kappa); "
                          "learned2d: the learned post-smoother on 2-D triangles (use --n 512)")
     ap.add_argument("--train-n2d", type=int, default=128)
@@ -387,6 +499,13 @@ def main():
     ap.add_argument("--train-epochs", type=int, default=8)
     ap.add_argument("--sweep", action="store_true", help="time a list of solver variants")
     ap.add_argument("--kernel", choices=["spmv"], help="time one kernel family alone")
+    ap.add_argument("--hex-n", type=int, default=96, help="c5: hexahedra per direction")
+    ap.add_argument("--hex-k", type=int, default=3)
+    ap.add_argument("--hex-perturb", type=float, default=0.15)
+    ap.add_argument("--hex-sigma", type=float, default=3.0)
+    ap.add_argument("--p-modes", default="1", help="c5: modes kept by the p-levels, e.g. 1 or 3,1")
+    ap.add_argument("--leg-steps", type=int, default=2, help="timed solves of each mixed / learned leg")
+    ap.add_argument("--cpu-iters", type=int, default=3, help="c5: oracle iterations timed on the full system")
     ap.add_argument("--reps", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -409,6 +528,8 @@ def main():
         return
     if args.kernel == "spmv":
         return kernel_bench(args, G, np, torch)
+    if args.config == "c5":
+        return c5_bench(args, G, np, torch)
     log("mesh and pattern of the HHO system n=%d k=%d (perturb %.2f)" % (args.n, args.k, args.perturb))
     P = G.HhoProblem(dim=3, mesh="stetra", n=args.n, k=args.k, perturb=args.perturb, seed=args.seed,
                      source="pisine", defer_assembly=True)

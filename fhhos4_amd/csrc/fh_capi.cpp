@@ -590,6 +590,12 @@ int fh_solver_export_levels(fh_solver* s, const char* dir) {
         s->s->ExportLevels(dir);
     });
 }
+int fh_solver_set_level_p(fh_solver* s, int level, int nfc) {
+    return guard([&] {
+        NEED(s);
+        s->s->SetLevelP(level, nfc);
+    });
+}
 int fh_solver_set_stop(fh_solver* s, double tol, int max_iter) {
     return guard([&] { NEED(s); s->s->SetStop(tol, max_iter); });
 }

@@ -1949,6 +1949,21 @@ static HhoDev make_device_problem(Ctx& c, const HhoProblem& H, const HhoProblem:
     d.modes = (const double*)up(modes.data(), sizeof(double) * modes.size());
     d.blk = (const int*)up(blk.data(), sizeof(int) * blk.size());
     d.frow = (const int*)up(frow.data(), sizeof(int) * frow.size());
+    if (I.kind == 2) {
+        std::vector<double> g1(I.g1x), l1(I.l1x);
+        g1.insert(g1.end(), I.g1w.begin(), I.g1w.end());
+        l1.insert(l1.end(), I.l1w.begin(), I.l1w.end());
+        std::vector<int> rexp, fexp;
+        for (const auto& e : I.rexp) rexp.insert(rexp.end(), {e[0], e[1], e[2]});
+        for (const auto& e : I.fexp) fexp.insert(fexp.end(), {e[0], e[1]});
+        d.ng1 = (int)I.g1x.size();
+        d.nl1 = (int)I.l1x.size();
+        d.g1 = (const double*)up(g1.data(), sizeof(double) * g1.size());
+        d.l1 = (const double*)up(l1.data(), sizeof(double) * l1.size());
+        d.rexp = (const int*)up(rexp.data(), sizeof(int) * rexp.size());
+        d.fexp = (const int*)up(fexp.data(), sizeof(int) * fexp.size());
+        c.sync();  // staging vectors of this block are released at its end
+    }
     c.sync();  // host staging vectors are released on return
     return d;
 }
@@ -1956,7 +1971,6 @@ static HhoDev make_device_problem(Ctx& c, const HhoProblem& H, const HhoProblem:
 double HhoProblem::L2ErrorDevice(Ctx& c, const double* x_dev) const {
     FH_CHECK(!c.host_only, -9, "device post-processing needs a device context");
     const Impl& I = *impl;
-    FH_CHECK(I.kind != 2, -2, "general (bilinear / trilinear) cells: device kernel not built in this library");
     std::vector<DevBuf> keep;
     HhoDev d = make_device_problem(c, *this, I, keep);
     DevBuf out(sizeof(double) * 2 * std::max(1, I.ncell));
@@ -1978,7 +1992,6 @@ double HhoProblem::L2ErrorDevice(Ctx& c, const double* x_dev) const {
 std::unique_ptr<Bcsr> HhoProblem::AssembleDevice(Ctx& c, double* b_dev, bool keep_host) const {
     FH_CHECK(!c.host_only, -9, "device assembly needs a device context");
     const Impl& I = *impl;
-    FH_CHECK(I.kind != 2, -2, "general (bilinear / trilinear) cells: device kernel not built in this library");
     auto t0 = std::chrono::steady_clock::now();
     std::vector<DevBuf> keep;
     HhoDev d = make_device_problem(c, *this, I, keep);

@@ -35,6 +35,16 @@ struct HhoDev {
     int post;
     const double* xsol;
     double* cell_err;
+    // general cells (kind 2, bilinear / trilinear maps): Gauss rules on
+    // [-1, 1] (ng1 points for the operators, nl1 for the L2 error: nodes
+    // then weights), reconstruction exponents (nr x 3, graded; the first nc
+    // are the cell basis) and face exponents (nfb x 2) of the tensor
+    // Legendre bases
+    int ng1, nl1;
+    const double* g1;
+    const double* l1;
+    const int* rexp;
+    const int* fexp;
 };
 
 // LDS bytes per cell (one workgroup per cell).

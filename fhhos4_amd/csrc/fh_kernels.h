@@ -76,6 +76,9 @@ void launch_agg_prolong_add(Ctx& c, int nb, int nf, Prec p, const int* agg, cons
 void launch_p_restrict(Ctx& c, int nb, int nf, int nfc, Prec p, const void* r, void* rc);
 void launch_p_prolong_add(Ctx& c, int nb, int nf, int nfc, Prec p, const void* ec, void* x);
 
+// S[p] = leading nfc x nfc sub-block of the nf x nf block A[p], p < nnzb (FP64).
+void launch_mode_submatrix(Ctx& c, int64_t nnzb, int nf, int nfc, const double* A, double* S);
+
 // y = M x with M dense symmetric n x n (precision mp), vectors in vp.
 void launch_dense_symv(Ctx& c, int n, const void* M, Prec mp, Prec vp, const void* x, void* y);
 

@@ -1385,6 +1385,25 @@ void launch_p_prolong_add(Ctx& c, int nb, int nf, int nfc, Prec p, const void* e
     FH_HIP(hipGetLastError());
 }
 
+// Leading nfc x nfc sub-block of every nf x nf block (FP64 values): the
+// Galerkin operator R A P of the p-injection P = [I_nfc; 0] per face.
+__global__ void k_mode_submatrix(int64_t nnzb, int nf, int nfc, const double* __restrict__ A, double* __restrict__ S) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t n = nnzb * nfc * nfc;
+    if (e >= n) return;
+    const int64_t p = e / (nfc * nfc);
+    const int r = (int)(e - p * nfc * nfc);
+    const int a = r / nfc, b = r - a * nfc;
+    S[e] = A[p * nf * nf + a * nf + b];
+}
+
+void launch_mode_submatrix(Ctx& c, int64_t nnzb, int nf, int nfc, const double* A, double* S) {
+    const int64_t n = nnzb * nfc * nfc;
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_mode_submatrix, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream, nnzb, nf, nfc, A, S);
+    FH_HIP(hipGetLastError());
+}
+
 void launch_dense_symv(Ctx& c, int n, const void* M, Prec mp, Prec vp, const void* x, void* y) {
     if (n <= 0) return;
     int g = (n + 3) / 4;

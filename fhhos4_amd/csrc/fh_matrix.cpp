@@ -432,9 +432,29 @@ std::unique_ptr<Bcsr> galerkin_rap(Ctx& c, const Bcsr& A, const Bcsr& P) {
 }
 
 std::unique_ptr<Bcsr> bcsr_mode_submatrix(Ctx& c, const Bcsr& A, int nfc) {
-    FH_CHECK(!A.h_vals.empty(), -3, "p-coarsening needs host values");
     FH_CHECK(nfc >= 1 && nfc <= A.bs_r && A.bs_r == A.bs_c, -2, "invalid number of kept modes");
     const int bs = A.bs_r;
+    if (A.h_vals.empty()) {
+        // operator on the device only (the GPU assembly): gather on the
+        // device, keep a host copy of the (small) coarse values for the
+        // set-up steps below it (Galerkin products, aggregation)
+        FH_CHECK(!c.host_only && A.prec == Prec::F64 && A.vals.p, -3, "p-coarsening needs FP64 values");
+        auto S = std::make_unique<Bcsr>();
+        S->nbr = A.nbr; S->nbc = A.nbc; S->bs_r = S->bs_c = nfc; S->nnzb = A.nnzb; S->prec = Prec::F64;
+        S->h_rowptr = A.h_rowptr;
+        S->h_colidx = A.h_colidx;
+        S->rowptr.alloc(sizeof(int) * (A.nbr + 1));
+        S->colidx.alloc(sizeof(int) * std::max<int64_t>(1, A.nnzb));
+        FH_HIP(hipMemcpyAsync(S->rowptr.p, A.rowptr.p, sizeof(int) * (A.nbr + 1), hipMemcpyDeviceToDevice, c.stream));
+        FH_HIP(hipMemcpyAsync(S->colidx.p, A.colidx.p, sizeof(int) * A.nnzb, hipMemcpyDeviceToDevice, c.stream));
+        const size_t nv = (size_t)A.nnzb * nfc * nfc;
+        S->vals.alloc(sizeof(double) * std::max<size_t>(1, nv));
+        launch_mode_submatrix(c, A.nnzb, bs, nfc, A.vals.as<double>(), S->vals.as<double>());
+        S->h_vals.resize(nv);
+        FH_HIP(hipMemcpyAsync(S->h_vals.data(), S->vals.p, sizeof(double) * nv, hipMemcpyDeviceToHost, c.stream));
+        c.sync();
+        return S;
+    }
     std::vector<double> v((size_t)A.nnzb * nfc * nfc);
     for (int64_t p = 0; p < A.nnzb; p++)
         for (int a = 0; a < nfc; a++)

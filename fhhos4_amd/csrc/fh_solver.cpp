@@ -243,18 +243,35 @@ public:
     int coarse_n = 0;
     double op_complexity = 0;
 
-    void setup(const Bcsr& A, const std::vector<std::pair<const Bcsr*, const Bcsr*>>& given, bool amg) {
+    void setup(const Bcsr& A, const std::vector<std::pair<const Bcsr*, const Bcsr*>>& given, bool amg,
+               const std::vector<int>& pmodes = {}) {
         L.clear();
         L.emplace_back();
         L[0].A64 = &A;
         if (amg) build_amg();
         else {
+            auto pm = [&pmodes](size_t l) { return l < pmodes.size() ? pmodes[l] : 0; };
             for (size_t l = 1; l < given.size(); l++) {
-                FH_CHECK(a_.galerkin || given[l].first != nullptr, -2, "missing operator of level " + std::to_string(l));
+                FH_CHECK(a_.galerkin || pm(l - 1) > 0 || given[l].first != nullptr, -2,
+                         "missing operator of level " + std::to_string(l));
                 L.emplace_back();
                 L[l].A64 = given[l].first;
             }
             for (size_t l = 0; l + 1 < given.size(); l++) {
+                if (pm(l) > 0) {
+                    // p-injection of hierarchical bases: keep the first nfc
+                    // modes; Galerkin operator = leading sub-blocks
+                    FH_CHECK(pm(l) < L[l].A64->bs_r, -2, "a p-transfer must keep fewer modes than the level has");
+                    L[l].ptrunc = true;
+                    L[l].nfc = pm(l);
+                    if (!L[l + 1].A64) {
+                        L[l + 1].A64_own = bcsr_mode_submatrix(c_, *L[l].A64, pm(l));
+                        L[l + 1].A64 = L[l + 1].A64_own.get();
+                    }
+                    FH_CHECK(L[l + 1].A64->nbr == L[l].A64->nbr && L[l + 1].A64->bs_r == pm(l), -2,
+                             "the operator below a p-transfer must have the kept modes of the same faces");
+                    continue;
+                }
                 FH_CHECK(given[l].second != nullptr, -2, "missing prolongation of level " + std::to_string(l));
                 L[l].P64 = given[l].second;
                 if (a_.galerkin) {
@@ -583,6 +600,13 @@ void Solver::SetLevel(int l, const Bcsr* A, const Bcsr* P) {
     given_levels_[l] = {A, P};
 }
 
+void Solver::SetLevelP(int l, int nfc) {
+    FH_CHECK(l >= 0 && nfc >= 1, -2, "p-transfer: level >= 0 and at least one kept mode");
+    if ((int)given_p_.size() <= l) given_p_.resize(l + 1, 0);
+    given_p_[l] = nfc;
+    if ((int)given_levels_.size() <= l + 1) given_levels_.resize(l + 2, {nullptr, nullptr});
+}
+
 void Solver::SetLearnedSmoother(Mpnn* nn, const NnGraph* g, double omega, bool unit_step) {
     FH_CHECK((nn == nullptr) == (g == nullptr), -2, "network and graph go together");
     nn_ = nn;
@@ -662,7 +686,7 @@ void Solver::Setup(const Bcsr& A) {
             if (g.empty()) g.push_back({&A, nullptr});
             g[0].first = &A;
         }
-        mg_->setup(A, g, p == "aggregamg");
+        mg_->setup(A, g, p == "aggregamg", given_p_);
     } else if (p == "bj" || p == "bj23" || p == "bgs") {
         prec_sm_ = std::make_unique<LevelSmoother>();
         prec_sm_->spec = SmootherSpec::parse(p, 1, args_.relax);

@@ -107,6 +107,12 @@ public:
     // GMG hierarchy input: operator and prolongation of level l (l >= 1 for
     // the operator; P of level l maps level l+1 to level l).
     void SetLevel(int l, const Bcsr* A, const Bcsr* P);
+    // The transfer from level l + 1 to level l is the p-injection keeping the
+    // first nfc modes of every face block (hierarchical bases; the natural
+    // injection of P_LevelForHHO::SetupProlongation, P_MultigridForHHO.h:62-86);
+    // the operator of level l + 1 is then its Galerkin operator, the leading
+    // nfc x nfc sub-blocks (unless SetLevel gives one).
+    void SetLevelP(int l, int nfc);
     // Learned post-smoother inside the preconditioner of FCG: after the
     // multigrid cycle z0 = M r, r1 = r - A z0, delta = net(r1); z = z0 +
     // (delta.r1 / delta.A delta) delta when delta.r1 > 0 (unit_step: z0 +
@@ -143,6 +149,7 @@ private:
     std::unique_ptr<Multigrid> mg_;
     std::unique_ptr<LevelSmoother> prec_sm_;   // bj / bgs as preconditioner
     std::vector<std::pair<const Bcsr*, const Bcsr*>> given_levels_;
+    std::vector<int> given_p_;  // kept modes of a p-transfer below level l (0: none)
     // U-AMG hierarchy and the level matrices made from it
     std::unique_ptr<UamgHierarchy> uamg_;
     std::vector<std::unique_ptr<Bcsr>> uamg_mats_;

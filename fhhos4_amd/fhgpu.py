@@ -72,6 +72,7 @@ _SIGS = {
     "fh_nn_train_step": [_vp, _vp, _vp, _vp, _vp, C.c_float, C.POINTER(C.c_double)],
     "fh_solver_set_nn": [_vp, _vp, _vp, C.c_double, C.c_int],
     "fh_solver_set_stop": [_vp, C.c_double, C.c_int],
+    "fh_solver_set_level_p": [_vp, C.c_int, C.c_int],
     "fh_solver_export_levels": [_vp, C.c_char_p],
     "fh_hho_assemble_gpu": [_vp, _vp, C.c_int, _pp, _vp],
     "fh_solver_apply_prec": [_vp, _vp, _vp],
@@ -533,6 +534,11 @@ class Solver:
         """Learned post-smoother inside FCG (before setup)."""
         self._nn = (nn, graph)
         _check(lib().fh_solver_set_nn(self.h, nn.h, graph.h, omega, int(unit_step)))
+
+    def set_level_p(self, level, nfc):
+        """p-injection from level+1 to level keeping nfc modes per face
+        (Galerkin operator below unless set_level gives one)."""
+        _check(lib().fh_solver_set_level_p(self.h, level, nfc))
 
     def set_level(self, level, A=None, P=None):
         self._levels.append((A, P))

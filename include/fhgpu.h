@@ -212,6 +212,14 @@ int fh_solver_destroy(fh_solver* s);
  * src/Solver/Multigrid/Level.h:11,32; LevelForHHO.h:182-213).  Matrices must
  * outlive the solver (Solver.h:9,25-28). */
 int fh_solver_set_level(fh_solver* s, int level, const fh_mat* A_level, const fh_mat* P_level);
+/* p-multigrid level: the transfer from level+1 to level is the natural
+ * injection of hierarchical face bases keeping the first nfc modes of every
+ * face (P_LevelForHHO::SetupProlongation, src/Solver/Multigrid/
+ * MultigridForHHO/P_MultigridForHHO.h:62-86; R = P^T, :89-92); the operator
+ * of level+1 is its Galerkin operator R A P, the leading nfc x nfc sub-block
+ * of every block (Level::ComputeGalerkinOperator, Level.h:133-137), unless
+ * fh_solver_set_level gives one.  Levels below may be geometric (h) levels. */
+int fh_solver_set_level_p(fh_solver* s, int level, int nfc);
 /* Solver::Setup (Solver.h:25-28; Multigrid::Setup, Multigrid.h:88-313). */
 int fh_solver_setup(fh_solver* s, const fh_mat* A);
 /* IterativeSolver::Solve(b, "0") (IterativeSolver.h:55-73): x (device, FP64)

@@ -727,7 +727,13 @@ int or_solve(const or_solver_cfg* cfg, const or_level_desc* levels, const double
         for (int l = 0; l < cfg->nlevels; l++) {
             const or_level_desc& d = levels[l];
             mg.lv[l].A.view(d.n, d.n, d.rp, d.ci, d.v);
-            mg.lv[l].nf = nf;
+            mg.lv[l].nf = d.nf > 0 ? d.nf : nf;
+            if (l + 1 < cfg->nlevels && d.pmodes > 0) {
+                // p-injection of hierarchical bases (P_MultigridForHHO.h:62-86)
+                mg.lv[l].ptrunc = true;
+                mg.lv[l].nfc = d.pmodes;
+                continue;
+            }
             if (l + 1 < cfg->nlevels) {
                 Level& L = mg.lv[l];
                 L.P.view(d.n, d.nc, d.prp, d.pci, d.pv);
@@ -782,9 +788,9 @@ int or_solve(const or_solver_cfg* cfg, const or_level_desc* levels, const double
         // IterativeSolver::StoppingCriteriaReached (IterativeSolver.h:224-247)
         if (cfg->max_iter == 0) return true;
         if (it == 0) return false;
+        if (it >= cfg->max_iter) { status = res < cfg->tol ? 0 : 1; return true; }
         if (!std::isfinite(res)) { status = -4; return true; }
         if (res < cfg->tol) { status = 0; return true; }
-        if (it >= cfg->max_iter) { status = 1; return true; }
         return false;
     };
     if (cfg->krylov == 1) {

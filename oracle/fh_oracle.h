@@ -66,6 +66,10 @@ typedef struct or_level_desc {
     /* prolongation to this level from the next coarser one (NULL for the
      * coarsest level or for aggregation levels): n x nc scalar CSR */
     int nc; const int* prp; const int* pci; const double* pv;
+    /* block size of this level (0: the configuration's nf), and, instead of
+     * a prolongation matrix, a p-transfer to the next level keeping the first
+     * pmodes modes of every block (0: use prp/pci/pv) */
+    int nf; int pmodes;
 } or_level_desc;
 
 typedef struct or_solver_cfg {

@@ -18,7 +18,8 @@ _f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 
 class LevelDesc(C.Structure):
     _fields_ = [("n", C.c_int), ("rp", C.c_void_p), ("ci", C.c_void_p), ("v", C.c_void_p),
-                ("nc", C.c_int), ("prp", C.c_void_p), ("pci", C.c_void_p), ("pv", C.c_void_p)]
+                ("nc", C.c_int), ("prp", C.c_void_p), ("pci", C.c_void_p), ("pv", C.c_void_p),
+                ("nf", C.c_int), ("pmodes", C.c_int)]
 
 
 class SolverCfg(C.Structure):
@@ -152,17 +153,24 @@ def solve(A, b, solver="cg", prec="mg", nf=1, cycle="V", nu=(1, 1), smoothers=("
           levels=None, tol=1e-8, max_iter=200, max_coarse=1000, theta=0.25, coarse_smoother=None, coarse_omega=0.0,
           fcg_truncation=1, fcg_restart=False):
     """Runs the oracle solver; `levels` = [(A_l, P_l), ...] for GMG (P of the
-    coarsest level is None).  Returns (x, hist, out)."""
+    coarsest level is None).  A level may be (A_l, ("p", nfc), nf_l): a
+    p-transfer keeping nfc modes of every nf_l-block; (A_l, P_l, nf_l) gives
+    the block size of a level whose size differs from nf.  Returns (x, hist,
+    out)."""
     keep = []
     if levels is None:
         levels = [(A, None)]
     descs = (LevelDesc * len(levels))()
-    for l, (Al, Pl) in enumerate(levels):
+    for l, lev in enumerate(levels):
+        Al, Pl = lev[0], lev[1]
+        descs[l].nf = lev[2] if len(lev) > 2 else 0
         rp, ci, v = _csr(Al)
         keep += [rp, ci, v]
         descs[l].n = Al.shape[0]
         descs[l].rp, descs[l].ci, descs[l].v = rp.ctypes.data, ci.ctypes.data, v.ctypes.data
-        if Pl is not None:
+        if isinstance(Pl, tuple) and Pl[0] == "p":
+            descs[l].pmodes = Pl[1]
+        elif Pl is not None:
             prp, pci, pv = _csr(Pl)
             keep += [prp, pci, pv]
             descs[l].nc = Pl.shape[1]

@@ -87,11 +87,17 @@ ASSEMBLY_CASES = [
     dict(dim=2, mesh="stri", n=8, k=2, perturb=0.2, kappa="lognormal", sigma=2.0, source="modes", source_seed=7),
     dict(dim=3, mesh="cart", n=4, k=2, kappa="subdomain", sigma=3.0, subdomain=2),
     dict(dim=2, mesh="cart", n=8, k=3),
+    dict(dim=3, mesh="hex", n=4, k=3, perturb=0.15, kappa="lognormal", sigma=3.0, seed=11),
+    dict(dim=3, mesh="hex", n=5, k=1, perturb=0.15, seed=12, source="modes", source_seed=3),
+    dict(dim=3, mesh="hex", n=4, k=0, perturb=0.15, seed=13),
+    dict(dim=2, mesh="quad", n=8, k=2, perturb=0.3, kappa="lognormal", sigma=2.0, seed=14),
 ]
 
 
 @pytest.mark.parametrize("kw", ASSEMBLY_CASES, ids=["tet-k2-perturbed", "tet-k0", "tet-k3-lognormal",
-                                                    "tri-k2-modes", "hex-k2-subdomains", "quad-k3"])
+                                                    "tri-k2-modes", "hex-k2-subdomains", "quad-k3",
+                                                    "trilinear-hex-k3-lognormal", "trilinear-hex-k1-modes",
+                                                    "trilinear-hex-k0", "bilinear-quad-k2-lognormal"])
 def test_gpu_assembly_matches_host(ctx, kw):
     """Local operators, static condensation and assembly on the GPU give the
     host assembly's operator and right-hand side (to rounding: the GPU fuses
@@ -171,7 +177,9 @@ def test_heterogeneous_mixed_precision_fcg(ctx, store):
 
 
 @pytest.mark.parametrize("kw", ASSEMBLY_CASES, ids=["tet-k2-perturbed", "tet-k0", "tet-k3-lognormal",
-                                                    "tri-k2-modes", "hex-k2-subdomains", "quad-k3"])
+                                                    "tri-k2-modes", "hex-k2-subdomains", "quad-k3",
+                                                    "trilinear-hex-k3-lognormal", "trilinear-hex-k1-modes",
+                                                    "trilinear-hex-k0", "bilinear-quad-k2-lognormal"])
 def test_gpu_l2_error_matches_host(ctx, kw):
     """Cell recovery, k+1 reconstruction and the quadrature of the L2 error on
     the GPU give the host's relative error, for the discrete solution and for
@@ -234,3 +242,19 @@ def test_gmg_chebyshev_on_sliced_and_chunked_levels_matches_oracle(ctx):
     assert st["converged"] == 1 and st["iterations"] == out.iterations
     xg = x.cpu().numpy()
     assert np.linalg.norm(xg - xo) <= 1e-10 * np.linalg.norm(xo)
+
+
+@pytest.mark.parametrize("kw", [dict(dim=3, mesh="hex", n=4, k=3, perturb=0.15, kappa="lognormal", sigma=3.0, seed=11),
+                                dict(dim=2, mesh="quad", n=8, k=2, perturb=0.3, seed=14)],
+                         ids=["trilinear-hex-k3", "bilinear-quad-k2"])
+def test_gpu_l2_error_general_cells(ctx, kw):
+    """Cell recovery, reconstruction and L2 quadrature of the general-cell
+    kernel against the host on the same face solution."""
+    P = G.HhoProblem(source="pisine", **kw)
+    import scipy.sparse.linalg as spl
+    x = spl.spsolve(P.scipy().tocsc(), P.rhs())
+    xd = torch.from_numpy(x).cuda()
+    torch.cuda.synchronize()
+    eh = P.l2_error(x)
+    eg = P.l2_error_gpu(ctx, xd)
+    assert abs(eg - eh) <= 1e-10 * eh

@@ -315,3 +315,31 @@ def test_oracle_stopping_rules():
     assert out.iterations == 0 and not x.any() and len(hist) == 1
     with pytest.raises(RuntimeError):
         O.solve(A, np.zeros(A.shape[0]), "cg", "none", nf=2)
+
+
+def test_oracle_p_transfer_is_the_injection_galerkin_closed_form():
+    """The oracle's p-transfer (truncation to the first nfc modes, injection
+    back) is the explicit p-injection P = [I_nfc; 0] per block, and its
+    Galerkin operator P^T A P is the leading nfc x nfc sub-block of every
+    block: a two-level solve through the p-transfer equals the solve through
+    the explicit matrix P with the coarse operator formed as P^T A P."""
+    import scipy.sparse as sp
+    from conftest import csr, load
+    d = load("cart8_k2_cg_mg_bj23")
+    A = csr(d, "A")
+    nf, nfc = 6, 1
+    nb = A.shape[0] // nf
+    rows = (np.arange(nb)[:, None] * nf + np.arange(nfc)[None, :]).ravel()
+    cols = (np.arange(nb)[:, None] * nfc + np.arange(nfc)[None, :]).ravel()
+    P = sp.csr_matrix((np.ones(rows.size), (rows, cols)), shape=(nb * nf, nb * nfc))
+    Ac = (P.T @ A @ P).tocsr()
+    Ab = sp.bsr_matrix(A, blocksize=(nf, nf))
+    sub = sp.bsr_matrix((Ab.data[:, :nfc, :nfc], Ab.indices, Ab.indptr), shape=(nb * nfc, nb * nfc)).tocsr()
+    assert abs(Ac - sub).max() == 0.0
+    b = d["b"]
+    kw = dict(nf=nf, smoothers=("bj23", "bj23"), max_iter=100)
+    x1, h1, o1 = O.solve(A, b, "cg", "mg", levels=[(A, ("p", nfc), nf), (sub, None, nfc)], **kw)
+    x2, h2, o2 = O.solve(A, b, "cg", "mg", levels=[(A, P, nf), (Ac, None, nfc)], **kw)
+    assert o1.iterations == o2.iterations
+    np.testing.assert_allclose(h1, h2, rtol=1e-10, atol=1e-14)
+    assert np.linalg.norm(x1 - x2) <= 1e-10 * np.linalg.norm(x2)
