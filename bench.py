@@ -377,22 +377,73 @@ def c5_bench(args, G, np, torch):
         % (hier_s, pm, [h[0].n_unknowns for h in H["h"]]))
     x = torch.zeros_like(b)
     torch.cuda.synchronize()
-    base = dict(prec="mg", nu=(args.nu, args.nu), smoothers=(args.smoother, args.smoother),
-                coarse_smoother=args.coarse_smoother, galerkin=True, tol=1e-8, max_iter=args.max_iter)
-    label = "hp-MG V(%d,%d) p%s+Galerkin h(k=0) %s, coarse levels %s" % (args.nu, args.nu, args.p_modes,
-                                                                         args.smoother, args.coarse_smoother)
+    amg = args.c5_prec == "aggregamg"
+    base = dict(prec=args.c5_prec, nu=(args.nu, args.nu), smoothers=(args.smoother, args.smoother),
+                coarse_smoother=args.coarse_smoother, galerkin=not amg, tol=1e-8, max_iter=args.max_iter,
+                cycle=args.c5_cycle, kcycle_levels=args.kcycle_levels)
+    if amg:
+        label = ("p-first aggregation AMG (p-injection to the face means, double pairwise aggregation) %s(%d,%d)%s "
+                 "%s, coarse levels %s" % (args.c5_cycle, args.nu, args.nu,
+                                           " K on levels 1-%d" % args.kcycle_levels if args.c5_cycle == "K" else "",
+                                           args.smoother, args.coarse_smoother))
+    else:
+        label = "hp-MG %s(%d,%d) p%s+Galerkin h(k=0) %s, coarse levels %s" % (
+            args.c5_cycle, args.nu, args.nu, args.p_modes, args.smoother, args.coarse_smoother)
+    krylov = "fcg" if args.c5_cycle == "K" else "cg"
 
     def solver(**kw):
         a = dict(base)
         a.update(kw)
         s = G.Solver(ctx, **a)
-        gmg.attach_hp(s, H)
+        if not amg:
+            gmg.attach_hp(s, H)
         t0 = time.perf_counter()
         s.setup(A)
         return s, time.perf_counter() - t0
 
+    if args.sweep:
+        # FP64 CG with hierarchy / cycle / smoother variants: "key=value,..."
+        # with p (modes kept, "3:1"), cycle, sm, csm, prec (mg | aggregamg)
+        for spec in args.sweep_list.split(";"):
+            kv = dict(t.split("=") for t in spec.split(",") if t)
+            pmv = tuple(int(t) for t in kv.get("p", args.p_modes).replace(":", ",").split(","))
+            prec = kv.get("prec", "mg")
+            a = dict(base, prec=prec, cycle=kv.get("cycle", "V"), smoothers=(kv.get("sm", args.smoother),) * 2,
+                     coarse_smoother=kv.get("csm", args.coarse_smoother), solver=kv.get("solver", "cg"))
+            if "nu" in kv:
+                a["nu"] = (int(kv["nu"]), int(kv["nu"]))
+            if "m" in kv:
+                a["fcg_truncation"] = int(kv["m"])
+            if "kl" in kv:
+                a["kcycle_levels"] = int(kv["kl"])
+            if "mc" in kv:
+                a["max_coarse"] = int(kv["mc"])
+            if "store" in kv:
+                a.update(mg_prec="f32", mg_store=kv["store"])
+            if "cstore" in kv:
+                a.update(mg_prec="f32", coarse_store=kv["cstore"])
+            try:
+                Hv = H if pmv == pm or prec != "mg" else gmg.build_hp_hierarchy(ctx, P, p_modes=pmv)
+                if prec != "mg":
+                    a["galerkin"] = False
+                s = G.Solver(ctx, **a)
+                if prec == "mg":
+                    gmg.attach_hp(s, Hv)
+                t0 = time.perf_counter()
+                s.setup(A)
+                su = time.perf_counter() - t0
+                ts, st = timed_solves(ctx, s, b, x, 1, 0)
+                out = {"variant": spec, "s": ts[0], "iterations": st["iterations"], "converged": st["converged"],
+                       "setup_s": su, "ms_per_it": 1e3 * ts[0] / max(1, st["iterations"]),
+                       "levels": [s.level_size(l) for l in range(s64_levels(s))]}
+                del s
+            except Exception as e:  # report and continue with the next variant
+                out = {"variant": spec, "error": str(e)}
+            log(json.dumps(out))
+            print(json.dumps(out), flush=True)
+        return
     log("set-up FP64 MG-PCG:", label)
-    s64, setup64 = solver(solver="cg")
+    s64, setup64 = solver(solver=krylov)
     levels = [s64.level_size(l) for l in range(s64_levels(s64))]
     log("levels", levels, "setup %.1fs" % setup64)
     ts64, st64 = timed_solves(ctx, s64, b, x, args.steps, args.warmup)
@@ -427,7 +478,7 @@ def c5_bench(args, G, np, torch):
     dom = max(kern, key=lambda k: kern[k]["ms_per_solve"])
     mixed = {}
     if not args.no_mixed:
-        for store, cstore in (("f32", None), ("bf16", None), ("f32", "bf16")):
+        for store, cstore in tuple(tuple(v.split(":")) if ":" in v else (v, None) for v in args.c5_mixed.split(",")):
             key = store if cstore is None else "%s_coarse_%s" % (store, cstore)
             log("set-up FCG + %s multigrid" % key)
             s, su = solver(solver="fcg", mg_prec="f32", mg_store=store, coarse_store=cstore)
@@ -447,8 +498,8 @@ def c5_bench(args, G, np, torch):
         "ms_per_step": t64 * 1e3, "higher_is_better": False, "scaling": "strong", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic",
         "config": {"workload": "3-D diffusion, %d^3 trilinear hexahedra perturbed +-%.2fh, HHO k=%d (%d unknowns), "
-                               "per-cell lognormal kappa sigma %.1f; CG + %s, tol 1e-8"
-                               % (args.hex_n, args.hex_perturb, args.hex_k, N, args.hex_sigma, label),
+                               "per-cell lognormal kappa sigma %.1f; %s + %s, tol 1e-8"
+                               % (args.hex_n, args.hex_perturb, args.hex_k, N, args.hex_sigma, krylov.upper(), label),
                    "unknowns": N, "faces": nb, "blocks": nnzb, "levels": levels},
         "iterations": st64["iterations"], "setup_s": setup64, "hierarchy_s": hier_s, "l2_error": l2_64,
         "postprocess_s": post_s, "dof_it_per_s": N * st64["iterations"] / t64,
@@ -498,6 +549,9 @@ def main():
     ap.add_argument("--train-rhs", type=int, default=100)
     ap.add_argument("--train-epochs", type=int, default=8)
     ap.add_argument("--sweep", action="store_true", help="time a list of solver variants")
+    ap.add_argument("--sweep-list", default="cycle=V;cycle=W;p=3:1;p=3:1,cycle=W;sm=cheb3;cycle=W,sm=cheb3;"
+                                            "prec=aggregamg,cycle=K,solver=fcg;prec=aggregamg,cycle=V",
+                    help="c5 --sweep: variants separated by ';'")
     ap.add_argument("--kernel", choices=["spmv"], help="time one kernel family alone")
     ap.add_argument("--hex-n", type=int, default=96, help="c5: hexahedra per direction")
     ap.add_argument("--hex-k", type=int, default=3)
@@ -506,6 +560,11 @@ def main():
     ap.add_argument("--p-modes", default="1", help="c5: modes kept by the p-levels, e.g. 1 or 3,1")
     ap.add_argument("--leg-steps", type=int, default=2, help="timed solves of each mixed / learned leg")
     ap.add_argument("--cpu-iters", type=int, default=3, help="c5: oracle iterations timed on the full system")
+    ap.add_argument("--c5-prec", default="aggregamg", choices=["aggregamg", "mg"],
+                    help="c5 multigrid: p-first aggregation AMG, or the geometric p-then-h hierarchy")
+    ap.add_argument("--c5-cycle", default="K", choices=["V", "W", "K"])
+    ap.add_argument("--kcycle-levels", type=int, default=2, help="K-cycle on levels 1..this one, V below")
+    ap.add_argument("--c5-mixed", default="f32", help="c5 mixed legs: fine[:coarse] storage, comma-separated")
     ap.add_argument("--reps", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1:

@@ -447,6 +447,7 @@ int fh_solver_create(fh_ctx* ctx, const fh_solver_args* a, fh_solver** out) {
         if (a->coarse_smoother) s.coarse_smoother = a->coarse_smoother;
         s.coarse_relax = a->coarse_relax;
         s.fcg_truncation = a->fcg_truncation > 0 ? a->fcg_truncation : 1;
+        s.kcycle_levels = a->kcycle_levels > 0 ? a->kcycle_levels : 0;
         FH_CHECK(s.fcg_truncation <= 32, -2, "FCG truncation is limited to 32 directions");
         s.fcg_restart = a->fcg_restart != 0;
         if (!s.coarse_smoother.empty())

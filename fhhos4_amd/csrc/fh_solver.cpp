@@ -544,7 +544,7 @@ public:
         MgLevel& C = L[l + 1];
         restrict_to(l, V.r, C.b);
         bool coarse_is_last = l + 2 == (int)L.size();
-        if (a_.cycle == 'K' && !coarse_is_last) {
+        if (a_.cycle == 'K' && !coarse_is_last && (a_.kcycle_levels <= 0 || l + 1 <= a_.kcycle_levels)) {
             kcycle(l + 1, C.b, C.x);
         } else {
             int loops = a_.cycle == 'W' ? std::max(2, a_.wloops) : 1;

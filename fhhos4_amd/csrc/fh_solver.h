@@ -98,6 +98,11 @@ struct SolverArgs {
     double coarse_relax = 0.0;
     int fcg_truncation = 1;          // FlexibleConjugateGradient::Truncation (FCG(1) by default)
     bool fcg_restart = false;        // FlexibleConjugateGradient::Restart
+    // K-cycle only on levels 1 .. kcycle_levels (0: every level below the
+    // finest, the reference's K-cycle); V-cycle recursion below.  Not in the
+    // reference: bounds the number of visits of the coarsest levels, which
+    // doubles with every K level.
+    int kcycle_levels = 0;
 };
 
 class Solver {

@@ -19,12 +19,13 @@ def build_hierarchy(ctx, problem, max_coarse=1000, keep=None):
     return out
 
 
-def _operator(ctx, problem):
+def _operator(ctx, problem, keep_host=True):
     """Level operator: assembled on the GPU for problems created with
     defer_assembly (the right-hand side is kept as problem.b_device), else
-    uploaded from the host assembly."""
+    uploaded from the host assembly.  keep_host keeps a host copy of the
+    values (needed by host set-up steps that read the operator)."""
     if problem.args.get("defer_assembly"):
-        A, b = problem.assemble_gpu(ctx, keep_host=True)
+        A, b = problem.assemble_gpu(ctx, keep_host=keep_host)
         problem.b_device = b
         return A
     return problem.matrix(ctx, keep_host=True)
@@ -54,7 +55,7 @@ def build_hp_hierarchy(ctx, problem, p_modes=(1,), h_degree=0, max_coarse=1000):
     nf_h = (h_degree + 1) if args["dim"] == 2 else (h_degree + 1) * (h_degree + 2) // 2
     if not p_modes or p_modes[-1] != nf_h:
         raise ValueError("the last p-level must keep the %d modes of the degree-%d h-levels" % (nf_h, h_degree))
-    A = _operator(ctx, problem)
+    A = _operator(ctx, problem, keep_host=False)  # the p-levels gather their sub-blocks on the device
     kw = dict(args)
     kw.update(k=h_degree, defer_assembly=True, ref_n=args.get("ref_n") or args["n"])
     chain = [G.HhoProblem(**kw)]

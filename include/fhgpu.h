@@ -188,6 +188,9 @@ typedef struct fh_solver_args {
     /* Not in the reference: storage of the level matrices below the finest
      * (0 = as mg_store, FH_F32, FH_BF16; needs FP32 multigrid vectors). */
     int coarse_store;
+    /* Not in the reference: K-cycle (cycle 'K') only on levels 1..kcycle_levels
+     * (<= 0: all, as Multigrid.h:611-656), V-cycle recursion below. */
+    int kcycle_levels;
 } fh_solver_args;
 
 typedef struct fh_solve_stats {
