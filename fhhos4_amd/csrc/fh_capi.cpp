@@ -338,6 +338,44 @@ int fh_bjac_sweep(fh_ctx* ctx, const fh_bjac* bj, const fh_mat* A, const void* b
                                   ctx->c.stream));
     });
 }
+int fh_bjac_sweep_noalloc(fh_ctx* ctx, const fh_bjac* bj, const fh_mat* A, const void* b, void* x, int nsweeps,
+                          int x_is_zero, int vprec) {
+    return guard([&] {
+        NEED(ctx); NEED_DEV(ctx); NEED(bj); NEED(A); NEED(b); NEED(x);
+        const fh::Bcsr& M = *A->A;
+        fh::Prec vp = P(vprec);
+        FH_CHECK(M.nbr == M.nbc && M.bs_r == M.bs_c, -2, "block Jacobi needs a square operator");
+        const size_t need = (size_t)M.rows() * fh::prec_bytes(vp);
+        FH_CHECK(need <= fh::kNoAllocScratchDoubles * sizeof(double), -5,
+                 "fh_bjac_sweep_noalloc: " + std::to_string(M.rows()) + " unknowns exceed the scratch of " +
+                     std::to_string(fh::kNoAllocScratchDoubles * sizeof(double) / fh::prec_bytes(vp)) +
+                     " entries; use fh_bjac_sweep with a work vector");
+        void* work = fh::noalloc_scratch(ctx->c.device);
+        struct Flag {
+            fh::Ctx& c;
+            explicit Flag(fh::Ctx& c_) : c(c_) { c.no_alloc = true; }
+            ~Flag() { c.no_alloc = false; }
+        } flag(ctx->c);
+        const int nf = M.bs_r;
+        void* cur = x;
+        void* nxt = work;
+        bool xz = x_is_zero != 0;
+        for (int s2 = 0; s2 < nsweeps; s2++) {
+            if (xz)
+                fh::launch_bjac_zero(ctx->c, M.nbr, nf, vp, b, cur, bj->S.spec.omega, bj->S.invD.p, bj->S.dprec,
+                                     nullptr, 0);
+            else {
+                fh::launch_bcsr(ctx->c, M, fh::OP_SMOOTH, vp, cur, b, nxt, 1.0, 0.0, bj->S.spec.omega, bj->S.invD.p,
+                                bj->S.dprec, nullptr, 0);
+                std::swap(cur, nxt);
+            }
+            xz = false;
+        }
+        if (cur != x)
+            FH_HIP(hipMemcpyAsync(x, cur, (size_t)M.rows() * fh::prec_bytes(vp), hipMemcpyDeviceToDevice,
+                                  ctx->c.stream));
+    });
+}
 int fh_coloring_create(fh_ctx* ctx, const fh_mat* A, fh_coloring** out) {
     return guard([&] {
         NEED(ctx); NEED(A); NEED(out);

@@ -48,6 +48,9 @@ struct Ctx {
     static constexpr int kScalars = 512;  // K-cycle uses 32 + 8 * level + 7 (levels < 40)
     // In-situ timing of the block-row kernel on one matrix (HIP events on
     // this stream around every launch of that operation).
+    // set while an allocation-free entry point runs: no execution-layout
+    // decision (which allocates) and no event-timed launches
+    bool no_alloc = false;
     const void* prof_matrix = nullptr;
     int prof_mode = -1;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_events;

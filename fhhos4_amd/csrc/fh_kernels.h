@@ -34,6 +34,15 @@ bool set_sliced(Ctx& c, const Bcsr& A, bool on);
 // matrix; a graph captured before a change must not be replayed after it.
 uint64_t layout_epoch();
 
+// Scratch of the allocation-free entry points: 2^21 doubles (16 MiB) per
+// device, a file-scope device array.  noalloc_scratch_init(device) looks up
+// the device's copy (call it with that device current, once per device);
+// noalloc_scratch(device) returns that address.
+constexpr size_t kNoAllocScratchDoubles = size_t(1) << 21;
+constexpr int kMaxDevices = 64;
+void noalloc_scratch_init(int device);
+void* noalloc_scratch(int device);
+
 // y = omega * D^-1 b over all block rows (smoothing from a zero guess); the
 // same values also into y2 when it is not null (first Chebyshev step).
 void launch_bjac_zero(Ctx& c, int nbr, int nf, Prec vprec, const void* b, void* y, double omega,

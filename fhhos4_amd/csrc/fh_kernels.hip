@@ -22,6 +22,7 @@
 #include "fh_kernels.h"
 
 #include <atomic>
+#include <mutex>
 
 #include <algorithm>
 #include <type_traits>
@@ -1153,6 +1154,39 @@ bool set_sliced(Ctx& c, const Bcsr& A, bool on) {
 
 uint64_t layout_epoch() { return g_layout_epoch.load(); }
 
+// ------------------------------------------------ allocation-free scratch
+// Device memory of the allocation-free entry points: one array per GPU the
+// library runs on, held until the process ends.  Host code never passes the
+// array's name to a kernel (on the host it is a stand-in, not a device
+// address): the address of each device's copy is looked up once, at context
+// creation with that device current, and only that address is used.
+static __device__ __align__(256) double scratch[kNoAllocScratchDoubles];
+
+namespace {
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+void noalloc_scratch_init(int device) {
+    FH_CHECK(device >= 0 && device < kMaxDevices, -2, "device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return;
+    int cur = -1;
+    FH_HIP(hipGetDevice(&cur));
+    FH_CHECK(cur == device, -3, "the scratch address must be looked up with its device current");
+    void* ptr = nullptr;
+    FH_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    FH_CHECK(ptr != nullptr, -3, "no device address for the scratch array");
+    g_scratch_ptr[device] = ptr;
+}
+
+void* noalloc_scratch(int device) {
+    FH_CHECK(device >= 0 && device < kMaxDevices, -2, "device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    FH_CHECK(g_scratch_ptr[device] != nullptr, -3, "scratch array not set up on this device (create a context first)");
+    return g_scratch_ptr[device];
+}
+
 static void launch_bcsr_impl(Ctx& c, const Bcsr& A, BcsrMode mode, Prec vprec, const void* x, const void* b, void* y,
                              double alpha, double beta, double omega, const void* invD, Prec dprec, const int* rows,
                              int nrows, void* aux);
@@ -1160,8 +1194,8 @@ static void launch_bcsr_impl(Ctx& c, const Bcsr& A, BcsrMode mode, Prec vprec, c
 void launch_bcsr(Ctx& c, const Bcsr& A, BcsrMode mode, Prec vprec, const void* x, const void* b, void* y,
                  double alpha, double beta, double omega, const void* invD, Prec dprec, const int* rows,
                  int nrows) {
-    if (!rows && A.sliced_state == 0) sliced_decide(c, A);
-    const bool prof = c.prof_matrix == (const void*)&A && c.prof_mode == (int)mode && !rows;
+    if (!rows && A.sliced_state == 0 && !c.no_alloc) sliced_decide(c, A);
+    const bool prof = c.prof_matrix == (const void*)&A && c.prof_mode == (int)mode && !rows && !c.no_alloc;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (prof) {
         FH_HIP(hipEventCreate(&e0));
@@ -1178,8 +1212,8 @@ void launch_bcsr(Ctx& c, const Bcsr& A, BcsrMode mode, Prec vprec, const void* x
 void launch_bcsr_cheb(Ctx& c, const Bcsr& A, Prec vprec, const void* x, const void* b, void* xnew, void* d,
                       double a, double cc, const void* invD, Prec dprec) {
     FH_CHECK(xnew != x && d != nullptr && d != x && d != xnew, -3, "Chebyshev step: x, xnew and d must differ");
-    if (A.sliced_state == 0) sliced_decide(c, A);
-    const bool prof = c.prof_matrix == (const void*)&A && c.prof_mode == (int)OP_CHEB;
+    if (A.sliced_state == 0 && !c.no_alloc) sliced_decide(c, A);
+    const bool prof = c.prof_matrix == (const void*)&A && c.prof_mode == (int)OP_CHEB && !c.no_alloc;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (prof) {
         FH_HIP(hipEventCreate(&e0));

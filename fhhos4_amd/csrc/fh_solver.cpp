@@ -32,6 +32,7 @@ Ctx::Ctx(int dev) : device(dev) {
     FH_HIP(hipHostMalloc(&h_scalars, sizeof(double) * kScalars, hipHostMallocDefault));
     FH_HIP(hipMemsetAsync(d_scalars, 0, sizeof(double) * kScalars, stream));
     FH_HIP(hipStreamSynchronize(stream));
+    noalloc_scratch_init(dev);  // this device is current
 }
 
 Ctx::~Ctx() {

@@ -89,6 +89,7 @@ _SIGS = {
     "fh_bjac_destroy": [_vp],
     "fh_bjac_inverses": [_vp, _vp, _f64p],
     "fh_bjac_sweep": [_vp, _vp, _vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int],
+    "fh_bjac_sweep_noalloc": [_vp, _vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int],
     "fh_coloring_create": [_vp, _vp, _pp],
     "fh_coloring_destroy": [_vp],
     "fh_coloring_get": [_vp, C.POINTER(C.c_int), _i32p],
@@ -429,6 +430,15 @@ class BlockJacobi:
         pb = _dev(b, n, like=x, name="b")
         pw = _dev(work, n, like=x, name="work")
         _check(lib().fh_bjac_sweep(self.ctx.h, self.h, M.h, pb, px, pw, nsweeps, int(x_is_zero), _vprec(x)))
+
+    @_ordered
+    def sweep_noalloc(self, b, x, nsweeps=1, x_is_zero=False, A=None):
+        """The same sweeps without a work vector (fh_bjac_sweep_noalloc)."""
+        M = A or self.A
+        n = M.rows
+        px = _dev(x, n, name="x")
+        pb = _dev(b, n, like=x, name="b")
+        _check(lib().fh_bjac_sweep_noalloc(self.ctx.h, self.h, M.h, pb, px, nsweeps, int(x_is_zero), _vprec(x)))
 
     def __del__(self):
         try:

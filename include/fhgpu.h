@@ -133,6 +133,17 @@ int fh_bjac_inverses(fh_ctx* ctx, const fh_bjac* bj, double* host_out);
  * Replaces BlockJacobi::ExecuteOneIteration (BlockJacobi.h:77-127). */
 int fh_bjac_sweep(fh_ctx* ctx, const fh_bjac* bj, const fh_mat* A, const void* b, void* x, void* work, int nsweeps,
                   int x_is_zero, int vprec);
+/* No counterpart in the reference.  fh_bjac_sweep for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): the ping-pong vector of the sweeps is a
+ * file-scope device array of 2^21 doubles (16 MiB) per GPU, held until the
+ * process ends, instead of the caller's `work`.  Vectors longer than
+ * 2^21 FP64 or 2^22 FP32 entries are rejected (-5).  The call allocates
+ * nothing: it never builds A's sliced / chunked copy (a matrix whose layout
+ * is not decided yet runs on the block-CSR kernels) and times no launches.
+ * Calls that overlap on one GPU share the array and race. */
+int fh_bjac_sweep_noalloc(fh_ctx* ctx, const fh_bjac* bj, const fh_mat* A, const void* b, void* x, int nsweeps,
+                          int x_is_zero, int vprec);
 /* Greedy colouring of the block graph (block-row order). */
 int fh_coloring_create(fh_ctx* ctx, const fh_mat* A, fh_coloring** out);
 int fh_coloring_destroy(fh_coloring* col);
