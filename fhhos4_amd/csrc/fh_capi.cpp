@@ -142,8 +142,15 @@ int fh_mat_blocks(const fh_mat* A, int* rowptr, int* colidx, double* vals) {
         if (rowptr) std::memcpy(rowptr, M.h_rowptr.data(), sizeof(int) * M.h_rowptr.size());
         if (colidx) std::memcpy(colidx, M.h_colidx.data(), sizeof(int) * M.h_colidx.size());
         if (vals) {
-            FH_CHECK(!M.h_vals.empty(), -3, "matrix created without host values (keep_host = 0)");
-            std::memcpy(vals, M.h_vals.data(), sizeof(double) * M.h_vals.size());
+            if (!M.h_vals.empty())
+                std::memcpy(vals, M.h_vals.data(), sizeof(double) * M.h_vals.size());
+            else {
+                // device-only FP64 values (GPU assembly): copied from HBM after
+                // the device's pending work
+                FH_CHECK(M.prec == fh::Prec::F64 && M.vals.p, -3, "matrix without FP64 values");
+                FH_HIP(hipDeviceSynchronize());
+                FH_HIP(hipMemcpy(vals, M.vals.p, M.value_bytes(), hipMemcpyDeviceToHost));
+            }
         }
     });
 }

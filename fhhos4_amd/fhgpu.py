@@ -307,6 +307,16 @@ class Matrix:
         """Writes the scalar operator in the reference's A.dat format."""
         _check(lib().fh_mat_export_market(self.ctx.h, self.h, os.fsencode(path)))
 
+    def blocks(self):
+        """Host copies of the block arrays (row pointers, block columns,
+        nnzb x bs x bs FP64 values; device-only matrices are read from HBM)."""
+        i = self.info()
+        rp = np.zeros(i["nbr"] + 1, dtype=np.int32)
+        ci = np.zeros(i["nnzb"], dtype=np.int32)
+        v = np.empty(i["nnzb"] * i["bs"] * i["bs"])
+        _check(lib().fh_mat_blocks(self.h, rp, ci, v))
+        return rp, ci, v.reshape(-1, i["bs"], i["bs"])
+
     def scipy(self):
         """Host copy as a scipy CSR matrix (matrix created with keep_host)."""
         import scipy.sparse as sp

@@ -51,8 +51,10 @@ int fh_mat_from_blocks(fh_ctx* ctx, int nbr, int nbc, int bs, const int* rowptr,
 int fh_mat_convert(fh_ctx* ctx, const fh_mat* A, int prec, fh_mat** out);
 int fh_mat_destroy(fh_mat* A);
 int fh_mat_info(const fh_mat* A, int* nbr, int* nbc, int* bs, int64_t* nnzb, int* prec);
-/* Host copy of the block arrays of a matrix created with keep_host (rowptr:
- * nbr+1, colidx: nnzb, vals: nnzb*bs*bs FP64); NULL pointers are skipped. */
+/* Host copy of the block arrays of a matrix (rowptr: nbr+1, colidx: nnzb,
+ * vals: nnzb*bs*bs FP64); NULL pointers are skipped.  Values come from the
+ * host copy (keep_host) or, for a device-only FP64 matrix, from HBM after
+ * the device's pending work. */
 int fh_mat_blocks(const fh_mat* A, int* rowptr, int* colidx, double* vals);
 /* Device pointers of the block structure and values (for profiling tools). */
 int fh_mat_device_arrays(const fh_mat* A, const int** rowptr, const int** colidx, const void** vals);

@@ -9,6 +9,8 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <unordered_map>
 #include <vector>
@@ -22,11 +24,39 @@ struct Csr {
     const int* prp = nullptr;
     const int* pci = nullptr;
     const double* pv = nullptr;
+    // block storage instead of the scalar arrays (bs > 0): n / bs block rows,
+    // block row pointers brp, block columns bci, dense bs x bs row-major
+    // blocks bv (64-bit value offsets: the finest level of the hexahedral
+    // k = 3 system holds 2.9e9 values)
+    int bs = 0;
+    const int* brp = nullptr;
+    const int* bci = nullptr;
+    const double* bv = nullptr;
     void view(int n_, int m_, const int* r, const int* c, const double* val) {
         n = n_; m = m_; prp = r; pci = c; pv = val;
     }
     void own() { prp = rp.data(); pci = ci.data(); pv = v.data(); }
 };
+
+// y = A x over the block rows of a block-stored matrix.
+void bspmv(const Csr& A, const double* x, double* y) {
+    const int bs = A.bs, nb = A.n / bs;
+#pragma omp parallel for schedule(static)
+    for (int I = 0; I < nb; I++) {
+        double acc[64];
+        for (int a = 0; a < bs; a++) acc[a] = 0;
+        for (int p = A.brp[I]; p < A.brp[I + 1]; p++) {
+            const double* blk = A.bv + (size_t)p * bs * bs;
+            const double* xj = x + (size_t)A.bci[p] * bs;
+            for (int a = 0; a < bs; a++) {
+                double t = 0;
+                for (int c = 0; c < bs; c++) t += blk[a * bs + c] * xj[c];
+                acc[a] += t;
+            }
+        }
+        for (int a = 0; a < bs; a++) y[(size_t)I * bs + a] = acc[a];
+    }
+}
 
 double dot(int n, const double* a, const double* b) {
     double s = 0;
@@ -36,6 +66,10 @@ double dot(int n, const double* a, const double* b) {
 double nrm2(int n, const double* a) { return std::sqrt(dot(n, a, a)); }
 
 void spmv(const Csr& A, const double* x, double* y) {
+    if (A.bs) {
+        bspmv(A, x, y);
+        return;
+    }
 #pragma omp parallel for schedule(static)
     for (int i = 0; i < A.n; i++) {
         double s = 0;
@@ -77,6 +111,10 @@ int block_inverse(const Csr& A, int nf, std::vector<double>& invD) {
 #pragma omp parallel for schedule(static)
     for (int i = 0; i < nb; i++) {
         std::vector<double> D(nf * nf, 0.0);
+        if (A.bs) {
+            for (int p = A.brp[i]; p < A.brp[i + 1]; p++)
+                if (A.bci[p] == i) std::copy(A.bv + (size_t)p * nf * nf, A.bv + (size_t)(p + 1) * nf * nf, D.begin());
+        } else
         for (int k = 0; k < nf; k++) {
             int row = i * nf + k;
             for (int p = A.prp[row]; p < A.prp[row + 1]; p++) {
@@ -330,6 +368,18 @@ void dinv_resid(const Csr& A, int nf, const double* invD, const double* b, const
 #pragma omp parallel for schedule(static)
     for (int i = 0; i < nb; i++) {
         double tmp[64];
+        if (A.bs) {
+            for (int k = 0; k < nf; k++) tmp[k] = b ? b[(size_t)i * nf + k] : 0.0;
+            for (int p = A.brp[i]; p < A.brp[i + 1]; p++) {
+                const double* blk = A.bv + (size_t)p * nf * nf;
+                const double* xj = x + (size_t)A.bci[p] * nf;
+                for (int k = 0; k < nf; k++) {
+                    double t = 0;
+                    for (int c = 0; c < nf; c++) t += blk[k * nf + c] * xj[c];
+                    tmp[k] -= t;
+                }
+            }
+        } else
         for (int k = 0; k < nf; k++) {
             int row = i * nf + k;
             double t = b ? b[row] : 0.0;
@@ -406,6 +456,10 @@ struct MG {
                 }
                 continue;
             }
+            if (L.A.bs) {  // block-stored levels: Chebyshev smoothing only
+                std::fprintf(stderr, "oracle: block-stored levels are smoothed by Chebyshev only\n");
+                std::abort();
+            }
             if (kind == SM_BJ) bj_sweep(L.A, nf, L.invD.data(), omega, b, x);
             else if (kind >= SM_MC_F) {
                 if (L.cptr.empty()) greedy_colour_blocks(L.A, nf, L.cptr, L.crows);
@@ -478,7 +532,7 @@ struct MG {
         restrict_(l, r, rc);
         int nc = (int)rc.size();
         ec.assign(nc, 0.0);
-        if (cfg->cycle == 'K' && l + 1 < (int)lv.size() - 1)
+        if (cfg->cycle == 'K' && l + 1 < (int)lv.size() - 1 && (cfg->kcycle_levels <= 0 || l + 1 <= cfg->kcycle_levels))
             fcg_kcycle(l + 1, rc, ec);
         else {
             int loops = cfg->cycle == 'W' ? std::max(2, cfg->wloops) : 1;
@@ -566,9 +620,16 @@ int build_amg(MG& mg, const Csr& A0, int nf, double theta, int max_coarse, int m
         C.A.n = C.A.m = nb;
         C.A.rp.assign(nb + 1, 0);
         for (int i = 0; i < nb; i++) {
-            int row = i * nf;
-            for (int p = A0.prp[row]; p < A0.prp[row + 1]; p++)
-                if (A0.pci[p] % nf == 0) { C.A.ci.push_back(A0.pci[p] / nf); C.A.v.push_back(A0.pv[p]); }
+            if (A0.bs) {
+                for (int p = A0.brp[i]; p < A0.brp[i + 1]; p++) {
+                    C.A.ci.push_back(A0.bci[p]);
+                    C.A.v.push_back(A0.bv[(size_t)p * nf * nf]);
+                }
+            } else {
+                int row = i * nf;
+                for (int p = A0.prp[row]; p < A0.prp[row + 1]; p++)
+                    if (A0.pci[p] % nf == 0) { C.A.ci.push_back(A0.pci[p] / nf); C.A.v.push_back(A0.pv[p]); }
+            }
             C.A.rp[i + 1] = (int)C.A.ci.size();
         }
         C.A.own();
@@ -711,11 +772,30 @@ void or_last_times(double* setup_s, double* solve_s) {
     *solve_s = g_solve_s;
 }
 
+static int solve_impl(const or_solver_cfg* cfg, const Csr& A, const or_level_desc* levels, const double* b, double* x,
+                      double* hist, or_solver_out* out);
+
 int or_solve(const or_solver_cfg* cfg, const or_level_desc* levels, const double* b, double* x,
              double* hist, or_solver_out* out) {
+    Csr A; A.view(levels[0].n, levels[0].n, levels[0].rp, levels[0].ci, levels[0].v);
+    return solve_impl(cfg, A, levels, b, x, hist, out);
+}
+
+int or_solve_bcsr(const or_solver_cfg* cfg, int nb, const int* brp, const int* bci, const double* bv, const double* b,
+                  double* x, double* hist, or_solver_out* out) {
+    if (cfg->prec != 0 && cfg->prec != 4) return -5;  // identity or the aggregation AMG (p-first)
+    if (cfg->prec == 0 && cfg->krylov == 0) return -5;
+    Csr A;
+    A.n = A.m = nb * cfg->nf;
+    A.bs = cfg->nf;
+    A.brp = brp; A.bci = bci; A.bv = bv;
+    return solve_impl(cfg, A, nullptr, b, x, hist, out);
+}
+
+static int solve_impl(const or_solver_cfg* cfg, const Csr& A, const or_level_desc* levels, const double* b, double* x,
+                      double* hist, or_solver_out* out) {
     auto t0 = std::chrono::steady_clock::now();
     const int nf = cfg->nf;
-    Csr A; A.view(levels[0].n, levels[0].n, levels[0].rp, levels[0].ci, levels[0].v);
     const int n = A.n;
     MG mg;
     mg.cfg = cfg;

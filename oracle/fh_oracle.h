@@ -94,6 +94,7 @@ typedef struct or_solver_cfg {
     double coarse_omega;
     int fcg_truncation;  /* FCG: number of kept directions (<= 0: 1) */
     int fcg_restart;     /* FCG: clear instead of dropping the oldest */
+    int kcycle_levels;   /* K-cycle only on levels 1..kcycle_levels (<= 0: all), V recursion below */
 } or_solver_cfg;
 
 typedef struct or_solver_out {
@@ -109,6 +110,12 @@ typedef struct or_solver_out {
  * negative on error. */
 int or_solve(const or_solver_cfg* cfg, const or_level_desc* levels, const double* b, double* x,
              double* hist, or_solver_out* out);
+/* The same with the system matrix in block storage (nb block rows of nf x nf
+ * dense blocks, cfg->nf; int32 block indices, 64-bit value offsets), for
+ * systems whose scalar entries overflow int32: no preconditioner, or the
+ * p-first aggregation AMG (prec 4) with Chebyshev smoothing. */
+int or_solve_bcsr(const or_solver_cfg* cfg, int nb, const int* brp, const int* bci, const double* bv, const double* b,
+                  double* x, double* hist, or_solver_out* out);
 
 /* Aggregation AMG hierarchy only: number of levels, unknowns per level, and
  * the first-level aggregate map (length n/nf). */

@@ -28,7 +28,7 @@ class SolverCfg(C.Structure):
                 ("post_omega", C.c_double), ("nu1", C.c_int), ("nu2", C.c_int), ("cycle", C.c_char),
                 ("wloops", C.c_int), ("max_coarse", C.c_int), ("theta", C.c_double), ("tol", C.c_double),
                 ("max_iter", C.c_int), ("nlevels", C.c_int), ("coarse_kind", C.c_int), ("coarse_omega", C.c_double),
-                ("fcg_truncation", C.c_int), ("fcg_restart", C.c_int)]
+                ("fcg_truncation", C.c_int), ("fcg_restart", C.c_int), ("kcycle_levels", C.c_int)]
 
 
 class SolverOut(C.Structure):
@@ -151,7 +151,7 @@ PREC_KIND = {"": 0, "none": 0, "bj": 1, "bgs": 2, "mg": 3, "aggregamg": 4}
 
 def solve(A, b, solver="cg", prec="mg", nf=1, cycle="V", nu=(1, 1), smoothers=("bj", "bj"), omega=1.0,
           levels=None, tol=1e-8, max_iter=200, max_coarse=1000, theta=0.25, coarse_smoother=None, coarse_omega=0.0,
-          fcg_truncation=1, fcg_restart=False):
+          fcg_truncation=1, fcg_restart=False, kcycle_levels=0):
     """Runs the oracle solver; `levels` = [(A_l, P_l), ...] for GMG (P of the
     coarsest level is None).  A level may be (A_l, ("p", nfc), nf_l): a
     p-transfer keeping nfc modes of every nf_l-block; (A_l, P_l, nf_l) gives
@@ -186,6 +186,7 @@ def solve(A, b, solver="cg", prec="mg", nf=1, cycle="V", nu=(1, 1), smoothers=("
     cfg.post_omega = 2.0 / 3.0 if smoothers[1] == "bj23" else omega
     cfg.nu1, cfg.nu2 = nu
     cfg.fcg_truncation, cfg.fcg_restart = fcg_truncation, int(fcg_restart)
+    cfg.kcycle_levels = kcycle_levels
     if coarse_smoother:
         cfg.coarse_kind = SMOOTHER_KIND[coarse_smoother]
         cfg.coarse_omega = 2.0 / 3.0 if coarse_smoother == "bj23" else (coarse_omega if coarse_omega > 0 else omega)
@@ -200,6 +201,50 @@ def solve(A, b, solver="cg", prec="mg", nf=1, cycle="V", nu=(1, 1), smoothers=("
     hist = np.zeros(max_iter + 2)
     out = SolverOut()
     st = lib().or_solve(C.byref(cfg), descs, np.ascontiguousarray(b, dtype=np.float64), x, hist, C.byref(out))
+    if st < 0:
+        raise RuntimeError("oracle solve failed with status %d" % st)
+    return x, hist[: out.iterations + 1], out
+
+
+def solve_bcsr(rowptr, colidx, vals, nf, b, solver="fcg", prec="aggregamg", cycle="K", nu=(1, 1),
+               smoothers=("cheb2", "cheb2"), tol=1e-8, max_iter=200, max_coarse=1000, theta=0.25,
+               coarse_smoother=None, kcycle_levels=0):
+    """Oracle solve with the matrix in block storage (block row pointers,
+    block columns, nnzb x nf x nf values): no preconditioner or the p-first
+    aggregation AMG with Chebyshev smoothing.  Returns (x, hist, out)."""
+    rp = np.ascontiguousarray(rowptr, dtype=np.int32)
+    ci = np.ascontiguousarray(colidx, dtype=np.int32)
+    v = np.ascontiguousarray(vals, dtype=np.float64).reshape(-1)
+    nb = len(rp) - 1
+    cfg = SolverCfg()
+    cfg.krylov = {"none": 0, "cg": 1, "fcg": 2}[solver]
+    cfg.prec = PREC_KIND[prec]
+    cfg.nf = nf
+    cfg.omega = 1.0
+    cfg.pre_kind = SMOOTHER_KIND[smoothers[0]]
+    cfg.post_kind = SMOOTHER_KIND[smoothers[1]]
+    cfg.pre_omega = cfg.post_omega = 1.0
+    cfg.nu1, cfg.nu2 = nu
+    cfg.fcg_truncation, cfg.fcg_restart = 1, 0
+    if coarse_smoother:
+        cfg.coarse_kind = SMOOTHER_KIND[coarse_smoother]
+        cfg.coarse_omega = 1.0
+    cfg.cycle = cycle.encode()
+    cfg.wloops = 2 if cycle == "W" else 1
+    cfg.max_coarse = max_coarse
+    cfg.theta = theta
+    cfg.tol = tol
+    cfg.max_iter = max_iter
+    cfg.nlevels = 0
+    cfg.kcycle_levels = kcycle_levels
+    x = np.zeros(nb * nf)
+    hist = np.zeros(max_iter + 2)
+    out = SolverOut()
+    L = lib()
+    L.or_solve_bcsr.argtypes = [C.POINTER(SolverCfg), C.c_int, _i32p, _i32p, _f64p, _f64p, _f64p, _f64p,
+                                C.POINTER(SolverOut)]
+    st = L.or_solve_bcsr(C.byref(cfg), nb, rp, ci, v, np.ascontiguousarray(b, dtype=np.float64), x, hist,
+                         C.byref(out))
     if st < 0:
         raise RuntimeError("oracle solve failed with status %d" % st)
     return x, hist[: out.iterations + 1], out

@@ -83,3 +83,30 @@ def test_gpu_assembled_hierarchy_and_mixed_precision(ctx):
     l2h = Ph.l2_error(xs["host"])
     l2g = Pd.l2_error_gpu(ctx, torch.from_numpy(xs["gpu"]).cuda())
     assert abs(l2g - l2h) <= 1e-8 * l2h
+
+
+@pytest.mark.parametrize("k,n,kl", [(3, 16, 2), (3, 16, 0), (1, 16, 2)], ids=["k3-K2", "k3-Kall", "k1-K2"])
+def test_pfirst_amg_kcycle_matches_oracle(ctx, k, n, kl):
+    """FCG + the p-first aggregation AMG (p-injection to the face means,
+    double pairwise aggregation of the scalar level) with a K-cycle on the
+    levels 1..kl, the benchmark solver, against the oracle's own hierarchy
+    and cycle on the same scalar matrix: the same level sizes and iteration
+    count, the solution to 1e-10."""
+    P = G.HhoProblem(dim=3, mesh="hex", n=n, k=k, perturb=0.15, seed=2026, kappa="lognormal", sigma=3.0,
+                     source="pisine")
+    As = P.scipy()
+    A = G.Matrix.from_csr(ctx, As, P.nf, keep_host=True)
+    kw = dict(nu=(1, 1), smoothers=("cheb2", "cheb2"), coarse_smoother="cheb1", max_iter=400)
+    s = G.Solver(ctx, solver="fcg", prec="aggregamg", cycle="K", kcycle_levels=kl, tol=1e-8, **kw)
+    s.setup(A)
+    b = torch.from_numpy(P.rhs()).cuda()
+    x = torch.zeros_like(b)
+    torch.cuda.synchronize()
+    st = s.solve(b, x)
+    assert st["converged"]
+    xo, ho, oo = O.solve(As, P.rhs(), "fcg", "aggregamg", nf=P.nf, cycle="K", kcycle_levels=kl, **kw)
+    assert [s.level_size(l) for l in range(oo.nlevels)] == list(oo.level_n[:oo.nlevels])
+    assert oo.nlevels >= 4  # a p-level and at least two aggregation levels
+    assert st["iterations"] == oo.iterations, (st["iterations"], oo.iterations)
+    xg = x.cpu().numpy()
+    assert np.linalg.norm(xg - xo) <= 1e-10 * np.linalg.norm(xo)

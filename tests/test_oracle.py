@@ -343,3 +343,22 @@ def test_oracle_p_transfer_is_the_injection_galerkin_closed_form():
     assert o1.iterations == o2.iterations
     np.testing.assert_allclose(h1, h2, rtol=1e-10, atol=1e-14)
     assert np.linalg.norm(x1 - x2) <= 1e-10 * np.linalg.norm(x2)
+
+
+def test_oracle_block_storage_path_equals_scalar_path():
+    """The oracle's block-stored system matrix (the CPU baseline of the
+    26-million-unknown hexahedral system, whose scalar entries overflow int32)
+    runs the same p-first aggregation AMG K-cycle FCG as the scalar path,
+    which the reference's fixtures pin: the same level sizes and residual
+    history, the same solution to rounding."""
+    import fhhos4_amd.fhgpu as G
+    P = G.HhoProblem(dim=3, mesh="hex", n=10, k=2, perturb=0.15, seed=2026, kappa="lognormal", sigma=3.0,
+                     source="pisine")
+    rp, ci, v = P.blocks()
+    kw = dict(nu=(1, 1), smoothers=("cheb2", "cheb2"), coarse_smoother="cheb1", max_iter=400)
+    x1, h1, o1 = O.solve(P.scipy(), P.rhs(), "fcg", "aggregamg", nf=P.nf, cycle="K", kcycle_levels=2, **kw)
+    x2, h2, o2 = O.solve_bcsr(rp, ci, v, P.nf, P.rhs(), "fcg", "aggregamg", cycle="K", kcycle_levels=2, **kw)
+    assert o1.iterations == o2.iterations and o1.nlevels == o2.nlevels >= 3
+    assert list(o1.level_n[:o1.nlevels]) == list(o2.level_n[:o2.nlevels])
+    np.testing.assert_allclose(h1, h2, rtol=1e-9, atol=1e-15)
+    assert np.linalg.norm(x1 - x2) <= 1e-12 * np.linalg.norm(x1)
