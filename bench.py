@@ -395,6 +395,56 @@ def c5_cpu_baseline(args, A, b, nf, gpu_iterations, base, krylov):
     return res
 
 
+def c5_learned_leg(args, G, ctx, P, A, b, x, x64, t64, base, mixed_ref):
+    """FCG with the FP32 multigrid and the learned post-smoother on the
+    hexahedral system.  The network (n_f = 10) is trained on the GPU by the
+    library's kernels on (residual, error) pairs of plain FCG solves of the
+    same family on a smaller mesh with another kappa field (--train-n, random
+    sinusoid right-hand sides), then runs on the full system."""
+    import numpy as np
+    from fhhos4_amd import learned as Lr
+    t0 = time.perf_counter()
+    fam = dict(dim=3, mesh="hex", n=args.train_n, k=args.hex_k, perturb=args.hex_perturb, seed=args.seed + 1,
+               kappa="lognormal", sigma=args.hex_sigma)
+    train_kw = dict(base)
+    train_kw.pop("tol", None)
+    train_kw.pop("max_iter", None)
+    sy = Lr.System(ctx, G.HhoProblem(source="pisine", **fam), train_kw, tol=1e-12, max_iter=args.max_iter)
+    rhs = [G.HhoProblem(source="modes", source_seed=1000 + j, **fam).rhs() for j in range(args.train_rhs)]
+    pairs = Lr.make_pairs(ctx, sy, rhs, iters=(1, 3, 6))
+    t_pairs = time.perf_counter() - t0
+    nn = G.Mpnn(ctx, P.nf, layers=3, seed=1)
+    before = Lr.mean_loss(nn, pairs)
+    t1 = time.perf_counter()
+    hist = Lr.train(nn, pairs, epochs=args.train_epochs, lr=2e-3, log=log)
+    t_train = time.perf_counter() - t1
+    log("learned smoother (n_f %d, %d parameters): %d pairs (%.1fs), training %.1fs, energy error %.3f -> %.3f"
+        % (P.nf, nn.num_params(), len(pairs), t_pairs, t_train, before, hist[-1]))
+    graph = G.NnGraph(ctx, A, G.face_features(P))
+    out = {}
+    for m in sorted({1, args.learned_fcg_m}):
+        s = G.Solver(ctx, **dict(base, solver="fcg", mg_prec="f32", mg_store="f32", fcg_truncation=m))
+        s.set_learned_smoother(nn, graph, 1.0)
+        t2 = time.perf_counter()
+        s.setup(A)
+        su = time.perf_counter() - t2
+        ts, st = timed_solves(ctx, s, b, x, args.leg_steps, 1)
+        xm = x.cpu().numpy()
+        out["fcg%d" % m] = {"s": float(np.mean(ts)), "iterations": st["iterations"], "setup_s": su,
+                            "converged": st["converged"], "accepted": st["nn_used"], "fallback": st["nn_fallback"],
+                            "rel_err_vs_fp64": float(np.linalg.norm(xm - x64) / np.linalg.norm(x64)),
+                            "speedup_vs_fp64": t64 / float(np.mean(ts)),
+                            "vs_mixed_without": float(np.mean(ts)) / mixed_ref if mixed_ref else None}
+        log("FCG(%d) + f32 MG + learned: %d iterations (%d accepted, %d fallback), %.3f s"
+            % (m, st["iterations"], st["nn_used"], st["nn_fallback"], out["fcg%d" % m]["s"]))
+        del s
+    out["train"] = {"n": args.train_n, "pairs": len(pairs), "pairs_s": t_pairs, "train_s": t_train,
+                    "epochs": args.train_epochs, "energy_error_untrained": before, "energy_error_trained": hist[-1],
+                    "params": nn.num_params()}
+    del graph
+    return out
+
+
 def c5_bench(args, G, np, torch):
     """The hexahedral k = 3 workload (see the module docstring)."""
     from fhhos4_amd import gmg
@@ -531,6 +581,9 @@ def c5_bench(args, G, np, torch):
                 % (key, st["iterations"], mixed[key]["s"], t64 / mixed[key]["s"], mixed[key]["rel_err_vs_fp64"]))
             del s
     best = min(mixed.values(), key=lambda v: v["s"]) if mixed else None
+    learned = None
+    if not args.no_learned:
+        learned = c5_learned_leg(args, G, ctx, P, A, b, x, x64, t64, base, best["s"] if best else None)
     cpu = None
     if not args.no_cpu_baseline:
         cpu = c5_cpu_baseline(args, A, b.cpu().numpy(), nf, st64["iterations"], base, krylov)
@@ -553,6 +606,7 @@ def c5_bench(args, G, np, torch):
         "postprocess_s": post_s, "dof_it_per_s": N * st64["iterations"] / t64,
         "fine_kernels": kern, "fine_layout": layout,
         "mixed_fcg": mixed,
+        "learned_fcg": learned,
         "best_mixed_vs_fp64": (best["s"] / t64) if best else None,
         "whole_solve_GBs": None if gb_iter is None else gb_iter * st64["iterations"] / t64 / 1e9,
         "roofline": {"bound": "hbm", "achieved": kern[dom]["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",

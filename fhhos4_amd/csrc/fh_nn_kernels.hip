@@ -546,7 +546,8 @@ __global__ void k_nn_gather_index(int64_t nnzb, const int* __restrict__ idx, con
 __global__ void k_nn_select(int64_t count, int nf, const double* s, int slot_s1, int slot_s2,
                             const double* __restrict__ delta, const double* __restrict__ r1,
                             const double* __restrict__ invD, double omega, int unit_step, double* __restrict__ z,
-                            int* __restrict__ counters) {
+                            int* __restrict__ counters, const double* __restrict__ r,
+                            const double* __restrict__ Adelta, double* __restrict__ Az) {
     const double s1 = s[slot_s1], s2 = s[slot_s2];
     const bool use = unit_step ? (s2 - 2.0 * s1 < 0) : (s1 > 0 && s2 > 0);
     const double alpha = unit_step ? 1.0 : s1 / s2;
@@ -555,6 +556,8 @@ __global__ void k_nn_select(int64_t count, int nf, const double* s, int slot_s1,
     if (t >= count) return;
     if (use) {
         z[t] += alpha * delta[t];
+        // A z = A z0 + alpha A delta with A z0 = r - r1 (no product)
+        if (Az) Az[t] = (r[t] - r1[t]) + alpha * Adelta[t];
     } else {
         const int64_t i = t / nf;
         const int a = (int)(t - i * nf);
@@ -669,9 +672,10 @@ void nn_gather_index(Ctx& c, int64_t nnzb, const int* idx, const float* src, flo
     FH_HIP(hipGetLastError());
 }
 void nn_select(Ctx& c, int64_t count, int nf, int slot_s1, int slot_s2, const double* delta, const double* r1,
-               const double* invD, double omega, bool unit_step, double* z, int* counters) {
+               const double* invD, double omega, bool unit_step, double* z, int* counters, const double* r,
+               const double* Adelta, double* Az) {
     hipLaunchKernelGGL(k_nn_select, dim3(g1(count)), dim3(256), 0, c.stream, count, nf, c.d_scalars, slot_s1, slot_s2,
-                       delta, r1, invD, omega, (int)unit_step, z, counters);
+                       delta, r1, invD, omega, (int)unit_step, z, counters, r, Adelta, Az);
     FH_HIP(hipGetLastError());
 }
 

@@ -817,8 +817,23 @@ void Solver::apply_prec(const DVec& r, DVec& z, DVec* Az) {
         const void* ys[2] = {r1_.data(), Adelta_.data()};
         int sl[2] = {kNnSlotS1, kNnSlotS2};
         launch_dots(c_, n, P, 2, xs, ys, sl);
+        // an accepted correction also gives A z = (r - r1) + alpha A delta;
+        // a block-Jacobi fallback (rare) needs the product
         nn_select(c_, n, A_->bs_r, kNnSlotS1, kNnSlotS2, (const double*)delta_.data(), (const double*)r1_.data(),
-                  nng_->invD64.as<double>(), nn_omega_, nn_unit_step_, (double*)z.data(), nn_counters_.as<int>());
+                  nng_->invD64.as<double>(), nn_omega_, nn_unit_step_, (double*)z.data(), nn_counters_.as<int>(),
+                  Az ? (const double*)r.data() : nullptr, (const double*)Adelta_.data(),
+                  Az ? (double*)Az->data() : nullptr);
+        if (Az) {
+            int cnt[2];
+            FH_HIP(hipMemcpyAsync(cnt, nn_counters_.p, sizeof(cnt), hipMemcpyDeviceToHost, c_.stream));
+            c_.sync();
+            if (cnt[1] != nn_fallbacks_seen_) {
+                nn_fallbacks_seen_ = cnt[1];
+                launch_bcsr(c_, *A_, OP_SPMV, Prec::F64, z.data(), nullptr, Az->data(), 1.0, 0.0, 0.0, nullptr,
+                            Prec::F64, nullptr, 0);
+            }
+        }
+        return;
     }
     if (Az) launch_bcsr(c_, *A_, OP_SPMV, Prec::F64, z.data(), nullptr, Az->data(), 1.0, 0.0, 0.0, nullptr, Prec::F64, nullptr, 0);
 }
@@ -828,7 +843,10 @@ void Solver::Solve(const double* b, double* x) {
     c_.sync();
     auto t0 = std::chrono::steady_clock::now();
     hist_.clear();
-    if (nn_) launch_fill(c_, 2, nn_counters_.p, Prec::F32, 0.0);  // two int zeros (same bit pattern as 0.0f)
+    if (nn_) {
+        launch_fill(c_, 2, nn_counters_.p, Prec::F32, 0.0);  // two int zeros (same bit pattern as 0.0f)
+        nn_fallbacks_seen_ = 0;
+    }
     if (args_.solver == "cg") solve_cg(b, x);
     else if (args_.solver == "fcg") solve_fcg(b, x);
     else solve_stationary(b, x);
@@ -926,9 +944,10 @@ void Solver::solve_fcg(const double* b, double* x) {
         while (std::find(kept.begin(), kept.end(), cur) != kept.end()) cur++;
         DVec& d = fd_[cur];
         DVec& Ad = fAd_[cur];
-        apply_prec(r_, z_, fused_ ? &Az_ : nullptr);
+        const bool haz = fused_ || nn_ != nullptr;  // the preconditioner returns A z
+        apply_prec(r_, z_, haz ? &Az_ : nullptr);
         copy(c_, n, P, z_.data(), d.data());
-        if (fused_) copy(c_, n, P, Az_.data(), Ad.data());
+        if (haz) copy(c_, n, P, Az_.data(), Ad.data());
         // z . Adk for every kept direction (three per launch)
         for (size_t q = 0; q < kept.size(); q += 3) {
             const void* xs[3];
@@ -945,10 +964,10 @@ void Solver::solve_fcg(const double* b, double* x) {
         for (int k : kept) {
             // d -= (z.Adk / dk.Adk) dk
             launch_axpby(c_, n, P, fd_[k].data(), d.data(), -1.0, slot_zad(k), slot_dad(k), 1.0, -1, -1);
-            if (fused_)
+            if (haz)
                 launch_axpby(c_, n, P, fAd_[k].data(), Ad.data(), -1.0, slot_zad(k), slot_dad(k), 1.0, -1, -1);
         }
-        if (!fused_)
+        if (!haz)
             launch_bcsr(c_, *A_, OP_SPMV, P, d.data(), nullptr, Ad.data(), 1.0, 0.0, 0.0, nullptr, P, nullptr, 0);
         dot2(c_, n, P, d.data(), Ad.data(), S_DAD, r_.data(), d.data(), S_RD);
         launch_cg_update(c_, n, P, x, d.data(), r_.data(), Ad.data(), S_RD, S_DAD, 1.0);

@@ -155,6 +155,7 @@ private:
     std::unique_ptr<LevelSmoother> prec_sm_;   // bj / bgs as preconditioner
     std::vector<std::pair<const Bcsr*, const Bcsr*>> given_levels_;
     std::vector<int> given_p_;  // kept modes of a p-transfer below level l (0: none)
+    int nn_fallbacks_seen_ = 0;  // block-Jacobi fallbacks of the learned post-smoother in this solve
     // U-AMG hierarchy and the level matrices made from it
     std::unique_ptr<UamgHierarchy> uamg_;
     std::vector<std::unique_ptr<Bcsr>> uamg_mats_;
