@@ -226,22 +226,24 @@ def hetero_bench(args, G, np, torch):
 
 
 def learned2d_bench(args, G, np, torch):
-    """Learned post-smoother on 2-D perturbed triangles (HHO k = 2, per-cell
-    lognormal kappa, sigma = 2): trained on the --train-n2d mesh with random
+    """Learned post-smoother on 2-D perturbed quadrilaterals (--mesh2d quad;
+    bilinear cells, HHO k = 2, per-cell lognormal kappa, sigma = 2; "stri":
+    the perturbed triangles): trained on the --train-n2d mesh with random
     sinusoid right-hand sides (pairs from plain FCG solves), evaluated on the
     --n mesh with unseen kappa and right-hand-side seeds.  FCG with the
-    aggregation AMG (symmetric multicolour block Gauss-Seidel) without a
-    post-step, with the block-Jacobi post-step (the untrained network) and
-    with the trained network; energy-norm gain on held-out pairs of the
-    evaluation mesh.  Perturbed quadrilaterals are not affine, so the
-    affine triangles of the same vertex perturbation stand in for them."""
+    multigrid without a post-step, with the block-Jacobi post-step (the
+    untrained network) and with the trained network; energy-norm gain on
+    held-out pairs of the evaluation mesh."""
     from fhhos4_amd import gmg
     from fhhos4_amd import learned as Lr
     ctx = G.Context(0)
     if args.prec2d == "aggregamg":
-        # K-cycle, as the reference's aggregation AMG defaults to (main.cpp:1874-1884)
-        solver = dict(prec="aggregamg", cycle="K", nu=(1, 1), smoothers=("sbgs", "sbgs"), fcg_truncation=8)
-        label = "FCG(8) + aggregation AMG K(1,1) sbgs"
+        # the p-first aggregation AMG with a K-cycle on the two top levels
+        # (the hexahedral benchmark's preconditioner)
+        solver = dict(prec="aggregamg", cycle=args.cycle2d, kcycle_levels=2, nu=(1, 1),
+                      smoothers=("cheb2", "cheb2"), coarse_smoother="cheb1", fcg_truncation=args.learned_fcg_m)
+        label = "FCG(%d) + p-first aggregation AMG %s(1,1)%s, cheb2 / cheb1" % (
+            args.learned_fcg_m, args.cycle2d, " on levels 1-2" if args.cycle2d == "K" else "")
     else:
         solver = dict(prec="mg", cycle="V", nu=(1, 1), smoothers=("cheb2", "cheb2"), galerkin=True,
                       fcg_truncation=8)
@@ -251,7 +253,7 @@ def learned2d_bench(args, G, np, torch):
         hier = gmg.build_hierarchy(ctx, problem) if args.prec2d == "gmg" else None
         return Lr.System(ctx, problem, solver, tol=tol, max_iter=args.max_iter, hierarchy=hier)
 
-    base = dict(dim=2, mesh="stri", k=2, perturb=0.2, kappa="lognormal", sigma=2.0)
+    base = dict(dim=2, mesh=args.mesh2d, k=2, perturb=0.2, kappa="lognormal", sigma=2.0)
     t0 = time.perf_counter()
     train_sys = system(G.HhoProblem(n=args.train_n2d, seed=11, source="pisine", **base))
     rhs = [G.HhoProblem(n=args.train_n2d, seed=11, source="modes", source_seed=1000 + j, **base).rhs()
@@ -273,9 +275,10 @@ def learned2d_bench(args, G, np, torch):
     untrained = G.Mpnn(ctx, nf, layers=3, seed=1)
     gain = {"untrained": Lr.mean_loss(untrained, held), "trained": Lr.mean_loss(nn, held), "pairs": len(held)}
     b = P.rhs()
-    out = {"workload": "2-D perturbed triangles, %d^2 x 2 cells, HHO k=2 (%d unknowns), lognormal kappa sigma 2; "
-                       "trained on %d^2 x 2 (%d pairs, %d epochs)" % (args.n, P.n_unknowns, args.train_n2d,
-                                                                      len(pairs), args.train_epochs),
+    cells = "quadrilaterals" if args.mesh2d == "quad" else "triangles (2 per square)"
+    out = {"workload": "2-D perturbed %s, %d^2 squares, HHO k=2 (%d unknowns), lognormal kappa sigma 2; "
+                       "trained on %d^2 (%d pairs, %d epochs)" % (cells, args.n, P.n_unknowns, args.train_n2d,
+                                                                  len(pairs), args.train_epochs),
            "solver": label,
            "train": {"pairs_s": t_pairs, "train_s": t_train, "energy_error_before": before, "after": hist[-1]},
            "held_out_energy_error": gain}
@@ -631,7 +634,9 @@ def main():
                          "learned2d: the learned post-smoother on 2-D triangles (use --n 512)")
     ap.add_argument("--train-n2d", type=int, default=128)
     ap.add_argument("--learned-fcg-m", type=int, default=8, help="FCG truncation of the learned leg")
-    ap.add_argument("--prec2d", default="gmg", choices=["gmg", "aggregamg"])
+    ap.add_argument("--prec2d", default="aggregamg", choices=["gmg", "aggregamg"])
+    ap.add_argument("--mesh2d", default="quad", choices=["quad", "stri"])
+    ap.add_argument("--cycle2d", default="K", choices=["V", "K"])
     ap.add_argument("--k", type=int, default=2)
     ap.add_argument("--perturb", type=float, default=0.2)
     ap.add_argument("--seed", type=int, default=2026)
