@@ -383,7 +383,7 @@ def c5_cpu_baseline(args, A, b, nf, gpu_iterations, base, krylov):
         % (v.nbytes / 1e9, dl, args.cpu_iters))
     x, hist, out = O.solve_bcsr(rp, ci, v, nf, b, krylov, base["prec"], cycle=base["cycle"], nu=base["nu"],
                                 smoothers=base["smoothers"], coarse_smoother=base["coarse_smoother"],
-                                kcycle_levels=base["kcycle_levels"], max_iter=args.cpu_iters)
+                                kcycle_levels=base["kcycle_levels"], theta=base["theta"], max_iter=args.cpu_iters)
     setup_s, solve_s = O.last_times()
     del rp, ci, v
     per_it = solve_s / max(1, out.iterations)
@@ -471,16 +471,17 @@ def c5_bench(args, G, np, torch):
     torch.cuda.synchronize()
     amg = args.c5_prec == "aggregamg"
     base = dict(prec=args.c5_prec, nu=(args.nu, args.nu), smoothers=(args.smoother, args.smoother),
-                coarse_smoother=args.coarse_smoother, galerkin=not amg, tol=1e-8, max_iter=args.max_iter,
-                cycle=args.c5_cycle, kcycle_levels=args.kcycle_levels)
+                coarse_smoother=args.c5_coarse_smoother, galerkin=not amg, tol=1e-8, max_iter=args.max_iter,
+                cycle=args.c5_cycle, kcycle_levels=args.kcycle_levels, theta=args.c5_theta)
     if amg:
         label = ("p-first aggregation AMG (p-injection to the face means, double pairwise aggregation) %s(%d,%d)%s "
                  "%s, coarse levels %s" % (args.c5_cycle, args.nu, args.nu,
                                            " K on levels 1-%d" % args.kcycle_levels if args.c5_cycle == "K" else "",
-                                           args.smoother, args.coarse_smoother))
+                                           args.smoother, args.c5_coarse_smoother))
+        label += ", strong-coupling threshold %g" % args.c5_theta
     else:
         label = "hp-MG %s(%d,%d) p%s+Galerkin h(k=0) %s, coarse levels %s" % (
-            args.c5_cycle, args.nu, args.nu, args.p_modes, args.smoother, args.coarse_smoother)
+            args.c5_cycle, args.nu, args.nu, args.p_modes, args.smoother, args.c5_coarse_smoother)
     krylov = "fcg" if args.c5_cycle == "K" else "cg"
 
     def solver(**kw):
@@ -501,7 +502,7 @@ def c5_bench(args, G, np, torch):
             pmv = tuple(int(t) for t in kv.get("p", args.p_modes).replace(":", ",").split(","))
             prec = kv.get("prec", "mg")
             a = dict(base, prec=prec, cycle=kv.get("cycle", "V"), smoothers=(kv.get("sm", args.smoother),) * 2,
-                     coarse_smoother=kv.get("csm", args.coarse_smoother), solver=kv.get("solver", "cg"))
+                     coarse_smoother=kv.get("csm", args.c5_coarse_smoother), solver=kv.get("solver", "cg"))
             if "nu" in kv:
                 a["nu"] = (int(kv["nu"]), int(kv["nu"]))
             if "m" in kv:
@@ -510,6 +511,8 @@ def c5_bench(args, G, np, torch):
                 a["kcycle_levels"] = int(kv["kl"])
             if "mc" in kv:
                 a["max_coarse"] = int(kv["mc"])
+            if "theta" in kv:
+                a["theta"] = float(kv["theta"])
             if "store" in kv:
                 a.update(mg_prec="f32", mg_store=kv["store"])
             if "cstore" in kv:
@@ -671,6 +674,8 @@ def main():
     ap.add_argument("--c5-prec", default="aggregamg", choices=["aggregamg", "mg"],
                     help="c5 multigrid: p-first aggregation AMG, or the geometric p-then-h hierarchy")
     ap.add_argument("--c5-cycle", default="K", choices=["V", "W", "K"])
+    ap.add_argument("--c5-coarse-smoother", default="cheb3", help="c5: smoother of the levels below the finest")
+    ap.add_argument("--c5-theta", type=float, default=0.1, help="c5: strong-coupling threshold of the aggregation")
     ap.add_argument("--kcycle-levels", type=int, default=2, help="K-cycle on levels 1..this one, V below")
     ap.add_argument("--c5-mixed", default="f32", help="c5 mixed legs: fine[:coarse] storage, comma-separated")
     ap.add_argument("--reps", type=int, default=20)

@@ -134,6 +134,48 @@ __global__ __launch_bounds__(kWG) void k_bcsr(int nbr, const int* __restrict__ r
     }
 }
 
+// Scalar rows (n_f = 1): a group of G lanes per row (64 / G rows per
+// wavefront), the lanes of a group striding over the row's entries, the
+// partial sums reduced with xor-shuffles inside the group.  For the
+// aggregation levels, whose rows hold 10-60 entries: one wavefront per row
+// (k_bcsr<1>) leaves most lanes idle and launches a wavefront per row.
+template <int G, class TA, class TV, class TD, int MODE>
+__global__ __launch_bounds__(kWG) void k_csr_grp(int nbr, const int* __restrict__ rp, const int* __restrict__ ci,
+                                                  const TA* __restrict__ val, const TV* x, const TV* __restrict__ b,
+                                                  TV* y, double alpha, double beta, double omega,
+                                                  const TD* __restrict__ invD, const int* __restrict__ rows,
+                                                  int nrows, TV* __restrict__ aux) {
+    using AT = typename Acc<TV>::T;
+    const int lane = threadIdx.x & 63;
+    const int wg = xcd_remap(blockIdx.x, gridDim.x);
+    const int w = wg * (kWG / 64) + (threadIdx.x >> 6);
+    const int count = rows ? nrows : nbr;
+    if (w * (64 / G) >= count) return;  // wave-uniform
+    const int g = lane / G, gl = lane - g * G;
+    const int idx = w * (64 / G) + g;
+    AT acc = 0;
+    int br = 0;
+    if (idx < count) {
+        br = rows ? rows[idx] : idx;
+        const int s = rp[br], e = rp[br + 1];
+        for (int p = s + gl; p < e; p += G) acc += (AT)ldv(val + p) * (AT)x[ci[p]];
+    }
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+    if (idx >= count || gl != 0) return;
+    const int64_t row = br;
+    if (MODE == OP_SPMV) {
+        AT v = (AT)alpha * acc;
+        if (beta != 0.0) v += (AT)beta * (AT)y[row];
+        y[row] = (TV)v;
+    } else if (MODE == OP_RESID) {
+        y[row] = (TV)((AT)b[row] - acc);
+    } else {
+        const AT z = (AT)ldv(invD + row) * ((AT)b[row] - acc);
+        smooth_out<MODE>(row, z, x, y, aux, alpha, beta, omega);
+    }
+}
+
 // Two consecutive values (same block row of the block: NF is even).
 __device__ __forceinline__ void ld2(const double* p, double& a, double& b) {
     const double2 v = *(const double2*)p;
@@ -838,6 +880,32 @@ void bcsr_dispatch_mode(Ctx& c, const Bcsr& A, BcsrMode mode, const void* x, con
             FH_HIP(hipGetLastError());
             return;
         }
+    }
+    if constexpr (NF == 1) {
+        // rows of scalar levels: G lanes per row from the mean row length
+        const double mean = A.nbr > 0 ? (double)A.nnzb / A.nbr : 0.0;
+        const int G = mean <= 6 ? 4 : (mean <= 12 ? 8 : (mean <= 24 ? 16 : 32));
+        const int nwg = (int)(((int64_t)count * G + kWG - 1) / kWG);
+#define FH_CSR(GG, MD)                                                                                              \
+        hipLaunchKernelGGL((k_csr_grp<GG, TA, TV, TD, MD>), dim3(nwg), dim3(kWG), 0, c.stream, A.nbr, rp, ci, val, \
+                           xx, bb, yy, alpha, beta, omega, D, rows, nrows, ax)
+#define FH_CSR_G(MD)                          \
+        switch (G) {                           \
+            case 4: FH_CSR(4, MD); break;      \
+            case 8: FH_CSR(8, MD); break;      \
+            case 16: FH_CSR(16, MD); break;    \
+            default: FH_CSR(32, MD); break;    \
+        }
+        switch (mode) {
+            case OP_SPMV: FH_CSR_G(OP_SPMV); break;
+            case OP_RESID: FH_CSR_G(OP_RESID); break;
+            case OP_CHEB: FH_CSR_G(OP_CHEB); break;
+            default: FH_CSR_G(OP_SMOOTH); break;
+        }
+#undef FH_CSR_G
+#undef FH_CSR
+        FH_HIP(hipGetLastError());
+        return;
     }
     const int nwg = (count + 3) / 4;
     switch (mode) {
