@@ -44,24 +44,25 @@ def spmv_bytes(nb, nf, nnzb, sv=8, sx=8):
 
 # rocprofv3 FETCH_SIZE / WRITE_SIZE summaries of the fine-level SpMV, per layout
 TRAFFIC_SUMMARY = {"sliced": os.path.join(REPO, "profiles", "sliced_spmv_traffic_tet64_k2.md"),
-                   "bcsr": os.path.join(REPO, "profiles", "bcsr_spmv_traffic_tet64_k2.md")}
+                   "bcsr": os.path.join(REPO, "profiles", "bcsr_spmv_traffic_tet64_k2.md"),
+                   "c5": os.path.join(REPO, "profiles", "c5_hex96_k3_fp64_traffic.md")}
 
 
-def measured_traffic(workload_key, layout):
-    """HBM bytes per launch of the fine-level SpMV from the committed
+def measured_traffic(workload_key, summary, field="f64_bytes"):
+    """HBM bytes per launch of a fine-level kernel from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE summary of the same workload and
     kernel (scaled by the counters' reading for a streaming kernel of known
     size), or None."""
     import re
     try:
-        text = open(TRAFFIC_SUMMARY[layout]).read()
+        text = open(TRAFFIC_SUMMARY[summary]).read()
     except OSError:
         return None
     m = re.search(r"<!-- traffic: (\{.*?\}) -->", text)
     if not m:
         return None
     d = json.loads(m.group(1))
-    return d.get("f64_bytes") if d.get("workload") == workload_key else None
+    return d.get(field) if d.get("workload") == workload_key else None
 
 
 def timed_solves(ctx, solver, b, x, steps, warmup):
@@ -616,7 +617,10 @@ def c5_bench(args, G, np, torch):
         "best_mixed_vs_fp64": (best["s"] / t64) if best else None,
         "whole_solve_GBs": None if gb_iter is None else gb_iter * st64["iterations"] / t64 / 1e9,
         "roofline": {"bound": "hbm", "achieved": kern[dom]["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": kern[dom]["GBs"] / HBM_PEAK_GBS, "traffic": None,
+                     "frac": kern[dom]["GBs"] / HBM_PEAK_GBS,
+                     "traffic": measured_traffic("hex n=%d k=%d perturb %g sigma %g seed %d" % (
+                         args.hex_n, args.hex_k, args.hex_perturb, args.hex_sigma, args.seed), "c5", dom)
+                     if layout == "sliced" else None,
                      "kernel": "fine-level %s (%s layout, n_f = %d)" % (dom, layout, nf),
                      "bytes_per_launch": kern[dom]["bytes"], "mean_ms": kern[dom]["mean_ms"],
                      "launches": kern[dom]["launches_per_solve"]},

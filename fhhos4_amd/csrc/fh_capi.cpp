@@ -9,6 +9,7 @@
 #include "fh_nn.h"
 #include "fh_solver.h"
 #include "fh_hho.h"
+#include "fh_condense.h"
 #include "fh_uamg.h"
 
 struct fh_ctx {
@@ -539,9 +540,12 @@ int fh_solver_setup_uncondensed(fh_solver* s, const fh_mat* A, const fh_csr* A_T
     });
 }
 int fh_solver_solve(fh_solver* s, const double* b, double* x, fh_solve_stats* st) {
+    return fh_solver_solve_x0(s, b, x, 1, st);
+}
+int fh_solver_solve_x0(fh_solver* s, const double* b, double* x, int x_is_zero, fh_solve_stats* st) {
     return guard([&] {
         NEED(s); NEED(b); NEED(x);
-        s->s->Solve(b, x);
+        s->s->Solve(b, x, x_is_zero != 0);
         if (st) {
             const auto& t = s->s->Stats();
             st->iterations = t.iterations;
@@ -866,6 +870,23 @@ int fh_hho_face_cells(const fh_hho* h, int* out) {
         std::vector<int> v;
         h->p->FaceCells(v);
         std::memcpy(out, v.data(), sizeof(int) * v.size());
+    });
+}
+int fh_condense_batched(fh_ctx* ctx, int ncells, int nfc, int nc, int nf, int nfaces, const int* cell_faces,
+                        const double* local, const double* B_T, const double* B_F, double* b_out, fh_mat** out) {
+    return guard([&] {
+        NEED(ctx); NEED_DEV(ctx); NEED(cell_faces); NEED(local); NEED(out);
+        auto A = fh::condense_batched(ctx->c, ncells, nfc, nc, nf, nfaces, cell_faces, local, B_T, B_F, b_out);
+        auto m = new fh_mat;
+        m->A = std::move(A);
+        *out = m;
+    });
+}
+int fh_cell_recover_batched(fh_ctx* ctx, int ncells, int nfc, int nc, int nf, int nfaces, const int* cell_faces,
+                            const double* local, const double* B_T, const double* x_F, double* x_T) {
+    return guard([&] {
+        NEED(ctx); NEED_DEV(ctx); NEED(cell_faces); NEED(local); NEED(x_F); NEED(x_T);
+        fh::cell_recover_batched(ctx->c, ncells, nfc, nc, nf, nfaces, cell_faces, local, B_T, x_F, x_T);
     });
 }
 int fh_hho_cell_kappa(const fh_hho* h, double* out) {

@@ -304,82 +304,209 @@ __device__ __forceinline__ void ldnt4(const bf16_t* p, float& a00, float& a01, f
 }
 template <class T> __device__ __forceinline__ void ldnt4(const T*, float&, float&, float&, float&) {}
 
+// Non-temporal raw buffer loads (the resource's base and row offset in
+// scalar registers, the lane's byte offset in one VGPR).  Word 3 of a gfx9
+// buffer resource: 32-bit data format, no swizzle.
+constexpr int kBufferWord3 = 0x00020000;
+constexpr int kBufferNt = 2;  // cache policy bits of the load: non-temporal
+template <class T> constexpr bool kRawSliceLoads = std::is_same<T, double>::value || std::is_same<T, float>::value;
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ void ldbuf2(__amdgpu_buffer_rsrc_t r, int voff, int soff, double& a, double& b) {
+    const u32x4 u = __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, kBufferNt);
+    a = __hiloint2double((int)u.y, (int)u.x);
+    b = __hiloint2double((int)u.w, (int)u.z);
+}
+__device__ __forceinline__ void ldbuf2(__amdgpu_buffer_rsrc_t r, int voff, int soff, float& a, float& b) {
+    const u32x2 u = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, kBufferNt);
+    a = __uint_as_float(u.x);
+    b = __uint_as_float(u.y);
+}
+__device__ __forceinline__ void ldbuf1(__amdgpu_buffer_rsrc_t r, int voff, int soff, double& a) {
+    const u32x2 u = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, kBufferNt);
+    a = __hiloint2double((int)u.y, (int)u.x);
+}
+__device__ __forceinline__ void ldbuf1(__amdgpu_buffer_rsrc_t r, int voff, int soff, float& a) {
+    a = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, kBufferNt));
+}
+template <class T> __device__ __forceinline__ void ldbuf2(__amdgpu_buffer_rsrc_t, int, int, float&, float&) {}
+template <class T> __device__ __forceinline__ void ldbuf1(__amdgpu_buffer_rsrc_t, int, int, float&) {}
+
 constexpr int kSlicedWaves = kWG / 64;  // slices per workgroup
 
-// Block-row operation on the sliced layout (all rows).  Lane
-// (r, b, jp) multiplies column group jp of block slot b of row r by its x
-// entries, for the NF rows i of the block; the NF partial sums per lane go
-// to LDS and lane (r, i) < RW * NF adds the W * JP sums of its row.
-template <int NF, class TA, class TV, class TD, int MODE>
+// Orders this wavefront's LDS accesses: the lanes of a wavefront run in
+// lockstep and the LDS serves one wavefront's requests in order, so only
+// the compiler has to be kept from moving accesses across this point.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Block-row operation on the sliced layout (all rows), one slice per
+// wavefront.  Lane (r, b, jp) multiplies column group jp of block slot b of
+// row r by its x entries, for the NF rows i of the block; the NF partial
+// sums per lane go to LDS and 64 / (RW * NF) lanes per (r, i) add the
+// W * JP sums of its row.  Every wavefront works on its own LDS rows, so no
+// workgroup barrier is taken.  The loads are issued in one burst: column
+// indices, values, the entries of b, x, d (and y) of the slice's rows and,
+// for the smoothing modes, its inverse diagonal blocks; only the gather of
+// x waits, for the column indices.  The inverse blocks are spread over the
+// lanes (lane (r, i, m) holds entries CW m .. CW m + CW - 1 of row i of
+// block r) when RW * NF * JP <= 64, and D^-1 times the residual is then
+// summed with shuffles.  MAP: the rows of this part are rowmap[0 .. nbr)
+// (else 0 .. nbr).
+template <int NF, class TA, class TV, class TD, int MODE, bool MAP>
 __global__ __launch_bounds__(kWG) void k_sliced(int nbr, int nslices, int W, int RW, const int* __restrict__ cols,
-                                                const TA* __restrict__ sv, const TV* x, const TV* __restrict__ b,
-                                                TV* y, double alpha, double beta, double omega,
+                                                const TA* __restrict__ sv, const TV* x, const TV* b, TV* y,
+                                                double alpha, double beta, double omega,
                                                 const TD* __restrict__ invD, TV* __restrict__ aux,
                                                 const int* __restrict__ rowmap) {
     using AT = typename Acc<TV>::T;
     constexpr int CW = NF % 2 == 0 ? 2 : 1;
     constexpr int JP = NF / CW;
+    constexpr bool SMOOTHING = MODE == OP_CHEB || MODE == OP_SMOOTH;
     const int LPR = W * JP, ACT = RW * LPR;
-    __shared__ AT red[kSlicedWaves][64][NF + 1];
-    __shared__ AT res[kSlicedWaves][64];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    __shared__ AT red[kSlicedWaves][NF][64];
+    // the wavefront index as a uniform value: slice offsets stay scalar
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int s = xcd_remap(blockIdx.x, gridDim.x) * kSlicedWaves + wave;
+    if (s >= nslices) return;  // wavefront-uniform; no workgroup barrier follows
+
+    // product lane (rv, bs, jp)
+    const int rv = lane / LPR, rl = lane - rv * LPR, bs = rl / JP, jp = rl - bs * JP;
+    const bool act = lane < ACT;
+    const int c = act ? cols[(int64_t)s * RW * W + rv * W + bs] : -1;
+    AT a[NF][CW];
+    if (act) {
+        // The slice's value loads: its base in a buffer resource (scalar
+        // registers), the row offset scalar, the lane offset one VGPR.
+        // Padding slots hold zero values.
+        const TA* sbase = sv + (int64_t)s * NF * ACT * CW;
+        if constexpr (kRawSliceLoads<TA>) {
+            const auto rs =
+                __builtin_amdgcn_make_buffer_rsrc((void*)sbase, 0, (int)(NF * ACT * CW * sizeof(TA)), kBufferWord3);
+#pragma unroll
+            for (int k = 0; k < NF; k++) {
+                if constexpr (CW == 2)
+                    ldbuf2(rs, lane * CW * (int)sizeof(TA), k * ACT * CW * (int)sizeof(TA), a[k][0], a[k][1]);
+                else
+                    ldbuf1(rs, lane * (int)sizeof(TA), k * ACT * (int)sizeof(TA), a[k][0]);
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < NF; k++) {
+                if constexpr (CW == 2)
+                    ldnt2(sbase + (int64_t)k * ACT * 2 + lane * 2, a[k][0], a[k][1]);
+                else
+                    ldnt(sbase + (int64_t)k * ACT + lane, a[k][0]);
+            }
+        }
+    }
+
+    // output lane (r, i, part 0) of row r, entry i; parts 1 .. NP - 1 (NP a
+    // power of two) help to sum
+    const int NP = 1 << (31 - __clz(64 / (RW * NF)));
+    const int r = lane / (NF * NP), rem = lane - r * (NF * NP), i = rem % NF, part = rem / NF;
+    const int row = s * RW + r;
+    const bool out = r < RW && part == 0 && row < nbr;  // nbr: rows of this part
+    const int64_t e = (int64_t)(MAP && out ? rowmap[row] : row) * NF + i;
+    AT bv = 0, xv = 0, dv = 0, yv = 0;
+    if (out) {
+        if (MODE != OP_SPMV) bv = (AT)b[e];
+        if (SMOOTHING) xv = (AT)x[e];
+        if (MODE == OP_CHEB && alpha != 0.0) dv = (AT)aux[e];
+        if (MODE == OP_SPMV && beta != 0.0) yv = (AT)y[e];
+    }
+    // inverse-block lane (dr, di, dm)
+    const bool dist = SMOOTHING && RW * NF * JP <= 64;
+    const bool dlane = dist && lane < RW * NF * JP;
+    const int dr = dlane ? lane / (NF * JP) : 0;
+    const int di = dlane ? (lane - dr * (NF * JP)) / JP : 0;
+    const int dm = dlane ? lane - dr * (NF * JP) - di * JP : 0;
+    AT d0 = 0, d1 = 0;
+    const int drow = s * RW + dr;
+    if (dlane && drow < nbr) {
+        const TD* D = invD + ((int64_t)(MAP ? rowmap[drow] : drow) * NF + di) * NF + CW * dm;
+        d0 = (AT)ldv(D);
+        if constexpr (CW == 2) d1 = (AT)ldv(D + 1);
+    }
+
     AT acc[NF];
 #pragma unroll
     for (int k = 0; k < NF; k++) acc[k] = 0;
-    if (s < nslices && lane < ACT) {
-        // The value loads do not wait for the column index: padding slots
-        // hold zero values, and their x is taken as zero without a load.
-        const int rv = lane / LPR, rem = lane - rv * LPR, bs = rem / JP, jp = rem - bs * JP;
-        const int c = cols[(int64_t)s * RW * W + rv * W + bs];
-        const TA* base = sv + ((int64_t)s * NF * ACT + lane) * CW;
+    if (act) {
+        // x of padding slots is taken as zero without a load
         if constexpr (CW == 2) {
-            AT a[NF][2];
-#pragma unroll
-            for (int k = 0; k < NF; k++) ldnt2(base + (int64_t)k * ACT * 2, a[k][0], a[k][1]);
             TV x0 = 0, x1 = 0;
             if (c >= 0) ld2(x + (int64_t)c * NF + 2 * jp, x0, x1);
 #pragma unroll
             for (int k = 0; k < NF; k++) acc[k] = a[k][0] * (AT)x0 + a[k][1] * (AT)x1;
         } else {
-            AT a[NF];
-#pragma unroll
-            for (int k = 0; k < NF; k++) ldnt(base + (int64_t)k * ACT, a[k]);
             const AT x0 = c >= 0 ? (AT)x[(int64_t)c * NF + jp] : (AT)0;
 #pragma unroll
-            for (int k = 0; k < NF; k++) acc[k] = a[k] * x0;
+            for (int k = 0; k < NF; k++) acc[k] = a[k][0] * x0;
         }
     }
+    // Row k of the wavefront's LDS holds the 64 partial sums of entry k,
+    // rotated by k so that the lanes summing different entries read
+    // different banks.
 #pragma unroll
-    for (int k = 0; k < NF; k++) red[wave][lane][k] = acc[k];
-    __syncthreads();
-    const bool fin = s < nslices && lane < RW * NF;
-    const int r = lane / NF, i = lane - r * NF;
-    const int row = s * RW + r;
+    for (int k = 0; k < NF; k++) red[wave][k][(lane + k) & 63] = acc[k];
+    wave_lds_sync();
+    // Row sums: NP lanes per (row r, entry i) each add a run of the LPR
+    // partial sums, then a shuffle tree over the parts leaves the row sum in
+    // part 0 (the other parts end with partial or foreign sums, not used).
+    const int QC = (LPR + NP - 1) / NP;
     AT sum = 0;
-    if (fin) {
-        for (int q = 0; q < LPR; q++) sum += red[wave][r * LPR + q][i];
+    if (r < RW) {
+        const int q1 = min(LPR, (part + 1) * QC);
+        for (int q = part * QC; q < q1; q++) sum += red[wave][i][(r * LPR + q + i) & 63];
     }
-    const bool out = fin && row < nbr;  // nbr: rows of this part
-    const int64_t e = (int64_t)(out && rowmap ? rowmap[row] : row) * NF + i;
+    for (int off = NP >> 1; off > 0; off >>= 1) sum += __shfl(sum, min(lane + NF * off, 63), 64);
     if (MODE == OP_SPMV) {
         if (out) {
             AT v = (AT)alpha * sum;
-            if (beta != 0.0) v += (AT)beta * (AT)y[e];
+            if (beta != 0.0) v += (AT)beta * yv;
             y[e] = (TV)v;
         }
     } else if (MODE == OP_RESID) {
-        if (out) y[e] = (TV)((AT)b[e] - sum);
+        if (out) y[e] = (TV)(bv - sum);
     } else {
-        // y = x + omega D^-1 (b - A x): the row's NF residuals through LDS
-        if (out) res[wave][lane] = (AT)b[e] - sum;
-        __syncthreads();
-        if (out) {
-            const TD* D = invD + e * NF;
-            AT z = 0;
+        // z = D^-1 (b - A x) of the row's NF residuals
+        const AT res = bv - sum;
+        AT z = 0;
+        if (dist) {
+            const int src = min(dr * NF * NP + CW * dm, 63);
+            AT zp = __shfl(res, src, 64) * d0;
+            if constexpr (CW == 2) zp += __shfl(res, min(src + 1, 63), 64) * d1;
+            z = zp;
+            for (int p = 1; p < JP; p++) {
+                const AT o = __shfl(zp, min(lane + p, 63), 64);
+                if (dm == 0) z += o;
+            }
+            z = __shfl(z, min(r * NF * JP + i * JP, 63), 64);
+        } else {
+            // the residuals go through the wavefront's LDS rows, whose
+            // partial sums have all been read
+            AT* rs = &red[wave][0][0];
+            wave_lds_sync();
+            if (out) rs[r * NF + i] = res;
+            wave_lds_sync();
+            if (out) {
+                const TD* D = invD + e * NF;
 #pragma unroll
-            for (int k = 0; k < NF; k++) z += (AT)ldv(D + k) * res[wave][r * NF + k];
-            smooth_out<MODE>(e, z, x, y, aux, alpha, beta, omega);
+                for (int k = 0; k < NF; k++) z += (AT)ldv(D + k) * rs[r * NF + k];
+            }
+        }
+        if (out) {
+            if constexpr (MODE == OP_CHEB) {
+                const AT dn = (alpha != 0.0 ? (AT)alpha * dv : (AT)0) + (AT)beta * z;
+                aux[e] = (TV)dn;
+                y[e] = (TV)(xv + dn);
+            } else {
+                y[e] = (TV)(xv + (AT)omega * z);
+            }
         }
     }
 }
@@ -828,29 +955,21 @@ void bcsr_dispatch_mode(Ctx& c, const Bcsr& A, BcsrMode mode, const void* x, con
         (A.sliced->cw == 1 || ((uintptr_t)x % (2 * sizeof(TV))) == 0)) {
         for (const Sliced* P = A.sliced.get(); P; P = P->next.get()) {
             const Sliced& S = *P;
-            const int nwg = (S.nslices + kSlicedWaves - 1) / kSlicedWaves;
             auto sc = S.cols.as<int>();
             auto sv = S.vals.as<TA>();
             auto rm = S.rowmap.as<int>();
             const int nrp = S.nrows;
-            switch (mode) {
-                case OP_SPMV:
-                    hipLaunchKernelGGL((k_sliced<NF, TA, TV, TD, OP_SPMV>), dim3(nwg), dim3(kWG), 0, c.stream, nrp,
-                                       S.nslices, S.W, S.RW, sc, sv, xx, bb, yy, alpha, beta, omega, D, ax, rm);
-                    break;
-                case OP_RESID:
-                    hipLaunchKernelGGL((k_sliced<NF, TA, TV, TD, OP_RESID>), dim3(nwg), dim3(kWG), 0, c.stream, nrp,
-                                       S.nslices, S.W, S.RW, sc, sv, xx, bb, yy, alpha, beta, omega, D, ax, rm);
-                    break;
-                case OP_CHEB:
-                    hipLaunchKernelGGL((k_sliced<NF, TA, TV, TD, OP_CHEB>), dim3(nwg), dim3(kWG), 0, c.stream, nrp,
-                                       S.nslices, S.W, S.RW, sc, sv, xx, bb, yy, alpha, beta, omega, D, ax, rm);
-                    break;
-                default:
-                    hipLaunchKernelGGL((k_sliced<NF, TA, TV, TD, OP_SMOOTH>), dim3(nwg), dim3(kWG), 0, c.stream, nrp,
-                                       S.nslices, S.W, S.RW, sc, sv, xx, bb, yy, alpha, beta, omega, D, ax, rm);
-                    break;
-            }
+            auto kern = rm ? (mode == OP_SPMV    ? k_sliced<NF, TA, TV, TD, OP_SPMV, true>
+                              : mode == OP_RESID ? k_sliced<NF, TA, TV, TD, OP_RESID, true>
+                              : mode == OP_CHEB  ? k_sliced<NF, TA, TV, TD, OP_CHEB, true>
+                                                 : k_sliced<NF, TA, TV, TD, OP_SMOOTH, true>)
+                           : (mode == OP_SPMV    ? k_sliced<NF, TA, TV, TD, OP_SPMV, false>
+                              : mode == OP_RESID ? k_sliced<NF, TA, TV, TD, OP_RESID, false>
+                              : mode == OP_CHEB  ? k_sliced<NF, TA, TV, TD, OP_CHEB, false>
+                                                 : k_sliced<NF, TA, TV, TD, OP_SMOOTH, false>);
+            const int nwg = (S.nslices + kSlicedWaves - 1) / kSlicedWaves;
+            hipLaunchKernelGGL(kern, dim3(nwg), dim3(kWG), 0, c.stream, nrp, S.nslices, S.W, S.RW, sc, sv, xx, bb, yy,
+                               alpha, beta, omega, D, ax, rm);
             FH_HIP(hipGetLastError());
         }
         return;

@@ -838,7 +838,7 @@ void Solver::apply_prec(const DVec& r, DVec& z, DVec* Az) {
     if (Az) launch_bcsr(c_, *A_, OP_SPMV, Prec::F64, z.data(), nullptr, Az->data(), 1.0, 0.0, 0.0, nullptr, Prec::F64, nullptr, 0);
 }
 
-void Solver::Solve(const double* b, double* x) {
+void Solver::Solve(const double* b, double* x, bool x_is_zero) {
     FH_CHECK(A_ != nullptr, -2, "Setup() must be called before Solve()");
     c_.sync();
     auto t0 = std::chrono::steady_clock::now();
@@ -847,9 +847,9 @@ void Solver::Solve(const double* b, double* x) {
         launch_fill(c_, 2, nn_counters_.p, Prec::F32, 0.0);  // two int zeros (same bit pattern as 0.0f)
         nn_fallbacks_seen_ = 0;
     }
-    if (args_.solver == "cg") solve_cg(b, x);
-    else if (args_.solver == "fcg") solve_fcg(b, x);
-    else solve_stationary(b, x);
+    if (args_.solver == "cg") solve_cg(b, x, x_is_zero);
+    else if (args_.solver == "fcg") solve_fcg(b, x, x_is_zero);
+    else solve_stationary(b, x, x_is_zero);
     c_.sync();
     if (nn_) {
         int cnt[2];
@@ -875,19 +875,35 @@ void Solver::Solve(const double* b, double* x) {
         return false;                                                                         \
     }
 
-void Solver::solve_cg(const double* b, double* x) {
-    // ConjugateGradient::Solve (ConjugateGradient.h:27-104), x0 = 0
+// The first residual of the Krylov loops (ConjugateGradient.h:36-45,
+// FlexibleConjugateGradient.h:58-66): r = b from x = 0, else b - A x, and
+// the normalised norm of the first iteration result (IterationResult.h:121-139).
+double Solver::initial_residual(const double* b, double* x, bool xz, double& res) {
     const int64_t n = A_->rows();
     const Prec P = Prec::F64;
-    launch_fill(c_, n, x, P, 0.0);
-    copy(c_, n, P, b, r_.data());
-    dot1(c_, n, P, b, b, S_BB);
-    double bn = std::sqrt(read_scalar(S_BB));
+    if (xz) {
+        launch_fill(c_, n, x, P, 0.0);
+        copy(c_, n, P, b, r_.data());
+    } else {
+        launch_bcsr(c_, *A_, OP_RESID, P, x, b, r_.data(), 1.0, 0.0, 0.0, nullptr, P, nullptr, 0);
+    }
+    dot2(c_, n, P, b, b, S_BB, r_.data(), r_.data(), S_RR);
+    const double bn = std::sqrt(read_scalar(S_BB));
+    const double rn = std::sqrt(read_scalar(S_RR));
+    res = xz ? 1.0 : (bn > 0 ? rn / bn : rn);
+    return bn;
+}
+
+void Solver::solve_cg(const double* b, double* x, bool xz) {
+    // ConjugateGradient::Solve (ConjugateGradient.h:27-104)
+    const int64_t n = A_->rows();
+    const Prec P = Prec::F64;
+    double res = 1.0;
+    const double bn = initial_residual(b, x, xz, res);
     apply_prec(r_, z_, nullptr);
     copy(c_, n, P, z_.data(), d_.data());
     dot1(c_, n, P, r_.data(), z_.data(), S_RZ);
     int it = 0;
-    double res = 1.0;
     hist_.push_back(res);
     stats_.converged = 0;
     auto stop = FH_STOP_TEST();
@@ -908,9 +924,9 @@ void Solver::solve_cg(const double* b, double* x) {
     stats_.final_res = res;
 }
 
-void Solver::solve_fcg(const double* b, double* x) {
-    // FlexibleConjugateGradient::Solve (FlexibleConjugateGradient.h:47-171),
-    // x0 = 0.  z is A-orthogonalised against the kept previous directions,
+void Solver::solve_fcg(const double* b, double* x, bool xz) {
+    // FlexibleConjugateGradient::Solve (FlexibleConjugateGradient.h:47-171).
+    // z is A-orthogonalised against the kept previous directions,
     // oldest first; when `Truncation` directions are kept the oldest is
     // dropped (or all are, with Restart).  Directions live in a ring of
     // Truncation + 1 buffers, so no vector is copied.
@@ -929,12 +945,9 @@ void Solver::solve_fcg(const double* b, double* x) {
     }
     auto slot_dad = [](int k) { return S_FCG + 2 * k; };
     auto slot_zad = [](int k) { return S_FCG + 2 * k + 1; };
-    launch_fill(c_, n, x, P, 0.0);
-    copy(c_, n, P, b, r_.data());
-    dot1(c_, n, P, b, b, S_BB);
-    double bn = std::sqrt(read_scalar(S_BB));
-    int it = 0;
     double res = 1.0;
+    const double bn = initial_residual(b, x, xz, res);
+    int it = 0;
     hist_.push_back(res);
     stats_.converged = 0;
     std::vector<int> kept;  // ring indices, oldest first
@@ -987,20 +1000,17 @@ void Solver::solve_fcg(const double* b, double* x) {
     stats_.final_res = res;
 }
 
-void Solver::solve_stationary(const double* b, double* x) {
+void Solver::solve_stationary(const double* b, double* x, bool xz) {
     // IterativeSolver::Solve (IterativeSolver.h:82-191) of a multigrid or a
-    // smoother used as the solver, x0 = 0.
+    // smoother used as the solver.
     const int64_t n = A_->rows();
     const Prec P = Prec::F64;
-    launch_fill(c_, n, x, P, 0.0);
-    dot1(c_, n, P, b, b, S_BB);
-    double bn = std::sqrt(read_scalar(S_BB));
-    int it = 0;
     double res = 1.0;
+    const double bn = initial_residual(b, x, xz, res);
+    int it = 0;
     hist_.push_back(res);
     stats_.converged = 0;
     auto stop = FH_STOP_TEST();
-    bool xz = true;
     while (!stop()) {
         if (mg_) mg_->iterate(b, x);
         else {

@@ -132,7 +132,8 @@ public:
     void SetupUncondensed(const Bcsr& A, const Csr& ATT, const Csr& ATF, const Csr& AFF, int cell_bs);
     const UamgHierarchy* Uamg() const { return uamg_.get(); }
     // Solves A x = b from x = 0 (FP64 device vectors of length A.rows()).
-    void Solve(const double* b, double* x);
+    // x holds the initial guess unless x_is_zero (then it is set to zero)
+    void Solve(const double* b, double* x, bool x_is_zero = true);
     // IterativeSolver::Tolerance / MaxIterations (IterativeSolver.h:11-23).
     void SetStop(double tol, int max_iter);
     // The configured preconditioner alone: z = M r (Preconditioner::Apply,
@@ -172,9 +173,11 @@ private:
     DevBuf nn_counters_;
     void apply_prec(const DVec& r, DVec& z, DVec* Az);
     double read_scalar(int slot);
-    void solve_cg(const double* b, double* x);
-    void solve_fcg(const double* b, double* x);
-    void solve_stationary(const double* b, double* x);
+    void solve_cg(const double* b, double* x, bool xz);
+    void solve_fcg(const double* b, double* x, bool xz);
+    void solve_stationary(const double* b, double* x, bool xz);
+    // r = b (x = 0) or b - A x; returns ||b|| and the first normalised residual
+    double initial_residual(const double* b, double* x, bool xz, double& res);
 };
 
 // Smoother application on one level (in the level's working precision).

@@ -102,6 +102,7 @@ _SIGS = {
     "fh_solver_set_level": [_vp, C.c_int, _vp, _vp],
     "fh_solver_setup": [_vp, _vp],
     "fh_solver_solve": [_vp, _vp, _vp, C.POINTER(SolveStats)],
+    "fh_solver_solve_x0": [_vp, _vp, _vp, C.c_int, C.POINTER(SolveStats)],
     "fh_solver_history": [_vp, _f64p, C.c_int],
     "fh_solver_level_size": [_vp, C.c_int],
     "fh_profile_begin": [_vp, _vp, C.c_int],
@@ -116,6 +117,8 @@ _SIGS = {
     "fh_hho_prolongation": [_vp, _vp, _vp, C.c_int, _pp],
     "fh_hho_face_cells": [_vp, _i32p],
     "fh_hho_cell_kappa": [_vp, _f64p],
+    "fh_condense_batched": [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _i32p, _vp, _vp, _vp, _vp, _pp],
+    "fh_cell_recover_batched": [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _i32p, _vp, _vp, _vp, _vp],
     "fh_hho_uncondensed": [_vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int64), _vp, _vp, _vp],
     "fh_hho_cell_unknowns": [_vp, C.POINTER(C.c_int)],
     "fh_hho_l2_error_gpu": [_vp, _vp, _vp, C.POINTER(C.c_double)],
@@ -484,6 +487,49 @@ def mcgs_sweep(ctx, A, bj, col, direction, b, x):
     _check(lib().fh_mcgs_sweep(ctx.h, A.h, bj.h, col.h, direction, pb, px, _vprec(x)))
 
 
+def _cell_args(cell_faces, local, nc, nf):
+    import torch
+    faces = np.ascontiguousarray(cell_faces, dtype=np.int32)
+    if faces.ndim != 2:
+        raise FhError("cell_faces must be a (cells, faces per cell) array")
+    ncells, nfc = faces.shape
+    nl = nc + nfc * nf
+    _dev(local, ncells * nl * nl, dtype=torch.float64, name="local")
+    return faces, ncells, nfc
+
+
+@_ordered
+def condense_batched(ctx, cell_faces, local, nc, nf, nfaces, B_T=None, B_F=None, rhs=True):
+    """Static condensation of per-cell local matrices on the GPU
+    (fh_condense_batched): cell_faces (cells x faces per cell, host, -1 for
+    an eliminated face), local (device FP64, cells x nl x nl with the cell
+    unknowns first), optional device B_T (cells x nc) and B_F (nfaces x nf).
+    Returns (Matrix, b) with b a device FP64 tensor, or None without rhs."""
+    import torch
+    faces, ncells, nfc = _cell_args(cell_faces, local, nc, nf)
+    bt = None if B_T is None else _dev(B_T, ncells * nc, dtype=torch.float64, name="B_T")
+    bf = None if B_F is None else _dev(B_F, nfaces * nf, dtype=torch.float64, name="B_F")
+    b = torch.empty(nfaces * nf, dtype=torch.float64, device="cuda") if rhs else None
+    h = C.c_void_p()
+    _check(lib().fh_condense_batched(ctx.h, ncells, nfc, nc, nf, nfaces, faces.reshape(-1), _ptr(local), bt, bf,
+                                     None if b is None else b.data_ptr(), C.byref(h)))
+    return Matrix(ctx, h), b
+
+
+@_ordered
+def cell_recover_batched(ctx, cell_faces, local, nc, nf, nfaces, x_F, B_T=None):
+    """Cell unknowns x_T = A_TT^-1 (B_T - A_TF x_F) of every cell
+    (fh_cell_recover_batched); returns a device FP64 tensor (cells x nc)."""
+    import torch
+    faces, ncells, nfc = _cell_args(cell_faces, local, nc, nf)
+    bt = None if B_T is None else _dev(B_T, ncells * nc, dtype=torch.float64, name="B_T")
+    xf = _dev(x_F, nfaces * nf, dtype=torch.float64, name="x_F")
+    xt = torch.empty(ncells * nc, dtype=torch.float64, device="cuda")
+    _check(lib().fh_cell_recover_batched(ctx.h, ncells, nfc, nc, nf, nfaces, faces.reshape(-1), _ptr(local), bt, xf,
+                                         xt.data_ptr()))
+    return xt.reshape(ncells, nc)
+
+
 def aggregate_pairwise(ctx, A, theta=0.25):
     nbr = A.info()["nbr"]
     agg = np.zeros(nbr, dtype=np.int32)
@@ -577,12 +623,17 @@ class Solver:
         _check(lib().fh_solver_setup_uncondensed(self.h, A.h, *[C.byref(k.s) for k in keep], int(cell_bs)))
 
     @_ordered
-    def solve(self, b, x):
+    def solve(self, b, x, guess=False):
+        """Solves A x = b from x = 0, or from the initial guess held in x
+        when guess is true (fh_solver_solve_x0)."""
         n = self._n()
         pb = _dev(b, n, dtype=_f64(), name="b")
         px = _dev(x, n, dtype=_f64(), name="x")
         st = SolveStats()
-        _check(lib().fh_solver_solve(self.h, pb, px, C.byref(st)))
+        if guess:
+            _check(lib().fh_solver_solve_x0(self.h, pb, px, 0, C.byref(st)))
+        else:
+            _check(lib().fh_solver_solve(self.h, pb, px, C.byref(st)))
         return st.as_dict()
 
     def history(self):

@@ -241,6 +241,14 @@ int fh_solver_setup(fh_solver* s, const fh_mat* A);
 /* IterativeSolver::Solve(b, "0") (IterativeSolver.h:55-73): x (device, FP64)
  * receives the solution from a zero initial guess. */
 int fh_solver_solve(fh_solver* s, const double* b, double* x, fh_solve_stats* stats);
+/* IterativeSolver::Solve(b, x, xEquals0) (IterativeSolver.h:75-80): x
+ * (device, FP64) holds the initial guess on input, unless x_is_zero, and the
+ * solution on output.  With a guess the first residual is b - A x and the
+ * first history entry its norm over ||b|| (ConjugateGradient.h:36-45,
+ * IterationResult.h:121-139); the initial-guess codes "1" and "rand" of
+ * IterativeSolver::Solve(b, code) (:55-73) are this call with the caller's
+ * vector of ones or random values. */
+int fh_solver_solve_x0(fh_solver* s, const double* b, double* x, int x_is_zero, fh_solve_stats* stats);
 /* ------------------------------------------------- learned post-smoother */
 /* No counterpart in the reference.  A message-passing network over the face
  * graph (hidden width 32, `layers` layers; a few thousand parameters)
@@ -372,6 +380,39 @@ int fh_hho_uncondensed(fh_hho* h, int which, int* nrows, int* ncols, int64_t* nn
                        double* vals);
 /* Number of cell unknowns per cell (HHO->nCellUnknowns). */
 int fh_hho_cell_unknowns(const fh_hho* h, int* nc);
+
+/* ------------------------------------- batched condensation and recovery */
+/* Static condensation of the caller's element matrices on the GPU: the face
+ * system A = sum over cells (A_FF - A_TF^T A_TT^-1 A_TF) and b = B_F - sum
+ * over cells A_TF^T A_TT^-1 B_T.  Replaces the condensation step of
+ * Diffusion_HHO::Assemble (Discretizations/HHO/Diffusion_HHO.h:624-637,
+ * A = (A_ndF_ndF - A_T_ndF^T Solve_A_T_T(A_T_ndF)).pruned(), and
+ * CondensedRHS :1026-1032) applied cell by cell to Diff_HHOElement::A
+ * (Diff_HHOElement.h:144).
+ *   ncells cells of nfc faces (nf unknowns each) and nc cell unknowns;
+ *   nfaces system faces (block rows of the result);
+ *   cell_faces: HOST int32 [ncells*nfc], the system number of each face of
+ *     each cell in element order, -1 for a face whose unknowns are
+ *     eliminated (Dirichlet faces, Diffusion_HHO.h:596-602);
+ *   local: DEVICE FP64 [ncells*nl*nl], nl = nc + nfc*nf, each cell's local
+ *     matrix row-major with the cell unknowns first, then the faces in
+ *     element order (FirstDOFNumber, Diff_HHOElement.h:522-525); only the
+ *     A_TT, A_TF and A_FF parts are read;
+ *   B_T: DEVICE [ncells*nc] or NULL (zero); B_F: DEVICE [nfaces*nf] or NULL
+ *     (zero); b_out: DEVICE [nfaces*nf], or NULL not to form b.
+ * *out: a new FP64 device matrix of nfaces x nfaces blocks of nf x nf whose
+ * pattern is every pair of faces of one cell (ascending columns); blocks
+ * that sum to zero are kept (the reference prunes exact zeros of its scalar
+ * pattern).  -4 if an A_TT is not positive definite, -3 if a cell's matrices
+ * do not fit one workgroup's LDS. */
+int fh_condense_batched(fh_ctx* ctx, int ncells, int nfc, int nc, int nf, int nfaces, const int* cell_faces,
+                        const double* local, const double* B_T, const double* B_F, double* b_out, fh_mat** out);
+/* Cell unknowns from the face solution, x_T = A_TT^-1 (B_T - A_TF x_F) per
+ * cell (Diffusion_HHO::SolveCellUnknowns, Diffusion_HHO.h:960-966); faces
+ * numbered -1 contribute nothing.  cell_faces, local and B_T as for
+ * fh_condense_batched; x_F: DEVICE [nfaces*nf]; x_T: DEVICE [ncells*nc]. */
+int fh_cell_recover_batched(fh_ctx* ctx, int ncells, int nfc, int nc, int nf, int nfaces, const int* cell_faces,
+                            const double* local, const double* B_T, const double* x_F, double* x_T);
 
 /* --------------------------------------------- uncondensed AMG (U-AMG) */
 /* Host scalar CSR (RowMajor, int32 indices, strictly ascending columns in

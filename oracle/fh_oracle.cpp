@@ -857,13 +857,20 @@ static int solve_impl(const or_solver_cfg* cfg, const Csr& A, const or_level_des
 
     double bn = nrm2(n, b);
     auto nres = [&](const double* r) { double rn = nrm2(n, r); return bn > 0 ? rn / bn : rn; };
-    std::fill(x, x + n, 0.0);
     int it = 0;
     int nh = 0;
     double res = 1.0;
-    hist[nh++] = res;
     int status = 1;
     std::vector<double> r(b, b + n), z(n), d(n), Ad(n), Az(n);
+    // first residual: b from x = 0, else b - A x (ConjugateGradient.h:36-45)
+    if (cfg->x_given) {
+        spmv(A, x, Ad.data());
+        for (int i = 0; i < n; i++) r[i] = b[i] - Ad[i];
+        res = nres(r.data());
+    } else {
+        std::fill(x, x + n, 0.0);
+    }
+    hist[nh++] = res;
     auto stop = [&]() {
         // IterativeSolver::StoppingCriteriaReached (IterativeSolver.h:224-247)
         if (cfg->max_iter == 0) return true;

@@ -95,6 +95,7 @@ typedef struct or_solver_cfg {
     int fcg_truncation;  /* FCG: number of kept directions (<= 0: 1) */
     int fcg_restart;     /* FCG: clear instead of dropping the oldest */
     int kcycle_levels;   /* K-cycle only on levels 1..kcycle_levels (<= 0: all), V recursion below */
+    int x_given;         /* nonzero: x holds the initial guess (IterativeSolver::Solve(b, x, false)) */
 } or_solver_cfg;
 
 typedef struct or_solver_out {

@@ -28,7 +28,8 @@ class SolverCfg(C.Structure):
                 ("post_omega", C.c_double), ("nu1", C.c_int), ("nu2", C.c_int), ("cycle", C.c_char),
                 ("wloops", C.c_int), ("max_coarse", C.c_int), ("theta", C.c_double), ("tol", C.c_double),
                 ("max_iter", C.c_int), ("nlevels", C.c_int), ("coarse_kind", C.c_int), ("coarse_omega", C.c_double),
-                ("fcg_truncation", C.c_int), ("fcg_restart", C.c_int), ("kcycle_levels", C.c_int)]
+                ("fcg_truncation", C.c_int), ("fcg_restart", C.c_int), ("kcycle_levels", C.c_int),
+                ("x_given", C.c_int)]
 
 
 class SolverOut(C.Structure):
@@ -151,12 +152,12 @@ PREC_KIND = {"": 0, "none": 0, "bj": 1, "bgs": 2, "mg": 3, "aggregamg": 4}
 
 def solve(A, b, solver="cg", prec="mg", nf=1, cycle="V", nu=(1, 1), smoothers=("bj", "bj"), omega=1.0,
           levels=None, tol=1e-8, max_iter=200, max_coarse=1000, theta=0.25, coarse_smoother=None, coarse_omega=0.0,
-          fcg_truncation=1, fcg_restart=False, kcycle_levels=0):
+          fcg_truncation=1, fcg_restart=False, kcycle_levels=0, x0=None):
     """Runs the oracle solver; `levels` = [(A_l, P_l), ...] for GMG (P of the
     coarsest level is None).  A level may be (A_l, ("p", nfc), nf_l): a
     p-transfer keeping nfc modes of every nf_l-block; (A_l, P_l, nf_l) gives
-    the block size of a level whose size differs from nf.  Returns (x, hist,
-    out)."""
+    the block size of a level whose size differs from nf.  x0: initial guess
+    (default zero).  Returns (x, hist, out)."""
     keep = []
     if levels is None:
         levels = [(A, None)]
@@ -198,6 +199,9 @@ def solve(A, b, solver="cg", prec="mg", nf=1, cycle="V", nu=(1, 1), smoothers=("
     cfg.max_iter = max_iter
     cfg.nlevels = len(levels)
     x = np.zeros(A.shape[0])
+    if x0 is not None:
+        x[:] = x0
+        cfg.x_given = 1
     hist = np.zeros(max_iter + 2)
     out = SolverOut()
     st = lib().or_solve(C.byref(cfg), descs, np.ascontiguousarray(b, dtype=np.float64), x, hist, C.byref(out))

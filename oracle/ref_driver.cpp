@@ -101,6 +101,7 @@ struct Opts
 	int aggregBlock = -1;
 	bool solve = true;
 	string kappaFile; // per-element isotropic kappa (int64 count + doubles, mesh element order)
+	bool dumpElements = false;
 };
 
 // Gives every element a physical part of its own with the isotropic tensor
@@ -240,6 +241,35 @@ static int Run(const Opts& o)
 		dat.ExportVector(problem->b, "b");
 	}
 
+	if (o.dumpElements)
+	{
+		// The element matrices the condensation is built from: every
+		// element's local matrix Diff_HHOElement::A (cell unknowns first,
+		// then its faces in element order; Diff_HHOElement.h:144,
+		// FirstDOFNumber :522-525), the system number of each face (-1 for a
+		// Dirichlet face, whose columns are eliminated), and the cell part of
+		// the right-hand side, AssembleSourceTerm (Diffusion_HHO.h:794).
+		const int nc = hho->nCellUnknowns, nfu = hho->nFaceUnknowns;
+		const size_t ne = mesh->Elements.size();
+		const size_t nfc = mesh->Elements[0]->Faces.size();
+		const int nloc = nc + (int)nfc * nfu;
+		vector<int64_t> faces(ne * nfc, -1);
+		Vector loc(ne * nloc * nloc);
+		for (Element<Dim>* e : mesh->Elements)
+		{
+			Diff_HHOElement<Dim>* he = problem->HHOElement(e);
+			const size_t en = e->Number;
+			for (size_t j = 0; j < nfc; j++)
+				faces[en * nfc + j] = he->Faces[j]->HasDirichletBC() ? -1 : (int64_t)he->Faces[j]->Number();
+			for (int r = 0; r < nloc; r++)
+				for (int c = 0; c < nloc; c++)
+					loc[(en * nloc + r) * nloc + c] = he->A(r, c);
+		}
+		write_ivec(o.out + "/elem_faces.ivec", faces);
+		write_vec(o.out + "/elem_A.vec", loc);
+		write_vec(o.out + "/elem_BT.vec", problem->AssembleSourceTerm(testCase->SourceFunction));
+	}
+
 	FILE* info = fopen((o.out + "/info.txt").c_str(), "w");
 	fprintf(info, "rows %ld\nnnz %ld\nnf %d\nassemble_ms %.3f\n", (long)problem->A.rows(), (long)problem->A.nonZeros(), faceBasis->Size(),
 		chrono::duration<double, milli>(t1 - t0).count());
@@ -331,6 +361,7 @@ int main(int argc, char** argv)
 		else if (a == "-aggreg-block") o.aggregBlock = stoi(next());
 		else if (a == "-nosolve") o.solve = false;
 		else if (a == "-kappa-file") o.kappaFile = next();
+		else if (a == "-dump-elements") o.dumpElements = true;
 		else { fprintf(stderr, "unknown option %s\n", a.c_str()); return 2; }
 	}
 	if (o.dim == 2) return Run<2>(o);

@@ -17,6 +17,12 @@ repository compiled against the reference sources in place, see
 * ``uamg_factors`` : coarsening factor after every U-AMG pass, as the
                  reference prints it (UncondensedLevel.h:121-124).
 
+* ``elem_faces``, ``elem_A``, ``elem_BT`` : every element's local matrix
+                 (Diff_HHOElement::A: cell unknowns, then the faces in element
+                 order), the system number of each face (-1: Dirichlet) and
+                 the cell part of the right-hand side (AssembleSourceTerm),
+                 the input of the batched static condensation.
+
 * ``dat_*``    : the reference's Matrix Market export (``A.dat``, ``b.dat``,
                  ``solutionFaces.dat``; ExportModule.h:68-87) as raw bytes.
 
@@ -80,7 +86,8 @@ def cell_kappa(dim, n, sigma, seed):
     return P.cell_kappa()
 
 
-def run(name, args, export_mg=False, keep_A=True, aggreg=None, export_dat=False, uamg=False, kappa=None):
+def run(name, args, export_mg=False, keep_A=True, aggreg=None, export_dat=False, uamg=False, kappa=None,
+        elements=False):
     if ONLY and name not in ONLY:
         return
     with tempfile.TemporaryDirectory() as tmp:
@@ -95,6 +102,8 @@ def run(name, args, export_mg=False, keep_A=True, aggreg=None, export_dat=False,
             cmd.append("-export-mg")
         if export_dat:
             cmd.append("-export-dat")
+        if elements:
+            cmd.append("-dump-elements")
         out = subprocess.run(cmd, check=True, stdout=subprocess.PIPE).stdout.decode(errors="replace")
         info = read_info(os.path.join(tmp, "info.txt"))
         d = {"cmd": np.array(" ".join(["refdrv"] + args))}
@@ -108,6 +117,10 @@ def run(name, args, export_mg=False, keep_A=True, aggreg=None, export_dat=False,
             d["hist"] = read_vec(os.path.join(tmp, "hist.vec"))
         for k, v in info.items():
             d["info_" + k] = np.array(v)
+        if elements:
+            d["elem_faces"] = read_vec(os.path.join(tmp, "elem_faces.ivec"), dtype=np.int64).astype(np.int32)
+            d["elem_A"] = read_vec(os.path.join(tmp, "elem_A.vec"))
+            d["elem_BT"] = read_vec(os.path.join(tmp, "elem_BT.vec"))
         if uamg:
             for blk, f in (("ATT", "A_T_T"), ("ATF", "A_T_F"), ("AFF", "A_F_F")):
                 pack_csr(blk, read_csr(os.path.join(tmp, f + ".csr")), d)
@@ -191,6 +204,15 @@ def main():
                               "-prec", "bj"], kappa=cell_kappa(3, 4, 3.0, 7))
     run("quad8_k2_lognormal", ["-dim", "2", "-mesh", "cart", "-k", "2", "-n", "8", "-source", "pisine", "-s", "cg",
                                "-prec", "bj"], kappa=cell_kappa(2, 8, 2.0, 5))
+    # Element matrices of small systems: the input of the batched static
+    # condensation and cell recovery (hexahedra with a lognormal kappa,
+    # tetrahedra k = 2, quadrilaterals k = 2)
+    run("elem_cart4_k1", ["-dim", "3", "-mesh", "cart", "-k", "1", "-n", "4", "-source", "pisine", "-nosolve"],
+        kappa=cell_kappa(3, 4, 3.0, 7), elements=True)
+    run("elem_stetra2_k2", ["-dim", "3", "-mesh", "stetra", "-k", "2", "-n", "2", "-source", "sine", "-nosolve"],
+        elements=True)
+    run("elem_quad4_k2", ["-dim", "2", "-mesh", "cart", "-k", "2", "-n", "4", "-source", "pisine", "-nosolve"],
+        elements=True)
     # Exported linear systems (Matrix Market text) of tiny 2-D and 3-D problems
     run("mm_c1_n4", c1s + ["-n", "4", "-s", "cg", "-prec", "mg", "-smoothers", "bj,bj"], export_dat=True)
     run("mm_tet2_k1", ["-dim", "3", "-mesh", "stetra", "-k", "1", "-n", "2", "-source", "sine", "-s", "cg",

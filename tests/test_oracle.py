@@ -362,3 +362,29 @@ def test_oracle_block_storage_path_equals_scalar_path():
     assert list(o1.level_n[:o1.nlevels]) == list(o2.level_n[:o2.nlevels])
     np.testing.assert_allclose(h1, h2, rtol=1e-9, atol=1e-15)
     assert np.linalg.norm(x1 - x2) <= 1e-12 * np.linalg.norm(x1)
+
+
+@pytest.mark.parametrize("solver,prec", [("cg", "aggregamg"), ("fcg", "aggregamg"), ("none", "aggregamg")])
+def test_oracle_initial_guess_is_a_shift(solver, prec):
+    """From an initial guess x0 the iterates are x0 plus those of the
+    problem A e = b - A x0 from zero (the first residual is b - A x0,
+    ConjugateGradient.h:36-45): same iteration count, the same solution,
+    and histories that differ only by the norms of the two right-hand sides.
+    An explicit zero guess reproduces the default run exactly."""
+    d = load("tet8_k0_fcg_amg")
+    A, b = csr(d, "A"), d["b"]
+    kw = dict(nf=1, cycle="K", smoothers=("bgs", "rbgs"), max_iter=12, tol=1e-30)  # 12 iterations each
+    x_ref, h_ref, o_ref = O.solve(A, b, solver, prec, **kw)
+    x_zero, h_zero, o_zero = O.solve(A, b, solver, prec, x0=np.zeros_like(b), **kw)
+    assert o_zero.iterations == o_ref.iterations and np.array_equal(h_zero, h_ref) and np.array_equal(x_zero, x_ref)
+    x0 = np.ones_like(b)
+    x1, h1, o1 = O.solve(A, b, solver, prec, x0=x0, **kw)
+    r0 = b - A @ x0
+    e, he, oe = O.solve(A, r0, solver, prec, **kw)
+    nb, nr = np.linalg.norm(b), np.linalg.norm(r0)
+    assert o1.iterations == oe.iterations == 12
+    assert h1[0] == pytest.approx(nr / nb, rel=1e-14)
+    # same Krylov space: histories scaled by ||r0|| / ||b||
+    k = min(len(h1), len(he))
+    assert np.allclose(h1[:k], he[:k] * nr / nb, rtol=1e-8, atol=0)
+    assert np.linalg.norm(x1 - (x0 + e)) <= 1e-12 * np.linalg.norm(x1)
