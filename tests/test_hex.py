@@ -107,3 +107,35 @@ def test_prolongation_reproduces_constants_inside(k):
     assert np.abs(vf[rows, 0] - 1).max() < 1e-12
     if nf > 1:
         assert np.abs(vf[rows, 1:]).max() < 1e-12
+
+
+@pytest.mark.parametrize("name,kw,nf", [
+    ("c1_n64_cg_mg_bj", dict(dim=2, mesh="quad", n=64, k=1, source="pisine"), 2),
+    ("cart8_k2_cg_mg_bj23", dict(dim=3, mesh="hex", n=8, k=2, source="pisine"), 6),
+])
+def test_geometric_hierarchy_equals_reference_levels(name, kw, nf):
+    """The CellInterp_Trace prolongation (LevelForHHO.h:182-213, 743-800,
+    899-954) and the re-discretised coarse operators of the general-cell
+    producer against the reference's own exported levels (level<l>_P,
+    level<l>_A of its -export mg) on unperturbed Cartesian meshes: entries to
+    1e-13 relative on every level, and the oracle's CG + V(1,1) block-Jacobi
+    solve on this hierarchy takes the reference's iteration count."""
+    from oracle import oracle as O
+    d = load(name)
+    probs = [G.HhoProblem(**kw)]
+    levels = []
+    for l in range(int(d["nlevels"]) - 1):
+        coarse = probs[-1].coarsened()
+        Pm = probs[-1].prolongation(G.Context(-1), coarse).scipy()
+        Pr = csr(d, "level%d_P" % l)
+        assert Pm.shape == Pr.shape
+        assert abs(Pm - Pr).max() <= 1e-13 * abs(Pr).max(), l
+        Ac, Ar = coarse.scipy(), csr(d, "level%d_A" % (l + 1))
+        assert abs(Ac - Ar).max() <= 1e-13 * abs(Ar).max(), l
+        levels.append((probs[-1].scipy(), Pm))
+        probs.append(coarse)
+    levels.append((probs[-1].scipy(), None))
+    sm = "bj23" if "bj23" in name else "bj"
+    x, hist, out = O.solve(probs[0].scipy(), d["b"], "cg", "mg", nf=nf, smoothers=(sm, sm), levels=levels)
+    assert out.iterations == int(d["info_iterations"])
+    assert np.linalg.norm(x - d["x"]) <= 1e-8 * np.linalg.norm(d["x"])
