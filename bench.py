@@ -520,15 +520,14 @@ def c5_bench(args, G, np, torch):
                 a.update(mg_prec="f32", coarse_store=kv["cstore"])
             try:
                 Hv = H if pmv == pm or prec != "mg" else gmg.build_hp_hierarchy(ctx, P, p_modes=pmv)
-                if prec != "mg":
-                    a["galerkin"] = False
+                a["galerkin"] = prec == "mg"  # hp-MG h-levels are Galerkin products of the given P
                 s = G.Solver(ctx, **a)
                 if prec == "mg":
                     gmg.attach_hp(s, Hv)
                 t0 = time.perf_counter()
                 s.setup(A)
                 su = time.perf_counter() - t0
-                ts, st = timed_solves(ctx, s, b, x, 1, 0)
+                ts, st = timed_solves(ctx, s, b, x, 1, 1)
                 out = {"variant": spec, "s": ts[0], "iterations": st["iterations"], "converged": st["converged"],
                        "setup_s": su, "ms_per_it": 1e3 * ts[0] / max(1, st["iterations"]),
                        "levels": [s.level_size(l) for l in range(s64_levels(s))]}

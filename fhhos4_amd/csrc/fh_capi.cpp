@@ -265,7 +265,7 @@ int fh_convert(fh_ctx* ctx, int64_t n, const void* src, int sprec, void* dst, in
 int fh_profile_begin(fh_ctx* ctx, const fh_mat* A, int mode) {
     return guard([&] {
         NEED(ctx); NEED_DEV(ctx); NEED(A);
-        FH_CHECK(mode >= 0 && mode <= 3, -2, "mode must be 0, 1, 2 or 3");
+        FH_CHECK(mode >= 0 && mode <= 4, -2, "mode must be 0, 1, 2, 3 or 4");
         ctx->c.prof_matrix = A->A.get();
         ctx->c.prof_mode = mode;
         ctx->c.prof_events.clear();
@@ -494,6 +494,10 @@ int fh_solver_create(fh_ctx* ctx, const fh_solver_args* a, fh_solver** out) {
         s.coarse_relax = a->coarse_relax;
         s.fcg_truncation = a->fcg_truncation > 0 ? a->fcg_truncation : 1;
         s.kcycle_levels = a->kcycle_levels > 0 ? a->kcycle_levels : 0;
+        FH_CHECK(a->nu_change_op == 0 || a->nu_change_op == '+' || a->nu_change_op == '-' || a->nu_change_op == '*',
+                 -2, "smoothing change operator must be '+', '-' or '*'");
+        s.nu_change_op = a->nu_change_op;
+        s.nu_change_coeff = a->nu_change_coeff;
         FH_CHECK(s.fcg_truncation <= 32, -2, "FCG truncation is limited to 32 directions");
         s.fcg_restart = a->fcg_restart != 0;
         if (!s.coarse_smoother.empty())

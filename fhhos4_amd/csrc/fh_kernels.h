@@ -10,6 +10,7 @@ enum BcsrMode : int {
     OP_RESID = 1,   // y = b - A x
     OP_SMOOTH = 2,  // y = x + omega * D^-1 (b - A x)   (block Jacobi / GS colour)
     OP_CHEB = 3,    // d = alpha * d + beta * D^-1 (b - A x), y = x + d   (Chebyshev step)
+    OP_RESID_HEAD = 4,  // rc[i] = (b - A x)[i nf]: launch_resid_head (in-situ timing code only)
 };
 
 // Block-row operation over all block rows, or over `rows[0..nrows)` when
@@ -24,6 +25,13 @@ void launch_bcsr(Ctx& c, const Bcsr& A, BcsrMode mode, Prec vprec, const void* x
 // (d not read when a = 0) and xnew = x + d.  xnew must not alias x.
 void launch_bcsr_cheb(Ctx& c, const Bcsr& A, Prec vprec, const void* x, const void* b, void* xnew, void* d,
                       double a, double cc, const void* invD, Prec dprec);
+
+// rc[i] = (b - A x)[i nf] for every block row i: the residual truncated to
+// the first mode of each block (the p-restriction to one mode per face),
+// reading entry 0 of every block only.  Returns false, having launched
+// nothing, unless A has a one-part sliced layout (and x is 2-value aligned
+// for even nf); the caller then forms the residual and restricts it.
+bool launch_resid_head(Ctx& c, const Bcsr& A, Prec vprec, const void* x, const void* b, void* rc);
 
 // Builds (on = true) or drops the sliced execution layout of A used by
 // full-matrix products; returns whether A now has one (false when a row of
@@ -71,6 +79,11 @@ void launch_dots(Ctx& c, int64_t n, Prec p, int nd, const void* const* xs, const
 // Device scalar arithmetic: s[dst] = op(s[a], s[b]); op 0: a/b, 1: sqrt(a),
 // 2: a*b, 3: a-b, 4: copy a.
 void launch_scalar_op(Ctx& c, int op, int dst, int a, int b);
+
+// K-cycle coefficients on the device: slots first .. first+6 hold rho1,
+// alpha1, |r|^2, |r_new|^2, gamma, beta, alpha2; ca and cb (cb = 0 when
+// |r_new| <= t |r|: the first correction alone) replace first+2, first+3.
+void launch_kcycle_coeffs(Ctx& c, int first, double t);
 
 // Aggregation transfers (piecewise-constant prolongation, block identity).
 // restrict: rc[I*nf+a] = sum_{k in members(I)} r[k*nf+a]

@@ -511,6 +511,89 @@ __global__ __launch_bounds__(kWG) void k_sliced(int nbr, int nslices, int W, int
     }
 }
 
+// Residual of the first entry of every block row, rc[row] = b[row NF] -
+// (A x)[row NF], on the sliced layout (one part, no row map): the residual
+// followed by the p-restriction to one mode per face (LevelForHHO.h:586-620
+// truncates b - A x to the lowest modes) without forming the rest of the
+// residual.  Only entry 0 of every slice's values is read (one contiguous
+// run per wavefront, 1/NF of the operator's bytes); the x gather is that of
+// the full product.  A wavefront takes kHeadSlices consecutive slices and
+// issues the loads of all of them before the first use (column indices,
+// then values and b, then the x gathers), since one slice alone is a short,
+// latency-bound chain.  Row sums as in k_sliced: NP lanes per row each add
+// a run of the LPR partial sums, then a shuffle tree.
+constexpr int kHeadSlices = 4;
+template <int NF, class TA, class TV>
+__global__ __launch_bounds__(kWG) void k_sliced_resid_head(int nbr, int nslices, int W, int RW,
+                                                           const int* __restrict__ cols, const TA* __restrict__ sv,
+                                                           const TV* x, const TV* __restrict__ b,
+                                                           TV* __restrict__ rc) {
+    using AT = typename Acc<TV>::T;
+    using LT = std::conditional_t<std::is_same<TA, double>::value, double, float>;  // loaded value type
+    constexpr int CW = NF % 2 == 0 ? 2 : 1;
+    constexpr int JP = NF / CW;
+    constexpr int H = kHeadSlices;
+    const int LPR = W * JP, ACT = RW * LPR;
+    __shared__ AT red[kSlicedWaves][64];
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int s0 = (xcd_remap(blockIdx.x, gridDim.x) * kSlicedWaves + wave) * H;
+    if (s0 >= nslices) return;  // wavefront-uniform; no workgroup barrier follows
+    const int rv = lane / LPR, rl = lane - rv * LPR, bs = rl / JP, jp = rl - bs * JP;
+    const bool act = lane < ACT;
+    const int NP = 1 << (31 - __clz(64 / RW));
+    const int r = lane / NP, part = lane - r * NP;
+    int c[H];
+    LT a0[H], a1[H];
+    AT bv[H];
+#pragma unroll
+    for (int h = 0; h < H; h++) {
+        const int s = s0 + h;
+        c[h] = act && s < nslices ? cols[(int64_t)s * RW * W + rv * W + bs] : -1;
+    }
+#pragma unroll
+    for (int h = 0; h < H; h++) {
+        const int s = s0 + h;
+        a0[h] = 0;
+        a1[h] = 0;
+        if (act && s < nslices) {
+            const TA* sbase = sv + (int64_t)s * NF * ACT * CW;  // entry 0 of the slice: its first ACT * CW values
+            if constexpr (CW == 2) ldnt2(sbase + lane * 2, a0[h], a1[h]);
+            else ldnt(sbase + lane, a0[h]);
+        }
+        const int row = s * RW + r;
+        bv[h] = r < RW && part == 0 && row < nbr ? (AT)b[(int64_t)row * NF] : (AT)0;
+    }
+    AT acc[H];
+#pragma unroll
+    for (int h = 0; h < H; h++) {
+        acc[h] = 0;
+        if (c[h] >= 0) {
+            if constexpr (CW == 2) {
+                TV x0, x1;
+                ld2(x + (int64_t)c[h] * NF + 2 * jp, x0, x1);
+                acc[h] = (AT)a0[h] * (AT)x0 + (AT)a1[h] * (AT)x1;
+            } else {
+                acc[h] = (AT)a0[h] * (AT)x[(int64_t)c[h] * NF + jp];
+            }
+        }
+    }
+    const int QC = (LPR + NP - 1) / NP;
+#pragma unroll
+    for (int h = 0; h < H; h++) {
+        red[wave][lane] = acc[h];
+        wave_lds_sync();
+        AT sum = 0;
+        if (r < RW) {
+            const int q1 = min(LPR, (part + 1) * QC);
+            for (int q = part * QC; q < q1; q++) sum += red[wave][r * LPR + q];
+        }
+        wave_lds_sync();  // every lane has read its partial sums before the next slice's are stored
+        for (int off = NP >> 1; off > 0; off >>= 1) sum += __shfl(sum, min(lane + off, 63), 64);
+        const int row = (s0 + h) * RW + r;
+        if (r < RW && part == 0 && row < nbr) rc[row] = (TV)(bv[h] - sum);
+    }
+}
+
 // Block-row operation on the chunked layout of wide rows (Galerkin coarse
 // operators, restrictions: 40-100 blocks per row).  One wavefront per row;
 // lane (b, jp) takes column group jp of block slot b of each chunk of W
@@ -689,6 +772,53 @@ __global__ void k_bjac_zero(int count, int nf, const TV* __restrict__ b, TV* __r
     if (y2) y2[(int64_t)br * nf + i] = v;  // first Chebyshev step: iterate and direction
 }
 
+// y = omega D^-1 b over all block rows, LDS-tiled: a workgroup takes
+// R = 256 / NF consecutive block rows.  Their inverse blocks (R NF^2 values,
+// one contiguous span) and their b entries are streamed into LDS with
+// consecutive lanes on consecutive values, so every load instruction reads
+// one contiguous run; then thread t of the tile forms output entry t from
+// LDS and the outputs are written contiguously.  The inverses are read once
+// (non-temporal).
+template <int NF, class TV, class TD>
+__global__ __launch_bounds__(kWG) void k_bjac_zero_tile(int nbr, const TV* __restrict__ b, TV* __restrict__ y,
+                                                        double omega, const TD* __restrict__ invD,
+                                                        TV* __restrict__ y2) {
+    using AT = typename Acc<TV>::T;
+    constexpr int R = kWG / NF;
+    constexpr int NV = R * NF;  // output entries of a full tile
+    __shared__ __align__(16) TD sD[R * NF * NF];
+    __shared__ AT sb[NV];
+    const int t = threadIdx.x;
+    const int r0 = blockIdx.x * R;
+    const int rows = min(R, nbr - r0);
+    const int nv = rows * NF;
+    const int ne = nv * NF;
+    const TD* Dg = invD + (int64_t)r0 * NF * NF;
+    if constexpr (NF % 2 == 0 && sizeof(TD) == 8) {
+        // two values per load (a tile starts at an even value offset; the
+        // launcher checks the base alignment)
+        typedef double d2_t __attribute__((ext_vector_type(2)));
+        const d2_t* D2 = (const d2_t*)Dg;
+        d2_t* S2 = (d2_t*)sD;
+        for (int e = t; e < ne / 2; e += kWG) S2[e] = __builtin_nontemporal_load(D2 + e);
+    } else {
+        for (int e = t; e < ne; e += kWG) sD[e] = __builtin_nontemporal_load(Dg + e);
+    }
+    if (t < nv) sb[t] = (AT)b[(int64_t)r0 * NF + t];
+    __syncthreads();
+    if (t >= nv) return;
+    const int r = t / NF;
+    const TD* Dr = sD + t * NF;  // row i = t - r NF of block r: (r NF + i) NF = t NF
+    const AT* br = sb + r * NF;
+    AT z = 0;
+#pragma unroll
+    for (int j = 0; j < NF; j++) z += (AT)Dr[j] * br[j];
+    const TV v = (TV)((AT)omega * z);
+    const int64_t o = (int64_t)r0 * NF + t;
+    y[o] = v;
+    if (y2) y2[o] = v;  // first Chebyshev step: iterate and direction
+}
+
 template <class TD>
 __global__ void k_block_inverse(int nbr, int nf, const double* __restrict__ val, const int* __restrict__ diag,
                                 TD* __restrict__ invD, int* flag) {
@@ -831,6 +961,28 @@ __global__ void k_scalar_op(double* s, int op, int dst, int a, int b) {
         default: r = s[a]; break;
     }
     s[dst] = r;
+}
+
+// Coefficients of the K-cycle's FCG step (Multigrid::FCGForKCycle,
+// Multigrid.h:611-656) from the device slots s[k .. k+6] = rho1, alpha1,
+// |r|^2, |r_new|^2, gamma, beta, alpha2: when |r_new| <= t |r| the first
+// correction alone is kept (ca = alpha1 / rho1, cb = 0), else
+// rho2 = beta - gamma^2 / rho1, ca = alpha1 / rho1 - gamma alpha2 / (rho1 rho2),
+// cb = alpha2 / rho2.  ca and cb replace s[k+2] and s[k+3].
+__global__ void k_kcycle_coeffs(double* s, int k, double t) {
+    const double rho1 = s[k], alpha1 = s[k + 1], rold = sqrt(s[k + 2]), rnew = sqrt(s[k + 3]);
+    double ca, cb;
+    if (rnew <= t * rold) {
+        ca = alpha1 / rho1;
+        cb = 0.0;
+    } else {
+        const double gamma = s[k + 4], beta = s[k + 5], alpha2 = s[k + 6];
+        const double rho2 = __dsub_rn(beta, __ddiv_rn(__dmul_rn(gamma, gamma), rho1));
+        ca = __dsub_rn(__ddiv_rn(alpha1, rho1), __ddiv_rn(__dmul_rn(gamma, alpha2), __dmul_rn(rho1, rho2)));
+        cb = alpha2 / rho2;
+    }
+    s[k + 2] = ca;
+    s[k + 3] = cb;
 }
 
 template <class T>
@@ -1414,6 +1566,62 @@ void launch_bcsr_cheb(Ctx& c, const Bcsr& A, Prec vprec, const void* x, const vo
     }
 }
 
+template <int NF, class TA, class TV>
+static void resid_head_launch(Ctx& c, const Sliced& S, int nbr, const void* x, const void* b, void* rc) {
+    const int waves = (S.nslices + kHeadSlices - 1) / kHeadSlices;
+    const int nwg = (waves + kSlicedWaves - 1) / kSlicedWaves;
+    hipLaunchKernelGGL((k_sliced_resid_head<NF, TA, TV>), dim3(nwg), dim3(kWG), 0, c.stream, nbr, S.nslices, S.W,
+                       S.RW, S.cols.as<int>(), S.vals.as<TA>(), (const TV*)x, (const TV*)b, (TV*)rc);
+    FH_HIP(hipGetLastError());
+}
+
+template <int NF>
+static bool resid_head_nf(Ctx& c, const Bcsr& A, Prec vp, const void* x, const void* b, void* rc) {
+    const Sliced& S = *A.sliced;
+    if (vp == Prec::F64 && A.prec == Prec::F64)
+        resid_head_launch<NF, double, double>(c, S, A.nbr, x, b, rc);
+    else if (vp == Prec::F32 && A.prec == Prec::F32)
+        resid_head_launch<NF, float, float>(c, S, A.nbr, x, b, rc);
+    else if (vp == Prec::F32 && A.prec == Prec::BF16)
+        resid_head_launch<NF, bf16_t, float>(c, S, A.nbr, x, b, rc);
+    else
+        return false;
+    return true;
+}
+
+bool launch_resid_head(Ctx& c, const Bcsr& A, Prec vp, const void* x, const void* b, void* rc) {
+    if (A.sliced_state == 0 && !c.no_alloc) sliced_decide(c, A);
+    if (A.sliced_state != 1 || A.sliced->kind != 0 || A.sliced->next || A.sliced->rowmap.p || A.bs_r != A.bs_c)
+        return false;
+    if (A.sliced->cw == 2 && (uintptr_t)x % (2 * prec_bytes(vp)) != 0) return false;
+    const bool prof = c.prof_matrix == (const void*)&A && c.prof_mode == (int)OP_RESID_HEAD && !c.no_alloc;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (prof) {
+        FH_HIP(hipEventCreate(&e0));
+        FH_HIP(hipEventCreate(&e1));
+        FH_HIP(hipEventRecord(e0, c.stream));
+    }
+    bool done;
+    switch (A.bs_r) {
+        case 1: done = resid_head_nf<1>(c, A, vp, x, b, rc); break;
+        case 2: done = resid_head_nf<2>(c, A, vp, x, b, rc); break;
+        case 3: done = resid_head_nf<3>(c, A, vp, x, b, rc); break;
+        case 4: done = resid_head_nf<4>(c, A, vp, x, b, rc); break;
+        case 6: done = resid_head_nf<6>(c, A, vp, x, b, rc); break;
+        case 10: done = resid_head_nf<10>(c, A, vp, x, b, rc); break;
+        default: done = false; break;
+    }
+    if (prof) {
+        FH_HIP(hipEventRecord(e1, c.stream));
+        if (done) c.prof_events.push_back({e0, e1});
+        else {
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+        }
+    }
+    return done;
+}
+
 static void launch_bcsr_impl(Ctx& c, const Bcsr& A, BcsrMode mode, Prec vprec, const void* x, const void* b, void* y,
                              double alpha, double beta, double omega, const void* invD, Prec dprec, const int* rows,
                              int nrows, void* aux) {
@@ -1429,10 +1637,43 @@ static void launch_bcsr_impl(Ctx& c, const Bcsr& A, BcsrMode mode, Prec vprec, c
     }
 }
 
+template <int NF>
+static bool bjac_zero_tile_nf(Ctx& c, int nbr, Prec vp, const void* b, void* y, double omega, const void* invD,
+                              Prec dp, void* y2) {
+    constexpr int R = kWG / NF;
+    const int g = (nbr + R - 1) / R;
+    if (vp == Prec::F64 && dp == Prec::F64)
+        hipLaunchKernelGGL((k_bjac_zero_tile<NF, double, double>), dim3(g), dim3(kWG), 0, c.stream, nbr,
+                           (const double*)b, (double*)y, omega, (const double*)invD, (double*)y2);
+    else if (vp == Prec::F32 && dp == Prec::F32)
+        hipLaunchKernelGGL((k_bjac_zero_tile<NF, float, float>), dim3(g), dim3(kWG), 0, c.stream, nbr,
+                           (const float*)b, (float*)y, omega, (const float*)invD, (float*)y2);
+    else
+        return false;
+    FH_HIP(hipGetLastError());
+    return true;
+}
+
+// The LDS-tiled zero-guess pass over all rows, for the block sizes of the
+// block-row kernels; false if it does not apply.
+static bool bjac_zero_tile(Ctx& c, int nbr, int nf, Prec vp, const void* b, void* y, double omega,
+                           const void* invD, Prec dp, void* y2) {
+    switch (nf) {
+        case 1: return bjac_zero_tile_nf<1>(c, nbr, vp, b, y, omega, invD, dp, y2);
+        case 2: return bjac_zero_tile_nf<2>(c, nbr, vp, b, y, omega, invD, dp, y2);
+        case 3: return bjac_zero_tile_nf<3>(c, nbr, vp, b, y, omega, invD, dp, y2);
+        case 4: return bjac_zero_tile_nf<4>(c, nbr, vp, b, y, omega, invD, dp, y2);
+        case 6: return bjac_zero_tile_nf<6>(c, nbr, vp, b, y, omega, invD, dp, y2);
+        case 10: return bjac_zero_tile_nf<10>(c, nbr, vp, b, y, omega, invD, dp, y2);
+        default: return false;
+    }
+}
+
 void launch_bjac_zero(Ctx& c, int nbr, int nf, Prec vp, const void* b, void* y, double omega, const void* invD,
                       Prec dp, const int* rows, int nrows, void* y2) {
     int count = rows ? nrows : nbr;
     if (count <= 0) return;
+    if (!rows && (uintptr_t)invD % 16 == 0 && bjac_zero_tile(c, nbr, nf, vp, b, y, omega, invD, dp, y2)) return;
     int64_t nt = (int64_t)count * nf;
     int g = (int)((nt + 255) / 256);
     const size_t vb = 2 * prec_bytes(vp), db = 2 * prec_bytes(dp);
@@ -1550,6 +1791,11 @@ void launch_dots(Ctx& c, int64_t n, Prec p, int nd, const void* const* xs, const
 
 void launch_scalar_op(Ctx& c, int op, int dst, int a, int b) {
     hipLaunchKernelGGL(k_scalar_op, dim3(1), dim3(1), 0, c.stream, c.d_scalars, op, dst, a, b);
+    FH_HIP(hipGetLastError());
+}
+
+void launch_kcycle_coeffs(Ctx& c, int first, double t) {
+    hipLaunchKernelGGL(k_kcycle_coeffs, dim3(1), dim3(1), 0, c.stream, c.d_scalars, first, t);
     FH_HIP(hipGetLastError());
 }
 

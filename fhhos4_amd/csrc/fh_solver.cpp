@@ -80,9 +80,17 @@ SmootherSpec SmootherSpec::parse(const std::string& code, int iterations, double
     s.iterations = iterations;
     if (code == "bj" || code == "j") { s.kind = SmootherKind::BlockJacobi; s.omega = relax; }
     else if (code == "bj23" || code == "j23") { s.kind = SmootherKind::BlockJacobi; s.omega = 2.0 / 3.0; }
-    else if (code == "bgs" || code == "gs" || code == "bsor" || code == "sor") { s.kind = SmootherKind::BlockGS; s.sweep = Sweep::Forward; }
-    else if (code == "rbgs" || code == "rgs" || code == "rbsor" || code == "rsor") { s.kind = SmootherKind::BlockGS; s.sweep = Sweep::Backward; }
-    else if (code == "sbgs" || code == "sgs") { s.kind = SmootherKind::BlockGS; s.sweep = Sweep::Symmetric; }
+    else if (code == "bgs" || code == "gs" || code == "bsor" || code == "sor" || code == "hbgs") {
+        s.kind = SmootherKind::BlockGS; s.sweep = Sweep::Forward;
+    } else if (code == "rbgs" || code == "rgs" || code == "rbsor" || code == "rsor" || code == "hrbgs") {
+        s.kind = SmootherKind::BlockGS; s.sweep = Sweep::Backward;
+    } else if (code == "sbgs" || code == "sgs" || code == "sbsor") {
+        s.kind = SmootherKind::BlockGS; s.sweep = Sweep::Symmetric;
+    } else if (code == "abgs" || code == "ags" || code == "absor") {
+        s.kind = SmootherKind::BlockGS; s.sweep = Sweep::AltForwardFirst;
+    } else if (code == "rabgs" || code == "rags" || code == "rabsor") {
+        s.kind = SmootherKind::BlockGS; s.sweep = Sweep::AltBackwardFirst;
+    }
     else if (code == "cheb1" || code == "cheb2" || code == "cheb3" || code == "cheb4") {
         s.kind = SmootherKind::Chebyshev;
         s.degree = code[4] - '0';
@@ -90,8 +98,10 @@ SmootherSpec SmootherSpec::parse(const std::string& code, int iterations, double
     }
     else throw Error(-2, "Unmanaged smoother code: '" + code + "'");
     if (s.kind == SmootherKind::BlockGS) {
-        FH_CHECK(relax == 1.0, -2, "block SOR (relaxation != 1) is not available; use bgs/rbgs with -relax 1");
-        s.omega = 1.0;
+        // SmootherFactory.h:27-80: the Gauss-Seidel codes with omega != 1 are
+        // BlockSOR (x_i <- x_i + omega D_i^-1 (b_i - sum_j A_ij x_j))
+        FH_CHECK(relax > 0 && relax < 2, -2, "SOR relaxation must lie in (0, 2)");
+        s.omega = relax;
     }
     return s;
 }
@@ -160,12 +170,16 @@ void smoother_setup(Ctx& c, const Bcsr& A64, LevelSmoother& S, Prec dprec) {
     }
 }
 
+// One multicolour block SOR sweep (omega = 1: Gauss-Seidel): the rows of a
+// colour are uncoupled, so x_i <- x_i + omega D_i^-1 (b_i - sum_j A_ij x_j)
+// over one colour at a time, in place, is the sequential sweep in the order
+// of the colours (BlockSOR::ProcessBlockRow, BlockSOR.h:193-223).
 static void gs_sweep(Ctx& c, const Bcsr& A, LevelSmoother& S, Prec vp, const void* b, DVec& x, bool backward) {
     for (int k = 0; k < S.ncolours; k++) {
         int cc = backward ? S.ncolours - 1 - k : k;
         int cnt = S.colour_ptr[cc + 1] - S.colour_ptr[cc];
         const int* rows = S.colour_rows.as<int>() + S.colour_ptr[cc];
-        launch_bcsr(c, A, OP_SMOOTH, vp, x.data(), b, x.data(), 1.0, 0.0, 1.0, S.invD.p, S.dprec, rows, cnt);
+        launch_bcsr(c, A, OP_SMOOTH, vp, x.data(), b, x.data(), 1.0, 0.0, S.spec.omega, S.invD.p, S.dprec, rows, cnt);
     }
 }
 
@@ -208,8 +222,11 @@ void smoother_apply(Ctx& c, const Bcsr& A, LevelSmoother& S, Prec vp, const void
             }
         } else {
             if (x_is_zero) launch_fill(c, x.n, x.data(), vp, 0.0);
-            if (S.spec.sweep == Sweep::Forward || S.spec.sweep == Sweep::Symmetric) gs_sweep(c, A, S, vp, b, x, false);
-            if (S.spec.sweep == Sweep::Backward || S.spec.sweep == Sweep::Symmetric) gs_sweep(c, A, S, vp, b, x, true);
+            Sweep sw = S.spec.sweep;
+            if (sw == Sweep::AltForwardFirst || sw == Sweep::AltBackwardFirst)
+                sw = (it % 2 == (sw == Sweep::AltForwardFirst ? 0 : 1)) ? Sweep::Forward : Sweep::Backward;
+            if (sw == Sweep::Forward || sw == Sweep::Symmetric) gs_sweep(c, A, S, vp, b, x, false);
+            if (sw == Sweep::Backward || sw == Sweep::Symmetric) gs_sweep(c, A, S, vp, b, x, true);
         }
         x_is_zero = false;
     }
@@ -293,16 +310,16 @@ public:
     }
 
     // Preconditioner application on FP64 vectors: z = M r (one cycle from 0).
-    // V- and W-cycles have no host decision inside, so the whole cycle can be
+    // No cycle has a host decision inside (the K-cycle's coefficients and
+    // its early exit are computed on the device), so the whole cycle can be
     // captured once per (r, z) pair into a HIP graph and replayed.  Whether
     // replay beats eager launching depends on the cycle (measured on MI355X:
     // faster for ~250 launches per cycle, slower for ~1000), so the first
     // application of a pair times one eager cycle and one replay and keeps
-    // the faster.  The K-cycle reads scalars on the host and always runs
-    // eagerly, as does every cycle while launches are being timed.  The
+    // the faster.  Cycles run eagerly while launches are being timed.  The
     // graph holds no memset node (vectors are cleared by kernels).
     void apply(const double* r, double* z) {
-        if (a_.cycle == 'K' || !graphs_enabled() || c_.prof_matrix != nullptr) {
+        if (!graphs_enabled() || c_.prof_matrix != nullptr) {
             apply_eager(r, z);
             return;
         }
@@ -478,11 +495,11 @@ private:
             if (!coarsest) {
                 if (l > 0 && !a_.coarse_smoother.empty()) {
                     const double w = a_.coarse_relax > 0 ? a_.coarse_relax : a_.relax;
-                    V.pre.spec = SmootherSpec::parse(a_.coarse_smoother, a_.nu1, w);
-                    V.post.spec = SmootherSpec::parse(a_.coarse_smoother, a_.nu2, w);
+                    V.pre.spec = SmootherSpec::parse(a_.coarse_smoother, a_.level_nu(a_.nu1, (int)l), w);
+                    V.post.spec = SmootherSpec::parse(a_.coarse_smoother, a_.level_nu(a_.nu2, (int)l), w);
                 } else {
-                    V.pre.spec = SmootherSpec::parse(a_.pre_smoother, a_.nu1, a_.relax);
-                    V.post.spec = SmootherSpec::parse(a_.post_smoother, a_.nu2, a_.relax);
+                    V.pre.spec = SmootherSpec::parse(a_.pre_smoother, a_.level_nu(a_.nu1, (int)l), a_.relax);
+                    V.post.spec = SmootherSpec::parse(a_.post_smoother, a_.level_nu(a_.nu2, (int)l), a_.relax);
                 }
                 smoother_setup(c_, *V.A64, V.pre, dprec);
                 smoother_setup(c_, *V.A64, V.post, dprec);
@@ -540,10 +557,14 @@ public:
         }
         bool xz = x_is_zero;
         smoother_apply(c_, *V.Aw, V.pre, vp_, b, x, V.work, xz);
-        if (xz) copy(c_, V.n, vp_, b, V.r.data());
-        else launch_bcsr(c_, *V.Aw, OP_RESID, vp_, x.data(), b, V.r.data(), 1.0, 0.0, 0.0, nullptr, vp_, nullptr, 0);
         MgLevel& C = L[l + 1];
-        restrict_to(l, V.r, C.b);
+        // A p-transfer to one mode per face needs only entry 0 of each block
+        // row of the residual: one pass over entry 0 of every block
+        if (xz || !V.ptrunc || V.nfc != 1 || !launch_resid_head(c_, *V.Aw, vp_, x.data(), b, C.b.data())) {
+            if (xz) copy(c_, V.n, vp_, b, V.r.data());
+            else launch_bcsr(c_, *V.Aw, OP_RESID, vp_, x.data(), b, V.r.data(), 1.0, 0.0, 0.0, nullptr, vp_, nullptr, 0);
+            restrict_to(l, V.r, C.b);
+        }
         bool coarse_is_last = l + 2 == (int)L.size();
         if (a_.cycle == 'K' && !coarse_is_last && (a_.kcycle_levels <= 0 || l + 1 <= a_.kcycle_levels)) {
             kcycle(l + 1, C.b, C.x);
@@ -563,6 +584,11 @@ public:
     }
 
     // Multigrid::FCGForKCycle (Multigrid.h:611-656); r is overwritten.
+    // Everything stays on the device: the second cycle, on the updated
+    // residual, is always run, and the coefficient kernel drops it
+    // (cb = 0) when |r_new| <= t |r|, as the reference's early return does.
+    // With no host decision inside, the whole preconditioner can be
+    // captured into a graph.
     void kcycle(int l, DVec& r, DVec& out) {
         MgLevel& V = L[l];
         const int s = S_KBASE + 8 * l;
@@ -570,25 +596,15 @@ public:
         cycle(l, r.data(), V.kc, true, &V.kv);
         // rho1 = c.v, alpha1 = c.r, |r|^2
         dot3(c_, V.n, vp_, V.kc.data(), V.kv.data(), s + 0, V.kc.data(), r.data(), s + 1, r.data(), r.data(), s + 2);
-        read_scalars(c_, s, 3);
-        double rho1 = c_.h_scalars[s + 0], alpha1 = c_.h_scalars[s + 1];
-        double rold = std::sqrt(c_.h_scalars[s + 2]);
-        launch_axpby(c_, V.n, vp_, V.kv.data(), r.data(), -(alpha1 / rho1), -1, -1, 1.0, -1, -1);
+        // r <- r - (alpha1 / rho1) v, then |r_new|^2
+        launch_axpby(c_, V.n, vp_, V.kv.data(), r.data(), -1.0, s + 1, s + 0, 1.0, -1, -1);
         dot1(c_, V.n, vp_, r.data(), r.data(), s + 3);
-        read_scalars(c_, s + 3, 1);
-        double rnew = std::sqrt(c_.h_scalars[s + 3]);
-        if (rnew <= t * rold) {
-            launch_axpby(c_, V.n, vp_, V.kc.data(), out.data(), alpha1 / rho1, -1, -1, 0.0, -1, -1);
-            return;
-        }
         cycle(l, r.data(), V.kd, true, &V.kw);
+        // gamma = d.v, beta = d.w, alpha2 = d.r
         dot3(c_, V.n, vp_, V.kd.data(), V.kv.data(), s + 4, V.kd.data(), V.kw.data(), s + 5, V.kd.data(), r.data(), s + 6);
-        read_scalars(c_, s + 4, 3);
-        double gamma = c_.h_scalars[s + 4], beta = c_.h_scalars[s + 5], alpha2 = c_.h_scalars[s + 6];
-        double rho2 = beta - gamma * gamma / rho1;
-        double ca = alpha1 / rho1 - gamma * alpha2 / (rho1 * rho2), cb = alpha2 / rho2;
-        launch_axpby(c_, V.n, vp_, V.kc.data(), out.data(), ca, -1, -1, 0.0, -1, -1);
-        launch_axpby(c_, V.n, vp_, V.kd.data(), out.data(), cb, -1, -1, 1.0, -1, -1);
+        launch_kcycle_coeffs(c_, s, t);
+        launch_axpby(c_, V.n, vp_, V.kc.data(), out.data(), 1.0, s + 2, -1, 0.0, -1, -1);
+        launch_axpby(c_, V.n, vp_, V.kd.data(), out.data(), 1.0, s + 3, -1, 1.0, -1, -1);
     }
 };
 
@@ -688,9 +704,13 @@ void Solver::Setup(const Bcsr& A) {
             g[0].first = &A;
         }
         mg_->setup(A, g, p == "aggregamg", given_p_);
-    } else if (p == "bj" || p == "bj23" || p == "bgs") {
+    } else if (p == "bj" || p == "bj23" || p == "bgs" || p == "rbgs" || p == "sbgs" || p == "bsor" || p == "rbsor" ||
+               p == "sbsor") {
+        // SolverFactory.h:57-84: the bgs codes are BlockSOR with omega = 1,
+        // the bsor codes take the relaxation parameter
         prec_sm_ = std::make_unique<LevelSmoother>();
-        prec_sm_->spec = SmootherSpec::parse(p, 1, args_.relax);
+        const bool gs = p == "bgs" || p == "rbgs" || p == "sbgs";
+        prec_sm_->spec = SmootherSpec::parse(p, 1, gs ? 1.0 : args_.relax);
         smoother_setup(c_, A, *prec_sm_, Prec::F64);
     } else if (!(p.empty() || p == "none" || p == "default"))
         throw Error(-2, "Unknown solver or preconditioner '" + p + "'");

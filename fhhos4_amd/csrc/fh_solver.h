@@ -22,11 +22,18 @@
 namespace fh {
 
 enum class SmootherKind { BlockJacobi, BlockGS, Chebyshev };
-enum class Sweep { Forward = 0, Backward = 1, Symmetric = 2 };
+// Alternating sweeps go forward on even smoothing iterations and backward
+// on odd ones (AltForwardFirst), or the other way (BlockGaussSeidel.h:105-120,
+// BlockSOR.h:153-182: IterationCount % 2).
+enum class Sweep { Forward = 0, Backward = 1, Symmetric = 2, AltForwardFirst = 3, AltBackwardFirst = 4 };
 
 // Smoother codes of SmootherFactory::Create (SmootherFactory.h:9-104)
-// restricted to the block smoothers: bj, bj23, j, j23, bgs, rbgs, sbgs, gs,
-// rgs, sgs (Gauss-Seidel runs as multicolour block Gauss-Seidel).
+// restricted to the block smoothers: bj, bj23, j, j23 (block Jacobi); bgs,
+// rbgs, sbgs, abgs, rabgs, gs, rgs, sgs, ags, rags (Gauss-Seidel) and bsor,
+// rbsor, sbsor, absor, rabsor, sor, rsor (SOR: Gauss-Seidel with the
+// relaxation parameter; bgs with relax != 1 is SOR too, as in the
+// reference); hbgs, hrbgs (the reference's chunk-parallel sweep).  Every
+// Gauss-Seidel / SOR sweep runs as a multicolour block sweep.
 struct SmootherSpec {
     SmootherKind kind = SmootherKind::BlockJacobi;
     Sweep sweep = Sweep::Forward;
@@ -103,6 +110,19 @@ struct SolverArgs {
     // reference: bounds the number of visits of the coarsest levels, which
     // doubles with every K level.
     int kcycle_levels = 0;
+    // Multigrid.h:224-242: smoothing iterations of each coarser level are its
+    // finer level's changed by op ('+', '-', '*'; 0 = unchanged) with coeff
+    char nu_change_op = 0;
+    int nu_change_coeff = 0;
+    // smoothing iterations of level l from the finest level's nu
+    int level_nu(int nu, int l) const {
+        for (int k = 0; k < l; k++) {
+            if (nu_change_op == '+') nu += nu_change_coeff;
+            else if (nu_change_op == '-') nu -= nu_change_coeff;
+            else if (nu_change_op == '*') nu *= nu_change_coeff;
+        }
+        return nu < 0 ? 0 : nu;
+    }
 };
 
 class Solver {

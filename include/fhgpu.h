@@ -115,7 +115,9 @@ int fh_axpby(fh_ctx* ctx, int64_t n, double a, const void* x, double b, void* y,
 int fh_convert(fh_ctx* ctx, int64_t n, const void* src, int sprec, void* dst, int dprec);
 
 /* In-situ timing of the block-row kernel: every launch of operation `mode`
- * (0 spmv, 1 residual, 2 smoothing, 3 fused Chebyshev step) on matrix A is
+ * (0 spmv, 1 residual, 2 smoothing, 3 fused Chebyshev step, 4 residual of
+ * the first entry of every block row, fused with the p-restriction to one
+ * mode per face) on matrix A is
  * bracketed by HIP events on the context's stream until fh_profile_end,
  * which returns the number of launches and their mean duration in ms.  Replaces the reference's Timer
  * accounting of matrix-vector products (src/Utils/Timer.h:60-123). */
@@ -204,6 +206,12 @@ typedef struct fh_solver_args {
     /* Not in the reference: K-cycle (cycle 'K') only on levels 1..kcycle_levels
      * (<= 0: all, as Multigrid.h:611-656), V-cycle recursion below. */
     int kcycle_levels;
+    /* Multigrid::CoarseLevelChangeSmoothingOperator / Coeff (Multigrid.h:224-242,
+     * -cycle V,nu1,nu2,<op><coeff>): each level's smoothing iterations are its
+     * finer level's changed by op ('+', '-', '*'; 0 = unchanged) with coeff;
+     * a count that falls below 0 is 0. */
+    char nu_change_op;
+    int nu_change_coeff;
 } fh_solver_args;
 
 typedef struct fh_solve_stats {

@@ -156,18 +156,33 @@ void bj_sweep(const Csr& A, int nf, const double* invD, double omega, const doub
     }
 }
 
-// BlockGaussSeidel::ProcessBlockRow (BlockGaussSeidel.h:193-223)
-inline void bgs_row(const Csr& A, int nf, const double* invD, int i, const double* b, double* x) {
-    double tmp[64];
+// BlockGaussSeidel::ProcessBlockRow (BlockGaussSeidel.h:193-223); with
+// omega != 1 BlockSOR::ProcessBlockRow (BlockSOR.h:193-223):
+// x_i <- D_i^-1 (omega b_i + (1 - omega) D_i x_i - omega sum_{j != i} A_ij x_j)
+inline void sor_rhs(int nf, const int* rp, const int* ci, const double* v, int i, double omega, const double* b,
+                    const double* x, double* tmp) {
     for (int k = 0; k < nf; k++) {
         int row = i * nf + k;
-        double t = b[row];
-        for (int p = A.prp[row]; p < A.prp[row + 1]; p++) {
-            int j = A.pci[p];
-            if (j / nf != i) t -= A.pv[p] * x[j];
+        if (omega == 1.0) {
+            double t = b[row];
+            for (int p = rp[row]; p < rp[row + 1]; p++)
+                if (ci[p] / nf != i) t -= v[p] * x[ci[p]];
+            tmp[k] = t;
+        } else {
+            double t = omega * b[row];
+            for (int p = rp[row]; p < rp[row + 1]; p++) {
+                if (ci[p] / nf == i) t += (1 - omega) * v[p] * x[ci[p]];
+                else t += -omega * v[p] * x[ci[p]];
+            }
+            tmp[k] = t;
         }
-        tmp[k] = t;
     }
+}
+
+inline void bgs_row(const Csr& A, int nf, const double* invD, int i, const double* b, double* x,
+                    double omega = 1.0) {
+    double tmp[64];
+    sor_rhs(nf, A.prp, A.pci, A.pv, i, omega, b, x, tmp);
     const double* Di = invD + (size_t)i * nf * nf;
     for (int k = 0; k < nf; k++) {
         double s = 0;
@@ -176,13 +191,15 @@ inline void bgs_row(const Csr& A, int nf, const double* invD, int i, const doubl
     }
 }
 
-// BlockGaussSeidel::ExecuteOneIteration, non-hybrid (BlockGaussSeidel.h:84-123)
-void bgs_sweep(const Csr& A, int nf, const double* invD, int dir, const double* b, double* x) {
+// BlockGaussSeidel::ExecuteOneIteration, non-hybrid (BlockGaussSeidel.h:84-123),
+// BlockSOR::ExecuteOneIteration (BlockSOR.h:98-188); dir 0 forward,
+// 1 backward, 2 symmetric
+void bgs_sweep(const Csr& A, int nf, const double* invD, int dir, const double* b, double* x, double omega = 1.0) {
     int nb = A.n / nf;
     if (dir == 0 || dir == 2)
-        for (int i = 0; i < nb; i++) bgs_row(A, nf, invD, i, b, x);
+        for (int i = 0; i < nb; i++) bgs_row(A, nf, invD, i, b, x, omega);
     if (dir == 1 || dir == 2)
-        for (int i = nb - 1; i >= 0; i--) bgs_row(A, nf, invD, i, b, x);
+        for (int i = nb - 1; i >= 0; i--) bgs_row(A, nf, invD, i, b, x, omega);
 }
 
 // Dense Cholesky (coarse solver; the reference uses SimplicialLDLT,
@@ -360,7 +377,10 @@ void galerkin(const Csr& A, int nf, const std::vector<int>& agg, int nagg, Csr& 
 
 // ----------------------------------------------------------------- multigrid
 enum { SM_BJ = 1, SM_BGS_F = 2, SM_BGS_B = 3, SM_BGS_S = 4, SM_MC_F = 5, SM_MC_B = 6, SM_MC_S = 7,
-       SM_CHEB2 = 8, SM_CHEB3 = 9, SM_CHEB4 = 10, SM_CHEB1 = 11 };
+       SM_CHEB2 = 8, SM_CHEB3 = 9, SM_CHEB4 = 10, SM_CHEB1 = 11,
+       // alternating directions (forward first / backward first), lexicographic and multicolour
+       SM_BGS_AF = 12, SM_BGS_AB = 13, SM_MC_AF = 14, SM_MC_AB = 15 };
+inline bool sm_is_cheb(int k) { return k >= SM_CHEB2 && k <= SM_CHEB1; }
 
 // out = D^-1 (b - A x), block by block
 void dinv_resid(const Csr& A, int nf, const double* invD, const double* b, const double* x, double* out) {
@@ -435,7 +455,7 @@ struct MG {
     void smooth(Level& L, int kind, double omega, int nu, const double* b, double* x) {
         const int nf = L.nf;
         for (int s = 0; s < nu; s++) {
-            if (kind >= SM_CHEB2) {
+            if (sm_is_cheb(kind)) {
                 // Chebyshev polynomial of D^-1 A on [0.25, 1.1] x lambda_max estimate
                 if (L.lmax == 0) L.lmax = cheb_lmax(L.A, nf, L.invD.data());
                 const int deg = kind == SM_CHEB1 ? 1 : 2 + (kind - SM_CHEB2);
@@ -460,15 +480,24 @@ struct MG {
                 std::fprintf(stderr, "oracle: block-stored levels are smoothed by Chebyshev only\n");
                 std::abort();
             }
-            if (kind == SM_BJ) bj_sweep(L.A, nf, L.invD.data(), omega, b, x);
-            else if (kind >= SM_MC_F) {
+            // alternating: forward on even smoothing iterations when forward
+            // comes first (IterationCount % 2, BlockGaussSeidel.h:105-120)
+            int k2 = kind;
+            if (kind == SM_BGS_AF || kind == SM_BGS_AB)
+                k2 = s % 2 == (kind == SM_BGS_AF ? 0 : 1) ? SM_BGS_F : SM_BGS_B;
+            if (kind == SM_MC_AF || kind == SM_MC_AB)
+                k2 = s % 2 == (kind == SM_MC_AF ? 0 : 1) ? SM_MC_F : SM_MC_B;
+            if (k2 == SM_BJ) bj_sweep(L.A, nf, L.invD.data(), omega, b, x);
+            else if (k2 >= SM_MC_F && k2 <= SM_MC_S) {
                 if (L.cptr.empty()) greedy_colour_blocks(L.A, nf, L.cptr, L.crows);
                 int nc = (int)L.cptr.size() - 1;
-                if (kind != SM_MC_B)
-                    or_mcbgs_sweep(L.A.n, nf, L.A.prp, L.A.pci, L.A.pv, L.invD.data(), nc, L.cptr.data(), L.crows.data(), 0, b, x);
-                if (kind != SM_MC_F)
-                    or_mcbgs_sweep(L.A.n, nf, L.A.prp, L.A.pci, L.A.pv, L.invD.data(), nc, L.cptr.data(), L.crows.data(), 1, b, x);
-            } else bgs_sweep(L.A, nf, L.invD.data(), kind - SM_BGS_F, b, x);
+                if (k2 != SM_MC_B)
+                    or_mcsor_sweep(L.A.n, nf, L.A.prp, L.A.pci, L.A.pv, L.invD.data(), nc, L.cptr.data(), L.crows.data(),
+                                   0, omega, b, x);
+                if (k2 != SM_MC_F)
+                    or_mcsor_sweep(L.A.n, nf, L.A.prp, L.A.pci, L.A.pv, L.invD.data(), nc, L.cptr.data(), L.crows.data(),
+                                   1, omega, b, x);
+            } else bgs_sweep(L.A, nf, L.invD.data(), k2 - SM_BGS_F, b, x, omega);
         }
     }
     void restrict_(int l, const std::vector<double>& r, std::vector<double>& rc) {
@@ -514,6 +543,16 @@ struct MG {
                 for (int a = 0; a < nf; a++) x[(size_t)i * nf + a] += ec[(size_t)L.agg[i] * nf + a];
         }
     }
+    // smoothing iterations of level l (Multigrid::Setup, Multigrid.h:224-242:
+    // each coarse level's count is its finer level's changed by op, coeff)
+    int level_nu(int nu, int l) const {
+        for (int k = 0; k < l; k++) {
+            if (cfg->nu_change_op == '+') nu += cfg->nu_change_coeff;
+            else if (cfg->nu_change_op == '-') nu -= cfg->nu_change_coeff;
+            else if (cfg->nu_change_op == '*') nu *= cfg->nu_change_coeff;
+        }
+        return nu < 0 ? 0 : nu;
+    }
     // Multigrid::MultigridCycle (Multigrid.h:438-608)
     void cycle(int l, const double* b, double* x, double* Ax) {
         Level& L = lv[l];
@@ -524,7 +563,8 @@ struct MG {
             return;
         }
         const bool cs = l > 0 && cfg->coarse_kind > 0;
-        smooth(L, cs ? cfg->coarse_kind : cfg->pre_kind, cs ? cfg->coarse_omega : cfg->pre_omega, cfg->nu1, b, x);
+        smooth(L, cs ? cfg->coarse_kind : cfg->pre_kind, cs ? cfg->coarse_omega : cfg->pre_omega, level_nu(cfg->nu1, l),
+               b, x);
         std::vector<double> r(n);
         spmv(L.A, x, r.data());
         for (int i = 0; i < n; i++) r[i] = b[i] - r[i];
@@ -542,7 +582,8 @@ struct MG {
             }
         }
         prolong_add(l, ec, x);
-        smooth(L, cs ? cfg->coarse_kind : cfg->post_kind, cs ? cfg->coarse_omega : cfg->post_omega, cfg->nu2, b, x);
+        smooth(L, cs ? cfg->coarse_kind : cfg->post_kind, cs ? cfg->coarse_omega : cfg->post_omega,
+               level_nu(cfg->nu2, l), b, x);
         if (Ax) spmv(L.A, x, Ax);
     }
     // Multigrid::FCGForKCycle (Multigrid.h:611-656); r is overwritten
@@ -699,10 +740,21 @@ void or_bgs_sweep(int n, int nf, const int* rp, const int* ci, const double* v, 
     bgs_sweep(A, nf, invD, dir, b, x);
 }
 
+void or_bsor_sweep(int n, int nf, const int* rp, const int* ci, const double* v, const double* invD,
+                   int dir, double omega, const double* b, double* x) {
+    Csr A; A.view(n, n, rp, ci, v);
+    bgs_sweep(A, nf, invD, dir, b, x, omega);
+}
+
 void or_mcbgs_sweep(int n, int nf, const int* rp, const int* ci, const double* v, const double* invD,
                     int ncol, const int* colour_ptr, const int* colour_rows, int dir,
                     const double* b, double* x) {
-    Csr A; A.view(n, n, rp, ci, v);
+    or_mcsor_sweep(n, nf, rp, ci, v, invD, ncol, colour_ptr, colour_rows, dir, 1.0, b, x);
+}
+
+void or_mcsor_sweep(int n, int nf, const int* rp, const int* ci, const double* v, const double* invD,
+                    int ncol, const int* colour_ptr, const int* colour_rows, int dir, double omega,
+                    const double* b, double* x) {
     for (int cc = 0; cc < ncol; cc++) {
         int c = dir == 0 ? cc : ncol - 1 - cc;
         // rows of one colour are not coupled: their updates read only the
@@ -711,13 +763,7 @@ void or_mcbgs_sweep(int n, int nf, const int* rp, const int* ci, const double* v
         for (int q = colour_ptr[c]; q < colour_ptr[c + 1]; q++) {
             int i = colour_rows[q];
             double tmp[64];
-            for (int k = 0; k < nf; k++) {
-                int row = i * nf + k;
-                double t = b[row];
-                for (int p = rp[row]; p < rp[row + 1]; p++)
-                    if (ci[p] / nf != i) t -= v[p] * x[ci[p]];
-                tmp[k] = t;
-            }
+            sor_rhs(nf, rp, ci, v, i, omega, b, x, tmp);
             const double* Di = invD + (size_t)i * nf * nf;
             for (int k = 0; k < nf; k++) {
                 double s = 0;
@@ -844,7 +890,8 @@ static int solve_impl(const or_solver_cfg* cfg, const Csr& A, const or_level_des
     }
     // MG::CanOptimizeResidualComputation (Multigrid.h:74-77): the point
     // Gauss-Seidel post-smoother returns A x (GaussSeidel.h:53-56).
-    bool fused = cfg->prec >= 3 && mg.lv.size() > 1 && nf == 1 && cfg->post_kind >= SM_BGS_F && cfg->nu2 > 0;
+    bool fused = cfg->prec >= 3 && mg.lv.size() > 1 && nf == 1 && cfg->post_kind >= SM_BGS_F && !sm_is_cheb(cfg->post_kind) &&
+                 cfg->nu2 > 0;
     auto t1 = std::chrono::steady_clock::now();
 
     auto apply = [&](const double* r, double* z, double* Az) {

@@ -45,6 +45,13 @@ void or_bgs_sweep(int n, int nf, const int* rp, const int* ci, const double* v, 
 void or_mcbgs_sweep(int n, int nf, const int* rp, const int* ci, const double* v, const double* invD,
                     int ncol, const int* colour_ptr, const int* colour_rows, int dir,
                     const double* b, double* x);
+/* Block SOR (BlockSOR.h:193-223): the Gauss-Seidel sweeps with relaxation
+ * omega (omega = 1 is Gauss-Seidel), lexicographic and multicolour. */
+void or_bsor_sweep(int n, int nf, const int* rp, const int* ci, const double* v, const double* invD,
+                   int dir, double omega, const double* b, double* x);
+void or_mcsor_sweep(int n, int nf, const int* rp, const int* ci, const double* v, const double* invD,
+                    int ncol, const int* colour_ptr, const int* colour_rows, int dir, double omega,
+                    const double* b, double* x);
 
 /* Greedy distance-1 colouring of the block graph: block rows in increasing
  * order take the smallest colour unused by their already-coloured
@@ -80,7 +87,10 @@ typedef struct or_solver_cfg {
     int pre_kind, post_kind;   /* 1 bj, 2 bgs forward, 3 bgs backward, 4 bgs symmetric,
                                   5/6/7 multicolour bgs forward/backward/symmetric,
                                   8/9/10/11 Chebyshev of degree 2/3/4/1 of D^-1 A
-                                  on [0.25, 1.1] x its power-iteration lambda_max */
+                                  on [0.25, 1.1] x its power-iteration lambda_max,
+                                  12/13 alternating bgs (forward / backward first),
+                                  14/15 the same, multicolour; the Gauss-Seidel
+                                  kinds relax with pre/post_omega (SOR) */
     double pre_omega, post_omega;
     int nu1, nu2;
     char cycle;        /* 'V', 'W' or 'K' */
@@ -96,6 +106,8 @@ typedef struct or_solver_cfg {
     int fcg_restart;     /* FCG: clear instead of dropping the oldest */
     int kcycle_levels;   /* K-cycle only on levels 1..kcycle_levels (<= 0: all), V recursion below */
     int x_given;         /* nonzero: x holds the initial guess (IterativeSolver::Solve(b, x, false)) */
+    char nu_change_op;   /* Multigrid.h:224-242: '+', '-', '*' or 0; each level's smoothing */
+    int nu_change_coeff; /* iterations are its finer level's changed by op with coeff (>= 0) */
 } or_solver_cfg;
 
 typedef struct or_solver_out {

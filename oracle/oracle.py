@@ -29,7 +29,7 @@ class SolverCfg(C.Structure):
                 ("wloops", C.c_int), ("max_coarse", C.c_int), ("theta", C.c_double), ("tol", C.c_double),
                 ("max_iter", C.c_int), ("nlevels", C.c_int), ("coarse_kind", C.c_int), ("coarse_omega", C.c_double),
                 ("fcg_truncation", C.c_int), ("fcg_restart", C.c_int), ("kcycle_levels", C.c_int),
-                ("x_given", C.c_int)]
+                ("x_given", C.c_int), ("nu_change_op", C.c_char), ("nu_change_coeff", C.c_int)]
 
 
 class SolverOut(C.Structure):
@@ -56,6 +56,9 @@ def lib():
         L.or_bgs_sweep.argtypes = [C.c_int, C.c_int, _i32p, _i32p, _f64p, _f64p, C.c_int, _f64p, _f64p]
         L.or_mcbgs_sweep.argtypes = [C.c_int, C.c_int, _i32p, _i32p, _f64p, _f64p, C.c_int, _i32p, _i32p,
                                      C.c_int, _f64p, _f64p]
+        L.or_bsor_sweep.argtypes = [C.c_int, C.c_int, _i32p, _i32p, _f64p, _f64p, C.c_int, C.c_double, _f64p, _f64p]
+        L.or_mcsor_sweep.argtypes = [C.c_int, C.c_int, _i32p, _i32p, _f64p, _f64p, C.c_int, _i32p, _i32p,
+                                     C.c_int, C.c_double, _f64p, _f64p]
         L.or_greedy_colouring.argtypes = [C.c_int, _i32p, _i32p, _i32p]
         L.or_pairwise_aggregate.argtypes = [C.c_int, C.c_int, _i32p, _i32p, _f64p, C.c_double, _i32p,
                                             C.POINTER(C.c_int)]
@@ -110,6 +113,24 @@ def bgs_sweep(A, nf, invD, direction, b, x):
     return x
 
 
+def bsor_sweep(A, nf, invD, direction, omega, b, x):
+    rp, ci, v = _csr(A)
+    x = np.array(x, dtype=np.float64)
+    lib().or_bsor_sweep(A.shape[0], nf, rp, ci, v, invD, direction, omega, np.ascontiguousarray(b, dtype=np.float64),
+                        x)
+    return x
+
+
+def mcsor_sweep(A, nf, invD, colour_ptr, colour_rows, direction, omega, b, x):
+    rp, ci, v = _csr(A)
+    x = np.array(x, dtype=np.float64)
+    lib().or_mcsor_sweep(A.shape[0], nf, rp, ci, v, invD, len(colour_ptr) - 1,
+                         np.ascontiguousarray(colour_ptr, dtype=np.int32),
+                         np.ascontiguousarray(colour_rows, dtype=np.int32), direction, omega,
+                         np.ascontiguousarray(b, dtype=np.float64), x)
+    return x
+
+
 def mcbgs_sweep(A, nf, invD, colour_ptr, colour_rows, direction, b, x):
     rp, ci, v = _csr(A)
     x = np.array(x, dtype=np.float64)
@@ -145,14 +166,18 @@ def amg_hierarchy(A, nf, theta=0.25, max_coarse=1000):
 
 
 SMOOTHER_KIND = {"bj": 1, "bj23": 1, "bgs": 2, "rbgs": 3, "sbgs": 4, "gs": 2, "rgs": 3, "sgs": 4,
-                 # multicolour block Gauss-Seidel (greedy colouring), the ordering of the HIP path
-                 "mcbgs": 5, "mcrbgs": 6, "mcsbgs": 7, "cheb2": 8, "cheb3": 9, "cheb4": 10, "cheb1": 11}
+                 "abgs": 12, "rabgs": 13, "ags": 12, "rags": 13,
+                 # block SOR: the Gauss-Seidel kinds with the relaxation `omega`
+                 "bsor": 2, "rbsor": 3, "sbsor": 4, "absor": 12, "rabsor": 13,
+                 # multicolour block Gauss-Seidel / SOR (greedy colouring), the ordering of the HIP path
+                 "mcbgs": 5, "mcrbgs": 6, "mcsbgs": 7, "mcabgs": 14, "mcrabgs": 15,
+                 "cheb2": 8, "cheb3": 9, "cheb4": 10, "cheb1": 11}
 PREC_KIND = {"": 0, "none": 0, "bj": 1, "bgs": 2, "mg": 3, "aggregamg": 4}
 
 
 def solve(A, b, solver="cg", prec="mg", nf=1, cycle="V", nu=(1, 1), smoothers=("bj", "bj"), omega=1.0,
           levels=None, tol=1e-8, max_iter=200, max_coarse=1000, theta=0.25, coarse_smoother=None, coarse_omega=0.0,
-          fcg_truncation=1, fcg_restart=False, kcycle_levels=0, x0=None):
+          fcg_truncation=1, fcg_restart=False, kcycle_levels=0, x0=None, nu_change=None):
     """Runs the oracle solver; `levels` = [(A_l, P_l), ...] for GMG (P of the
     coarsest level is None).  A level may be (A_l, ("p", nfc), nf_l): a
     p-transfer keeping nfc modes of every nf_l-block; (A_l, P_l, nf_l) gives
@@ -188,6 +213,8 @@ def solve(A, b, solver="cg", prec="mg", nf=1, cycle="V", nu=(1, 1), smoothers=("
     cfg.nu1, cfg.nu2 = nu
     cfg.fcg_truncation, cfg.fcg_restart = fcg_truncation, int(fcg_restart)
     cfg.kcycle_levels = kcycle_levels
+    if nu_change:  # e.g. "+1" (Multigrid.h:224-242)
+        cfg.nu_change_op, cfg.nu_change_coeff = nu_change[0].encode(), int(nu_change[1:])
     if coarse_smoother:
         cfg.coarse_kind = SMOOTHER_KIND[coarse_smoother]
         cfg.coarse_omega = 2.0 / 3.0 if coarse_smoother == "bj23" else (coarse_omega if coarse_omega > 0 else omega)

@@ -102,6 +102,9 @@ struct Opts
 	bool solve = true;
 	string kappaFile; // per-element isotropic kappa (int64 count + doubles, mesh element order)
 	bool dumpElements = false;
+	double relax = 1.0;        // Solver.RelaxationParameter (SOR smoothers)
+	char nuOp = '+';           // MG.CoarseLevelChangeSmoothingOperator / Coeff
+	int nuCoeff = 0;
 };
 
 // Gives every element a physical part of its own with the isotropic tensor
@@ -170,6 +173,9 @@ static int Run(const Opts& o)
 	mg.CycleLetter = o.cycle;
 	mg.PreSmoothingIterations = o.pre;
 	mg.PostSmoothingIterations = o.post;
+	mg.CoarseLevelChangeSmoothingOperator = o.nuOp;
+	mg.CoarseLevelChangeSmoothingCoeff = o.nuCoeff;
+	args.Solver.RelaxationParameter = o.relax;
 	size_t comma = o.smoothers.find(',');
 	mg.PreSmootherCode = o.smoothers.substr(0, comma);
 	mg.PostSmootherCode = o.smoothers.substr(comma + 1);
@@ -362,6 +368,8 @@ int main(int argc, char** argv)
 		else if (a == "-nosolve") o.solve = false;
 		else if (a == "-kappa-file") o.kappaFile = next();
 		else if (a == "-dump-elements") o.dumpElements = true;
+		else if (a == "-relax") o.relax = stod(next());
+		else if (a == "-coarse-nu") { string c = next(); o.nuOp = c[0]; o.nuCoeff = stoi(c.substr(1)); }
 		else { fprintf(stderr, "unknown option %s\n", a.c_str()); return 2; }
 	}
 	if (o.dim == 2) return Run<2>(o);

@@ -167,11 +167,18 @@ def main():
     run("c1_n32_fcg_mg_bj23", c1s + ["-n", "32", "-s", "fcg", "-prec", "mg", "-cycle", "V,1,1", "-smoothers", "bj23,bj23"], keep_A=False)
     run("c1_n32_cg_mg_w_bj23", c1s + ["-n", "32", "-s", "cg", "-prec", "mg", "-cycle", "W,1,1", "-smoothers", "bj23,bj23"], keep_A=False)
     run("c1_n32_fcg_mg_k_bgs", c1s + ["-n", "32", "-s", "fcg", "-prec", "mg", "-cycle", "K,1,1", "-smoothers", "bgs,rbgs"], keep_A=False)
+    # block SOR (BlockSOR.h), alternating Gauss-Seidel, per-level smoothing change (Multigrid.h:224-242)
+    run("c1_n32_cg_mg_bsor", c1s + ["-n", "32", "-s", "cg", "-prec", "mg", "-cycle", "V,1,1", "-smoothers", "bsor,rbsor",
+                                    "-relax", "1.2"], keep_A=False)
+    run("c1_n32_cg_mg_abgs", c1s + ["-n", "32", "-s", "cg", "-prec", "mg", "-cycle", "V,2,2", "-smoothers", "abgs,rabgs"],
+        keep_A=False)
     # Aggregation AMG on a scalar (k=0) tetrahedral system: FCG + K-cycle
     tet0 = ["-dim", "3", "-mesh", "stetra", "-k", "0", "-n", "8", "-source", "sine"]
     run("tet8_k0_fcg_amg", tet0 + ["-s", "fcg", "-prec", "aggregamg", "-cycle", "K,1,1", "-smoothers", "bgs,rbgs"],
         export_mg=True, aggreg=tet0 + ["-nosolve"])
     run("tet8_k0_cg_amg_v", tet0 + ["-s", "cg", "-prec", "aggregamg", "-cycle", "V,1,1", "-smoothers", "bgs,rbgs", "-max-iter", "25"], keep_A=False)
+    run("tet8_k0_fcg_amg_nu", tet0 + ["-s", "fcg", "-prec", "aggregamg", "-cycle", "K,1,1", "-smoothers", "bgs,rbgs",
+                                      "-coarse-nu", "+1"], keep_A=False)
     # Block (n_f = 6) tetrahedral system: SpMV / block smoothers / block aggregation
     tet2 = ["-dim", "3", "-mesh", "stetra", "-k", "2", "-n", "4", "-source", "sine"]
     run("tet4_k2", tet2 + ["-s", "cg", "-prec", "bj"], aggreg=tet2 + ["-nosolve"])

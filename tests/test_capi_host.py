@@ -99,6 +99,13 @@ def test_invalid_arguments_are_reported(hctx):
         G.Matrix.from_csr(hctx, A, 2)
     with pytest.raises(G.FhError, match="smoother|cycle"):
         s = G.Solver(hctx, solver="cg", prec="mg", cycle="X")
+    with pytest.raises(G.FhError, match="smoothing change"):
+        G.Solver(hctx, solver="cg", prec="mg", nu_change="/2")
+    with pytest.raises(G.FhError, match="relaxation"):
+        G.Solver(hctx, solver="cg", prec="mg", smoothers=("bsor", "rbsor"), relax=2.5, coarse_smoother="bsor")
+    # the reference's smoother codes are accepted
+    for code in ("bsor", "rbsor", "sbsor", "absor", "rabsor", "abgs", "rabgs", "ags", "rags", "hbgs", "hrbgs"):
+        G.Solver(hctx, solver="cg", prec="mg", smoothers=(code, code), relax=1.3, coarse_smoother=code)
 
 
 def test_execution_layout_query(hctx):

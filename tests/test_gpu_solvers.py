@@ -109,6 +109,40 @@ def test_gauss_seidel_solver_matches_oracle_multicolour(ctx, case):
     assert abs(st["iterations"] - int(f["info_iterations"])) <= 3
 
 
+MC_CASES = [
+    # fixture (reference run), system, solver, prec, cycle, smoothers, oracle's multicolour kinds, nu, relax,
+    # per-level smoothing change
+    ("c1_n32_cg_mg_bsor", "c1_n32_cg_mg_bj23", "cg", "mg", "V", ("bsor", "rbsor"), ("mcbgs", "mcrbgs"), (1, 1), 1.2,
+     None, 2),
+    ("c1_n32_cg_mg_abgs", "c1_n32_cg_mg_bj23", "cg", "mg", "V", ("abgs", "rabgs"), ("mcabgs", "mcrabgs"), (2, 2),
+     1.0, None, 2),
+    ("tet8_k0_fcg_amg_nu", "tet8_k0_fcg_amg", "fcg", "aggregamg", "K", ("bgs", "rbgs"), ("mcbgs", "mcrbgs"), (1, 1),
+     1.0, "+1", 1),
+]
+
+
+@pytest.mark.parametrize("case", MC_CASES, ids=[c[0] for c in MC_CASES])
+def test_sor_alternating_and_level_smoothing_match_oracle_multicolour(ctx, case):
+    """Block SOR (relaxation 1.2), alternating Gauss-Seidel and the per-level
+    change of the smoothing iterations (Multigrid.h:224-242) on the GPU
+    against the oracle run with the same multicolour ordering: iterations
+    equal, the solution to 1e-10; the reference's lexicographic count is
+    stated beside it."""
+    name, sysname, solver, prec, cycle, sm, osm, nu, relax, nuc, nf = case
+    print("case", name, flush=True)
+    f, d = load(name), load(sysname)
+    A = csr(d, "A")
+    levels = gmg_levels(d) if prec == "mg" else None
+    x, hist, st = run_gpu(ctx, d, nf, levels=levels, solver=solver, prec=prec, cycle=cycle, nu=nu, smoothers=sm,
+                          relax=relax, nu_change=nuc)
+    xo, ho, oo = O.solve(A, d["b"], solver, prec, nf=nf, cycle=cycle, nu=nu, smoothers=osm, omega=relax,
+                         levels=levels, nu_change=nuc)
+    assert st["iterations"] == oo.iterations
+    assert np.linalg.norm(x - xo) <= 1e-10 * np.linalg.norm(xo)
+    print("reference lexicographic iterations", int(f["info_iterations"]), "multicolour", st["iterations"])
+    assert abs(st["iterations"] - int(f["info_iterations"])) <= 3
+
+
 def test_amg_hierarchy_matches_reference_levels(ctx):
     d = load("tet8_k0_fcg_amg")
     A = csr(d, "A")

@@ -388,3 +388,84 @@ def test_oracle_initial_guess_is_a_shift(solver, prec):
     k = min(len(h1), len(he))
     assert np.allclose(h1[:k], he[:k] * nr / nb, rtol=1e-8, atol=0)
     assert np.linalg.norm(x1 - (x0 + e)) <= 1e-12 * np.linalg.norm(x1)
+
+
+SOR_CASES = [
+    # fixture (reference run), smoothers, nu, omega
+    ("c1_n32_cg_mg_bsor", ("bsor", "rbsor"), (1, 1), 1.2),
+    ("c1_n32_cg_mg_abgs", ("abgs", "rabgs"), (2, 2), 1.0),
+]
+
+
+@pytest.mark.parametrize("case", SOR_CASES, ids=[c[0] for c in SOR_CASES])
+def test_oracle_sor_and_alternating_match_reference(case):
+    """Block SOR (BlockSOR.h:193-223, relaxation 1.2) and alternating block
+    Gauss-Seidel (abgs / rabgs: forward and backward by smoothing iteration,
+    BlockGaussSeidel.h:105-120) against the reference's own CG + GMG runs:
+    the same iteration count and residual history, the solution to 1e-10."""
+    name, sm, nu, omega = case
+    f, d = load(name), load("c1_n32_cg_mg_bj23")
+    A = csr(d, "A")
+    x, hist, out = O.solve(A, d["b"], "cg", "mg", nf=2, smoothers=sm, nu=nu, omega=omega, levels=gmg_levels(d))
+    assert out.iterations == int(f["info_iterations"])
+    np.testing.assert_allclose(hist, f["hist"], rtol=1e-8, atol=1e-15)
+    assert np.linalg.norm(x - f["x"]) <= 1e-10 * np.linalg.norm(f["x"])
+
+
+def test_oracle_level_smoothing_change_matches_reference():
+    """The per-level change of the smoothing iterations (Multigrid.h:224-242,
+    '+1': one more smoothing iteration on every coarser level) in the
+    reference's FCG + aggregation AMG K-cycle: the same iteration count and
+    residual history (17 iterations, against 18 without the change), the
+    solution to 1e-10."""
+    f, d = load("tet8_k0_fcg_amg_nu"), load("tet8_k0_fcg_amg")
+    A = csr(d, "A")
+    kw = dict(nf=1, cycle="K", smoothers=("bgs", "rbgs"))
+    x, hist, out = O.solve(A, d["b"], "fcg", "aggregamg", nu_change="+1", **kw)
+    assert out.iterations == int(f["info_iterations"]) != int(d["info_iterations"])
+    np.testing.assert_allclose(hist, f["hist"], rtol=1e-9, atol=1e-15)
+    assert np.linalg.norm(x - f["x"]) <= 1e-10 * np.linalg.norm(f["x"])
+
+
+@pytest.mark.parametrize("omega", [0.7, 1.3])
+def test_bsor_sweep_solves_the_relaxed_triangular_system(omega):
+    """One forward block SOR sweep solves (D + omega L) x' = omega b -
+    (omega U + (omega - 1) D) x; backward swaps L and U (L, U: strictly block
+    lower / upper parts, D: diagonal blocks)."""
+    d = load("c1_n32_cg_mg_bj23")
+    A = csr(d, "A")
+    nf = 2
+    nb = A.shape[0] // nf
+    rng = np.random.default_rng(5)
+    b, x = rng.standard_normal(A.shape[0]), rng.standard_normal(A.shape[0])
+    inv = O.block_inverse(A, nf)
+    Ad = A.toarray()
+    blk = np.kron(np.eye(nb), np.ones((nf, nf)))
+    low = np.kron(np.tril(np.ones((nb, nb)), -1), np.ones((nf, nf)))
+    D, L = Ad * blk, Ad * low
+    U = Ad - D - L
+    got = O.bsor_sweep(A, nf, inv, 0, omega, b, x)
+    np.testing.assert_allclose((D + omega * L) @ got, omega * b - (omega * U + (omega - 1) * D) @ x,
+                               rtol=1e-9, atol=1e-9)
+    got = O.bsor_sweep(A, nf, inv, 1, omega, b, x)
+    np.testing.assert_allclose((D + omega * U) @ got, omega * b - (omega * L + (omega - 1) * D) @ x,
+                               rtol=1e-9, atol=1e-9)
+    # omega = 1 is Gauss-Seidel
+    np.testing.assert_allclose(O.bsor_sweep(A, nf, inv, 0, 1.0, b, x), O.bgs_sweep(A, nf, inv, 0, b, x),
+                               rtol=0, atol=0)
+
+
+def test_multicolour_sor_reduces_to_lexicographic():
+    """One colour per block row, visited in row order = lexicographic SOR."""
+    d = load("c1_n32_cg_mg_bj23")
+    A = csr(d, "A")
+    nf = 2
+    nb = A.shape[0] // nf
+    rng = np.random.default_rng(6)
+    b, x = rng.standard_normal(A.shape[0]), rng.standard_normal(A.shape[0])
+    inv = O.block_inverse(A, nf)
+    cp = np.arange(nb + 1, dtype=np.int32)
+    rows = np.arange(nb, dtype=np.int32)
+    for direction in (0, 1):
+        np.testing.assert_allclose(O.mcsor_sweep(A, nf, inv, cp, rows, direction, 1.25, b, x),
+                                   O.bsor_sweep(A, nf, inv, direction, 1.25, b, x), rtol=1e-12, atol=1e-12)
