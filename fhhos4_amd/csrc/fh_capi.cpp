@@ -177,7 +177,8 @@ int fh_mat_layout(const fh_mat* A, int* layout) {
         NEED(A); NEED(layout);
         const fh::Bcsr& M = *A->A;
         int v = 0;
-        if (M.sliced_state == 1 && M.sliced) v = M.sliced->kind == 1 ? 3 : (M.sliced->next ? 2 : 1);
+        if (M.sliced_state == 1 && M.sliced)
+            v = M.sliced->kind == 2 ? 4 : (M.sliced->kind == 1 ? 3 : (M.sliced->next ? 2 : 1));
         *layout = v;
     });
 }

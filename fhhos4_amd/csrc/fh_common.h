@@ -117,7 +117,9 @@ struct DVec {
 // further part (rows split by length into classes of their own width).
 // kind 1 (wide rows): every row split into chunks of W block slots
 // (RW = 1), chunk_ptr[row] .. chunk_ptr[row + 1] being its chunks, one
-// wavefront per row.
+// wavefront per row.  kind 2 (scalar square operators): SELL-64, slice s
+// of 64 rows with chunk_ptr[s + 1] - chunk_ptr[s] slots, entries and
+// columns slot-major from 64 chunk_ptr[s] (padding: value 0, column = row).
 struct Sliced {
     int kind = 0;
     int W = 0, RW = 0, cw = 1, nslices = 0;  // kind 1: nslices = number of chunks

@@ -742,6 +742,104 @@ __global__ void k_to_sliced(int nbr, int nslices, int W, int RW, const int* __re
         }
 }
 
+// SELL-64 layout of scalar operators (n_f = 1: the aggregation levels).
+// Slice s holds rows 64 s .. 64 s + 63 with w_s = sptr[s + 1] - sptr[s]
+// slots (its longest row); its entries are stored slot-major from
+// 64 sptr[s], lane = row, so every load instruction of a wavefront reads 64
+// consecutive values and every lane has w_s independent loads to issue.
+// Padding entries hold the value 0 and the lane's own row as column.
+constexpr int kSellC = 64;
+constexpr int kSellUnroll = 4;
+template <class TA, class TV, class TD, int MODE>
+__global__ __launch_bounds__(kWG) void k_sell(int nbr, int nslices, const int* __restrict__ sptr,
+                                              const int* __restrict__ cols, const TA* __restrict__ sv, const TV* x,
+                                              const TV* b, TV* y, double alpha, double beta, double omega,
+                                              const TD* __restrict__ invD, TV* __restrict__ aux) {
+    using AT = typename Acc<TV>::T;
+    using LT = std::conditional_t<std::is_same<TA, double>::value, double, float>;
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int s = xcd_remap(blockIdx.x, gridDim.x) * (kWG / 64) + wave;
+    if (s >= nslices) return;  // wavefront-uniform
+    const int p0 = sptr[s], w = sptr[s + 1] - p0;
+    const int row = s * kSellC + lane;
+    const int* cs = cols + (int64_t)p0 * kSellC + lane;
+    const TA* vs = sv + (int64_t)p0 * kSellC + lane;
+    // epilogue operands first: their loads overlap the product's
+    AT bv = 0, xv = 0, dv = 0, yv = 0, dinv = 0;
+    const bool out = row < nbr;
+    if (out) {
+        if (MODE != OP_SPMV) bv = (AT)b[row];
+        if (MODE == OP_CHEB || MODE == OP_SMOOTH) { xv = (AT)x[row]; dinv = (AT)ldv(invD + row); }
+        if (MODE == OP_CHEB && alpha != 0.0) dv = (AT)aux[row];
+        if (MODE == OP_SPMV && beta != 0.0) yv = (AT)y[row];
+    }
+    AT acc = 0;
+    int j = 0;
+    for (; j + kSellUnroll <= w; j += kSellUnroll) {
+        int c[kSellUnroll];
+        LT a[kSellUnroll];
+#pragma unroll
+        for (int u = 0; u < kSellUnroll; u++) c[u] = cs[u * kSellC];
+#pragma unroll
+        for (int u = 0; u < kSellUnroll; u++) ldnt(vs + u * kSellC, a[u]);
+#pragma unroll
+        for (int u = 0; u < kSellUnroll; u++) acc += (AT)a[u] * (AT)x[c[u]];
+        cs += kSellUnroll * kSellC;
+        vs += kSellUnroll * kSellC;
+    }
+    for (; j < w; j++) {
+        LT a;
+        const int c = cs[0];
+        ldnt(vs, a);
+        acc += (AT)a * (AT)x[c];
+        cs += kSellC;
+        vs += kSellC;
+    }
+    if (!out) return;
+    if constexpr (MODE == OP_SPMV) {
+        AT v = (AT)alpha * acc;
+        if (beta != 0.0) v += (AT)beta * yv;
+        y[row] = (TV)v;
+    } else if constexpr (MODE == OP_RESID) {
+        y[row] = (TV)(bv - acc);
+    } else if constexpr (MODE == OP_CHEB) {
+        const AT dn = (alpha != 0.0 ? (AT)alpha * dv : (AT)0) + (AT)beta * dinv * (bv - acc);
+        aux[row] = (TV)dn;
+        y[row] = (TV)(xv + dn);
+    } else {
+        y[row] = (TV)(xv + (AT)omega * dinv * (bv - acc));
+    }
+}
+
+// Block-CSR (n_f = 1) -> SELL-64 values and columns, one thread per
+// (slice, slot, lane).
+template <class TA>
+__global__ void k_to_sell(int nbr, int nslices, const int* __restrict__ sptr, const int* __restrict__ rp,
+                          const int* __restrict__ ci, const TA* __restrict__ val, int* __restrict__ cols,
+                          TA* __restrict__ sv, int64_t total) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    // slice of entry t: the last s with 64 sptr[s] <= t (binary search)
+    int lo = 0, hi = nslices;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if ((int64_t)sptr[mid] * kSellC <= t) lo = mid;
+        else hi = mid;
+    }
+    const int s = lo;
+    const int64_t off = t - (int64_t)sptr[s] * kSellC;
+    const int j = (int)(off / kSellC), lane = (int)(off - (int64_t)j * kSellC);
+    const int row = s * kSellC + lane;
+    int c = row < nbr ? row : 0;
+    TA v{};
+    if (row < nbr && j < rp[row + 1] - rp[row]) {
+        c = ci[rp[row] + j];
+        v = val[rp[row] + j];
+    }
+    cols[t] = c;
+    sv[t] = v;
+}
+
 template <class TV, class TD>
 __global__ void k_bjac_zero(int count, int nf, const TV* __restrict__ b, TV* __restrict__ y, double omega,
                             const TD* __restrict__ invD, const int* __restrict__ rows, TV* __restrict__ y2,
@@ -1073,6 +1171,35 @@ void bcsr_dispatch_mode(Ctx& c, const Bcsr& A, BcsrMode mode, const void* x, con
     auto yy = (TV*)y;
     auto D = (const TD*)invD;
     auto ax = (TV*)aux;
+    if constexpr (NF == 1) {
+        if (!rows && A.sliced_state == 1 && A.sliced->kind == 2) {
+            const Sliced& S = *A.sliced;
+            const int nwg = (S.nslices + kWG / 64 - 1) / (kWG / 64);
+            auto sp = S.chunk_ptr.as<int>();
+            auto sc = S.cols.as<int>();
+            auto sv = S.vals.as<TA>();
+            switch (mode) {
+                case OP_SPMV:
+                    hipLaunchKernelGGL((k_sell<TA, TV, TD, OP_SPMV>), dim3(nwg), dim3(kWG), 0, c.stream, A.nbr,
+                                       S.nslices, sp, sc, sv, xx, bb, yy, alpha, beta, omega, D, ax);
+                    break;
+                case OP_RESID:
+                    hipLaunchKernelGGL((k_sell<TA, TV, TD, OP_RESID>), dim3(nwg), dim3(kWG), 0, c.stream, A.nbr,
+                                       S.nslices, sp, sc, sv, xx, bb, yy, alpha, beta, omega, D, ax);
+                    break;
+                case OP_CHEB:
+                    hipLaunchKernelGGL((k_sell<TA, TV, TD, OP_CHEB>), dim3(nwg), dim3(kWG), 0, c.stream, A.nbr,
+                                       S.nslices, sp, sc, sv, xx, bb, yy, alpha, beta, omega, D, ax);
+                    break;
+                default:
+                    hipLaunchKernelGGL((k_sell<TA, TV, TD, OP_SMOOTH>), dim3(nwg), dim3(kWG), 0, c.stream, A.nbr,
+                                       S.nslices, sp, sc, sv, xx, bb, yy, alpha, beta, omega, D, ax);
+                    break;
+            }
+            FH_HIP(hipGetLastError());
+            return;
+        }
+    }
     if (!rows && A.sliced_state == 1 && A.sliced->kind == 1) {
         const Sliced& S = *A.sliced;
         const bool aligned = S.cw == 1 || ((uintptr_t)x % (2 * sizeof(TV))) == 0;
@@ -1271,6 +1398,49 @@ void wide_convert_prec(Ctx& c, const Bcsr& A, Sliced& S, const int* chunk_row) {
     else wide_convert<NF, bf16_t>(c, A, S, chunk_row);
 }
 
+// Slot counts of the SELL-64 layout of a scalar operator: sptr (nslices + 1
+// entries, in units of 64 entries); returns the total slots.
+int64_t sell_geometry(const Bcsr& A, std::vector<int>& sptr) {
+    const int nslices = (A.nbr + kSellC - 1) / kSellC;
+    sptr.assign(nslices + 1, 0);
+    for (int sl = 0; sl < nslices; sl++) {
+        int w = 0;
+        for (int i = sl * kSellC; i < std::min(A.nbr, (sl + 1) * kSellC); i++)
+            w = std::max(w, A.h_rowptr[i + 1] - A.h_rowptr[i]);
+        sptr[sl + 1] = sptr[sl] + w;
+    }
+    return (int64_t)sptr[nslices] * kSellC;
+}
+
+template <class TA>
+void sell_convert(Ctx& c, const Bcsr& A, Sliced& S, int64_t total) {
+    if (total == 0) return;
+    hipLaunchKernelGGL((k_to_sell<TA>), dim3((unsigned)((total + 255) / 256)), dim3(256), 0, c.stream, A.nbr,
+                       S.nslices, S.chunk_ptr.as<int>(), A.rowptr.as<int>(), A.colidx.as<int>(), A.vals.as<TA>(),
+                       S.cols.as<int>(), S.vals.as<TA>(), total);
+    FH_HIP(hipGetLastError());
+}
+
+void sell_build(Ctx& c, const Bcsr& A, const std::vector<int>& sptr, int64_t total) {
+    auto S = std::make_unique<Sliced>();
+    S->kind = 2;
+    S->W = 0; S->RW = kSellC; S->cw = 1;
+    S->nslices = (int)sptr.size() - 1;
+    S->nrows = A.nbr;
+    FH_CHECK(A.bs_r == 1 && A.bs_c == 1 && A.nbr == A.nbc && total == (int64_t)sptr.back() * kSellC, -10,
+             "SELL layout: square scalar operators");
+    S->chunk_ptr.alloc(sizeof(int) * sptr.size());
+    FH_HIP(hipMemcpyAsync(S->chunk_ptr.p, sptr.data(), sizeof(int) * sptr.size(), hipMemcpyHostToDevice, c.stream));
+    S->cols.alloc(sizeof(int) * std::max<int64_t>(1, total));
+    S->vals.alloc(prec_bytes(A.prec) * std::max<int64_t>(1, total));
+    if (A.prec == Prec::F64) sell_convert<double>(c, A, *S, total);
+    else if (A.prec == Prec::F32) sell_convert<float>(c, A, *S, total);
+    else sell_convert<bf16_t>(c, A, *S, total);
+    c.sync();  // the host slot table goes out of scope with the caller
+    A.sliced = std::move(S);
+    A.sliced_state = 1;
+}
+
 void wide_build(Ctx& c, const Bcsr& A, int W, int cw, int64_t nchunks) {
     auto S = std::make_unique<Sliced>();
     const int nf = A.bs_r;
@@ -1403,6 +1573,15 @@ int64_t sliced_split(const Bcsr& A, int cw, int& t_best) {
 // kernels read a layout copy is replayed only while the count is unchanged.
 std::atomic<uint64_t> g_layout_epoch{0};
 
+// FHGPU_SELL=0 keeps scalar operators on the other layouts.
+bool sell_enabled() {
+    static const bool on = [] {
+        const char* e = std::getenv("FHGPU_SELL");
+        return e ? std::atoi(e) != 0 : true;
+    }();
+    return on;
+}
+
 bool sliced_enabled() {
     static const bool on = [] {
         const char* e = std::getenv("FHGPU_SLICED");
@@ -1425,6 +1604,16 @@ void sliced_decide(Ctx& c, const Bcsr& A) {
     if (!sliced_enabled() || A.nbr < 4096) {
         A.sliced_state = -1;
         return;
+    }
+    // scalar square operators (the aggregation levels): SELL-64 when its
+    // padding stays within half the entries
+    if (A.bs_r == 1 && A.bs_c == 1 && A.nbr == A.nbc && A.h_rowptr.size() == (size_t)A.nbr + 1 && sell_enabled()) {
+        std::vector<int> sptr;
+        const int64_t total = sell_geometry(A, sptr);
+        if ((double)total <= 1.5 * (double)A.nnzb) {
+            sell_build(c, A, sptr, total);
+            return;
+        }
     }
     if (sliced_geometry(A, W, RW, cw)) {
         const int64_t nslices = (A.nbr + RW - 1) / RW;

@@ -76,7 +76,8 @@ int fh_mat_device_arrays(const fh_mat* A, const int** rowptr, const int** colidx
 int fh_mat_set_sliced(fh_ctx* ctx, const fh_mat* A, int on, int* active);
 /* Execution layout of A's full-matrix products: 0 block CSR (or not decided
  * yet), 1 sliced copy, 2 sliced copy in two parts (short and long rows, each
- * with its own width: uneven rows such as a prolongation's), 3 chunked copy.
+ * with its own width: uneven rows such as a prolongation's), 3 chunked copy,
+ * 4 SELL-64 copy (scalar square operators: 64 rows per slice, slot-major).
  * No reference counterpart (an execution detail). */
 int fh_mat_layout(const fh_mat* A, int* layout);
 

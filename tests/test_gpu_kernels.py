@@ -302,3 +302,45 @@ def test_sliced_two_part_layout_modes(ctx, nf):
         if prec == "f64":
             for k, (u, w) in enumerate(zip(res[True], want)):
                 assert np.linalg.norm(u - w) <= 1e-12 * np.linalg.norm(w), k
+
+
+def test_sell_layout_modes_for_scalar_levels(ctx):
+    """Scalar square operators with rows of 5-11 entries (the aggregation
+    levels' shape) get the SELL-64 copy (layout 4) at their first product:
+    the product, residual and block-Jacobi modes equal the block-CSR kernel's
+    (FP64 to summation-order roundoff, FP32 / BF16-stored to FP32 roundoff)
+    and the oracle's; a Chebyshev-smoothed FCG + AMG solve on such a level
+    hierarchy is checked by the K-cycle parity tests."""
+    import scipy.sparse as sp
+    rng = np.random.default_rng(90)
+    n = 20011  # a partial last slice
+    lens = rng.integers(5, 12, size=n)
+    rows, cols = [], []
+    for i, k in enumerate(lens):
+        c = np.unique((i + rng.integers(-300, 301, size=k)) % n)
+        c = np.unique(np.concatenate([[i], c]))
+        rows += [i] * len(c)
+        cols += list(c)
+    v = rng.standard_normal(len(rows))
+    A = sp.csr_matrix((v, (rows, cols)), shape=(n, n))
+    A = (A + sp.diags(4.0 * np.abs(A).sum(axis=1).A.ravel() + 1.0)).tocsr()
+    A.sort_indices()
+    x, y0, b = (rng.standard_normal(n) for _ in range(3))
+    inv = O.block_inverse(A, 1)
+    want = [1.5 * O.spmv(A, x) - 0.5 * y0, b - O.spmv(A, x),
+            O.bj_sweep(A, 1, inv, 0.7, b, O.bj_sweep(A, 1, inv, 0.7, b, x))]
+    for prec, vdt, tol in (("f64", torch.float64, 1e-13), ("f32", torch.float32, 2e-6), ("bf16", torch.float32, 2e-6)):
+        res = {}
+        for auto in (False, True):
+            M64 = G.Matrix.from_csr(ctx, A, 1)
+            M = M64 if prec == "f64" else M64.convert(prec)
+            if not auto:
+                assert not M.set_sliced(False)
+            bj = G.BlockJacobi(ctx, M64, 0.7, dprec="f64" if prec == "f64" else "f32")
+            res[auto] = _modes(ctx, M, x, y0, b, vdt, bj)
+            assert M.layout() == (4 if auto else 0), (prec, M.layout())
+        for k, (u, w) in enumerate(zip(res[False], res[True])):
+            assert np.linalg.norm(u - w) <= tol * np.linalg.norm(u), (prec, k)
+        if prec == "f64":
+            for k, (u, w) in enumerate(zip(res[True], want)):
+                assert np.linalg.norm(u - w) <= 1e-12 * np.linalg.norm(w), k
