@@ -182,6 +182,9 @@ def main():
     # Block (n_f = 6) tetrahedral system: SpMV / block smoothers / block aggregation
     tet2 = ["-dim", "3", "-mesh", "stetra", "-k", "2", "-n", "4", "-source", "sine"]
     run("tet4_k2", tet2 + ["-s", "cg", "-prec", "bj"], aggreg=tet2 + ["-nosolve"])
+    # geometric hierarchy on tetrahedra (CellInterp_Trace; the reference's orthogonalised monomial face basis)
+    run("tet4_k2_cg_mg_bj23", tet2 + ["-s", "cg", "-prec", "mg", "-cycle", "V,1,1", "-smoothers", "bj23,bj23"],
+        export_mg=True)
     # finer tetrahedral k=2 system: L2 error and iteration count only
     run("tet8_k2_cg_bj", ["-dim", "3", "-mesh", "stetra", "-k", "2", "-n", "8", "-source", "sine", "-s", "cg",
                           "-prec", "bj"], keep_A=False)

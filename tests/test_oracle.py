@@ -25,6 +25,7 @@ GMG_CASES = [
     ("c1_n32_cg_mg_w_bj23", "c1_n32_cg_mg_bj23", "cg", "mg", "W", ("bj23", "bj23"), 2, 200),
     ("c1_n32_fcg_mg_k_bgs", "c1_n32_cg_mg_bj23", "fcg", "mg", "K", ("bgs", "rbgs"), 2, 200),
     ("cart8_k2_cg_mg_bj23", "cart8_k2_cg_mg_bj23", "cg", "mg", "V", ("bj23", "bj23"), 6, 200),
+    ("tet4_k2_cg_mg_bj23", "tet4_k2_cg_mg_bj23", "cg", "mg", "V", ("bj23", "bj23"), 6, 200),
 ]
 
 
