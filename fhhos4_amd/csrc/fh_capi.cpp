@@ -317,7 +317,7 @@ int fh_bjac_inverses(fh_ctx* ctx, const fh_bjac* bj, double* host_out) {
     return guard([&] {
         NEED(ctx); NEED_DEV(ctx); NEED(bj); NEED(host_out);
         FH_CHECK(bj->S.dprec == fh::Prec::F64, -2, "download available for FP64 inverses");
-        FH_HIP(hipMemcpyAsync(host_out, bj->S.invD.p, bj->S.invD.bytes, hipMemcpyDeviceToHost, ctx->c.stream));
+        FH_HIP(hipMemcpyAsync(host_out, bj->S.invD->p, bj->S.invD->bytes, hipMemcpyDeviceToHost, ctx->c.stream));
         ctx->c.sync();
     });
 }
@@ -333,10 +333,10 @@ int fh_bjac_sweep(fh_ctx* ctx, const fh_bjac* bj, const fh_mat* A, const void* b
         bool xz = x_is_zero != 0;
         for (int s = 0; s < nsweeps; s++) {
             if (xz)
-                fh::launch_bjac_zero(ctx->c, M.nbr, nf, vp, b, cur, bj->S.spec.omega, bj->S.invD.p, bj->S.dprec,
+                fh::launch_bjac_zero(ctx->c, M.nbr, nf, vp, b, cur, bj->S.spec.omega, bj->S.invD->p, bj->S.dprec,
                                      nullptr, 0);
             else {
-                fh::launch_bcsr(ctx->c, M, fh::OP_SMOOTH, vp, cur, b, nxt, 1.0, 0.0, bj->S.spec.omega, bj->S.invD.p,
+                fh::launch_bcsr(ctx->c, M, fh::OP_SMOOTH, vp, cur, b, nxt, 1.0, 0.0, bj->S.spec.omega, bj->S.invD->p,
                                 bj->S.dprec, nullptr, 0);
                 std::swap(cur, nxt);
             }
@@ -371,10 +371,10 @@ int fh_bjac_sweep_noalloc(fh_ctx* ctx, const fh_bjac* bj, const fh_mat* A, const
         bool xz = x_is_zero != 0;
         for (int s2 = 0; s2 < nsweeps; s2++) {
             if (xz)
-                fh::launch_bjac_zero(ctx->c, M.nbr, nf, vp, b, cur, bj->S.spec.omega, bj->S.invD.p, bj->S.dprec,
+                fh::launch_bjac_zero(ctx->c, M.nbr, nf, vp, b, cur, bj->S.spec.omega, bj->S.invD->p, bj->S.dprec,
                                      nullptr, 0);
             else {
-                fh::launch_bcsr(ctx->c, M, fh::OP_SMOOTH, vp, cur, b, nxt, 1.0, 0.0, bj->S.spec.omega, bj->S.invD.p,
+                fh::launch_bcsr(ctx->c, M, fh::OP_SMOOTH, vp, cur, b, nxt, 1.0, 0.0, bj->S.spec.omega, bj->S.invD->p,
                                 bj->S.dprec, nullptr, 0);
                 std::swap(cur, nxt);
             }
@@ -426,7 +426,7 @@ int fh_mcgs_sweep(fh_ctx* ctx, const fh_mat* A, const fh_bjac* bj, const fh_colo
             for (int k = 0; k < S.ncolours; k++) {
                 int cc = backward ? S.ncolours - 1 - k : k;
                 int cnt = S.colour_ptr[cc + 1] - S.colour_ptr[cc];
-                fh::launch_bcsr(ctx->c, *A->A, fh::OP_SMOOTH, P(vprec), x, b, x, 1.0, 0.0, 1.0, bj->S.invD.p,
+                fh::launch_bcsr(ctx->c, *A->A, fh::OP_SMOOTH, P(vprec), x, b, x, 1.0, 0.0, 1.0, bj->S.invD->p,
                                 bj->S.dprec, S.colour_rows.as<int>() + S.colour_ptr[cc], cnt);
             }
         };

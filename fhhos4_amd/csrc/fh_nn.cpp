@@ -52,8 +52,8 @@ std::unique_ptr<NnGraph> nn_graph_create(Ctx& c, const Bcsr& A, const double* fa
     LevelSmoother s32, s64;
     smoother_setup(c, A, s32, Prec::F32);
     smoother_setup(c, A, s64, Prec::F64);
-    g->invD32 = std::move(s32.invD);
-    g->invD64 = std::move(s64.invD);
+    g->invD32 = std::move(*s32.invD);
+    g->invD64 = std::move(*s64.invD);
     c.sync();
     return g;
 }

@@ -138,21 +138,21 @@ void smoother_setup(Ctx& c, const Bcsr& A64, LevelSmoother& S, Prec dprec) {
     std::vector<int> diag = diag_blocks(A64);
     DevBuf d_diag(sizeof(int) * diag.size());
     FH_HIP(hipMemcpyAsync(d_diag.p, diag.data(), sizeof(int) * diag.size(), hipMemcpyHostToDevice, c.stream));
-    S.invD.alloc((size_t)A64.nbr * nf * nf * prec_bytes(dprec));
+    S.invD = std::make_shared<DevBuf>((size_t)A64.nbr * nf * nf * prec_bytes(dprec));
     DevBuf flag(sizeof(int));
     FH_HIP(hipMemsetAsync(flag.p, 0, sizeof(int), c.stream));
-    launch_block_inverse(c, A64, d_diag.as<int>(), S.invD.p, dprec, flag.as<int>());
+    launch_block_inverse(c, A64, d_diag.as<int>(), S.invD->p, dprec, flag.as<int>());
     int h_flag = 0;
     FH_HIP(hipMemcpyAsync(&h_flag, flag.p, sizeof(int), hipMemcpyDeviceToHost, c.stream));
     c.sync();
     FH_CHECK(h_flag == 0, -6, "singular diagonal block in block row " + std::to_string(h_flag - 1));
     if (S.spec.kind == SmootherKind::Chebyshev) {
         if (dprec == Prec::F64)
-            S.lmax = estimate_lmax(c, A64, S.invD.p);
+            S.lmax = estimate_lmax(c, A64, S.invD->p);
         else {
             LevelSmoother S64;
             smoother_setup(c, A64, S64, Prec::F64);
-            S.lmax = estimate_lmax(c, A64, S64.invD.p);
+            S.lmax = estimate_lmax(c, A64, S64.invD->p);
         }
         S.d.resize(A64.rows(), dprec);
     }
@@ -179,7 +179,7 @@ static void gs_sweep(Ctx& c, const Bcsr& A, LevelSmoother& S, Prec vp, const voi
         int cc = backward ? S.ncolours - 1 - k : k;
         int cnt = S.colour_ptr[cc + 1] - S.colour_ptr[cc];
         const int* rows = S.colour_rows.as<int>() + S.colour_ptr[cc];
-        launch_bcsr(c, A, OP_SMOOTH, vp, x.data(), b, x.data(), 1.0, 0.0, S.spec.omega, S.invD.p, S.dprec, rows, cnt);
+        launch_bcsr(c, A, OP_SMOOTH, vp, x.data(), b, x.data(), 1.0, 0.0, S.spec.omega, S.invD->p, S.dprec, rows, cnt);
     }
 }
 
@@ -189,9 +189,9 @@ void smoother_apply(Ctx& c, const Bcsr& A, LevelSmoother& S, Prec vp, const void
     for (int it = 0; it < S.spec.iterations; it++) {
         if (S.spec.kind == SmootherKind::BlockJacobi) {
             if (x_is_zero)
-                launch_bjac_zero(c, A.nbr, nf, vp, b, x.data(), S.spec.omega, S.invD.p, S.dprec, nullptr, 0);
+                launch_bjac_zero(c, A.nbr, nf, vp, b, x.data(), S.spec.omega, S.invD->p, S.dprec, nullptr, 0);
             else {
-                launch_bcsr(c, A, OP_SMOOTH, vp, x.data(), b, work.data(), 1.0, 0.0, S.spec.omega, S.invD.p, S.dprec,
+                launch_bcsr(c, A, OP_SMOOTH, vp, x.data(), b, work.data(), 1.0, 0.0, S.spec.omega, S.invD->p, S.dprec,
                             nullptr, 0);
                 std::swap(x.buf, work.buf);
             }
@@ -203,7 +203,7 @@ void smoother_apply(Ctx& c, const Bcsr& A, LevelSmoother& S, Prec vp, const void
             for (int k = 0; k < S.spec.degree; k++) {
                 if (k == 0 && x_is_zero) {
                     // from x = 0: d = D^-1 b / theta and x = d in one pass
-                    launch_bjac_zero(c, A.nbr, nf, vp, b, x.data(), 1.0 / theta, S.invD.p, S.dprec, nullptr, 0,
+                    launch_bjac_zero(c, A.nbr, nf, vp, b, x.data(), 1.0 / theta, S.invD->p, S.dprec, nullptr, 0,
                                      S.d.data());
                 } else {
                     // d = a d + c D^-1 (b - A x) and x + d in one block-row pass
@@ -215,7 +215,7 @@ void smoother_apply(Ctx& c, const Bcsr& A, LevelSmoother& S, Prec vp, const void
                         cc = 2.0 * rn / delta;
                         rho = rn;
                     }
-                    launch_bcsr_cheb(c, A, vp, x.data(), b, work.data(), S.d.data(), a, cc, S.invD.p, S.dprec);
+                    launch_bcsr_cheb(c, A, vp, x.data(), b, work.data(), S.d.data(), a, cc, S.invD->p, S.dprec);
                     std::swap(x.buf, work.buf);
                 }
                 x_is_zero = false;
@@ -502,7 +502,15 @@ private:
                     V.post.spec = SmootherSpec::parse(a_.post_smoother, a_.level_nu(a_.nu2, (int)l), a_.relax);
                 }
                 smoother_setup(c_, *V.A64, V.pre, dprec);
-                smoother_setup(c_, *V.A64, V.post, dprec);
+                if (V.post.spec.kind == V.pre.spec.kind && V.pre.spec.kind != SmootherKind::BlockGS) {
+                    // the same block inverses and lambda_max estimate serve both
+                    V.post.dprec = V.pre.dprec;
+                    V.post.invD = V.pre.invD;
+                    V.post.lmax = V.pre.lmax;
+                    if (V.post.spec.kind == SmootherKind::Chebyshev) V.post.d.resize(V.A64->rows(), dprec);
+                } else {
+                    smoother_setup(c_, *V.A64, V.post, dprec);
+                }
                 if (!V.agg && !V.ptrunc) {
                     Prec tp = vp_ == Prec::F64 ? Prec::F64 : Prec::F32;
                     if (tp != Prec::F64) {

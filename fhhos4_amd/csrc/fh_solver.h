@@ -12,6 +12,7 @@
 // one scalar per Krylov iteration (the residual norm for the stopping test)
 // and, in the K-cycle, the two norms that decide its branch.
 #pragma once
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -53,7 +54,7 @@ constexpr int kChebPowerIts = 20;
 // for Gauss-Seidel, a colouring of the block graph.
 struct LevelSmoother {
     SmootherSpec spec;
-    DevBuf invD;
+    std::shared_ptr<DevBuf> invD;  // block inverses (shared by a level's pre- and post-smoother)
     Prec dprec = Prec::F64;
     int ncolours = 0;
     std::vector<int> colour_ptr;   // host: colour c = rows[colour_ptr[c] .. colour_ptr[c+1])
