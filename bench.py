@@ -186,6 +186,60 @@ def kernel_bench(args, G, np, torch):
     print(json.dumps(out), flush=True)
 
 
+def condense_bench(args, G, np, torch):
+    """Batched static condensation (fh_condense_batched) of synthetic element
+    matrices of the headline cell shape: hexahedra at k = 3, 20 cell unknowns
+    and 6 faces x 10 face unknowns (local matrices 80 x 80), on an n^3 grid
+    of cells (--cond-n) with the structured face numbering (boundary faces
+    eliminated).  Local matrices: symmetric, R + R^T over 2 plus 80 I with R
+    standard normal (seeded), so A_TT is positive definite.  Per cell the
+    Schur product A_TF^T (A_TT^-1 A_TF) is 60 x 20 x 60 (144 kflop, on MFMA),
+    the Cholesky and triangular solves about 52 kflop; 51 KB of local matrix
+    are read and 36 blocks of 10 x 10 added."""
+    n = args.cond_n
+    nc, nfc, nf = 20, 6, 10
+    nl = nc + nfc * nf
+    ncells = n ** 3
+    # faces: x-normal (n+1) n n, then y-normal, then z-normal; interior ones numbered
+    idx = np.arange(ncells)
+    cx, cy, cz = idx % n, (idx // n) % n, idx // (n * n)
+    nx = (n + 1) * n * n
+    xf = lambda i, y, z: (z * n + y) * (n + 1) + i
+    yf = lambda x, j, z: nx + (z * (n + 1) + j) * n + x
+    zf = lambda x, y, k: 2 * nx + (k * n + y) * n + x
+    raw = np.stack([xf(cx, cy, cz), xf(cx + 1, cy, cz), yf(cx, cy, cz), yf(cx, cy + 1, cz), zf(cx, cy, cz),
+                    zf(cx, cy, cz + 1)], axis=1)
+    bnd = np.stack([cx == 0, cx == n - 1, cy == 0, cy == n - 1, cz == 0, cz == n - 1], axis=1)
+    used = np.unique(raw[~bnd])
+    renum = -np.ones(3 * nx, dtype=np.int64)
+    renum[used] = np.arange(len(used))
+    faces = np.where(bnd, -1, renum[raw]).astype(np.int32)
+    nfaces = len(used)
+    gen = torch.Generator(device="cuda").manual_seed(args.seed)
+    local = torch.randn(ncells, nl, nl, dtype=torch.float64, device="cuda", generator=gen)
+    local = 0.5 * (local + local.transpose(1, 2).contiguous())
+    local += nl * torch.eye(nl, dtype=torch.float64, device="cuda")
+    bt = torch.randn(ncells * nc, dtype=torch.float64, device="cuda", generator=gen)
+    torch.cuda.synchronize()
+    ctx = G.Context(0)
+    G.condense_batched(ctx, faces[:64], local[:64], nc, nf, nfaces, B_T=bt[:64 * nc])  # warm-up (code objects)
+    ts = []
+    for _ in range(args.steps):
+        t = time.perf_counter()
+        M, b = G.condense_batched(ctx, faces, local, nc, nf, nfaces, B_T=bt)
+        ctx.sync()
+        ts.append(time.perf_counter() - t)
+        del M, b
+    flop = ncells * (2.0 * (nfc * nf) ** 2 * nc + nc ** 3 / 3.0 + 2.0 * nc * nc * (nfc * nf + 1))
+    byt = ncells * (nl * nl * 8 + nc * 8 + nfc * nfc * nf * nf * 8 * 2)
+    out = {"kernel": "condense_batched", "cells": ncells, "faces": nfaces, "nc": nc, "face_unknowns": nfc * nf,
+           "call_s": float(np.median(ts)), "flop": flop, "bytes": byt,
+           "note": "call time includes the host block pattern; the kernel's own duration is in the rocprof summary"}
+    log("condensation of %d cells (%d faces): %.3f s per call (pattern on the host, kernel, right-hand side)"
+        % (ncells, nfaces, out["call_s"]))
+    print(json.dumps(out), flush=True)
+
+
 def hetero_bench(args, G, np, torch):
     """Heterogeneous diffusion on Cartesian hexahedra (n^3 cells, HHO k): kappa
     piecewise constant per 8^3-cell subdomain, lognormal with sigma = 3
@@ -204,16 +258,21 @@ def hetero_bench(args, G, np, torch):
     log("hexahedral hierarchy", [h[0].n_unknowns for h in hier], "%.1fs" % (time.perf_counter() - t))
     A, b = hier[0][1], P.b_device
     x = torch.zeros_like(b)
-    v = Variant("gmg", False, args.smoother, args.relax, args.nu)
-    s64, su = make_solver(ctx, G, hier, A, v, max_iter=args.max_iter)
+    # --prec amg: the p-first aggregation AMG (K-cycle on two levels with
+    # --cycle K, the coarse smoother of --coarse-smoother)
+    amg = args.prec == "amg"
+    v = Variant(args.prec, False, args.smoother, args.relax, args.nu, args.cycle if amg else "V",
+                args.coarse_smoother if amg else None)
+    krylov = "fcg" if v.cycle == "K" else "cg"
+    s64, su = make_solver(ctx, G, hier, A, v, solver=krylov, max_iter=args.max_iter)
     ts, st = timed_solves(ctx, s64, b, x, args.steps, args.warmup)
     t64 = float(np.mean(ts))
     x64 = x.cpu().numpy()
     out = {"workload": "3-D heterogeneous diffusion, %d^3 hexahedra, HHO k=%d (%d unknowns), kappa lognormal per 8^3 "
                        "subdomain (sigma 3), 1/8 anisotropic diag(1,1,1e-2)" % (args.n, args.k, N),
-           "solver": "gmg %s" % v.label(), "fp64_cg": {"s": t64, "iterations": st["iterations"], "setup_s": su,
-                                                      "converged": st["converged"]}}
-    log("FP64 CG + %s: %d iterations, %.3f s" % (v.label(), st["iterations"], t64))
+           "solver": v.label(), "fp64_" + krylov: {"s": t64, "iterations": st["iterations"], "setup_s": su,
+                                                  "converged": st["converged"]}}
+    log("FP64 %s + %s: %d iterations, %.3f s" % (krylov.upper(), v.label(), st["iterations"], t64))
     del s64
     for store, cstore in (("f32", None), ("bf16", None), ("f32", "bf16")):
         key = store if cstore is None else "%s_coarse_%s" % (store, cstore)
@@ -600,10 +659,16 @@ def c5_bench(args, G, np, torch):
     dom = max(kern, key=lambda k: kern[k]["ms_per_solve"])
     mixed = {}
     if not args.no_mixed:
-        for store, cstore in tuple(tuple(v.split(":")) if ":" in v else (v, None) for v in args.c5_mixed.split(",")):
-            key = store if cstore is None else "%s_coarse_%s" % (store, cstore)
+        for spec in args.c5_mixed.split(","):
+            # "f32", "f32:bf16" (coarse levels BF16), a trailing "o": FCG's own
+            # product with the FP32 copy of the operator (residual replacement)
+            outer = "f32" if spec.endswith("o") else "f64"
+            spec = spec.rstrip("o")
+            store, cstore = tuple(spec.split(":")) if ":" in spec else (spec, None)
+            key = (store if cstore is None else "%s_coarse_%s" % (store, cstore)) + ("_outer_f32" if outer == "f32" else "")
             log("set-up FCG + %s multigrid" % key)
-            s, su = solver(solver="fcg", mg_prec="f32", mg_store=store, coarse_store=cstore)
+            s, su = solver(solver="fcg", mg_prec="f32", mg_store=store, coarse_store=cstore, outer_prec=outer,
+                           replace_every=args.replace_every)
             ts, st = timed_solves(ctx, s, b, x, args.leg_steps, 1)
             xm = x.cpu().numpy()
             mixed[key] = {"s": float(np.mean(ts)), "iterations": st["iterations"], "setup_s": su,
@@ -698,7 +763,8 @@ def main():
     ap.add_argument("--sweep-list", default="cycle=V;cycle=W;p=3:1;p=3:1,cycle=W;sm=cheb3;cycle=W,sm=cheb3;"
                                             "prec=aggregamg,cycle=K,solver=fcg;prec=aggregamg,cycle=V",
                     help="c5 --sweep: variants separated by ';'")
-    ap.add_argument("--kernel", choices=["spmv"], help="time one kernel family alone")
+    ap.add_argument("--kernel", choices=["spmv", "condense"], help="time one kernel family alone")
+    ap.add_argument("--cond-n", type=int, default=48, help="--kernel condense: cells per direction")
     ap.add_argument("--hex-n", type=int, default=96, help="c5: hexahedra per direction")
     ap.add_argument("--hex-k", type=int, default=3)
     ap.add_argument("--hex-perturb", type=float, default=0.15)
@@ -713,8 +779,11 @@ def main():
     ap.add_argument("--c5-coarse-smoother", default="cheb4", help="c5: smoother of the levels below the finest")
     ap.add_argument("--c5-theta", type=float, default=0.1, help="c5: strong-coupling threshold of the aggregation")
     ap.add_argument("--max-coarse", type=int, default=2500, help="c5: largest coarsest level (dense solve)")
+    ap.add_argument("--replace-every", type=int, default=4,
+                    help="c5 FP32 outer product: iterations between residual replacements")
     ap.add_argument("--kcycle-levels", type=int, default=2, help="K-cycle on levels 1..this one, V below")
-    ap.add_argument("--c5-mixed", default="f32", help="c5 mixed legs: fine[:coarse] storage, comma-separated")
+    ap.add_argument("--c5-mixed", default="f32,f32o",
+                    help="c5 mixed legs: fine[:coarse] storage, comma-separated; a trailing o: FP32 outer product")
     ap.add_argument("--reps", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -737,6 +806,8 @@ def main():
         return
     if args.kernel == "spmv":
         return kernel_bench(args, G, np, torch)
+    if args.kernel == "condense":
+        return condense_bench(args, G, np, torch)
     if args.config == "c5":
         return c5_bench(args, G, np, torch)
     log("mesh and pattern of the HHO system n=%d k=%d (perturb %.2f)" % (args.n, args.k, args.perturb))

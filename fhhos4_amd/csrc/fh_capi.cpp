@@ -499,6 +499,11 @@ int fh_solver_create(fh_ctx* ctx, const fh_solver_args* a, fh_solver** out) {
                  -2, "smoothing change operator must be '+', '-' or '*'");
         s.nu_change_op = a->nu_change_op;
         s.nu_change_coeff = a->nu_change_coeff;
+        FH_CHECK(a->outer_prec == 0 || a->outer_prec == 1, -2, "outer_prec: FH_F64 or FH_F32");
+        s.outer_prec = P(a->outer_prec);
+        s.replace_every = a->replace_every > 0 ? a->replace_every : 10;
+        FH_CHECK(s.outer_prec == fh::Prec::F64 || (s.solver == "fcg" && s.mg_store == fh::Prec::F32), -2,
+                 "an FP32 outer product needs FCG and an FP32-stored multigrid");
         FH_CHECK(s.fcg_truncation <= 32, -2, "FCG truncation is limited to 32 directions");
         s.fcg_restart = a->fcg_restart != 0;
         if (!s.coarse_smoother.empty())

@@ -23,6 +23,7 @@
 // without atomics: every cell stores its A_TF^T A_TT^-1 B_T, and every face
 // adds those of its cells in cell order.
 #include <algorithm>
+#include <cstdlib>
 #include <vector>
 
 #include "fh_condense.h"
@@ -79,6 +80,41 @@ __device__ void chol_solve_col(const double* L, int nc, double* Y, int ld, int c
     }
 }
 
+// Schur entries of a cell on MFMA: S = A_FF - A_TF^T X with X = A_TT^-1 A_TF,
+// an m x m x nc product cut into 16 x 16 output tiles, one wavefront per
+// tile, v_mfma_f64_16x16x4 over k = 4 cell unknowns at a time (zero-padded
+// beyond nc and m).  Operand lanes: A[i][k] = T[k][p] with i = lane % 16,
+// k = lane / 16; B[k][j] = X[k][q] with j = lane % 16, k = lane / 16; result
+// element v of a lane is D[lane / 16 + 4 v][lane % 16] (the layout measured
+// on gfx950 with one-hot operands).
+typedef double d4_t __attribute__((ext_vector_type(4)));
+__device__ void schur_mfma(const CondDev& a, const double* T, const double* X, const double* loc, const int* slot) {
+    const int nc = a.nc, m = a.m, nl = a.nl, nf = a.nf, nfc = a.nfc;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const int tiles = (m + 15) / 16;
+    for (int t = wave; t < tiles * tiles; t += nw) {  // wavefront-uniform trip count
+        const int ti = t / tiles, tj = t - ti * tiles;
+        const int p_op = 16 * ti + (lane & 15), q = 16 * tj + (lane & 15);
+        d4_t acc = {0.0, 0.0, 0.0, 0.0};
+        for (int kk = 0; kk < nc; kk += 4) {
+            const int r = kk + (lane >> 4);
+            const double av = (r < nc && p_op < m) ? T[r * m + p_op] : 0.0;
+            const double bv = (r < nc && q < m) ? X[r * (m + 1) + q] : 0.0;
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
+        }
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int p = 16 * ti + (lane >> 4) + 4 * v;
+            const int s = p < m && q < m ? slot[(p / nf) * nfc + q / nf] : -1;
+            if (s >= 0) {
+                const double val = loc[(int64_t)(nc + p) * nl + nc + q] - acc[v];
+                atomicAdd(a.vals + (int64_t)s * nf * nf + (p % nf) * nf + q % nf, val);
+            }
+        }
+    }
+}
+
+template <bool MFMA>
 __global__ __launch_bounds__(kCondWG) void k_condense(CondDev a) {
     extern __shared__ double lds[];
     const int nc = a.nc, m = a.m, nl = a.nl, nf = a.nf, nfc = a.nfc;
@@ -103,13 +139,17 @@ __global__ __launch_bounds__(kCondWG) void k_condense(CondDev a) {
     __syncthreads();
     // Schur entries (p, q) of the cell's faces j1 = p / nf, j2 = q / nf
     const int* slot = a.slot + (int64_t)cell * nfc * nfc;
-    for (int t = threadIdx.x; t < m * m; t += blockDim.x) {
-        const int p = t / m, q = t - p * m;
-        const int s = slot[(p / nf) * nfc + q / nf];
-        if (s < 0) continue;
-        double v = loc[(int64_t)(nc + p) * nl + nc + q];
-        for (int r = 0; r < nc; r++) v -= T[r * m + p] * X[r * (m + 1) + q];
-        atomicAdd(a.vals + (int64_t)s * nf * nf + (p % nf) * nf + q % nf, v);
+    if constexpr (MFMA) {
+        schur_mfma(a, T, X, loc, slot);
+    } else {
+        for (int t = threadIdx.x; t < m * m; t += blockDim.x) {
+            const int p = t / m, q = t - p * m;
+            const int s = slot[(p / nf) * nfc + q / nf];
+            if (s < 0) continue;
+            double v = loc[(int64_t)(nc + p) * nl + nc + q];
+            for (int r = 0; r < nc; r++) v -= T[r * m + p] * X[r * (m + 1) + q];
+            atomicAdd(a.vals + (int64_t)s * nf * nf + (p % nf) * nf + q % nf, v);
+        }
     }
     if (a.cstage) {
         for (int p = threadIdx.x; p < m; p += blockDim.x) {
@@ -159,6 +199,15 @@ __global__ __launch_bounds__(kCondWG) void k_cell_recover(CondDev a, const doubl
     if (threadIdx.x == 0) chol_solve_col(L, nc, y, 1, 0);
     __syncthreads();
     for (int r = threadIdx.x; r < nc; r += blockDim.x) xT[(int64_t)cell * nc + r] = y[r];
+}
+
+// FHGPU_MFMA=0 keeps the Schur product on the vector ALUs.
+bool mfma_enabled() {
+    static const bool on = [] {
+        const char* e = std::getenv("FHGPU_MFMA");
+        return e ? std::atoi(e) != 0 : true;
+    }();
+    return on;
 }
 
 size_t cond_lds_bytes(int nc, int m) { return sizeof(double) * ((size_t)nc * nc + (size_t)nc * m + (size_t)nc * (m + 1)); }
@@ -245,8 +294,15 @@ std::unique_ptr<Bcsr> condense_batched(Ctx& c, int ncells, int nfc, int nc, int 
     CondDev d{ncells, nfc, nc, nf, nl, m, nullptr, dslot.as<int>(), local, BT, A->vals.as<double>(),
               b_out ? cstage.as<double>() : nullptr, derr.as<int>()};
     const size_t lds = cond_lds_bytes(nc, m);
-    set_lds((const void*)k_condense, lds);
-    hipLaunchKernelGGL(k_condense, dim3(ncells), dim3(kCondWG), lds, c.stream, d);
+    // the Schur product on MFMA where the element blocks are dense enough
+    // for 16 x 16 x 4 tiles (at least 2 x 2 output tiles and 2 steps of k)
+    if (m >= 32 && nc >= 8 && mfma_enabled()) {
+        set_lds((const void*)k_condense<true>, lds);
+        hipLaunchKernelGGL(k_condense<true>, dim3(ncells), dim3(kCondWG), lds, c.stream, d);
+    } else {
+        set_lds((const void*)k_condense<false>, lds);
+        hipLaunchKernelGGL(k_condense<false>, dim3(ncells), dim3(kCondWG), lds, c.stream, d);
+    }
     FH_HIP(hipGetLastError());
     if (b_out) {
         // (cell, local face) entries of every face, in cell order

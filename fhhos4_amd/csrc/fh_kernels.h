@@ -33,6 +33,11 @@ void launch_bcsr_cheb(Ctx& c, const Bcsr& A, Prec vprec, const void* x, const vo
 // for even nf); the caller then forms the residual and restricts it.
 bool launch_resid_head(Ctx& c, const Bcsr& A, Prec vprec, const void* x, const void* b, void* rc);
 
+// y = A x for an FP32-stored A with FP64 vectors and FP64 accumulation, on
+// the one-part sliced copy (n_f >= 2) or the SELL-64 copy (n_f = 1).
+// Returns false, having launched nothing, when A has neither.
+bool launch_spmv_f32vals(Ctx& c, const Bcsr& A, const void* x, void* y);
+
 // Builds (on = true) or drops the sliced execution layout of A used by
 // full-matrix products; returns whether A now has one (false when a row of
 // A does not fit one wavefront).  Without a call the layout is chosen at the

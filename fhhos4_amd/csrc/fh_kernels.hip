@@ -377,7 +377,10 @@ __global__ __launch_bounds__(kWG) void k_sliced(int nbr, int nslices, int W, int
     const int rv = lane / LPR, rl = lane - rv * LPR, bs = rl / JP, jp = rl - bs * JP;
     const bool act = lane < ACT;
     const int c = act ? cols[(int64_t)s * RW * W + rv * W + bs] : -1;
-    AT a[NF][CW];
+    // loaded value type: FP64 values load as doubles, FP32 and BF16 ones as
+    // floats (widened to the accumulation type at use)
+    using LT = std::conditional_t<std::is_same<TA, double>::value, double, float>;
+    LT a[NF][CW];
     if (act) {
         // The slice's value loads: its base in a buffer resource (scalar
         // registers), the row offset scalar, the lane offset one VGPR.
@@ -441,11 +444,11 @@ __global__ __launch_bounds__(kWG) void k_sliced(int nbr, int nslices, int W, int
             TV x0 = 0, x1 = 0;
             if (c >= 0) ld2(x + (int64_t)c * NF + 2 * jp, x0, x1);
 #pragma unroll
-            for (int k = 0; k < NF; k++) acc[k] = a[k][0] * (AT)x0 + a[k][1] * (AT)x1;
+            for (int k = 0; k < NF; k++) acc[k] = (AT)a[k][0] * (AT)x0 + (AT)a[k][1] * (AT)x1;
         } else {
             const AT x0 = c >= 0 ? (AT)x[(int64_t)c * NF + jp] : (AT)0;
 #pragma unroll
-            for (int k = 0; k < NF; k++) acc[k] = a[k][0] * x0;
+            for (int k = 0; k < NF; k++) acc[k] = (AT)a[k][0] * x0;
         }
     }
     // Row k of the wavefront's LDS holds the 64 partial sums of entry k,
@@ -1753,6 +1756,63 @@ void launch_bcsr_cheb(Ctx& c, const Bcsr& A, Prec vprec, const void* x, const vo
         FH_HIP(hipEventRecord(e1, c.stream));
         c.prof_events.push_back({e0, e1});
     }
+}
+
+// y = A x with FP32-stored values and FP64 vectors (FP64 accumulation), on
+// the one-part sliced copy (n_f >= 2) or the SELL-64 copy (n_f = 1).
+template <int NF>
+static bool spmv_f32vals_nf(Ctx& c, const Bcsr& A, const void* x, void* y) {
+    const Sliced& S = *A.sliced;
+    if constexpr (NF == 1) {
+        if (S.kind != 2) return false;
+        const int nwg = (S.nslices + kWG / 64 - 1) / (kWG / 64);
+        hipLaunchKernelGGL((k_sell<float, double, double, OP_SPMV>), dim3(nwg), dim3(kWG), 0, c.stream, A.nbr,
+                           S.nslices, S.chunk_ptr.as<int>(), S.cols.as<int>(), S.vals.as<float>(), (const double*)x,
+                           (const double*)nullptr, (double*)y, 1.0, 0.0, 0.0, (const double*)nullptr,
+                           (double*)nullptr);
+    } else {
+        if (S.kind != 0 || S.next || S.rowmap.p) return false;
+        if (S.cw == 2 && (uintptr_t)x % 16 != 0) return false;
+        const int nwg = (S.nslices + kSlicedWaves - 1) / kSlicedWaves;
+        hipLaunchKernelGGL((k_sliced<NF, float, double, double, OP_SPMV, false>), dim3(nwg), dim3(kWG), 0, c.stream,
+                           S.nrows, S.nslices, S.W, S.RW, S.cols.as<int>(), S.vals.as<float>(), (const double*)x,
+                           (const double*)nullptr, (double*)y, 1.0, 0.0, 0.0, (const double*)nullptr,
+                           (double*)nullptr, (const int*)nullptr);
+    }
+    FH_HIP(hipGetLastError());
+    return true;
+}
+
+bool launch_spmv_f32vals(Ctx& c, const Bcsr& A, const void* x, void* y) {
+    if (A.prec != Prec::F32 || A.bs_r != A.bs_c) return false;
+    if (A.sliced_state == 0 && !c.no_alloc) sliced_decide(c, A);
+    if (A.sliced_state != 1) return false;
+    const bool prof = c.prof_matrix == (const void*)&A && c.prof_mode == (int)OP_SPMV && !c.no_alloc;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (prof) {
+        FH_HIP(hipEventCreate(&e0));
+        FH_HIP(hipEventCreate(&e1));
+        FH_HIP(hipEventRecord(e0, c.stream));
+    }
+    bool done;
+    switch (A.bs_r) {
+        case 1: done = spmv_f32vals_nf<1>(c, A, x, y); break;
+        case 2: done = spmv_f32vals_nf<2>(c, A, x, y); break;
+        case 3: done = spmv_f32vals_nf<3>(c, A, x, y); break;
+        case 4: done = spmv_f32vals_nf<4>(c, A, x, y); break;
+        case 6: done = spmv_f32vals_nf<6>(c, A, x, y); break;
+        case 10: done = spmv_f32vals_nf<10>(c, A, x, y); break;
+        default: done = false; break;
+    }
+    if (prof) {
+        FH_HIP(hipEventRecord(e1, c.stream));
+        if (done) c.prof_events.push_back({e0, e1});
+        else {
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+        }
+    }
+    return done;
 }
 
 template <int NF, class TA, class TV>

@@ -973,12 +973,17 @@ void Solver::solve_fcg(const double* b, double* x, bool xz) {
     }
     auto slot_dad = [](int k) { return S_FCG + 2 * k; };
     auto slot_zad = [](int k) { return S_FCG + 2 * k + 1; };
+    // FCG's own product with the FP32 copy of the finest multigrid level
+    const Bcsr* A32 = nullptr;
+    if (args_.outer_prec == Prec::F32 && mg_ && !mg_->L.empty() && mg_->L[0].Aw && mg_->L[0].Aw->prec == Prec::F32)
+        A32 = mg_->L[0].Aw;
     double res = 1.0;
     const double bn = initial_residual(b, x, xz, res);
     int it = 0;
     hist_.push_back(res);
     stats_.converged = 0;
     std::vector<int> kept;  // ring indices, oldest first
+    int since_replace = 0;  // FP32-stored products since the last residual replacement
     auto stop = FH_STOP_TEST();
     while (!stop()) {
         int cur = 0;
@@ -1008,8 +1013,13 @@ void Solver::solve_fcg(const double* b, double* x, bool xz) {
             if (haz)
                 launch_axpby(c_, n, P, fAd_[k].data(), Ad.data(), -1.0, slot_zad(k), slot_dad(k), 1.0, -1, -1);
         }
-        if (!haz)
-            launch_bcsr(c_, *A_, OP_SPMV, P, d.data(), nullptr, Ad.data(), 1.0, 0.0, 0.0, nullptr, P, nullptr, 0);
+        bool approx = false;  // A d formed with FP32-stored values
+        if (!haz) {
+            approx = A32 && launch_spmv_f32vals(c_, *A32, d.data(), Ad.data());
+            if (!approx)
+                launch_bcsr(c_, *A_, OP_SPMV, P, d.data(), nullptr, Ad.data(), 1.0, 0.0, 0.0, nullptr, P, nullptr, 0);
+        }
+        since_replace = approx ? since_replace + 1 : since_replace;
         dot2(c_, n, P, d.data(), Ad.data(), S_DAD, r_.data(), d.data(), S_RD);
         launch_cg_update(c_, n, P, x, d.data(), r_.data(), Ad.data(), S_RD, S_DAD, 1.0);
         if ((int)kept.size() == m) {
@@ -1022,6 +1032,16 @@ void Solver::solve_fcg(const double* b, double* x, bool xz) {
         it++;
         double rr = read_scalar(S_RR);
         res = bn > 0 ? std::sqrt(rr) / bn : std::sqrt(rr);
+        if (since_replace > 0 && (res < args_.tol || since_replace >= args_.replace_every)) {
+            // residual replacement: the recursive residual carries the
+            // rounding of the FP32-stored products; the stopping test and the
+            // next directions use the FP64 b - A x
+            launch_bcsr(c_, *A_, OP_RESID, P, x, b, r_.data(), 1.0, 0.0, 0.0, nullptr, P, nullptr, 0);
+            dot1(c_, n, P, r_.data(), r_.data(), S_RR);
+            rr = read_scalar(S_RR);
+            res = bn > 0 ? std::sqrt(rr) / bn : std::sqrt(rr);
+            since_replace = 0;
+        }
         hist_.push_back(res);
     }
     stats_.iterations = it;

@@ -115,6 +115,11 @@ struct SolverArgs {
     // finer level's changed by op ('+', '-', '*'; 0 = unchanged) with coeff
     char nu_change_op = 0;
     int nu_change_coeff = 0;
+    // FCG's own product A p with the FP32 copy of the finest multigrid level
+    // (FP64 vectors), the recursive residual replaced by the FP64 b - A x
+    // every replace_every iterations and before convergence is declared
+    Prec outer_prec = Prec::F64;
+    int replace_every = 10;
     // smoothing iterations of level l from the finest level's nu
     int level_nu(int nu, int l) const {
         for (int k = 0; k < l; k++) {

@@ -27,7 +27,8 @@ class SolverArgs(C.Structure):
                 ("theta", C.c_double), ("mg_prec", C.c_int), ("mg_store", C.c_int), ("galerkin", C.c_int),
                 ("coarse_smoother", C.c_char_p), ("coarse_relax", C.c_double), ("fcg_truncation", C.c_int),
                 ("fcg_restart", C.c_int), ("coarse_store", C.c_int), ("kcycle_levels", C.c_int),
-                ("nu_change_op", C.c_char), ("nu_change_coeff", C.c_int)]
+                ("nu_change_op", C.c_char), ("nu_change_coeff", C.c_int), ("outer_prec", C.c_int),
+                ("replace_every", C.c_int)]
 
 
 class SolveStats(C.Structure):
@@ -557,7 +558,7 @@ class Solver:
     def __init__(self, ctx, solver="cg", prec="mg", cycle="V", nu=(1, 1), smoothers=("bj", "bj"), relax=1.0,
                  tol=1e-8, max_iter=200, max_coarse=1000, theta=0.25, mg_prec="f64", mg_store=None, wloops=0,
                  galerkin=False, coarse_smoother=None, coarse_relax=0.0, fcg_truncation=1, fcg_restart=False,
-                 coarse_store=None, kcycle_levels=0, nu_change=None):
+                 coarse_store=None, kcycle_levels=0, nu_change=None, outer_prec="f64", replace_every=10):
         a = SolverArgs()
         self._strs = [s.encode() for s in (solver, prec or "", smoothers[0], smoothers[1], coarse_smoother or "")]
         a.solver, a.prec, a.pre_smoother, a.post_smoother, a.coarse_smoother = self._strs
@@ -573,6 +574,7 @@ class Solver:
         a.galerkin = int(galerkin)
         a.coarse_store = 0 if coarse_store is None else _PREC[coarse_store]
         a.kcycle_levels = kcycle_levels
+        a.outer_prec, a.replace_every = _PREC[outer_prec], replace_every
         if nu_change:  # e.g. "+1": Multigrid::CoarseLevelChangeSmoothingOperator / Coeff
             a.nu_change_op, a.nu_change_coeff = nu_change[0].encode(), int(nu_change[1:])
         h = C.c_void_p()

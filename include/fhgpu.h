@@ -213,6 +213,14 @@ typedef struct fh_solver_args {
      * a count that falls below 0 is 0. */
     char nu_change_op;
     int nu_change_coeff;
+    /* Not in the reference: storage of the operator in FCG's own product A p
+     * (FH_F64, 0: the operator itself; FH_F32: the FP32 copy of the
+     * multigrid's finest level, FP64 vectors and accumulation, with the
+     * recursive residual replaced by the FP64 b - A x every replace_every
+     * iterations (<= 0: 10) and before convergence is declared, so the
+     * stopping test always holds for the FP64 residual). */
+    int outer_prec;
+    int replace_every;
 } fh_solver_args;
 
 typedef struct fh_solve_stats {

@@ -223,6 +223,9 @@ def main():
         elements=True)
     run("elem_quad4_k2", ["-dim", "2", "-mesh", "cart", "-k", "2", "-n", "4", "-source", "pisine", "-nosolve"],
         elements=True)
+    # the headline cells: hexahedra k = 3 (60 face unknowns per cell: the MFMA Schur product)
+    run("elem_cart3_k3", ["-dim", "3", "-mesh", "cart", "-k", "3", "-n", "3", "-source", "pisine", "-nosolve"],
+        kappa=cell_kappa(3, 3, 3.0, 9), elements=True)
     # Exported linear systems (Matrix Market text) of tiny 2-D and 3-D problems
     run("mm_c1_n4", c1s + ["-n", "4", "-s", "cg", "-prec", "mg", "-smoothers", "bj,bj"], export_dat=True)
     run("mm_tet2_k1", ["-dim", "3", "-mesh", "stetra", "-k", "1", "-n", "2", "-source", "sine", "-s", "cg",

@@ -99,6 +99,9 @@ def test_invalid_arguments_are_reported(hctx):
         G.Matrix.from_csr(hctx, A, 2)
     with pytest.raises(G.FhError, match="smoother|cycle"):
         s = G.Solver(hctx, solver="cg", prec="mg", cycle="X")
+    with pytest.raises(G.FhError, match="outer product"):
+        G.Solver(hctx, solver="fcg", prec="aggregamg", outer_prec="f32")
+    G.Solver(hctx, solver="fcg", prec="aggregamg", mg_prec="f32", mg_store="f32", outer_prec="f32")
     with pytest.raises(G.FhError, match="smoothing change"):
         G.Solver(hctx, solver="cg", prec="mg", nu_change="/2")
     with pytest.raises(G.FhError, match="relaxation"):

@@ -16,7 +16,7 @@ from conftest import csr, load
 F = pytest.importorskip("fhhos4_amd")
 G = F.fhgpu
 
-CASES = ["elem_cart4_k1", "elem_stetra2_k2", "elem_quad4_k2"]
+CASES = ["elem_cart4_k1", "elem_stetra2_k2", "elem_quad4_k2", "elem_cart3_k3"]
 
 
 def elements(d):

@@ -206,3 +206,38 @@ def test_initial_guess_matches_oracle(ctx, guess):
     hg = s.history()
     assert abs(hg[0] - ho[0]) <= 1e-12 * ho[0] and ho[0] != 1.0
     assert np.linalg.norm(x.cpu().numpy() - xo) <= 1e-10 * np.linalg.norm(xo)
+
+
+def test_fp32_outer_product_with_residual_replacement(ctx):
+    """FCG whose own product A p uses the FP32 copy of the finest multigrid
+    level (FP64 vectors and accumulation), with the recursive residual
+    replaced by the FP64 b - A x every few iterations and before convergence
+    is declared: the FP64 residual of the returned x meets the tolerance, the
+    solution agrees with the FP64 FCG's, and the iteration count stays
+    within two of the FCG with the FP64 product.  An FP32 outer product
+    needs an FP32-stored multigrid."""
+    P = G.HhoProblem(dim=3, mesh="hex", n=16, k=3, perturb=0.15, seed=2026, kappa="lognormal", sigma=3.0,
+                     source="pisine")
+    As = P.scipy()
+    A = G.Matrix.from_csr(ctx, As, P.nf, keep_host=True)
+    bh = P.rhs()
+    kw = dict(solver="fcg", prec="aggregamg", cycle="K", kcycle_levels=2, nu=(1, 1), smoothers=("cheb1", "cheb1"),
+              coarse_smoother="cheb4", theta=0.1, max_coarse=2500, tol=1e-8, max_iter=400)
+    xs, its = {}, {}
+    for key, extra in (("f64", {}), ("f32", dict(mg_prec="f32", mg_store="f32")),
+                       ("f32o", dict(mg_prec="f32", mg_store="f32", outer_prec="f32", replace_every=5))):
+        s = G.Solver(ctx, **kw, **extra)
+        s.setup(A)
+        b = torch.from_numpy(bh).cuda()
+        x = torch.zeros_like(b)
+        torch.cuda.synchronize()
+        st = s.solve(b, x)
+        assert st["converged"] == 1, key
+        xs[key], its[key] = x.cpu().numpy(), st["iterations"]
+    print("iterations", its)
+    r = bh - As @ xs["f32o"]
+    assert np.linalg.norm(r) < 1e-8 * np.linalg.norm(bh)
+    assert np.linalg.norm(xs["f32o"] - xs["f64"]) <= 1e-6 * np.linalg.norm(xs["f64"])
+    assert its["f32o"] <= its["f32"] + 2
+    with pytest.raises(G.FhError, match="outer product"):
+        G.Solver(ctx, **kw, outer_prec="f32")
