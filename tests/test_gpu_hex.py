@@ -241,3 +241,28 @@ def test_fp32_outer_product_with_residual_replacement(ctx):
     assert its["f32o"] <= its["f32"] + 2
     with pytest.raises(G.FhError, match="outer product"):
         G.Solver(ctx, **kw, outer_prec="f32")
+
+
+def test_tetrahedral_amg_block_jacobi_matches_oracle(ctx):
+    """The tetrahedral family's specified solver at small size: 8^3 x 6
+    perturbed tetrahedra (+-0.2h under the quality floor), HHO k = 2 (face
+    blocks of 6), FCG + the p-first aggregation AMG K-cycle with damped block
+    Jacobi (omega = 0.5, two sweeps) on every level, against the oracle's
+    own hierarchy and cycle on the same matrix: the same level sizes and
+    iteration count, the solution to 1e-10."""
+    P = G.HhoProblem(dim=3, mesh="stetra", n=8, k=2, perturb=0.2, seed=2026, source="pisine")
+    As = P.scipy()
+    A = G.Matrix.from_csr(ctx, As, P.nf, keep_host=True)
+    kw = dict(nu=(2, 2), smoothers=("bj", "bj"), max_iter=400, theta=0.1, max_coarse=500)
+    s = G.Solver(ctx, solver="fcg", prec="aggregamg", cycle="K", kcycle_levels=2, relax=0.5, tol=1e-8, **kw)
+    s.setup(A)
+    b = torch.from_numpy(P.rhs()).cuda()
+    x = torch.zeros_like(b)
+    torch.cuda.synchronize()
+    st = s.solve(b, x)
+    assert st["converged"]
+    xo, ho, oo = O.solve(As, P.rhs(), "fcg", "aggregamg", nf=P.nf, cycle="K", kcycle_levels=2, omega=0.5, **kw)
+    assert [s.level_size(l) for l in range(oo.nlevels)] == list(oo.level_n[:oo.nlevels])
+    assert st["iterations"] == oo.iterations, (st["iterations"], oo.iterations)
+    xg = x.cpu().numpy()
+    assert np.linalg.norm(xg - xo) <= 1e-10 * np.linalg.norm(xo)
