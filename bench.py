@@ -633,8 +633,9 @@ def c5_bench(args, G, np, torch):
     t = time.perf_counter()
     l2_64 = P.l2_error_gpu(ctx, x)
     post_s = time.perf_counter() - t
-    log("FP64 MG-PCG: %d iterations, %.3f s (%.1f ms/it), res %.2e, L2 %.4e"
-        % (st64["iterations"], t64, 1e3 * t64 / max(1, st64["iterations"]), st64["final_res"], l2_64))
+    log("FP64 MG-PCG: %d iterations, %.3f s (%.1f ms/it), res %.2e, L2 %.4e, cycle %s"
+        % (st64["iterations"], t64, 1e3 * t64 / max(1, st64["iterations"]), st64["final_res"], l2_64,
+           {0: "eager", 1: "eager", 2: "graph replay"}[st64["graph_mode"]]))
     # in-situ durations of the fine-level kernels: SpMV (CG product and
     # residual kernel family) and the fused Chebyshev step
     prof = {}
@@ -702,6 +703,7 @@ def c5_bench(args, G, np, torch):
                                % (args.hex_n, args.hex_perturb, args.hex_k, N, args.hex_sigma, krylov.upper(), label),
                    "unknowns": N, "faces": nb, "blocks": nnzb, "levels": levels},
         "iterations": st64["iterations"], "setup_s": setup64, "assembly_s": hier_s,
+        "cycle_execution": {0: "eager", 1: "eager", 2: "graph replay"}[st64["graph_mode"]],
         # no closed-form solution exists for the lognormal kappa field: the
         # discrete L2 distance of the reconstruction to prod sin(pi x_i) (the
         # kappa = 1 solution) is compared between the legs

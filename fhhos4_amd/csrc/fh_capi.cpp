@@ -567,6 +567,7 @@ int fh_solver_solve_x0(fh_solver* s, const double* b, double* x, int x_is_zero, 
             st->op_complexity = t.op_complexity;
             st->nn_used = t.nn_used;
             st->nn_fallback = t.nn_fallback;
+            st->graph_mode = t.graph_mode;
         }
     });
 }

@@ -879,6 +879,7 @@ void Solver::Solve(const double* b, double* x, bool x_is_zero) {
     else if (args_.solver == "fcg") solve_fcg(b, x, x_is_zero);
     else solve_stationary(b, x, x_is_zero);
     c_.sync();
+    stats_.graph_mode = mg_ ? mg_->graph_mode_ : 0;
     if (nn_) {
         int cnt[2];
         FH_HIP(hipMemcpy(cnt, nn_counters_.p, sizeof(cnt), hipMemcpyDeviceToHost));

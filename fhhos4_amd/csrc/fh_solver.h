@@ -75,6 +75,7 @@ struct SolveStats {
     int levels = 0;
     int converged = 0;
     double op_complexity = 0;
+    int graph_mode = 0;  // multigrid cycle: 0 eager (undecided), 1 eager (faster), 2 graph replay
 };
 
 class Multigrid;

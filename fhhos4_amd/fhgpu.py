@@ -32,12 +32,12 @@ class SolverArgs(C.Structure):
 
 
 class SolveStats(C.Structure):
-    _fields_ = [("iterations", C.c_int), ("converged", C.c_int), ("levels", C.c_int), ("pad", C.c_int),
+    _fields_ = [("iterations", C.c_int), ("converged", C.c_int), ("levels", C.c_int), ("graph_mode", C.c_int),
                 ("final_res", C.c_double), ("setup_s", C.c_double), ("solve_s", C.c_double),
                 ("op_complexity", C.c_double), ("nn_used", C.c_int), ("nn_fallback", C.c_int)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 _lib = None

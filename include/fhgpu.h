@@ -227,7 +227,8 @@ typedef struct fh_solve_stats {
     int iterations;
     int converged;
     int levels;
-    int pad;
+    int graph_mode;            /* multigrid cycle execution: 0 eager (undecided), 1 eager
+                                  (faster than replay), 2 HIP graph replay */
     double final_res;
     double setup_s;
     double solve_s;
