@@ -448,7 +448,8 @@ def c5_cpu_baseline(args, A, b, nf, gpu_iterations, base, krylov):
     x, hist, out = O.solve_bcsr(rp, ci, v, nf, b, krylov, base["prec"], cycle=base["cycle"], nu=base["nu"],
                                 smoothers=base["smoothers"], coarse_smoother=base["coarse_smoother"],
                                 kcycle_levels=base["kcycle_levels"], theta=base["theta"],
-                                max_coarse=base["max_coarse"], max_iter=args.cpu_iters)
+                                max_coarse=base["max_coarse"], fcg_truncation=base["fcg_truncation"],
+                                max_iter=args.cpu_iters)
     setup_s, solve_s = O.last_times()
     del rp, ci, v
     per_it = solve_s / max(1, out.iterations)
@@ -504,7 +505,7 @@ def c5_learned_leg(args, G, ctx, P, A, b, x, x64, t64, base, mixed_ref, l2_64):
         % (held_gain["untrained"], held_gain["trained"]))
     del held, held_sys, untrained
     out = {}
-    for m in sorted({1, args.learned_fcg_m}):
+    for m in sorted({args.fcg_m, args.learned_fcg_m}):
         s = G.Solver(ctx, **dict(base, solver="fcg", mg_prec="f32", mg_store="f32", fcg_truncation=m))
         s.set_learned_smoother(nn, graph, 1.0)
         t2 = time.perf_counter()
@@ -559,7 +560,7 @@ def c5_bench(args, G, np, torch):
     base = dict(prec=args.c5_prec, nu=(args.nu, args.nu), smoothers=(args.c5_smoother, args.c5_smoother),
                 coarse_smoother=args.c5_coarse_smoother, galerkin=not amg, tol=1e-8, max_iter=args.max_iter,
                 cycle=args.c5_cycle, kcycle_levels=args.kcycle_levels, theta=args.c5_theta,
-                max_coarse=args.max_coarse)
+                max_coarse=args.max_coarse, fcg_truncation=args.fcg_m)
     if amg:
         label = ("p-first aggregation AMG (p-injection to the face means, double pairwise aggregation) %s(%d,%d)%s "
                  "%s, coarse levels %s" % (args.c5_cycle, args.nu, args.nu,
@@ -592,6 +593,12 @@ def c5_bench(args, G, np, torch):
                      coarse_smoother=kv.get("csm", args.c5_coarse_smoother), solver=kv.get("solver", "cg"))
             if "nu" in kv:
                 a["nu"] = (int(kv["nu"]), int(kv["nu"]))
+            if "cnu" in kv:  # coarse levels' (pre, post) counts, e.g. cnu=1:1
+                a["coarse_nu"] = tuple(int(t) for t in kv["cnu"].split(":"))
+            if "nu1" in kv or "nu2" in kv:
+                a["nu"] = (int(kv.get("nu1", a["nu"][0])), int(kv.get("nu2", a["nu"][1])))
+            if "outer" in kv:  # FCG's own product on the FP32 copy, residual replacement every re
+                a.update(outer_prec=kv["outer"], replace_every=int(kv.get("re", args.replace_every)))
             if "m" in kv:
                 a["fcg_truncation"] = int(kv["m"])
             if "kl" in kv:
@@ -700,7 +707,8 @@ def c5_bench(args, G, np, torch):
         "dtype": "f64", "data": "synthetic",
         "config": {"workload": "3-D diffusion, %d^3 trilinear hexahedra perturbed +-%.2fh, HHO k=%d (%d unknowns), "
                                "per-cell lognormal kappa sigma %.1f; %s + %s, tol 1e-8"
-                               % (args.hex_n, args.hex_perturb, args.hex_k, N, args.hex_sigma, krylov.upper(), label),
+                               % (args.hex_n, args.hex_perturb, args.hex_k, N, args.hex_sigma,
+                                  "FCG(%d)" % args.fcg_m if krylov == "fcg" else "CG", label),
                    "unknowns": N, "faces": nb, "blocks": nnzb, "levels": levels},
         "iterations": st64["iterations"], "setup_s": setup64, "assembly_s": hier_s,
         "cycle_execution": {0: "eager", 1: "eager", 2: "graph replay"}[st64["graph_mode"]],
@@ -780,6 +788,8 @@ def main():
     ap.add_argument("--c5-smoother", default="cheb1", help="c5: smoother of the finest level")
     ap.add_argument("--c5-coarse-smoother", default="cheb4", help="c5: smoother of the levels below the finest")
     ap.add_argument("--c5-theta", type=float, default=0.1, help="c5: strong-coupling threshold of the aggregation")
+    ap.add_argument("--fcg-m", type=int, default=2,
+                    help="c5: FCG truncation (directions kept) of the FP64 and mixed legs")
     ap.add_argument("--max-coarse", type=int, default=2500, help="c5: largest coarsest level (dense solve)")
     ap.add_argument("--replace-every", type=int, default=4,
                     help="c5 FP32 outer product: iterations between residual replacements")

@@ -502,6 +502,11 @@ int fh_solver_create(fh_ctx* ctx, const fh_solver_args* a, fh_solver** out) {
         FH_CHECK(a->outer_prec == 0 || a->outer_prec == 1, -2, "outer_prec: FH_F64 or FH_F32");
         s.outer_prec = P(a->outer_prec);
         s.replace_every = a->replace_every > 0 ? a->replace_every : 10;
+        if (a->coarse_nu_given) {
+            FH_CHECK(a->coarse_nu1 >= 0 && a->coarse_nu2 >= 0, -2, "coarse smoothing iterations must be >= 0");
+            s.coarse_nu1 = a->coarse_nu1;
+            s.coarse_nu2 = a->coarse_nu2;
+        }
         FH_CHECK(s.outer_prec == fh::Prec::F64 || (s.solver == "fcg" && s.mg_store == fh::Prec::F32), -2,
                  "an FP32 outer product needs FCG and an FP32-stored multigrid");
         FH_CHECK(s.fcg_truncation <= 32, -2, "FCG truncation is limited to 32 directions");

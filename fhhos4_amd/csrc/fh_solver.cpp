@@ -472,6 +472,9 @@ private:
 
     void finish() {
         const Prec dprec = vp_ == Prec::F64 ? Prec::F64 : Prec::F32;
+        // smoothing iterations of level l: the finest level's nu, changed
+        // level by level by nu_change_op, or the given coarse count
+        auto nu_of = [&](int nu, int coarse_nu, int l) { return l > 0 && coarse_nu >= 0 ? coarse_nu : a_.level_nu(nu, l); };
         double fine_nnz = (double)L[0].A64->nnzb;
         op_complexity = 0;
         for (size_t l = 0; l < L.size(); l++) {
@@ -495,11 +498,11 @@ private:
             if (!coarsest) {
                 if (l > 0 && !a_.coarse_smoother.empty()) {
                     const double w = a_.coarse_relax > 0 ? a_.coarse_relax : a_.relax;
-                    V.pre.spec = SmootherSpec::parse(a_.coarse_smoother, a_.level_nu(a_.nu1, (int)l), w);
-                    V.post.spec = SmootherSpec::parse(a_.coarse_smoother, a_.level_nu(a_.nu2, (int)l), w);
+                    V.pre.spec = SmootherSpec::parse(a_.coarse_smoother, nu_of(a_.nu1, a_.coarse_nu1, (int)l), w);
+                    V.post.spec = SmootherSpec::parse(a_.coarse_smoother, nu_of(a_.nu2, a_.coarse_nu2, (int)l), w);
                 } else {
-                    V.pre.spec = SmootherSpec::parse(a_.pre_smoother, a_.level_nu(a_.nu1, (int)l), a_.relax);
-                    V.post.spec = SmootherSpec::parse(a_.post_smoother, a_.level_nu(a_.nu2, (int)l), a_.relax);
+                    V.pre.spec = SmootherSpec::parse(a_.pre_smoother, nu_of(a_.nu1, a_.coarse_nu1, (int)l), a_.relax);
+                    V.post.spec = SmootherSpec::parse(a_.post_smoother, nu_of(a_.nu2, a_.coarse_nu2, (int)l), a_.relax);
                 }
                 smoother_setup(c_, *V.A64, V.pre, dprec);
                 if (V.post.spec.kind == V.pre.spec.kind && V.pre.spec.kind != SmootherKind::BlockGS) {

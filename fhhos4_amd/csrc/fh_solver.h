@@ -121,6 +121,9 @@ struct SolverArgs {
     // every replace_every iterations and before convergence is declared
     Prec outer_prec = Prec::F64;
     int replace_every = 10;
+    // smoothing iterations of the levels below the finest, when >= 0 (not in
+    // the reference); -1: the finest level's, changed by nu_change_op
+    int coarse_nu1 = -1, coarse_nu2 = -1;
     // smoothing iterations of level l from the finest level's nu
     int level_nu(int nu, int l) const {
         for (int k = 0; k < l; k++) {

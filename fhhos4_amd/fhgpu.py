@@ -28,7 +28,8 @@ class SolverArgs(C.Structure):
                 ("coarse_smoother", C.c_char_p), ("coarse_relax", C.c_double), ("fcg_truncation", C.c_int),
                 ("fcg_restart", C.c_int), ("coarse_store", C.c_int), ("kcycle_levels", C.c_int),
                 ("nu_change_op", C.c_char), ("nu_change_coeff", C.c_int), ("outer_prec", C.c_int),
-                ("replace_every", C.c_int)]
+                ("replace_every", C.c_int), ("coarse_nu_given", C.c_int), ("coarse_nu1", C.c_int),
+                ("coarse_nu2", C.c_int)]
 
 
 class SolveStats(C.Structure):
@@ -558,7 +559,8 @@ class Solver:
     def __init__(self, ctx, solver="cg", prec="mg", cycle="V", nu=(1, 1), smoothers=("bj", "bj"), relax=1.0,
                  tol=1e-8, max_iter=200, max_coarse=1000, theta=0.25, mg_prec="f64", mg_store=None, wloops=0,
                  galerkin=False, coarse_smoother=None, coarse_relax=0.0, fcg_truncation=1, fcg_restart=False,
-                 coarse_store=None, kcycle_levels=0, nu_change=None, outer_prec="f64", replace_every=10):
+                 coarse_store=None, kcycle_levels=0, nu_change=None, outer_prec="f64", replace_every=10,
+                 coarse_nu=None):
         a = SolverArgs()
         self._strs = [s.encode() for s in (solver, prec or "", smoothers[0], smoothers[1], coarse_smoother or "")]
         a.solver, a.prec, a.pre_smoother, a.post_smoother, a.coarse_smoother = self._strs
@@ -577,6 +579,8 @@ class Solver:
         a.outer_prec, a.replace_every = _PREC[outer_prec], replace_every
         if nu_change:  # e.g. "+1": Multigrid::CoarseLevelChangeSmoothingOperator / Coeff
             a.nu_change_op, a.nu_change_coeff = nu_change[0].encode(), int(nu_change[1:])
+        if coarse_nu is not None:  # (pre, post) smoothing iterations below the finest level
+            a.coarse_nu_given, (a.coarse_nu1, a.coarse_nu2) = 1, coarse_nu
         h = C.c_void_p()
         _check(lib().fh_solver_create(ctx.h, C.byref(a), C.byref(h)))
         self.ctx, self.h, self.max_iter = ctx, h, max_iter

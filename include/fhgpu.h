@@ -221,6 +221,12 @@ typedef struct fh_solver_args {
      * stopping test always holds for the FP64 residual). */
     int outer_prec;
     int replace_every;
+    /* Not in the reference: when coarse_nu_given is nonzero, the levels below
+     * the finest take coarse_nu1 / coarse_nu2 (>= 0) pre/post smoothing
+     * iterations instead of nu1 / nu2 changed by nu_change_op; the finest
+     * level keeps nu1 / nu2 (e.g. a pre-smoothing-only finest level). */
+    int coarse_nu_given;
+    int coarse_nu1, coarse_nu2;
 } fh_solver_args;
 
 typedef struct fh_solve_stats {

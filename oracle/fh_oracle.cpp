@@ -545,7 +545,10 @@ struct MG {
     }
     // smoothing iterations of level l (Multigrid::Setup, Multigrid.h:224-242:
     // each coarse level's count is its finer level's changed by op, coeff)
-    int level_nu(int nu, int l) const {
+    // (not in the reference: coarse_nu, the given count of the levels below
+    // the finest, when coarse_nu_given)
+    int level_nu(int nu, int l, int coarse_nu) const {
+        if (l > 0 && cfg->coarse_nu_given) return coarse_nu;
         for (int k = 0; k < l; k++) {
             if (cfg->nu_change_op == '+') nu += cfg->nu_change_coeff;
             else if (cfg->nu_change_op == '-') nu -= cfg->nu_change_coeff;
@@ -563,7 +566,7 @@ struct MG {
             return;
         }
         const bool cs = l > 0 && cfg->coarse_kind > 0;
-        smooth(L, cs ? cfg->coarse_kind : cfg->pre_kind, cs ? cfg->coarse_omega : cfg->pre_omega, level_nu(cfg->nu1, l),
+        smooth(L, cs ? cfg->coarse_kind : cfg->pre_kind, cs ? cfg->coarse_omega : cfg->pre_omega, level_nu(cfg->nu1, l, cfg->coarse_nu1),
                b, x);
         std::vector<double> r(n);
         spmv(L.A, x, r.data());
@@ -583,7 +586,7 @@ struct MG {
         }
         prolong_add(l, ec, x);
         smooth(L, cs ? cfg->coarse_kind : cfg->post_kind, cs ? cfg->coarse_omega : cfg->post_omega,
-               level_nu(cfg->nu2, l), b, x);
+               level_nu(cfg->nu2, l, cfg->coarse_nu2), b, x);
         if (Ax) spmv(L.A, x, Ax);
     }
     // Multigrid::FCGForKCycle (Multigrid.h:611-656); r is overwritten

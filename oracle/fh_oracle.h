@@ -108,6 +108,8 @@ typedef struct or_solver_cfg {
     int x_given;         /* nonzero: x holds the initial guess (IterativeSolver::Solve(b, x, false)) */
     char nu_change_op;   /* Multigrid.h:224-242: '+', '-', '*' or 0; each level's smoothing */
     int nu_change_coeff; /* iterations are its finer level's changed by op with coeff (>= 0) */
+    int coarse_nu_given; /* nonzero: levels below the finest take coarse_nu1 / coarse_nu2 */
+    int coarse_nu1, coarse_nu2;
 } or_solver_cfg;
 
 typedef struct or_solver_out {

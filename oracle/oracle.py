@@ -29,7 +29,8 @@ class SolverCfg(C.Structure):
                 ("wloops", C.c_int), ("max_coarse", C.c_int), ("theta", C.c_double), ("tol", C.c_double),
                 ("max_iter", C.c_int), ("nlevels", C.c_int), ("coarse_kind", C.c_int), ("coarse_omega", C.c_double),
                 ("fcg_truncation", C.c_int), ("fcg_restart", C.c_int), ("kcycle_levels", C.c_int),
-                ("x_given", C.c_int), ("nu_change_op", C.c_char), ("nu_change_coeff", C.c_int)]
+                ("x_given", C.c_int), ("nu_change_op", C.c_char), ("nu_change_coeff", C.c_int),
+                ("coarse_nu_given", C.c_int), ("coarse_nu1", C.c_int), ("coarse_nu2", C.c_int)]
 
 
 class SolverOut(C.Structure):
@@ -177,7 +178,7 @@ PREC_KIND = {"": 0, "none": 0, "bj": 1, "bgs": 2, "mg": 3, "aggregamg": 4}
 
 def solve(A, b, solver="cg", prec="mg", nf=1, cycle="V", nu=(1, 1), smoothers=("bj", "bj"), omega=1.0,
           levels=None, tol=1e-8, max_iter=200, max_coarse=1000, theta=0.25, coarse_smoother=None, coarse_omega=0.0,
-          fcg_truncation=1, fcg_restart=False, kcycle_levels=0, x0=None, nu_change=None):
+          fcg_truncation=1, fcg_restart=False, kcycle_levels=0, x0=None, nu_change=None, coarse_nu=None):
     """Runs the oracle solver; `levels` = [(A_l, P_l), ...] for GMG (P of the
     coarsest level is None).  A level may be (A_l, ("p", nfc), nf_l): a
     p-transfer keeping nfc modes of every nf_l-block; (A_l, P_l, nf_l) gives
@@ -215,6 +216,8 @@ def solve(A, b, solver="cg", prec="mg", nf=1, cycle="V", nu=(1, 1), smoothers=("
     cfg.kcycle_levels = kcycle_levels
     if nu_change:  # e.g. "+1" (Multigrid.h:224-242)
         cfg.nu_change_op, cfg.nu_change_coeff = nu_change[0].encode(), int(nu_change[1:])
+    if coarse_nu is not None:  # (pre, post) smoothing iterations below the finest level
+        cfg.coarse_nu_given, (cfg.coarse_nu1, cfg.coarse_nu2) = 1, coarse_nu
     if coarse_smoother:
         cfg.coarse_kind = SMOOTHER_KIND[coarse_smoother]
         cfg.coarse_omega = 2.0 / 3.0 if coarse_smoother == "bj23" else (coarse_omega if coarse_omega > 0 else omega)
@@ -239,7 +242,7 @@ def solve(A, b, solver="cg", prec="mg", nf=1, cycle="V", nu=(1, 1), smoothers=("
 
 def solve_bcsr(rowptr, colidx, vals, nf, b, solver="fcg", prec="aggregamg", cycle="K", nu=(1, 1),
                smoothers=("cheb2", "cheb2"), tol=1e-8, max_iter=200, max_coarse=1000, theta=0.25,
-               coarse_smoother=None, kcycle_levels=0):
+               coarse_smoother=None, kcycle_levels=0, fcg_truncation=1, coarse_nu=None):
     """Oracle solve with the matrix in block storage (block row pointers,
     block columns, nnzb x nf x nf values): no preconditioner or the p-first
     aggregation AMG with Chebyshev smoothing.  Returns (x, hist, out)."""
@@ -256,7 +259,9 @@ def solve_bcsr(rowptr, colidx, vals, nf, b, solver="fcg", prec="aggregamg", cycl
     cfg.post_kind = SMOOTHER_KIND[smoothers[1]]
     cfg.pre_omega = cfg.post_omega = 1.0
     cfg.nu1, cfg.nu2 = nu
-    cfg.fcg_truncation, cfg.fcg_restart = 1, 0
+    cfg.fcg_truncation, cfg.fcg_restart = fcg_truncation, 0
+    if coarse_nu is not None:
+        cfg.coarse_nu_given, (cfg.coarse_nu1, cfg.coarse_nu2) = 1, coarse_nu
     if coarse_smoother:
         cfg.coarse_kind = SMOOTHER_KIND[coarse_smoother]
         cfg.coarse_omega = 1.0

@@ -85,23 +85,34 @@ def test_gpu_assembled_hierarchy_and_mixed_precision(ctx):
     assert abs(l2g - l2h) <= 1e-8 * l2h
 
 
-@pytest.mark.parametrize("k,n,kl,theta,csm", [(3, 16, 2, 0.25, "cheb1"), (3, 16, 0, 0.25, "cheb1"),
-                                               (1, 16, 2, 0.25, "cheb1"), (2, 16, 2, 0.25, "cheb1"),
-                                               (3, 16, 2, 0.1, "cheb3")],
-                         ids=["k3-K2", "k3-Kall", "k1-K2", "k2-K2", "k3-K2-bench"])
-def test_pfirst_amg_kcycle_matches_oracle(ctx, k, n, kl, theta, csm):
+PFIRST_CASES = [
+    # k, n, K-cycle levels, theta, coarse smoother, other settings
+    (3, 16, 2, 0.25, "cheb1", {}), (3, 16, 0, 0.25, "cheb1", {}), (1, 16, 2, 0.25, "cheb1", {}),
+    (2, 16, 2, 0.25, "cheb1", {}), (3, 16, 2, 0.1, "cheb3", {}),
+    (3, 16, 2, 0.1, "cheb4", dict(smoothers=("cheb1", "cheb1"), max_coarse=2500)),
+    (3, 16, 2, 0.1, "cheb4", dict(smoothers=("cheb1", "cheb1"), max_coarse=2500, nu=(1, 0), coarse_nu=(1, 1))),
+]
+
+
+@pytest.mark.parametrize("k,n,kl,theta,csm,extra", PFIRST_CASES,
+                         ids=["k3-K2", "k3-Kall", "k1-K2", "k2-K2", "k3-K2-cheb3", "k3-K2-bench",
+                              "k3-K2-fine-pre-only"])
+def test_pfirst_amg_kcycle_matches_oracle(ctx, k, n, kl, theta, csm, extra):
     """FCG + the p-first aggregation AMG (p-injection to the face means,
     double pairwise aggregation of the scalar level) with a K-cycle on the
     levels 1..kl against the oracle's own hierarchy and cycle on the same
     matrix, for face blocks of 3, 6 and 10 unknowns: the same level sizes and
-    iteration count, the solution to 1e-10.  The last case has the settings
-    of the 96^3 benchmark (strong-coupling threshold 0.1, degree-3
-    Chebyshev below the finest level)."""
+    iteration count, the solution to 1e-10.  The last two cases have the
+    settings of the 96^3 benchmark (strong-coupling threshold 0.1, degree-1
+    Chebyshev on the finest level and degree 4 below, coarsest level up to
+    2500 rows), the second with pre-smoothing only on the finest level."""
+    print("case k=%d n=%d kl=%d theta=%g %s %s" % (k, n, kl, theta, csm, extra), flush=True)
     P = G.HhoProblem(dim=3, mesh="hex", n=n, k=k, perturb=0.15, seed=2026, kappa="lognormal", sigma=3.0,
                      source="pisine")
     As = P.scipy()
     A = G.Matrix.from_csr(ctx, As, P.nf, keep_host=True)
     kw = dict(nu=(1, 1), smoothers=("cheb2", "cheb2"), coarse_smoother=csm, max_iter=400, theta=theta)
+    kw.update(extra)
     s = G.Solver(ctx, solver="fcg", prec="aggregamg", cycle="K", kcycle_levels=kl, tol=1e-8, **kw)
     s.setup(A)
     b = torch.from_numpy(P.rhs()).cuda()

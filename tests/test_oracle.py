@@ -428,6 +428,24 @@ def test_oracle_level_smoothing_change_matches_reference():
     assert np.linalg.norm(x - f["x"]) <= 1e-10 * np.linalg.norm(f["x"])
 
 
+def test_given_coarse_smoothing_counts_reproduce_the_reference_nu_change():
+    """The coarse-level smoothing counts given directly (not in the
+    reference): on this three-level hierarchy level 1 is the only smoothed
+    coarse level, so coarse counts (2, 2) are the reference's "+1" change
+    of (1, 1), and the solve reproduces its fixture; counts (1, 1) give the
+    unchanged solve."""
+    f, d = load("tet8_k0_fcg_amg_nu"), load("tet8_k0_fcg_amg")
+    A = csr(d, "A")
+    kw = dict(nf=1, cycle="K", smoothers=("bgs", "rbgs"))
+    x, hist, out = O.solve(A, d["b"], "fcg", "aggregamg", coarse_nu=(2, 2), **kw)
+    assert out.nlevels == 3
+    assert out.iterations == int(f["info_iterations"])
+    np.testing.assert_allclose(hist, f["hist"], rtol=1e-9, atol=1e-15)
+    x, hist, out = O.solve(A, d["b"], "fcg", "aggregamg", coarse_nu=(1, 1), **kw)
+    assert out.iterations == int(d["info_iterations"])
+    np.testing.assert_allclose(hist, d["hist"], rtol=1e-9, atol=1e-15)
+
+
 @pytest.mark.parametrize("omega", [0.7, 1.3])
 def test_bsor_sweep_solves_the_relaxed_triangular_system(omega):
     """One forward block SOR sweep solves (D + omega L) x' = omega b -
