@@ -45,6 +45,16 @@ def spmv_bytes(nb, nf, nnzb, sv=8, sx=8):
     return nnzb * nf * nf * sv + 4 * nnzb + 4 * (nb + 1) + 2 * nb * nf * sx
 
 
+def elem_spmv_bytes(ncell, nb, nf, sv=8, sx=8, nfc=6):
+    """Algorithmic bytes of one product on the element layout: every cell's
+    matrix by its m/2 + 1 cyclic diagonals ((m/2 + 1) m values, m = nfc nf),
+    its nfc block rows, x read once and y written once.  The two-colour
+    scheme also reads x of every face twice and y once more (its actual
+    traffic is counted by rocprofv3, not here)."""
+    m = nfc * nf
+    return ncell * (m // 2 + 1) * m * sv + 4 * ncell * nfc + 2 * nb * nf * sx
+
+
 # rocprofv3 FETCH_SIZE / WRITE_SIZE summaries of the fine-level SpMV, per layout
 TRAFFIC_SUMMARY = {"sliced": os.path.join(REPO, "profiles", "sliced_spmv_traffic_tet64_k2.md"),
                    "bcsr": os.path.join(REPO, "profiles", "bcsr_spmv_traffic_tet64_k2.md"),
@@ -611,6 +621,7 @@ def c5_bench(args, G, np, torch):
                 a.update(mg_prec="f32", mg_store=kv["store"])
             if "cstore" in kv:
                 a.update(mg_prec="f32", coarse_store=kv["cstore"])
+            A.set_elem(kv.get("elem", "1") != "0")  # elem=0: block-row kernels on the finest level
             try:
                 Hv = H if pmv == pm or prec != "mg" else gmg.build_hp_hierarchy(ctx, P, p_modes=pmv)
                 a["galerkin"] = prec == "mg"  # hp-MG h-levels are Galerkin products of the given P
@@ -651,9 +662,9 @@ def c5_bench(args, G, np, torch):
         s64.solve(b, x)
         nl, ms = G.profile_end(ctx)
         prof[name] = (nl, ms)
-    layout = {0: "bcsr", 1: "sliced", 2: "sliced2", 3: "chunked", 4: "sell"}[A.layout()]
+    layout = {0: "bcsr", 1: "sliced", 2: "sliced2", 3: "chunked", 4: "sell", 5: "elem"}[A.layout()]
     del s64
-    bytes_spmv = spmv_bytes(nb, nf, nnzb)
+    bytes_spmv = spmv_bytes(nb, nf, nnzb) if layout != "elem" else elem_spmv_bytes(P.info["ncells"], nb, nf)
     # fused Chebyshev step: the product's bytes + D^-1 (nb nf^2 doubles) +
     # b read + direction read and written + new iterate written
     bytes_cheb = bytes_spmv + nb * nf * nf * 8 + 4 * N * 8

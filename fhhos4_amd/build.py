@@ -18,7 +18,7 @@ LIB = os.path.join(PKG, "libfhgpu.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 OMP_DIR = "/opt/rocm/llvm/lib"
 
-SOURCES = ["fh_kernels.hip", "fh_hho_kernels.hip", "fh_nn_kernels.hip", "fh_condense.hip", "fh_matrix.cpp", "fh_io.cpp", "fh_solver.cpp", "fh_nn.cpp",
+SOURCES = ["fh_kernels.hip", "fh_elem.hip", "fh_hho_kernels.hip", "fh_nn_kernels.hip", "fh_condense.hip", "fh_matrix.cpp", "fh_io.cpp", "fh_solver.cpp", "fh_nn.cpp",
            "fh_uamg.cpp", "fh_hho.cpp", "fh_capi.cpp"]
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950", "-fopenmp",

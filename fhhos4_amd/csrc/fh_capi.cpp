@@ -172,12 +172,22 @@ int fh_mat_set_sliced(fh_ctx* ctx, const fh_mat* A, int on, int* active) {
     });
 }
 
+int fh_mat_set_elem(fh_ctx* ctx, const fh_mat* A, int on, int* active) {
+    return guard([&] {
+        NEED(ctx); NEED(A);
+        const bool r = fh::set_elem(*A->A, on != 0) && fh::elem_enabled();
+        if (active) *active = r ? 1 : 0;
+    });
+}
+
 int fh_mat_layout(const fh_mat* A, int* layout) {
     return guard([&] {
         NEED(A); NEED(layout);
         const fh::Bcsr& M = *A->A;
         int v = 0;
-        if (M.sliced_state == 1 && M.sliced)
+        if (M.elem_on && fh::elem_enabled())
+            v = 5;
+        else if (M.sliced_state == 1 && M.sliced)
             v = M.sliced->kind == 2 ? 4 : (M.sliced->kind == 1 ? 3 : (M.sliced->next ? 2 : 1));
         *layout = v;
     });

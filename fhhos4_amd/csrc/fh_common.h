@@ -129,6 +129,28 @@ struct Sliced {
     int act(int nf) const { return RW * W * (nf / cw); }
 };
 
+// Element execution layout of a statically condensed operator
+// A = sum_c P_c^T S_c P_c (S_c: the condensed matrix of cell c over its
+// nfc faces, m = nfc * nf unknowns, m <= 64 and even), kept from the
+// assembly.  Each S_c is symmetric and stored by cyclic diagonals: diagonal
+// t = 0 .. m/2 holds S[l][(l + t) mod m] at t m + l (l = 0 .. m-1), so every
+// unordered pair of unknowns is stored once (twice on diagonal m/2);
+// elem_stride(m) = (m/2 + 1) m values per cell.  Cells are in two colours
+// such that every unknown face belongs to exactly one cell of each colour (a
+// bipartite cell graph, such as a hexahedral grid's); colour 0 first
+// (ncol[0] cells), then colour 1.  rows holds the block row of every
+// (cell, face), -1 for an eliminated face.
+inline int elem_stride(int m) { return (m / 2 + 1) * m; }
+struct ElemLayout {
+    int nfc = 0, nf = 0, m = 0, stride = 0;
+    int ncol[2] = {0, 0};
+    Prec prec = Prec::F64;
+    std::shared_ptr<DevBuf> rows;
+    DevBuf vals;
+    int64_t ncell() const { return (int64_t)ncol[0] + ncol[1]; }
+    size_t value_bytes() const { return (size_t)ncell() * stride * prec_bytes(prec); }
+};
+
 // Block-CSR matrix with dense nf_r x nf_c blocks stored row-major, block
 // rows contiguous.  Value offsets are 64-bit (block index * nf_r * nf_c).
 struct Bcsr {
@@ -150,6 +172,10 @@ struct Bcsr {
     // -1 not used.
     mutable std::unique_ptr<Sliced> sliced;
     mutable int sliced_state = 0;
+    // element layout (from the assembly; null if none) and whether full
+    // products, residuals and Chebyshev steps use it
+    std::shared_ptr<ElemLayout> elem;
+    mutable bool elem_on = false;
     int64_t rows() const { return (int64_t)nbr * bs_r; }
     int64_t cols() const { return (int64_t)nbc * bs_c; }
     size_t value_bytes() const { return (size_t)nnzb * bs_r * bs_c * prec_bytes(prec); }

@@ -11,6 +11,7 @@
 
 #include "fh_common.h"
 #include "fh_hho_dev.h"
+#include "fh_kernels.h"
 
 namespace fh {
 namespace {
@@ -1989,12 +1990,89 @@ double HhoProblem::L2ErrorDevice(Ctx& c, const double* x_dev) const {
     return nrm2 > 0 ? std::sqrt(err / nrm2) : std::sqrt(err);
 }
 
+// Two-colouring of the cells for the element layout: every unknown face
+// must lie in exactly two cells, of different colours (the cell graph over
+// unknown faces is bipartite).  Returns false (no layout) otherwise, or when
+// the cells do not have six faces.
+static bool elem_colouring(const HhoProblem::Impl& I, std::vector<int>& pos, int ncol[2]) {
+    const int m = I.nfc;
+    if (m != 6) return false;
+    std::vector<int> fc((size_t)2 * I.nface, -1);
+    for (int c = 0; c < I.ncell; c++)
+        for (int f = 0; f < m; f++) {
+            const int g = I.cf[(size_t)c * m + f];
+            if (I.funk[g] < 0) continue;
+            int* s = &fc[(size_t)2 * g];
+            if (s[0] < 0) s[0] = c;
+            else if (s[1] < 0) s[1] = c;
+            else return false;  // a face in three cells
+        }
+    for (int g = 0; g < I.nface; g++)
+        if (I.funk[g] >= 0 && fc[(size_t)2 * g + 1] < 0) return false;  // an unknown face in one cell
+    std::vector<int> col(I.ncell, -1), queue;
+    queue.reserve(I.ncell);
+    for (int s = 0; s < I.ncell; s++) {
+        if (col[s] >= 0) continue;
+        col[s] = 0;
+        queue.assign(1, s);
+        for (size_t h = 0; h < queue.size(); h++) {
+            const int c = queue[h];
+            for (int f = 0; f < m; f++) {
+                const int g = I.cf[(size_t)c * m + f];
+                if (I.funk[g] < 0) continue;
+                const int o = fc[(size_t)2 * g] == c ? fc[(size_t)2 * g + 1] : fc[(size_t)2 * g];
+                if (col[o] < 0) {
+                    col[o] = 1 - col[c];
+                    queue.push_back(o);
+                } else if (col[o] == col[c]) {
+                    return false;  // not bipartite
+                }
+            }
+        }
+    }
+    pos.assign(I.ncell, -1);
+    ncol[0] = ncol[1] = 0;
+    for (int c = 0; c < I.ncell; c++) ncol[col[c]]++;
+    int next[2] = {0, ncol[0]};
+    for (int c = 0; c < I.ncell; c++) pos[c] = next[col[c]]++;
+    return true;
+}
+
 std::unique_ptr<Bcsr> HhoProblem::AssembleDevice(Ctx& c, double* b_dev, bool keep_host) const {
     FH_CHECK(!c.host_only, -9, "device assembly needs a device context");
     const Impl& I = *impl;
     auto t0 = std::chrono::steady_clock::now();
     std::vector<DevBuf> keep;
     HhoDev d = make_device_problem(c, *this, I, keep);
+    // element layout: the condensed cell matrices kept beside the block rows
+    std::shared_ptr<ElemLayout> E;
+    std::vector<int> pos;
+    int ncol[2];
+    if (nf * I.nfc <= 64 && nf * I.nfc % 2 == 0 && elem_colouring(I, pos, ncol)) {
+        E = std::make_shared<ElemLayout>();
+        E->nfc = I.nfc;
+        E->nf = nf;
+        E->m = I.nfc * nf;
+        E->stride = elem_stride(E->m);
+        E->ncol[0] = ncol[0];
+        E->ncol[1] = ncol[1];
+        E->prec = Prec::F64;
+        std::vector<int> rows((size_t)I.ncell * I.nfc);
+        for (int cc = 0; cc < I.ncell; cc++)
+            for (int f = 0; f < I.nfc; f++)
+                rows[(size_t)pos[cc] * I.nfc + f] = I.funk[I.cf[(size_t)cc * I.nfc + f]];
+        E->rows = std::make_shared<DevBuf>(sizeof(int) * rows.size());
+        FH_HIP(hipMemcpyAsync(E->rows->p, rows.data(), sizeof(int) * rows.size(), hipMemcpyHostToDevice, c.stream));
+        const size_t nv = (size_t)I.ncell * E->stride;
+        E->vals.alloc(sizeof(double) * nv);
+        FH_HIP(hipMemsetAsync(E->vals.p, 0, sizeof(double) * nv, c.stream));
+        keep.emplace_back(sizeof(int) * pos.size());
+        FH_HIP(hipMemcpyAsync(keep.back().p, pos.data(), sizeof(int) * pos.size(), hipMemcpyHostToDevice, c.stream));
+        d.elem_pos = keep.back().as<int>();
+        d.elem_vals = E->vals.as<double>();
+        d.elem_stride = E->stride;
+        c.sync();  // host staging released
+    }
     auto A = std::make_unique<Bcsr>();
     A->nbr = A->nbc = nb;
     A->bs_r = A->bs_c = nf;
@@ -2013,6 +2091,10 @@ std::unique_ptr<Bcsr> HhoProblem::AssembleDevice(Ctx& c, double* b_dev, bool kee
     d.vals = A->vals.as<double>();
     d.b = b_dev;
     launch_hho_local(c, d);  // synchronises
+    if (E) {
+        A->elem = E;
+        set_elem(*A, true);
+    }
     if (keep_host) {
         A->h_vals.resize(nval);
         FH_HIP(hipMemcpyAsync(A->h_vals.data(), A->vals.p, sizeof(double) * nval, hipMemcpyDeviceToHost, c.stream));

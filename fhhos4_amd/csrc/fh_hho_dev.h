@@ -28,6 +28,13 @@ struct HhoDev {
     const int* frow;               // nfc per cell: block row of the face, -1 for a Dirichlet face
     double* vals;                  // block values (nf x nf row-major), accumulated
     double* b;                     // right-hand side, accumulated
+    // element layout (optional, elem_vals null: none): the condensed cell
+    // matrix by cyclic diagonals (entry (r, s) with t = (s - r) mod nF <=
+    // nF / 2 at t nF + r, nF = nfc * nfb unknowns), elem_stride values per
+    // cell, cell c at position elem_pos[c]
+    double* elem_vals;
+    const int* elem_pos;
+    int elem_stride;
     // post-processing mode (post = 1): instead of the scatter, recover the
     // cell unknowns from the face solution xsol (nb x nfb), reconstruct the
     // k+1 polynomial and write the cell's integrals of (u - p)^2 and u^2

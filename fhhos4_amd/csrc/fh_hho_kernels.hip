@@ -24,6 +24,14 @@ namespace {
 
 constexpr int kT = 64;
 
+// position of entry (r, s) of a cell matrix of n unknowns stored by cyclic
+// diagonals (diagonal t = (s - r) mod n at t n + r), or -1 when the entry is
+// stored as its transpose (t > n / 2)
+__device__ __forceinline__ int elem_entry(int n, int r, int s) {
+    const int t = s >= r ? s - r : s - r + n;
+    return t <= n / 2 ? t * n + r : -1;
+}
+
 __device__ double source_at(const HhoDev& d, double x, double y, double z) {
     const double X[3] = {x, y, z};
     if (d.src == 2) {
@@ -374,10 +382,12 @@ __global__ __launch_bounds__(kT) void k_hho_local(HhoDev d, int* err) {
         const int r = e / nF, s = e - r * nF;
         const int f = r / nfb, a = r - f * nfb, g = s / nfb, bb = s - g * nfb;
         const int p = blk[f * m + g];
-        if (p < 0) continue;
+        const int ep = d.elem_vals != nullptr ? elem_entry(nF, r, s) : -1;
+        if (p < 0 && ep < 0) continue;
         double v = A[(nc + r) * nh + nc + s];
         for (int i = 0; i < nc; i++) v -= A[i * nh + nc + r] * X[i * (nF + 1) + s];
-        atomicAdd(d.vals + (size_t)p * bsz + a * nfb + bb, v);
+        if (ep >= 0) d.elem_vals[(size_t)d.elem_pos[c] * d.elem_stride + ep] = v;
+        if (p >= 0) atomicAdd(d.vals + (size_t)p * bsz + a * nfb + bb, v);
     }
     for (int r = t; r < nF; r += kT) {
         const int f = r / nfb, a = r - f * nfb;
@@ -951,10 +961,12 @@ __global__ __launch_bounds__(kTG) void k_hho_general(HhoDev d, int* err) {
         const int r = e / nF, s2 = e - r * nF;
         const int f = r / nfb, aa = r - f * nfb, g = s2 / nfb, bb = s2 - g * nfb;
         const int p = blk[f * m + g];
-        if (p < 0) continue;
+        const int ep = d.elem_vals != nullptr ? elem_entry(nF, r, s2) : -1;
+        if (p < 0 && ep < 0) continue;
         double v = A[(nc + r) * nh + nc + s2];
         for (int i = 0; i < nc; i++) v -= A[i * nh + nc + r] * X[i * (nF + 1) + s2];
-        atomicAdd(d.vals + (size_t)p * bsz + aa * nfb + bb, v);
+        if (ep >= 0) d.elem_vals[(size_t)d.elem_pos[c] * d.elem_stride + ep] = v;
+        if (p >= 0) atomicAdd(d.vals + (size_t)p * bsz + aa * nfb + bb, v);
     }
     for (int r = t; r < nF; r += kTG) {
         const int f = r / nfb, aa = r - f * nfb;

@@ -46,6 +46,23 @@ bool set_sliced(Ctx& c, const Bcsr& A, bool on);
 // Counter of execution-layout changes (built, dropped or decided) of any
 // matrix; a graph captured before a change must not be replayed after it.
 uint64_t layout_epoch();
+void layout_epoch_bump();
+
+// Element layout (ElemLayout): y = alpha A x + beta y (OP_SPMV), y = b - A x
+// (OP_RESID) or the fused Chebyshev step (OP_CHEB, d in aux), as two
+// launches, one per cell colour.  Returns false, having launched nothing,
+// when A has no active element layout or the mode and precisions are not
+// covered (FP64 values and vectors; FP32 values and vectors; FP32 values
+// with FP64 vectors for OP_SPMV).  x must not alias y.
+bool launch_elem(Ctx& c, const Bcsr& A, BcsrMode mode, Prec vprec, const void* x, const void* b, void* y,
+                 double alpha, double beta, const void* invD, Prec dprec, void* aux);
+// Makes the element layout of A active (on, when A has one) or not; returns
+// whether it is active.  The layout's data stay with A either way.
+bool set_elem(const Bcsr& A, bool on);
+// false when the environment sets FHGPU_ELEM=0 (element layouts unused)
+bool elem_enabled();
+// FP32 copy of an FP64 element layout (cell rows shared).
+std::shared_ptr<ElemLayout> elem_convert(Ctx& c, const ElemLayout& E, Prec p);
 
 // Scratch of the allocation-free entry points: 2^21 doubles (16 MiB) per
 // device, a file-scope device array.  noalloc_scratch_init(device) looks up
@@ -75,6 +92,11 @@ void launch_fill(Ctx& c, int64_t n, void* x, Prec p, double v);
 // (hb, ic, id).
 void launch_axpby(Ctx& c, int64_t n, Prec p, const void* x, void* y, double ha, int ia, int ib, double hb,
                   int ic, int id);
+// FCG orthogonalisation (FP64): out = z - sum_k (s[num[k]] / s[den[k]]) vs[k],
+// the nv terms added in order, one pass per 4 of them (out may alias z; a
+// copy when nv = 0).
+void launch_fcg_orth(Ctx& c, int64_t n, const double* z, int nv, const double* const* vs, const int* num,
+                     const int* den, double* out);
 // x += a d ; r -= a q with a = s[inum] / s[iden]  (CG / FCG update).
 void launch_cg_update(Ctx& c, int64_t n, Prec p, void* x, const void* d, void* r, const void* q, int inum, int iden,
                       double sign);

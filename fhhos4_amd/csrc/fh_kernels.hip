@@ -988,6 +988,31 @@ __global__ void k_axpby(int64_t n, const T* __restrict__ x, T* __restrict__ y, c
     }
 }
 
+// FCG's orthogonalisation of z against up to kOrthVecs kept directions in
+// one pass: out = z + sum_k a_k v_k with a_k = -s[num_k] / s[den_k], added in
+// k order with the arithmetic of successive k_axpby updates.  out may alias z.
+constexpr int kOrthVecs = 4;
+struct OrthArgs {
+    const double* v[kOrthVecs];
+    int num[kOrthVecs], den[kOrthVecs];
+};
+template <int NV>
+__global__ void k_fcg_orth(int64_t n, const double* z, OrthArgs p, const double* s, double* out) {
+    double a[NV];
+#pragma unroll
+    for (int k = 0; k < NV; k++) a[k] = scal(s, -1.0, p.num[k], p.den[k]);
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+        double y = z[t];
+#pragma unroll
+        for (int k = 0; k < NV; k++) {
+            double v = a[k] * p.v[k][t];
+            v += 1.0 * y;
+            y = v;
+        }
+        out[t] = y;
+    }
+}
+
 // Chebyshev smoothing step: d = a d + c (t - x); x = x + d, where
 // t = x + D^-1 (b - A x) was produced by the block-row kernel.
 template <class T>
@@ -1684,6 +1709,7 @@ bool set_sliced(Ctx& c, const Bcsr& A, bool on) {
 }
 
 uint64_t layout_epoch() { return g_layout_epoch.load(); }
+void layout_epoch_bump() { g_layout_epoch.fetch_add(1); }
 
 // ------------------------------------------------ allocation-free scratch
 // Device memory of the allocation-free entry points: one array per GPU the
@@ -1725,7 +1751,10 @@ static void launch_bcsr_impl(Ctx& c, const Bcsr& A, BcsrMode mode, Prec vprec, c
 void launch_bcsr(Ctx& c, const Bcsr& A, BcsrMode mode, Prec vprec, const void* x, const void* b, void* y,
                  double alpha, double beta, double omega, const void* invD, Prec dprec, const int* rows,
                  int nrows) {
-    if (!rows && A.sliced_state == 0 && !c.no_alloc) sliced_decide(c, A);
+    // full products and residuals on the element layout when A has an
+    // active one; otherwise on the sliced copy or the block rows
+    const bool elem = !rows && A.elem_on && (mode == OP_SPMV || mode == OP_RESID);
+    if (!elem && !rows && A.sliced_state == 0 && !c.no_alloc) sliced_decide(c, A);
     const bool prof = c.prof_matrix == (const void*)&A && c.prof_mode == (int)mode && !rows && !c.no_alloc;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (prof) {
@@ -1733,7 +1762,10 @@ void launch_bcsr(Ctx& c, const Bcsr& A, BcsrMode mode, Prec vprec, const void* x
         FH_HIP(hipEventCreate(&e1));
         FH_HIP(hipEventRecord(e0, c.stream));
     }
-    launch_bcsr_impl(c, A, mode, vprec, x, b, y, alpha, beta, omega, invD, dprec, rows, nrows, nullptr);
+    if (!(elem && launch_elem(c, A, mode, vprec, x, b, y, alpha, beta, nullptr, dprec, nullptr))) {
+        if (elem && A.sliced_state == 0 && !c.no_alloc) sliced_decide(c, A);
+        launch_bcsr_impl(c, A, mode, vprec, x, b, y, alpha, beta, omega, invD, dprec, rows, nrows, nullptr);
+    }
     if (prof) {
         FH_HIP(hipEventRecord(e1, c.stream));
         c.prof_events.push_back({e0, e1});
@@ -1743,7 +1775,8 @@ void launch_bcsr(Ctx& c, const Bcsr& A, BcsrMode mode, Prec vprec, const void* x
 void launch_bcsr_cheb(Ctx& c, const Bcsr& A, Prec vprec, const void* x, const void* b, void* xnew, void* d,
                       double a, double cc, const void* invD, Prec dprec) {
     FH_CHECK(xnew != x && d != nullptr && d != x && d != xnew, -3, "Chebyshev step: x, xnew and d must differ");
-    if (A.sliced_state == 0 && !c.no_alloc) sliced_decide(c, A);
+    const bool elem = A.elem_on;
+    if (!elem && A.sliced_state == 0 && !c.no_alloc) sliced_decide(c, A);
     const bool prof = c.prof_matrix == (const void*)&A && c.prof_mode == (int)OP_CHEB && !c.no_alloc;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (prof) {
@@ -1751,7 +1784,10 @@ void launch_bcsr_cheb(Ctx& c, const Bcsr& A, Prec vprec, const void* x, const vo
         FH_HIP(hipEventCreate(&e1));
         FH_HIP(hipEventRecord(e0, c.stream));
     }
-    launch_bcsr_impl(c, A, OP_CHEB, vprec, x, b, xnew, a, cc, 1.0, invD, dprec, nullptr, 0, d);
+    if (!(elem && launch_elem(c, A, OP_CHEB, vprec, x, b, xnew, a, cc, invD, dprec, d))) {
+        if (elem && A.sliced_state == 0 && !c.no_alloc) sliced_decide(c, A);
+        launch_bcsr_impl(c, A, OP_CHEB, vprec, x, b, xnew, a, cc, 1.0, invD, dprec, nullptr, 0, d);
+    }
     if (prof) {
         FH_HIP(hipEventRecord(e1, c.stream));
         c.prof_events.push_back({e0, e1});
@@ -1785,8 +1821,9 @@ static bool spmv_f32vals_nf(Ctx& c, const Bcsr& A, const void* x, void* y) {
 
 bool launch_spmv_f32vals(Ctx& c, const Bcsr& A, const void* x, void* y) {
     if (A.prec != Prec::F32 || A.bs_r != A.bs_c) return false;
-    if (A.sliced_state == 0 && !c.no_alloc) sliced_decide(c, A);
-    if (A.sliced_state != 1) return false;
+    const bool elem = A.elem_on && A.elem->prec == Prec::F32;
+    if (!elem && A.sliced_state == 0 && !c.no_alloc) sliced_decide(c, A);
+    if (!elem && A.sliced_state != 1) return false;
     const bool prof = c.prof_matrix == (const void*)&A && c.prof_mode == (int)OP_SPMV && !c.no_alloc;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (prof) {
@@ -1794,8 +1831,9 @@ bool launch_spmv_f32vals(Ctx& c, const Bcsr& A, const void* x, void* y) {
         FH_HIP(hipEventCreate(&e1));
         FH_HIP(hipEventRecord(e0, c.stream));
     }
-    bool done;
-    switch (A.bs_r) {
+    bool done = elem && launch_elem(c, A, OP_SPMV, Prec::F64, x, nullptr, y, 1.0, 0.0, nullptr, Prec::F64, nullptr);
+    if (!done && A.sliced_state == 0 && !c.no_alloc) sliced_decide(c, A);
+    if (!done && A.sliced_state == 1) switch (A.bs_r) {
         case 1: done = spmv_f32vals_nf<1>(c, A, x, y); break;
         case 2: done = spmv_f32vals_nf<2>(c, A, x, y); break;
         case 3: done = spmv_f32vals_nf<3>(c, A, x, y); break;
@@ -2003,6 +2041,33 @@ void launch_axpby(Ctx& c, int64_t n, Prec p, const void* x, void* y, double ha, 
         hipLaunchKernelGGL((k_axpby<float>), dim3(g), dim3(256), 0, c.stream, n, (const float*)x, (float*)y,
                            c.d_scalars, ha, ia, ib, hb, ic, id);
     FH_HIP(hipGetLastError());
+}
+
+void launch_fcg_orth(Ctx& c, int64_t n, const double* z, int nv, const double* const* vs, const int* num,
+                     const int* den, double* out) {
+    if (n <= 0) return;
+    const int g = grid_for(n);
+    if (nv == 0) {
+        if (out != z) FH_HIP(hipMemcpyAsync(out, z, sizeof(double) * n, hipMemcpyDeviceToDevice, c.stream));
+        return;
+    }
+    for (int k0 = 0; k0 < nv; k0 += kOrthVecs) {
+        const int cnt = std::min(kOrthVecs, nv - k0);
+        OrthArgs a{};
+        for (int k = 0; k < cnt; k++) {
+            a.v[k] = vs[k0 + k];
+            a.num[k] = num[k0 + k];
+            a.den[k] = den[k0 + k];
+        }
+        const double* src = k0 == 0 ? z : out;
+        switch (cnt) {
+            case 1: hipLaunchKernelGGL((k_fcg_orth<1>), dim3(g), dim3(256), 0, c.stream, n, src, a, c.d_scalars, out); break;
+            case 2: hipLaunchKernelGGL((k_fcg_orth<2>), dim3(g), dim3(256), 0, c.stream, n, src, a, c.d_scalars, out); break;
+            case 3: hipLaunchKernelGGL((k_fcg_orth<3>), dim3(g), dim3(256), 0, c.stream, n, src, a, c.d_scalars, out); break;
+            default: hipLaunchKernelGGL((k_fcg_orth<4>), dim3(g), dim3(256), 0, c.stream, n, src, a, c.d_scalars, out); break;
+        }
+        FH_HIP(hipGetLastError());
+    }
 }
 
 void launch_cg_update(Ctx& c, int64_t n, Prec p, void* x, const void* d, void* r, const void* q, int inum, int iden,

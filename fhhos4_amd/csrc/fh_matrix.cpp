@@ -92,6 +92,11 @@ std::unique_ptr<Bcsr> bcsr_convert(Ctx& c, const Bcsr& A, Prec p) {
     launch_convert(c, nv, A.vals.p, A.prec, B->vals.p, p);
     B->h_rowptr = A.h_rowptr;
     B->h_colidx = A.h_colidx;
+    if (A.elem && A.elem->prec == Prec::F64 && p == Prec::F32) {
+        // the element layout follows to FP32 (BF16 copies keep the block rows only)
+        B->elem = elem_convert(c, *A.elem, p);
+        set_elem(*B, A.elem_on);
+    }
     c.sync();
     return B;
 }

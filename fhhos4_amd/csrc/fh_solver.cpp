@@ -996,8 +996,6 @@ void Solver::solve_fcg(const double* b, double* x, bool xz) {
         DVec& Ad = fAd_[cur];
         const bool haz = fused_ || nn_ != nullptr;  // the preconditioner returns A z
         apply_prec(r_, z_, haz ? &Az_ : nullptr);
-        copy(c_, n, P, z_.data(), d.data());
-        if (haz) copy(c_, n, P, Az_.data(), Ad.data());
         // z . Adk for every kept direction (three per launch)
         for (size_t q = 0; q < kept.size(); q += 3) {
             const void* xs[3];
@@ -1011,11 +1009,23 @@ void Solver::solve_fcg(const double* b, double* x, bool xz) {
             }
             launch_dots(c_, n, P, cnt, xs, ys, sl);
         }
-        for (int k : kept) {
-            // d -= (z.Adk / dk.Adk) dk
-            launch_axpby(c_, n, P, fd_[k].data(), d.data(), -1.0, slot_zad(k), slot_dad(k), 1.0, -1, -1);
+        // d = z - sum_k (z.Adk / dk.Adk) dk over the kept directions, oldest
+        // first, in one pass (and A d the same way when the preconditioner
+        // returned A z)
+        {
+            std::vector<const double*> vs, avs;
+            std::vector<int> num, den;
+            for (int k : kept) {
+                vs.push_back((const double*)fd_[k].data());
+                avs.push_back((const double*)fAd_[k].data());
+                num.push_back(slot_zad(k));
+                den.push_back(slot_dad(k));
+            }
+            launch_fcg_orth(c_, n, (const double*)z_.data(), (int)kept.size(), vs.data(), num.data(), den.data(),
+                            (double*)d.data());
             if (haz)
-                launch_axpby(c_, n, P, fAd_[k].data(), Ad.data(), -1.0, slot_zad(k), slot_dad(k), 1.0, -1, -1);
+                launch_fcg_orth(c_, n, (const double*)Az_.data(), (int)kept.size(), avs.data(), num.data(), den.data(),
+                                (double*)Ad.data());
         }
         bool approx = false;  // A d formed with FP32-stored values
         if (!haz) {

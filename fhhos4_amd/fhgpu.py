@@ -62,6 +62,7 @@ _SIGS = {
     "fh_mat_device_arrays": [_vp, _pp, _pp, _pp],
     "fh_mat_blocks": [_vp, _i32p, _i32p, _f64p],
     "fh_mat_set_sliced": [_vp, _vp, C.c_int, C.POINTER(C.c_int)],
+    "fh_mat_set_elem": [_vp, _vp, C.c_int, C.POINTER(C.c_int)],
     "fh_mat_layout": [_vp, C.POINTER(C.c_int)],
     "fh_nn_create": [_vp, C.c_int, C.c_int, C.c_uint64, _pp],
     "fh_nn_destroy": [_vp],
@@ -342,8 +343,17 @@ class Matrix:
         _check(lib().fh_mat_set_sliced(self.ctx.h, self.h, int(bool(on)), C.byref(a)))
         return bool(a.value)
 
+    def set_elem(self, on):
+        """Uses (on=True) or stops using the element layout (condensed cell
+        matrices kept by the GPU assembly of hexahedral meshes) for full
+        products, residuals and Chebyshev steps; returns whether it is used."""
+        a = C.c_int()
+        _check(lib().fh_mat_set_elem(self.ctx.h, self.h, int(bool(on)), C.byref(a)))
+        return bool(a.value)
+
     def layout(self):
-        """0 block CSR (or undecided), 1 sliced, 2 sliced in two parts, 3 chunked."""
+        """0 block CSR (or undecided), 1 sliced, 2 sliced in two parts, 3 chunked,
+        4 SELL-64, 5 element layout."""
         a = C.c_int()
         _check(lib().fh_mat_layout(self.h, C.byref(a)))
         return a.value

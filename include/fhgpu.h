@@ -74,11 +74,21 @@ int fh_mat_device_arrays(const fh_mat* A, const int** rowptr, const int** colidx
  * (src/Utils/Types.h:11-22); the copy is a GPU execution detail of the same
  * operator. */
 int fh_mat_set_sliced(fh_ctx* ctx, const fh_mat* A, int on, int* active);
+/* Uses (on != 0) or stops using the element layout of an operator assembled
+ * on the GPU from hexahedral cells (fh_hho_assemble_gpu keeps the condensed
+ * cell matrices, upper triangles, beside the block rows and uses them for
+ * full products, residuals and Chebyshev steps: about 55% of the block
+ * rows' bytes per product).  *active: whether A now uses one (0 when A has
+ * none, or when the environment sets FHGPU_ELEM=0).  No reference
+ * counterpart: the reference assembles A from the same cell matrices
+ * (Diffusion_HHO.h:633-637); this keeps them as a GPU execution detail. */
+int fh_mat_set_elem(fh_ctx* ctx, const fh_mat* A, int on, int* active);
 /* Execution layout of A's full-matrix products: 0 block CSR (or not decided
  * yet), 1 sliced copy, 2 sliced copy in two parts (short and long rows, each
  * with its own width: uneven rows such as a prolongation's), 3 chunked copy,
- * 4 SELL-64 copy (scalar square operators: 64 rows per slice, slot-major).
- * No reference counterpart (an execution detail). */
+ * 4 SELL-64 copy (scalar square operators: 64 rows per slice, slot-major),
+ * 5 element layout (fh_mat_set_elem).  No reference counterpart (an
+ * execution detail). */
 int fh_mat_layout(const fh_mat* A, int* layout);
 
 /* -------------------------------------------- exported linear system */
