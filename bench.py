@@ -459,7 +459,7 @@ def c5_cpu_baseline(args, A, b, nf, gpu_iterations, base, krylov):
                                 smoothers=base["smoothers"], coarse_smoother=base["coarse_smoother"],
                                 kcycle_levels=base["kcycle_levels"], theta=base["theta"],
                                 max_coarse=base["max_coarse"], fcg_truncation=base["fcg_truncation"],
-                                max_iter=args.cpu_iters)
+                                coarse_nu=base["coarse_nu"], max_iter=args.cpu_iters)
     setup_s, solve_s = O.last_times()
     del rp, ci, v
     per_it = solve_s / max(1, out.iterations)
@@ -567,19 +567,21 @@ def c5_bench(args, G, np, torch):
     x = torch.zeros_like(b)
     torch.cuda.synchronize()
     amg = args.c5_prec == "aggregamg"
-    base = dict(prec=args.c5_prec, nu=(args.nu, args.nu), smoothers=(args.c5_smoother, args.c5_smoother),
+    cnu = tuple(int(t) for t in args.c5_coarse_nu.split(","))
+    base = dict(prec=args.c5_prec, nu=(args.nu, args.c5_nu_post), smoothers=(args.c5_smoother, args.c5_smoother),
                 coarse_smoother=args.c5_coarse_smoother, galerkin=not amg, tol=1e-8, max_iter=args.max_iter,
                 cycle=args.c5_cycle, kcycle_levels=args.kcycle_levels, theta=args.c5_theta,
-                max_coarse=args.max_coarse, fcg_truncation=args.fcg_m)
+                max_coarse=args.max_coarse, fcg_truncation=args.fcg_m, coarse_nu=cnu)
     if amg:
-        label = ("p-first aggregation AMG (p-injection to the face means, double pairwise aggregation) %s(%d,%d)%s "
-                 "%s, coarse levels %s" % (args.c5_cycle, args.nu, args.nu,
-                                           " K on levels 1-%d" % args.kcycle_levels if args.c5_cycle == "K" else "",
-                                           args.c5_smoother, args.c5_coarse_smoother))
+        label = ("p-first aggregation AMG (p-injection to the face means, double pairwise aggregation) %s-cycle%s, "
+                 "finest level (%d,%d) %s, coarse levels (%d,%d) %s"
+                 % (args.c5_cycle, " on levels 1-%d" % args.kcycle_levels if args.c5_cycle == "K" else "",
+                    args.nu, args.c5_nu_post, args.c5_smoother, cnu[0], cnu[1], args.c5_coarse_smoother))
         label += ", strong-coupling threshold %g" % args.c5_theta
     else:
-        label = "hp-MG %s(%d,%d) p%s+Galerkin h(k=0) %s, coarse levels %s" % (
-            args.c5_cycle, args.nu, args.nu, args.p_modes, args.c5_smoother, args.c5_coarse_smoother)
+        label = "hp-MG %s-cycle p%s+Galerkin h(k=0), finest level (%d,%d) %s, coarse levels (%d,%d) %s" % (
+            args.c5_cycle, args.p_modes, args.nu, args.c5_nu_post, args.c5_smoother, cnu[0], cnu[1],
+            args.c5_coarse_smoother)
     krylov = "fcg" if args.c5_cycle == "K" else "cg"
 
     def solver(**kw):
@@ -799,10 +801,13 @@ def main():
     ap.add_argument("--c5-smoother", default="cheb1", help="c5: smoother of the finest level")
     ap.add_argument("--c5-coarse-smoother", default="cheb4", help="c5: smoother of the levels below the finest")
     ap.add_argument("--c5-theta", type=float, default=0.1, help="c5: strong-coupling threshold of the aggregation")
-    ap.add_argument("--fcg-m", type=int, default=2,
+    ap.add_argument("--c5-nu-post", type=int, default=0,
+                    help="c5: post-smoothing steps of the finest level (pre-smoothing: --nu)")
+    ap.add_argument("--c5-coarse-nu", default="1,1", help="c5: pre,post smoothing steps below the finest level")
+    ap.add_argument("--fcg-m", type=int, default=4,
                     help="c5: FCG truncation (directions kept) of the FP64 and mixed legs")
     ap.add_argument("--max-coarse", type=int, default=2500, help="c5: largest coarsest level (dense solve)")
-    ap.add_argument("--replace-every", type=int, default=4,
+    ap.add_argument("--replace-every", type=int, default=8,
                     help="c5 FP32 outer product: iterations between residual replacements")
     ap.add_argument("--kcycle-levels", type=int, default=2, help="K-cycle on levels 1..this one, V below")
     ap.add_argument("--c5-mixed", default="f32,f32o",

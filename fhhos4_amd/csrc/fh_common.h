@@ -133,14 +133,17 @@ struct Sliced {
 // A = sum_c P_c^T S_c P_c (S_c: the condensed matrix of cell c over its
 // nfc faces, m = nfc * nf unknowns, m <= 64 and even), kept from the
 // assembly.  Each S_c is symmetric and stored by cyclic diagonals: diagonal
-// t = 0 .. m/2 holds S[l][(l + t) mod m] at t m + l (l = 0 .. m-1), so every
-// unordered pair of unknowns is stored once (twice on diagonal m/2);
-// elem_stride(m) = (m/2 + 1) m values per cell.  Cells are in two colours
+// t = 0 .. m/2 holds S[l][(l + t) mod m] (l = 0 .. m-1), so every unordered
+// pair of unknowns is stored once (twice on diagonal m/2).  Diagonal 0 sits
+// at l, diagonals 2p + 1 and 2p + 2 interleaved at m + 2 m p + 2 l + {0, 1}
+// (one two-value load per lane and pair; an unused slot when m/2 is odd);
+// elem_stride(m) values per cell.  Cells are in two colours
 // such that every unknown face belongs to exactly one cell of each colour (a
 // bipartite cell graph, such as a hexahedral grid's); colour 0 first
 // (ncol[0] cells), then colour 1.  rows holds the block row of every
 // (cell, face), -1 for an eliminated face.
-inline int elem_stride(int m) { return (m / 2 + 1) * m; }
+inline int elem_stride(int m) { return m * (1 + 2 * ((m / 2 + 1) / 2)); }
+inline int elem_pos(int m, int t, int l) { return t == 0 ? l : m + 2 * m * ((t - 1) / 2) + 2 * l + (t - 1) % 2; }
 struct ElemLayout {
     int nfc = 0, nf = 0, m = 0, stride = 0;
     int ncol[2] = {0, 0};

@@ -13,7 +13,9 @@
 // S[l][(l + t) mod m] at position l, l = 0 .. m-1.  Every unordered pair
 // {i, j} lies on exactly one stored diagonal (t = (j - i) mod m or
 // (i - j) mod m, whichever is <= m/2; diagonal m/2 holds both orientations),
-// and every stored diagonal is a full run of m values.
+// and every stored diagonal is a full run of m values.  Diagonals 2p + 1
+// and 2p + 2 are interleaved, so that lane l reads both with one two-value
+// load, and the wavefront reads one contiguous run per pair.
 //
 // 64 / m cells per wavefront (one at k = 2 and 3, three at k = 1, ten at
 // k = 0), lane g m + l owning local unknown l of the g-th cell (face l / nf,
@@ -44,6 +46,11 @@ __device__ __forceinline__ T ld_stream(const T* p) {
     return __builtin_nontemporal_load(p);
 }
 
+typedef double elem_d2 __attribute__((ext_vector_type(2)));
+typedef float elem_f2 __attribute__((ext_vector_type(2)));
+template <class TA> struct ElemPair { using T = elem_d2; };
+template <> struct ElemPair<float> { using T = elem_f2; };
+
 // MODE: OP_SPMV  y = alpha A x + beta y
 //       OP_RESID y = b - A x
 //       OP_CHEB  d = alpha d + beta D^-1 (b - A x), y = x + d   (d in aux)
@@ -71,40 +78,54 @@ __global__ __launch_bounds__(kElemWaves * 64) void k_elem(int ncell, int64_t cel
     const int row = has ? frow[cell * kElemFaces + f] : -1;
     const int64_t e = (int64_t)row * NF + i;
     const AT xl = row >= 0 ? (AT)x[e] : (AT)0;
+    // the Chebyshev step's row of D^-1, loaded before the product
+    TD dinv[MODE == OP_CHEB && PASS == 1 ? NF : 1];
+    if constexpr (MODE == OP_CHEB && PASS == 1) {
+        const TD* D = invD + (row >= 0 ? e : 0) * NF;
+#pragma unroll
+        for (int k = 0; k < NF; k++) dinv[k] = D[k];
+    }
     // lanes without a cell read inside cell 0's values (l < m) and drop them
-    const TA* v = vals + cell * stride + l;
+    const TA* v = vals + cell * stride;
+    using PT = typename ElemPair<TA>::T;
+    const PT* vp = (const PT*)(v + M) + l;  // pair p of lane l at vp[p m]
     const int base = (q < G ? q : 0) * M;  // first lane of the group
-    AT acc = has ? (AT)ld_stream(v) * xl : (AT)0;  // diagonal 0
-    // Diagonals 1 .. m/2 in an even number NCH of chunks of C, two chunks
-    // per iteration of a loop that is not unrolled: the loads of one chunk
-    // are in flight while the other is used, and the compiler cannot hoist
-    // every load and shuffle of the cell (which would leave one wavefront
-    // per SIMD).  Loads are unconditional (a slot past the last diagonal
-    // re-reads it and is dropped), so that the loop body is one block.
-    constexpr int NCH = H > 16 ? 6 : 2;
-    constexpr int C = (H + NCH - 1) / NCH;
-    AT ua[C], ub[C];
-    auto load_chunk = [&](int ch, AT* u) {
+    AT acc = has ? (AT)ld_stream(v + l) * xl : (AT)0;  // diagonal 0
+    // Pairs of diagonals (2p + 1, 2p + 2), p < P, in an even number NCH of
+    // chunks of C pairs, two chunks per iteration of a loop that is not
+    // unrolled: the loads of one chunk are in flight while the other is
+    // used, and the compiler cannot hoist every load and shuffle of the cell
+    // (which would leave one wavefront per SIMD).  Loads are unconditional
+    // (a slot past the last pair re-reads it and is dropped), so that the
+    // loop body is one block.
+    constexpr int P = (H + 1) / 2;
+    constexpr int NCH = P > 10 ? 4 : 2;
+    constexpr int C = (P + NCH - 1) / NCH;
+    PT ua[C], ub[C];
+    auto load_chunk = [&](int ch, PT* u) {
 #pragma unroll
         for (int j = 0; j < C; j++) {
-            const int t = 1 + ch * C + j;
-            u[j] = (AT)ld_stream(v + (t <= H ? t : H) * M);
+            const int p = ch * C + j;
+            u[j] = ld_stream(vp + (p < P ? p : P - 1) * M);
         }
     };
-    auto use_chunk = [&](int ch, const AT* u) {
+    auto use_diag = [&](int t, AT uj) {
+        int src = l + t;
+        src = base + (src < M ? src : src - M);
+        acc += uj * __shfl(xl, src, 64);
+        if (t < H) {
+            int dst = l - t;
+            dst = base + (dst >= 0 ? dst : dst + M);
+            acc += __shfl(uj * xl, dst, 64);
+        }
+    };
+    auto use_chunk = [&](int ch, const PT* u) {
 #pragma unroll
         for (int j = 0; j < C; j++) {
-            const int t = 1 + ch * C + j;
-            if (t > H) break;  // wavefront-uniform
-            const AT uj = has ? u[j] : (AT)0;
-            int src = l + t;
-            src = base + (src < M ? src : src - M);
-            acc += uj * __shfl(xl, src, 64);
-            if (t < H) {
-                int dst = l - t;
-                dst = base + (dst >= 0 ? dst : dst + M);
-                acc += __shfl(uj * xl, dst, 64);
-            }
+            const int p = ch * C + j;
+            if (p >= P) break;  // wavefront-uniform
+            use_diag(2 * p + 1, has ? (AT)u[j].x : (AT)0);
+            if (2 * p + 2 <= H) use_diag(2 * p + 2, has ? (AT)u[j].y : (AT)0);
         }
     };
     load_chunk(0, ua);
@@ -134,10 +155,9 @@ __global__ __launch_bounds__(kElemWaves * 64) void k_elem(int ncell, int64_t cel
             // z = D^-1 (b - A x) over the face's nf residuals, held by the
             // face's lanes; every lane takes part in the shuffles
             const AT res = row >= 0 ? (AT)y[e] - acc : (AT)0;
-            const TD* D = invD + (row >= 0 ? e : 0) * NF;
             TD dk[NF];
 #pragma unroll
-            for (int k = 0; k < NF; k++) dk[k] = row >= 0 ? D[k] : (TD)0;
+            for (int k = 0; k < NF; k++) dk[k] = row >= 0 ? dinv[k] : (TD)0;
             AT z = 0;
 #pragma unroll
             for (int k = 0; k < NF; k++) z += (AT)dk[k] * __shfl(res, base + f * NF + k, 64);

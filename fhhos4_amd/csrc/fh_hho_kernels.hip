@@ -25,11 +25,13 @@ namespace {
 constexpr int kT = 64;
 
 // position of entry (r, s) of a cell matrix of n unknowns stored by cyclic
-// diagonals (diagonal t = (s - r) mod n at t n + r), or -1 when the entry is
-// stored as its transpose (t > n / 2)
+// diagonals (diagonal t = (s - r) mod n, position r; diagonal 0 first, then
+// diagonals 2p + 1 and 2p + 2 interleaved), or -1 when the entry is stored
+// as its transpose (t > n / 2)
 __device__ __forceinline__ int elem_entry(int n, int r, int s) {
     const int t = s >= r ? s - r : s - r + n;
-    return t <= n / 2 ? t * n + r : -1;
+    if (t > n / 2) return -1;
+    return t == 0 ? r : n + 2 * n * ((t - 1) / 2) + 2 * r + (t - 1) % 2;
 }
 
 __device__ double source_at(const HhoDev& d, double x, double y, double z) {
