@@ -70,7 +70,14 @@ __global__ __launch_bounds__(kElemWaves * 64) void k_elem(int ncell, int64_t cel
     constexpr int H = M / 2;   // last stored diagonal
     const int lane = threadIdx.x & 63;
     const int q = lane / M, l = lane - q * M;  // group, local unknown
-    const int64_t w = (int64_t)blockIdx.x * kElemWaves + (threadIdx.x >> 6);
+    // workgroup -> cells: every XCD takes a contiguous range of cells, so
+    // faces that share a cache line are fetched into one L2
+    int bid = blockIdx.x;
+    {
+        const int nwg = gridDim.x, xcd = bid & 7, idx = bid >> 3, per = nwg >> 3, rem = nwg & 7;
+        bid = xcd * per + (xcd < rem ? xcd : rem) + idx;
+    }
+    const int64_t w = (int64_t)bid * kElemWaves + (threadIdx.x >> 6);
     if (w * G >= ncell) return;  // wavefront-uniform; no barrier follows
     const bool has = q < G && w * G + q < ncell;
     const int64_t cell = cell0 + (has ? w * G + q : 0);
@@ -99,7 +106,7 @@ __global__ __launch_bounds__(kElemWaves * 64) void k_elem(int ncell, int64_t cel
     // (a slot past the last pair re-reads it and is dropped), so that the
     // loop body is one block.
     constexpr int P = (H + 1) / 2;
-    constexpr int NCH = P > 10 ? 4 : 2;
+    constexpr int NCH = P > 10 ? 8 : 2;
     constexpr int C = (P + NCH - 1) / NCH;
     PT ua[C], ub[C];
     auto load_chunk = [&](int ch, PT* u) {
