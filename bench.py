@@ -58,7 +58,8 @@ def elem_spmv_bytes(ncell, nb, nf, sv=8, sx=8, nfc=6):
 # rocprofv3 FETCH_SIZE / WRITE_SIZE summaries of the fine-level SpMV, per layout
 TRAFFIC_SUMMARY = {"sliced": os.path.join(REPO, "profiles", "sliced_spmv_traffic_tet64_k2.md"),
                    "bcsr": os.path.join(REPO, "profiles", "bcsr_spmv_traffic_tet64_k2.md"),
-                   "c5": os.path.join(REPO, "profiles", "c5_hex96_k3_fp64_traffic.md")}
+                   "c5": os.path.join(REPO, "profiles", "c5_hex96_k3_fp64_traffic.md"),
+                   "c5elem": os.path.join(REPO, "profiles", "c5_hex96_k3_elem_traffic.md")}
 
 
 def measured_traffic(workload_key, summary, field="f64_bytes"):
@@ -738,8 +739,9 @@ def c5_bench(args, G, np, torch):
         "roofline": {"bound": "hbm", "achieved": kern[dom]["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": kern[dom]["GBs"] / HBM_PEAK_GBS,
                      "traffic": measured_traffic("hex n=%d k=%d perturb %g sigma %g seed %d" % (
-                         args.hex_n, args.hex_k, args.hex_perturb, args.hex_sigma, args.seed), "c5", dom)
-                     if layout == "sliced" else None,
+                         args.hex_n, args.hex_k, args.hex_perturb, args.hex_sigma, args.seed),
+                         {"sliced": "c5", "elem": "c5elem"}[layout], dom)
+                     if layout in ("sliced", "elem") else None,
                      "kernel": "fine-level %s (%s layout, n_f = %d)" % (dom, layout, nf),
                      "bytes_per_launch": kern[dom]["bytes"], "mean_ms": kern[dom]["mean_ms"],
                      "launches": kern[dom]["launches_per_solve"]},
