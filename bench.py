@@ -517,7 +517,8 @@ def c5_learned_leg(args, G, ctx, P, A, b, x, x64, t64, base, mixed_ref, l2_64):
     del held, held_sys, untrained
     out = {}
     for m in sorted({args.fcg_m, args.learned_fcg_m}):
-        s = G.Solver(ctx, **dict(base, solver="fcg", mg_prec="f32", mg_store="f32", fcg_truncation=m))
+        s = G.Solver(ctx, **dict(base, solver="fcg", mg_prec="f32", mg_store="f32", fcg_truncation=m,
+                                 outer_prec="f32", replace_every=args.replace_every))
         s.set_learned_smoother(nn, graph, 1.0)
         t2 = time.perf_counter()
         s.setup(A)

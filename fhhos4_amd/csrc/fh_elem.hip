@@ -234,9 +234,12 @@ bool elem_dispatch(Ctx& c, const ElemLayout& E, BcsrMode mode, Prec vp, const vo
             return false;
         return true;
     }
-    if (E.prec == Prec::F32 && vp == Prec::F64 && mode == OP_SPMV) {
+    if (E.prec == Prec::F32 && vp == Prec::F64 && (mode == OP_SPMV || mode == OP_RESID)) {
         // FP32-stored values with FP64 vectors and accumulation
-        elem_two_pass<NF, float, double, double, OP_SPMV>(c, E, x, b, y, alpha, beta, nullptr, nullptr);
+        if (mode == OP_SPMV)
+            elem_two_pass<NF, float, double, double, OP_SPMV>(c, E, x, b, y, alpha, beta, nullptr, nullptr);
+        else
+            elem_two_pass<NF, float, double, double, OP_RESID>(c, E, x, b, y, alpha, beta, nullptr, nullptr);
         return true;
     }
     return false;

@@ -53,7 +53,7 @@ void layout_epoch_bump();
 // launches, one per cell colour.  Returns false, having launched nothing,
 // when A has no active element layout or the mode and precisions are not
 // covered (FP64 values and vectors; FP32 values and vectors; FP32 values
-// with FP64 vectors for OP_SPMV).  x must not alias y.
+// with FP64 vectors for OP_SPMV and OP_RESID).  x must not alias y.
 bool launch_elem(Ctx& c, const Bcsr& A, BcsrMode mode, Prec vprec, const void* x, const void* b, void* y,
                  double alpha, double beta, const void* invD, Prec dprec, void* aux);
 // Makes the element layout of A active (on, when A has one) or not; returns

@@ -841,9 +841,21 @@ void Solver::apply_prec(const DVec& r, DVec& z, DVec* Az) {
         copy(c_, n, Prec::F64, r.data(), z.data());
     if (nn_) {
         const Prec P = Prec::F64;
-        launch_bcsr(c_, *A_, OP_RESID, P, z.data(), r.data(), r1_.data(), 1.0, 0.0, 0.0, nullptr, P, nullptr, 0);
+        // with outer_prec FP32, the network's two products use the FP32 copy
+        // of the finest level when it has an element layout (FP64 vectors
+        // and accumulation); FCG then replaces its residual as for its own
+        // FP32-stored product
+        const Bcsr* A32 = nullptr;
+        if (args_.outer_prec == Prec::F32 && mg_ && !mg_->L.empty() && mg_->L[0].Aw &&
+            mg_->L[0].Aw->prec == Prec::F32 && mg_->L[0].Aw->elem_on && elem_enabled())
+            A32 = mg_->L[0].Aw;
+        nn_f32_products_ = A32 != nullptr;
+        launch_bcsr(c_, A32 ? *A32 : *A_, OP_RESID, P, z.data(), r.data(), r1_.data(), 1.0, 0.0, 0.0, nullptr, P,
+                    nullptr, 0);
         nn_->forward(*nng_, (const double*)r1_.data(), (double*)delta_.data(), true);
-        launch_bcsr(c_, *A_, OP_SPMV, P, delta_.data(), nullptr, Adelta_.data(), 1.0, 0.0, 0.0, nullptr, P, nullptr, 0);
+        if (!(A32 && launch_spmv_f32vals(c_, *A32, delta_.data(), Adelta_.data())))
+            launch_bcsr(c_, *A_, OP_SPMV, P, delta_.data(), nullptr, Adelta_.data(), 1.0, 0.0, 0.0, nullptr, P, nullptr,
+                        0);
         const void* xs[2] = {delta_.data(), delta_.data()};
         const void* ys[2] = {r1_.data(), Adelta_.data()};
         int sl[2] = {kNnSlotS1, kNnSlotS2};
@@ -1027,7 +1039,7 @@ void Solver::solve_fcg(const double* b, double* x, bool xz) {
                 launch_fcg_orth(c_, n, (const double*)Az_.data(), (int)kept.size(), avs.data(), num.data(), den.data(),
                                 (double*)Ad.data());
         }
-        bool approx = false;  // A d formed with FP32-stored values
+        bool approx = nn_ != nullptr && nn_f32_products_;  // A d formed with FP32-stored values
         if (!haz) {
             approx = A32 && launch_spmv_f32vals(c_, *A32, d.data(), Ad.data());
             if (!approx)

@@ -187,6 +187,7 @@ private:
     std::vector<std::pair<const Bcsr*, const Bcsr*>> given_levels_;
     std::vector<int> given_p_;  // kept modes of a p-transfer below level l (0: none)
     int nn_fallbacks_seen_ = 0;  // block-Jacobi fallbacks of the learned post-smoother in this solve
+    bool nn_f32_products_ = false;  // the learned post-smoother's products used FP32-stored values
     // U-AMG hierarchy and the level matrices made from it
     std::unique_ptr<UamgHierarchy> uamg_;
     std::vector<std::unique_ptr<Bcsr>> uamg_mats_;

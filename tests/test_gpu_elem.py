@@ -125,3 +125,40 @@ def test_elem_solve_matches_block_rows_and_oracle(ctx, k, nu):
     assert np.linalg.norm(bh - As @ xm) < 1e-8 * np.linalg.norm(bh)
     assert np.linalg.norm(xm - xo) <= 1e-6 * ref
     print("FP32 multigrid + FP32 element-layout outer product: %d iterations" % st["iterations"])
+
+
+def test_learned_post_smoother_with_fp32_products(ctx):
+    """The learned post-smoother inside FCG(8) with the FP32 multigrid: with
+    outer_prec FP32 its two products per application (r1 = r - A z0 and
+    A delta) run on the FP32 element layout with FP64 vectors, and FCG
+    replaces its residual by the FP64 b - A x: the solve reaches the
+    tolerance for the FP64 residual, with the solution of the FP64-product
+    solve to 1e-6 and an iteration count within two of it.  The untrained
+    network (its correction is the block-Jacobi step) is used, so every
+    application is accepted or falls back deterministically."""
+    print("case hex n=12 k=3 learned, FP32 products", flush=True)
+    P = _problem(12, 3)
+    A, b = P.assemble_gpu(ctx, keep_host=True)
+    As = A.scipy()
+    bh = b.cpu().numpy()
+    nn = G.Mpnn(ctx, P.nf, layers=3, seed=1)
+    graph = G.NnGraph(ctx, A, G.face_features(P))
+    kw = dict(solver="fcg", prec="aggregamg", cycle="K", kcycle_levels=2, nu=(1, 0), coarse_nu=(1, 1),
+              smoothers=("cheb1", "cheb1"), coarse_smoother="cheb4", theta=0.1, max_coarse=2500, tol=1e-8,
+              max_iter=400, fcg_truncation=8, mg_prec="f32", mg_store="f32")
+    xs, its = {}, {}
+    for outer in ("f64", "f32"):
+        s = G.Solver(ctx, **kw, outer_prec=outer, replace_every=8)
+        s.set_learned_smoother(nn, graph, 1.0)
+        s.setup(A)
+        x = torch.zeros_like(b)
+        torch.cuda.synchronize()
+        st = s.solve(b, x)
+        assert st["converged"] == 1, outer
+        assert st["nn_used"] + st["nn_fallback"] == st["iterations"]
+        xs[outer], its[outer] = x.cpu().numpy(), st["iterations"]
+        del s
+    print("iterations: FP64 products %d, FP32 products %d" % (its["f64"], its["f32"]))
+    assert np.linalg.norm(bh - As @ xs["f32"]) < 1e-8 * np.linalg.norm(bh)
+    assert np.linalg.norm(xs["f32"] - xs["f64"]) <= 1e-6 * np.linalg.norm(xs["f64"])
+    assert abs(its["f32"] - its["f64"]) <= 2
