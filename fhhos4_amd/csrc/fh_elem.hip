@@ -265,6 +265,7 @@ bool set_elem(const Bcsr& A, bool on) {
 bool launch_elem(Ctx& c, const Bcsr& A, BcsrMode mode, Prec vp, const void* x, const void* b, void* y, double alpha,
                  double beta, const void* invD, Prec dp, void* aux) {
     if (!A.elem || !A.elem_on || !elem_enabled()) return false;
+    FH_CHECK(x != y, -3, "element-layout product: x must not alias y");
     const ElemLayout& E = *A.elem;
     FH_CHECK(E.nf == A.bs_r && E.nfc == kElemFaces && E.m == E.nfc * E.nf && E.stride == elem_stride(E.m), -3,
              "element layout does not match");

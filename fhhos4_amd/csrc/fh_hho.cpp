@@ -2048,7 +2048,7 @@ std::unique_ptr<Bcsr> HhoProblem::AssembleDevice(Ctx& c, double* b_dev, bool kee
     std::shared_ptr<ElemLayout> E;
     std::vector<int> pos;
     int ncol[2];
-    if (nf * I.nfc <= 64 && nf * I.nfc % 2 == 0 && elem_colouring(I, pos, ncol)) {
+    if (elem_enabled() && nf * I.nfc <= 64 && nf * I.nfc % 2 == 0 && elem_colouring(I, pos, ncol)) {
         E = std::make_shared<ElemLayout>();
         E->nfc = I.nfc;
         E->nf = nf;
