@@ -97,6 +97,10 @@ void launch_axpby(Ctx& c, int64_t n, Prec p, const void* x, void* y, double ha, 
 // copy when nv = 0).
 void launch_fcg_orth(Ctx& c, int64_t n, const double* z, int nv, const double* const* vs, const int* num,
                      const int* den, double* out);
+// The FP64 update fused with s[slot_rr] = <r, r> of the updated residual
+// (the value a separate launch_dots gives).
+void launch_cg_update_rr(Ctx& c, int64_t n, double* x, const double* d, double* r, const double* q, int inum,
+                         int iden, double sign, int slot_rr);
 // x += a d ; r -= a q with a = s[inum] / s[iden]  (CG / FCG update).
 void launch_cg_update(Ctx& c, int64_t n, Prec p, void* x, const void* d, void* r, const void* q, int inum, int iden,
                       double sign);

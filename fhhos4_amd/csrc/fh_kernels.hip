@@ -1048,19 +1048,42 @@ __device__ __forceinline__ void block_reduce(double (&v)[3], double* sh) {
     }
 }
 
-template <class T, int ND>
+// SX: every product has the same x (read once; the products and sums are
+// those of separate x reads)
+template <class T, int ND, bool SX>
 __global__ __launch_bounds__(kWG) void k_dots(int64_t n, const T* x0, const T* y0, const T* x1, const T* y1,
                                                const T* x2, const T* y2, double* partials) {
     __shared__ double sh[3 * 16];
     double v[3] = {0, 0, 0};
     for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
-        v[0] += (double)x0[t] * (double)y0[t];
-        if (ND > 1) v[1] += (double)x1[t] * (double)y1[t];
-        if (ND > 2) v[2] += (double)x2[t] * (double)y2[t];
+        const double xv = (double)x0[t];
+        v[0] += xv * (double)y0[t];
+        if (ND > 1) v[1] += (SX ? xv : (double)x1[t]) * (double)y1[t];
+        if (ND > 2) v[2] += (SX ? xv : (double)x2[t]) * (double)y2[t];
     }
     block_reduce<ND>(v, sh);
     if (threadIdx.x == 0)
         for (int k = 0; k < ND; k++) partials[k * kDotBlocks + blockIdx.x] = v[k];
+}
+
+// The CG / FCG update fused with |r|^2 of the updated residual, on the
+// reduction grid of k_dots (the same per-thread order and tree, so the sum
+// is the one a separate dot product of r with itself gives).
+__global__ __launch_bounds__(kWG) void k_cg_update_rr(int64_t n, double* __restrict__ x, const double* __restrict__ d,
+                                                      double* __restrict__ r, const double* __restrict__ q,
+                                                      const double* s, int inum, int iden, double sign,
+                                                      double* partials) {
+    __shared__ double sh[3 * 16];
+    const double al = sign * s[inum] / s[iden];
+    double v[3] = {0, 0, 0};
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+        x[t] = x[t] + al * d[t];
+        const double rn = r[t] - al * q[t];
+        r[t] = rn;
+        v[0] += rn * rn;
+    }
+    block_reduce<1>(v, sh);
+    if (threadIdx.x == 0) partials[blockIdx.x] = v[0];
 }
 
 template <int ND>
@@ -2070,6 +2093,16 @@ void launch_fcg_orth(Ctx& c, int64_t n, const double* z, int nv, const double* c
     }
 }
 
+void launch_cg_update_rr(Ctx& c, int64_t n, double* x, const double* d, double* r, const double* q, int inum,
+                         int iden, double sign, int slot_rr) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_cg_update_rr, dim3(kDotBlocks), dim3(kWG), 0, c.stream, n, x, d, r, q, c.d_scalars, inum, iden,
+                       sign, c.d_partials);
+    hipLaunchKernelGGL((k_dots_final<1>), dim3(1), dim3(kDotBlocks), 0, c.stream, c.d_partials, c.d_scalars, slot_rr, 0,
+                       0);
+    FH_HIP(hipGetLastError());
+}
+
 void launch_cg_update(Ctx& c, int64_t n, Prec p, void* x, const void* d, void* r, const void* q, int inum, int iden,
                       double sign) {
     if (n <= 0) return;
@@ -2087,9 +2120,18 @@ void launch_dots(Ctx& c, int64_t n, Prec p, int nd, const void* const* xs, const
     FH_CHECK(nd >= 1 && nd <= 3, -3, "1 to 3 dot products per launch");
     const void* X[3] = {xs[0], nd > 1 ? xs[1] : xs[0], nd > 2 ? xs[2] : xs[0]};
     const void* Y[3] = {ys[0], nd > 1 ? ys[1] : ys[0], nd > 2 ? ys[2] : ys[0]};
-#define FH_DOTS(T, ND)                                                                                            \
-    hipLaunchKernelGGL((k_dots<T, ND>), dim3(kDotBlocks), dim3(kWG), 0, c.stream, n, (const T*)X[0], (const T*)Y[0], \
-                       (const T*)X[1], (const T*)Y[1], (const T*)X[2], (const T*)Y[2], c.d_partials)
+    const bool sx = X[1] == X[0] && X[2] == X[0];
+#define FH_DOTS(T, ND)                                                                                              \
+    do {                                                                                                            \
+        if (sx)                                                                                                     \
+            hipLaunchKernelGGL((k_dots<T, ND, true>), dim3(kDotBlocks), dim3(kWG), 0, c.stream, n, (const T*)X[0],   \
+                               (const T*)Y[0], (const T*)X[1], (const T*)Y[1], (const T*)X[2], (const T*)Y[2],       \
+                               c.d_partials);                                                                       \
+        else                                                                                                        \
+            hipLaunchKernelGGL((k_dots<T, ND, false>), dim3(kDotBlocks), dim3(kWG), 0, c.stream, n, (const T*)X[0],  \
+                               (const T*)Y[0], (const T*)X[1], (const T*)Y[1], (const T*)X[2], (const T*)Y[2],       \
+                               c.d_partials);                                                                       \
+    } while (0)
     if (p == Prec::F64) {
         if (nd == 1) FH_DOTS(double, 1); else if (nd == 2) FH_DOTS(double, 2); else FH_DOTS(double, 3);
     } else {

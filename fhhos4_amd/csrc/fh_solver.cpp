@@ -1046,15 +1046,17 @@ void Solver::solve_fcg(const double* b, double* x, bool xz) {
                 launch_bcsr(c_, *A_, OP_SPMV, P, d.data(), nullptr, Ad.data(), 1.0, 0.0, 0.0, nullptr, P, nullptr, 0);
         }
         since_replace = approx ? since_replace + 1 : since_replace;
-        dot2(c_, n, P, d.data(), Ad.data(), S_DAD, r_.data(), d.data(), S_RD);
-        launch_cg_update(c_, n, P, x, d.data(), r_.data(), Ad.data(), S_RD, S_DAD, 1.0);
+        // d.Ad and d.r (= r.d) in one pass over d
+        dot2(c_, n, P, d.data(), Ad.data(), S_DAD, d.data(), r_.data(), S_RD);
+        // x += a d, r -= a Ad and |r|^2 in one pass
+        launch_cg_update_rr(c_, n, x, (const double*)d.data(), (double*)r_.data(), (const double*)Ad.data(), S_RD,
+                            S_DAD, 1.0, S_RR);
         if ((int)kept.size() == m) {
             if (args_.fcg_restart) kept.clear();
             else kept.erase(kept.begin());
         }
         launch_scalar_op(c_, 4, slot_dad(cur), S_DAD, 0);
         kept.push_back(cur);
-        dot1(c_, n, P, r_.data(), r_.data(), S_RR);
         it++;
         double rr = read_scalar(S_RR);
         res = bn > 0 ? std::sqrt(rr) / bn : std::sqrt(rr);
