@@ -331,9 +331,12 @@ def learned2d_bench(args, G, np, torch):
     base = dict(dim=2, mesh=args.mesh2d, k=2, perturb=0.2, kappa="lognormal", sigma=2.0)
     t0 = time.perf_counter()
     train_sys = system(G.HhoProblem(n=args.train_n2d, seed=11, source="pisine", **base))
-    rhs = [G.HhoProblem(n=args.train_n2d, seed=11, source="modes", source_seed=1000 + j, **base).rhs()
-           for j in range(args.train_rhs)]
-    pairs = Lr.make_pairs(ctx, train_sys, rhs, iters=(1, 3, 6))
+    rhs = []
+    for j in range(args.train_rhs):
+        rhs.append(G.HhoProblem(n=args.train_n2d, seed=11, source="modes", source_seed=1000 + j, **base).rhs())
+        if j and j % 200 == 0:
+            log("training right-hand sides: %d" % j)
+    pairs = Lr.make_pairs(ctx, train_sys, rhs, iters=(1, 3, 6), log=log)
     t_pairs = time.perf_counter() - t0
     nf = train_sys.problem.nf
     nn = G.Mpnn(ctx, nf, layers=3, seed=1)

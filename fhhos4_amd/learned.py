@@ -36,9 +36,9 @@ class System:
         self.tol, self.max_iter = tol, max_iter
 
 
-def make_pairs(ctx, system, rhs, iters=(1, 3, 6)):
+def make_pairs(ctx, system, rhs, iters=(1, 3, 6), log=None):
     """(r1, e1) device pairs of the right-hand sides `rhs` (host arrays) at
-    the FCG iterations `iters`."""
+    the FCG iterations `iters`; `log(message)` every 100 right-hand sides."""
     n = system.n
     s = system.solver
     xs = torch.zeros(n, dtype=torch.float64, device="cuda")
@@ -46,7 +46,9 @@ def make_pairs(ctx, system, rhs, iters=(1, 3, 6)):
     rk = torch.zeros_like(xs)
     z0 = torch.zeros_like(xs)
     pairs = []
-    for b_host in rhs:
+    for j, b_host in enumerate(rhs):
+        if log and j and j % 100 == 0:
+            log("pairs: %d right-hand sides solved, %d pairs" % (j, len(pairs)))
         b = torch.from_numpy(np.ascontiguousarray(b_host)).cuda()
         s.set_stop(system.tol, system.max_iter)
         s.solve(b, xs)
