@@ -344,8 +344,9 @@ def learned2d_bench(args, G, np, torch):
     t1 = time.perf_counter()
     hist = Lr.train(nn, pairs, epochs=args.train_epochs, lr=2e-3, log=log)
     t_train = time.perf_counter() - t1
-    log("trained on %d^2 x 2 triangles: %d pairs (%.1fs), %.1fs, energy error %.3f -> %.3f"
-        % (args.train_n2d, len(pairs), t_pairs, t_train, before, hist[-1]))
+    log("trained on the %d^2 %s mesh: %d pairs (%.1fs), %.1fs, energy error %.3f -> %.3f"
+        % (args.train_n2d, "quadrilateral" if args.mesh2d == "quad" else "triangle", len(pairs), t_pairs, t_train,
+           before, hist[-1]))
     P = G.HhoProblem(n=args.n, seed=12, source="modes", source_seed=5000, **base)
     ev = system(P)
     held = Lr.make_pairs(ctx, ev, [G.HhoProblem(n=args.n, seed=12, source="modes", source_seed=6000 + j,
