@@ -814,7 +814,7 @@ def main():
     ap.add_argument("--fcg-m", type=int, default=4,
                     help="c5: FCG truncation (directions kept) of the FP64 and mixed legs")
     ap.add_argument("--max-coarse", type=int, default=2500, help="c5: largest coarsest level (dense solve)")
-    ap.add_argument("--replace-every", type=int, default=8,
+    ap.add_argument("--replace-every", type=int, default=12,
                     help="c5 FP32 outer product: iterations between residual replacements")
     ap.add_argument("--kcycle-levels", type=int, default=2, help="K-cycle on levels 1..this one, V below")
     ap.add_argument("--c5-mixed", default="f32,f32o",
